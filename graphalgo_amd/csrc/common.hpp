@@ -1,0 +1,189 @@
+// Internal runtime of the gfx950 library: errors, device context, device
+// buffers and the device-side primitives (wave scans, block scans) shared by
+// the kernels.  Wavefronts are 64 lanes wide on CDNA4; every primitive here
+// is written for that width.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "graphalgo.h"
+
+namespace ga {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+void set_error(int code, const std::string& msg);
+
+#define GA_HIP(x)                                                                               \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess) {                                                                     \
+      (void)hipGetLastError();                                                                  \
+      throw ::ga::Error{GA_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)};            \
+    }                                                                                           \
+  } while (0)
+
+#define GA_REQUIRE(cond, code, msg)                \
+  do {                                             \
+    if (!(cond)) throw ::ga::Error{(code), (msg)}; \
+  } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return GA_OK;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(GA_ERR_OOM, "host allocation failed");
+    return GA_ERR_OOM;
+  } catch (const std::exception& e) {
+    set_error(GA_ERR_HIP, e.what());
+    return GA_ERR_HIP;
+  }
+}
+
+// Grow-only device allocation owned by a workspace or handle.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; bytes = o.bytes;
+      o.p = nullptr; o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      throw Error{GA_ERR_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed"};
+    }
+    bytes = b;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  int64_t hbm_bytes = 0;
+  std::string name;
+  // kernel timing
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+  std::vector<hipEvent_t> event_pool;
+  int64_t prof_launches = 0;
+  int64_t prof_bytes_host = 0;  // host-known part of the algorithmic bytes
+  DevBuf prof_dev;              // [0] = wedges processed (u64)
+};
+
+Context& ctx();  // throws GA_ERR_STATE when ga_init has not run
+
+// Device view of a CSR matrix.
+struct DCsr {
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  const int32_t* rp = nullptr;
+  const int32_t* col = nullptr;
+  const void* val = nullptr;
+};
+
+// An owned device CSR (pattern or valued).
+struct OwnedCsr {
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  DevBuf rp, col, val;
+  DCsr view() const { return DCsr{nrows, ncols, nnz, rp.as<int32_t>(), col.as<int32_t>(), val.p}; }
+};
+
+size_t dtype_size(int dtype);
+void validate_host_csr(const ga_csr* A, const char* name);
+void upload_csr(const ga_csr* A, int dtype, OwnedCsr& out, hipStream_t s);
+
+// ---- device primitives --------------------------------------------------
+template <typename T>
+__device__ inline T wave_incl_scan(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ inline T wave_reduce_sum(T v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ inline void atomic_add_val(int32_t* p, int32_t v) { atomicAdd(p, v); }
+__device__ inline void atomic_add_val(int64_t* p, int64_t v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
+}
+__device__ inline void atomic_add_val(double* p, double v) { atomicAdd(p, v); }
+
+// ---- host launchers shared between translation units -------------------
+// Exclusive scan of n int32 counts into out[0..n]; out[n] = total.  Totals
+// must fit in int32 (they index CSR arrays).  tmp is grown as needed.
+void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf& tmp, hipStream_t s);
+// Fill n 32-bit words with a value (graph-capture safe: a kernel, not a memset).
+void fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s);
+void fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s);
+
+// Workspace of the masked SpGEMM (row items and counters).
+struct MxmWork {
+  DevBuf cnt, start, item_row, item_t0, scan_tmp, counter;
+};
+
+// C<M> = A*B (structural mask).  mode 0: per-entry values into Cval (dtype);
+// mode 1: pattern inputs, int64 sum of all C entries accumulated into
+// sums[row_group ? row_group[i] : 0] (caller zeroes sums).
+void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,
+                    unsigned long long* sums, const int32_t* row_group, MxmWork& w, hipStream_t s);
+
+// Strict lower triangle of a square pattern with sorted rows, on device.
+void lower_triangle_dev(const DCsr& A, OwnedCsr& L, DevBuf& tmp, hipStream_t s);
+// Transpose of a pattern on device (rows of the result sorted).
+void transpose_dev(const DCsr& A, OwnedCsr& T, DevBuf& tmp, hipStream_t s);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {
+  int64_t g = ceil_div(n, block);
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+}  // namespace ga

@@ -1,0 +1,316 @@
+// Library runtime: context, error reporting, uploads, scans, fills and the
+// kernel-timing hooks.
+#include "common.hpp"
+
+#include <algorithm>
+#include <mutex>
+
+namespace ga {
+
+namespace {
+thread_local std::string t_err;
+std::mutex g_ctx_mu;
+Context* g_ctx = nullptr;
+}  // namespace
+
+void set_error(int code, const std::string& msg) { t_err = "[" + std::to_string(code) + "] " + msg; }
+
+Context& ctx() {
+  GA_REQUIRE(g_ctx != nullptr, GA_ERR_STATE, "ga_init has not been called");
+  GA_HIP(hipSetDevice(g_ctx->device));
+  return *g_ctx;
+}
+
+size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case GA_INT32: return 4;
+    case GA_INT64: return 8;
+    case GA_FLOAT64: return 8;
+  }
+  throw Error{GA_ERR_ARG, "unknown dtype " + std::to_string(dtype)};
+}
+
+void validate_host_csr(const ga_csr* A, const char* name) {
+  std::string nm(name);
+  GA_REQUIRE(A != nullptr, GA_ERR_ARG, nm + ": null matrix");
+  GA_REQUIRE(A->nrows >= 0 && A->ncols >= 0 && A->nnz >= 0, GA_ERR_ARG, nm + ": negative size");
+  GA_REQUIRE(A->nrows < INT32_MAX && A->ncols < INT32_MAX && A->nnz < INT32_MAX, GA_ERR_LIMIT,
+             nm + ": sizes must fit int32 CSR indexing");
+  GA_REQUIRE(A->rowptr != nullptr, GA_ERR_ARG, nm + ": null rowptr");
+  GA_REQUIRE(A->nnz == 0 || A->col != nullptr, GA_ERR_ARG, nm + ": null col");
+  GA_REQUIRE(A->rowptr[0] == 0 && A->rowptr[A->nrows] == A->nnz, GA_ERR_ARG, nm + ": rowptr does not span nnz");
+  for (int64_t i = 0; i < A->nrows; ++i) {
+    int32_t b = A->rowptr[i], e = A->rowptr[i + 1];
+    GA_REQUIRE(b <= e, GA_ERR_ARG, nm + ": rowptr not monotone at row " + std::to_string(i));
+    for (int32_t t = b; t < e; ++t) {
+      int32_t c = A->col[t];
+      GA_REQUIRE(c >= 0 && c < A->ncols, GA_ERR_UNSORTED, nm + ": column out of range in row " + std::to_string(i));
+      GA_REQUIRE(t == b || A->col[t - 1] < c, GA_ERR_UNSORTED,
+                 nm + ": row " + std::to_string(i) + " not sorted / has duplicates");
+    }
+  }
+}
+
+void upload_csr(const ga_csr* A, int dtype, OwnedCsr& out, hipStream_t s) {
+  out.nrows = A->nrows;
+  out.ncols = A->ncols;
+  out.nnz = A->nnz;
+  out.rp.ensure(sizeof(int32_t) * (A->nrows + 1));
+  out.col.ensure(sizeof(int32_t) * std::max<int64_t>(A->nnz, 1));
+  GA_HIP(hipMemcpyAsync(out.rp.p, A->rowptr, sizeof(int32_t) * (A->nrows + 1), hipMemcpyHostToDevice, s));
+  if (A->nnz) GA_HIP(hipMemcpyAsync(out.col.p, A->col, sizeof(int32_t) * A->nnz, hipMemcpyHostToDevice, s));
+  if (A->val) {
+    size_t es = dtype_size(dtype);
+    out.val.ensure(es * std::max<int64_t>(A->nnz, 1));
+    if (A->nnz) GA_HIP(hipMemcpyAsync(out.val.p, A->val, es * A->nnz, hipMemcpyHostToDevice, s));
+  } else {
+    out.val.release();
+  }
+}
+
+// ---- fills ---------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (uint32_t*)p, v, n);
+  GA_HIP(hipGetLastError());
+}
+void fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, (uint64_t*)p, v, n);
+  GA_HIP(hipGetLastError());
+}
+
+// ---- exclusive scan (reduce, scan tile sums, apply) ----------------------
+constexpr int SCAN_T = 256;
+constexpr int SCAN_PER = 8;
+constexpr int SCAN_TILE = SCAN_T * SCAN_PER;
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_tile_sums(const int32_t* __restrict__ in, int64_t n,
+                                                           long long* __restrict__ bsum) {
+  __shared__ long long red[SCAN_T / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  long long s = 0;
+  for (int q = threadIdx.x; q < SCAN_TILE; q += SCAN_T) {
+    int64_t idx = base + q;
+    if (idx < n) s += in[idx];
+  }
+  s = wave_reduce_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int w = 0; w < SCAN_T / 64; ++w) t += red[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tile_offsets(long long* __restrict__ bsum, int64_t nb,
+                                                            int32_t* __restrict__ out_total) {
+  __shared__ long long wtot[16];
+  __shared__ long long carry;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < nb; c0 += 1024) {
+    const int64_t idx = c0 + tid;
+    const long long v = idx < nb ? bsum[idx] : 0;
+    const long long incl = wave_incl_scan(v);
+    if (lane == 63) wtot[wv] = incl;
+    __syncthreads();
+    long long wbase = 0, ctot = 0;
+    for (int k = 0; k < 16; ++k) {
+      if (k < wv) wbase += wtot[k];
+      ctot += wtot[k];
+    }
+    const long long c = carry;
+    if (idx < nb) bsum[idx] = c + wbase + incl - v;
+    __syncthreads();
+    if (tid == 0) carry = c + ctot;
+    __syncthreads();
+  }
+  if (tid == 0) *out_total = (int32_t)carry;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_tile_apply(const int32_t* __restrict__ in,
+                                                            int32_t* __restrict__ out, int64_t n,
+                                                            const long long* __restrict__ bsum) {
+  __shared__ long long wtot[SCAN_T / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)tid * SCAN_PER;
+  int32_t v[SCAN_PER];
+  long long tsum = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_PER; ++q) {
+    v[q] = (base + q < n) ? in[base + q] : 0;
+    tsum += v[q];
+  }
+  const long long incl = wave_incl_scan(tsum);
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  long long off = bsum[blockIdx.x] + incl - tsum;
+  for (int k = 0; k < wv; ++k) off += wtot[k];
+#pragma unroll
+  for (int q = 0; q < SCAN_PER; ++q) {
+    if (base + q < n) out[base + q] = (int32_t)off;
+    off += v[q];
+  }
+}
+
+void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf& tmp, hipStream_t s) {
+  if (n <= 0) {
+    fill_u32(out, 0u, 1, s);
+    return;
+  }
+  const int64_t nb = ceil_div(n, SCAN_TILE);
+  tmp.ensure(sizeof(long long) * nb);
+  long long* bsum = tmp.as<long long>();
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, n, bsum);
+  hipLaunchKernelGGL(k_scan_tile_offsets, dim3(1), dim3(1024), 0, s, bsum, nb, out + n);
+  hipLaunchKernelGGL(k_scan_tile_apply, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, n, (const long long*)bsum);
+  GA_HIP(hipGetLastError());
+}
+
+// ---- lower triangle --------------------------------------------------------
+__global__ void k_lower_count(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                              int32_t* __restrict__ cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = rp[i], hi = rp[i + 1];
+    const int b = lo;
+    while (lo < hi) {  // first column >= i
+      int mid = (lo + hi) >> 1;
+      if (col[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    cnt[i] = lo - b;
+  }
+}
+
+__global__ void k_lower_copy(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                             const int32_t* __restrict__ lrp, int32_t* __restrict__ lcol) {
+  // one wavefront per row
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
+    const int src = rp[i], dst = lrp[i], len = lrp[i + 1] - lrp[i];
+    for (int q = lane; q < len; q += 64) lcol[dst + q] = col[src + q];
+  }
+}
+
+void lower_triangle_dev(const DCsr& A, OwnedCsr& L, DevBuf& tmp, hipStream_t s) {
+  const int64_t n = A.nrows;
+  L.nrows = n;
+  L.ncols = A.ncols;
+  L.rp.ensure(sizeof(int32_t) * (n + 1));
+  DevBuf cnt;
+  cnt.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
+  hipLaunchKernelGGL(k_lower_count, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, A.rp, A.col, n,
+                     cnt.as<int32_t>());
+  scan_exclusive_i32(cnt.as<int32_t>(), L.rp.as<int32_t>(), n, tmp, s);
+  int32_t lnnz = 0;
+  GA_HIP(hipMemcpyAsync(&lnnz, L.rp.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  GA_HIP(hipStreamSynchronize(s));
+  L.nnz = lnnz;
+  L.col.ensure(sizeof(int32_t) * std::max<int64_t>(lnnz, 1));
+  hipLaunchKernelGGL(k_lower_copy, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp, A.col, n,
+                     (const int32_t*)L.rp.as<int32_t>(), L.col.as<int32_t>());
+  GA_HIP(hipGetLastError());
+}
+
+}  // namespace ga
+
+using namespace ga;
+
+extern "C" {
+
+const char* ga_last_error(void) { return t_err.c_str(); }
+
+ga_status ga_init(int device) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (g_ctx && g_ctx->device == device) return;
+    int count = 0;
+    GA_HIP(hipGetDeviceCount(&count));
+    GA_REQUIRE(device >= 0 && device < count, GA_ERR_ARG, "no HIP device " + std::to_string(device));
+    GA_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    GA_HIP(hipGetDeviceProperties(&prop, device));
+    auto* c = new Context();
+    c->device = device;
+    c->num_cu = prop.multiProcessorCount;
+    c->hbm_bytes = (int64_t)prop.totalGlobalMem;
+    c->name = prop.gcnArchName;
+    GA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->prof_dev.ensure(64);
+    GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
+    GA_HIP(hipStreamSynchronize(c->stream));
+    g_ctx = c;
+  });
+}
+
+ga_status ga_device_info(char* name, int name_len, int* compute_units, int64_t* hbm_bytes) {
+  return guarded([&] {
+    Context& c = ctx();
+    if (name && name_len > 0) {
+      std::strncpy(name, c.name.c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (compute_units) *compute_units = c.num_cu;
+    if (hbm_bytes) *hbm_bytes = c.hbm_bytes;
+  });
+}
+
+ga_status ga_synchronize(void) {
+  return guarded([&] { GA_HIP(hipStreamSynchronize(ctx().stream)); });
+}
+
+ga_status ga_check_csr(const ga_csr* A) {
+  return guarded([&] { validate_host_csr(A, "A"); });
+}
+
+void ga_free_host(void* p) { std::free(p); }
+
+ga_status ga_prof_enable(int on) {
+  return guarded([&] { ctx().prof = on != 0; });
+}
+
+ga_status ga_prof_read(ga_kernel_prof* out, int reset) {
+  return guarded([&] {
+    GA_REQUIRE(out != nullptr, GA_ERR_ARG, "null output");
+    Context& c = ctx();
+    GA_HIP(hipStreamSynchronize(c.stream));
+    double ms = 0.0;
+    for (auto& pr : c.prof_events) {
+      float t = 0.f;
+      GA_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
+      ms += t;
+    }
+    unsigned long long w = 0;
+    GA_HIP(hipMemcpy(&w, c.prof_dev.p, sizeof(w), hipMemcpyDeviceToHost));
+    out->launches = c.prof_launches;
+    out->kernel_ms = ms;
+    out->wedges = (int64_t)w;
+    out->bytes_alg = c.prof_bytes_host + 4 * (int64_t)w;
+    if (reset) {
+      for (auto& pr : c.prof_events) {
+        c.event_pool.push_back(pr.first);
+        c.event_pool.push_back(pr.second);
+      }
+      c.prof_events.clear();
+      c.prof_launches = 0;
+      c.prof_bytes_host = 0;
+      GA_HIP(hipMemsetAsync(c.prof_dev.p, 0, 64, c.stream));
+      GA_HIP(hipStreamSynchronize(c.stream));
+    }
+  });
+}
+
+}  // extern "C"

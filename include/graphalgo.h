@@ -1,0 +1,159 @@
+/*
+ * graphalgo.h — C ABI of the MI355X (gfx950) masked-SpGEMM library.
+ *
+ * The reference (UAlex322/GraphAlgo) is a header-only C++ template library;
+ * its hot path is the masked sparse product C<M> = A*B over CSR and the three
+ * algorithms built on it.  Every entry point below names the reference
+ * interface it replaces.  The C++ drop-in surface with the reference's own
+ * names and signatures (spMtx, mxmm_msa, k_truss, ...) is in
+ * include/graphalgo/ and forwards to these functions.
+ *
+ * Conventions
+ *  - CSR with int32 row pointers and int32 column indices (the reference's
+ *    `int* Rst`, `int* Col`, include/matrix.h:31-32); rows sorted ascending
+ *    with no duplicate columns.  `val == NULL` means a pattern matrix whose
+ *    values are all 1.
+ *  - Every function returns a ga_status; 0 is success.  On failure
+ *    ga_last_error() returns a message for the calling thread.
+ *  - Host-pointer entry points are synchronous: inputs are copied to the GPU,
+ *    outputs copied back before return.  ga_graph_* entry points work on
+ *    graphs kept resident in HBM.
+ *  - The `isParallel` flag of the reference has no counterpart: there is one
+ *    GPU path.
+ */
+#ifndef GRAPHALGO_H
+#define GRAPHALGO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int ga_status;
+#define GA_OK 0
+#define GA_ERR_ARG 1       /* bad argument (null pointer, negative size, bad enum) */
+#define GA_ERR_DIM 2       /* dimension mismatch between operands */
+#define GA_ERR_UNSORTED 3  /* CSR rows not sorted / duplicate or out-of-range column */
+#define GA_ERR_OOM 4       /* device or host allocation failed / admission refused */
+#define GA_ERR_HIP 5       /* HIP runtime error */
+#define GA_ERR_LIMIT 6     /* size beyond a documented limit */
+#define GA_ERR_STATE 7     /* library not initialised / handle misuse */
+
+typedef enum { GA_INT32 = 0, GA_INT64 = 1, GA_FLOAT64 = 2 } ga_dtype;
+
+/* A CSR matrix view (host or device pointers depending on the call). */
+typedef struct {
+  int64_t nrows;
+  int64_t ncols;
+  int64_t nnz;
+  const int32_t* rowptr; /* nrows + 1 */
+  const int32_t* col;    /* nnz */
+  const void* val;       /* nnz values of the call's dtype, or NULL = all ones */
+} ga_csr;
+
+/* ---- library state ----------------------------------------------------- */
+const char* ga_last_error(void);
+/* Selects the HIP device and creates the library's stream; idempotent. */
+ga_status ga_init(int device);
+/* Device name, compute-unit count and HBM size of the initialised device. */
+ga_status ga_device_info(char* name, int name_len, int* compute_units, int64_t* hbm_bytes);
+/* Waits for all work the library queued. */
+ga_status ga_synchronize(void);
+/* Checks a host CSR for the input contract (sorted rows, columns in range).
+ * Returns GA_OK or GA_ERR_UNSORTED / GA_ERR_ARG. */
+ga_status ga_check_csr(const ga_csr* A);
+
+/* ---- masked SpGEMM ----------------------------------------------------- */
+/* C<M> = A*B with a structural mask: C has exactly M's pattern (explicit zeros
+ * kept) and Cval[j] = sum_k A(i,k) B(k, M.col[j]) for every stored j of row i.
+ * Replaces mxmm_msa / mxmm_mca / mxmm_heap (include/matrix_la.h:700-713,
+ * :586-599, :1013-1028, bound through mxmOp<T>, src/main.cpp:11-12, :206-218).
+ * dtype selects the value type of A, B and C (plus-times semiring);
+ * integer products accumulate exactly (int32 wraps like the reference's int).
+ * Cval: M.nnz elements, host memory. */
+ga_status ga_mxm_masked(const ga_csr* A, const ga_csr* B, const ga_csr* M, ga_dtype dtype, void* Cval);
+
+/* C<!M> = A*B with a complement mask: C holds the product entries whose
+ * column is not stored in M's row, columns ascending.  Replaces
+ * mxmm_msa_cmask (include/matrix_la.h:788-891).  The library allocates the
+ * three output arrays with malloc (free with ga_free_host):
+ * *Crowptr (nrows+1), *Ccol (*Cnnz), *Cval (*Cnnz values of dtype). */
+ga_status ga_mxm_cmask(const ga_csr* A, const ga_csr* B, const ga_csr* M, ga_dtype dtype, int32_t** Crowptr,
+                       int32_t** Ccol, void** Cval, int64_t* Cnnz);
+
+/* Masked sparse x dense: C<M> = A*B with B and C dense row-major
+ * (A.ncols x ncols and A.nrows x ncols); row i of C is zero outside M's
+ * row i.  Replaces mxmm_spd (include/matrix_la.h:239-270).  FP64 only. */
+ga_status ga_mxm_spd(const ga_csr* A, const double* B, int64_t ncols, const ga_csr* M, double* C);
+
+void ga_free_host(void* p);
+
+/* ---- triangle counting ------------------------------------------------- */
+/* A: symmetric pattern, sorted rows.  T = sum of C<L> = L*L, L = tril(A,-1).
+ * Replaces triangle_counting_masked_sandia (src/main.cpp:65-86). */
+ga_status ga_triangle_count(const ga_csr* A, int64_t* triangles);
+/* T = sum(C<A> = L*L^T) / 2.  Replaces triangle_counting_masked_lu
+ * (src/main.cpp:40-62). */
+ga_status ga_triangle_count_lu(const ga_csr* A, int64_t* triangles);
+/* per_vertex[i] = (sum_j (A*A)<A>(i,j)) / 2.  Replaces
+ * triangle_counting_vertex (src/main.cpp:14-37); int64 instead of int. */
+ga_status ga_triangle_count_vertex(const ga_csr* A, int64_t* per_vertex);
+
+/* ---- k-truss ----------------------------------------------------------- */
+/* The k-truss of the symmetric pattern A: repeat { S<C> = C*C; keep S >= k-2 }
+ * until nnz is unchanged.  Replaces k_truss (src/main.cpp:89-146).
+ * out_rowptr (nrows+1) and out_col (capacity A.nnz) receive the pattern;
+ * *iterations = number of support passes (the reference's
+ * totalIterationNum). */
+ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_col, int64_t* out_nnz,
+                    int32_t* iterations);
+
+typedef struct {
+  int64_t passes;         /* support products computed */
+  int64_t steps;          /* filter steps (one per k and per removal round) */
+  int64_t wedges;         /* sum over passes of sum_{(i,k) in C} nnz(C_k) */
+  int64_t bytes_alg;      /* algorithmic bytes of the support passes */
+  double support_ms;      /* GPU time in the support kernels (when profiling) */
+  double total_ms;        /* wall time of the whole decomposition */
+} ga_truss_stats;
+
+/* Truss number of every stored entry of the symmetric pattern A: the largest
+ * k whose k-truss holds the entry, 2 if none (the caller-side loop over k of
+ * the reference's k_truss).  truss: A.nnz int32 (host).  stats may be NULL. */
+ga_status ga_truss_decomposition(const ga_csr* A, int32_t* truss, int32_t* kmax, ga_truss_stats* stats);
+
+/* ---- betweenness centrality -------------------------------------------- */
+/* bc[v] = sum over the given sources s of the Brandes dependency delta_s(v)
+ * (unnormalised, not halved), FP64 path counts and dependencies.  Replaces
+ * betweenness_centrality_batch (include/betweenness_centrality.h:125-237),
+ * whose sources are 0..batch-1; here they are an explicit list of distinct
+ * vertices.  A: square pattern (directed edges as stored). */
+ga_status ga_bc_batch(const ga_csr* A, const int32_t* sources, int64_t nsources, double* bc);
+
+/* ---- resident graphs --------------------------------------------------- */
+typedef struct ga_graph ga_graph;
+/* Uploads a symmetric pattern and keeps it in HBM with its workspaces. */
+ga_status ga_graph_create(const ga_csr* A, ga_graph** out);
+ga_status ga_graph_destroy(ga_graph* g);
+ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles);
+ga_status ga_graph_truss(ga_graph* g, int32_t* truss, int32_t* kmax, ga_truss_stats* stats);
+ga_status ga_graph_bc(ga_graph* g, const int32_t* sources, int64_t nsources, double* bc);
+
+/* ---- kernel timing ----------------------------------------------------- */
+/* When enabled, the library brackets every launch of the masked-SpGEMM
+ * kernel with HIP events on the stream it runs on and accumulates launch
+ * count, GPU milliseconds, wedges and algorithmic bytes. */
+typedef struct {
+  int64_t launches;
+  double kernel_ms;
+  int64_t wedges;
+  int64_t bytes_alg;
+} ga_kernel_prof;
+ga_status ga_prof_enable(int on);
+ga_status ga_prof_read(ga_kernel_prof* out, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAPHALGO_H */

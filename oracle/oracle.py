@@ -1,0 +1,197 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper of oracle/_build/liboracle.so (the C restatement in oracle.c)
+and of the reference harness oracle/_ref/ref_driver.  Only tests/, the smoke
+check and bench.py's cpu_baseline leg import this module, and only as the
+checker or the timed CPU baseline; the product (graphalgo_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+REF = HERE / "_ref" / "ref_driver"
+
+I32P = C.POINTER(C.c_int32)
+I64P = C.POINTER(C.c_int64)
+F64P = C.POINTER(C.c_double)
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _lib.orc_mxm_masked_i64.argtypes = [C.c_int64, C.c_int64, I32P, I32P, I32P, I32P, I32P, I32P, I32P, I32P,
+                                            I64P]
+        _lib.orc_mxm_masked_f64.argtypes = [C.c_int64, C.c_int64, I32P, I32P, F64P, I32P, I32P, F64P, I32P, I32P,
+                                            F64P]
+        _lib.orc_wedges.argtypes = [C.c_int64, I32P, I32P, I32P]
+        _lib.orc_wedges.restype = C.c_int64
+        _lib.orc_tc_sandia.argtypes = [C.c_int64, I32P, I32P]
+        _lib.orc_tc_sandia.restype = C.c_int64
+        _lib.orc_tc_lu.argtypes = [C.c_int64, I32P, I32P]
+        _lib.orc_tc_lu.restype = C.c_int64
+        _lib.orc_tc_vertex.argtypes = [C.c_int64, I32P, I32P, I64P]
+        _lib.orc_ktruss.argtypes = [C.c_int64, I32P, I32P, C.c_int, C.POINTER(I32P), C.POINTER(I32P), I64P,
+                                    C.POINTER(C.c_int)]
+        _lib.orc_truss_decomposition.argtypes = [C.c_int64, I32P, I32P, I32P, I32P, I64P, I64P, I64P, C.c_int64]
+        _lib.orc_mxm_cmask_f64.argtypes = [C.c_int64, C.c_int64, I32P, I32P, F64P, I32P, I32P, F64P, I32P, I32P,
+                                           C.POINTER(I32P), C.POINTER(I32P), C.POINTER(F64P), I64P]
+        _lib.orc_mxm_spd_f64.argtypes = [C.c_int64, C.c_int64, I32P, I32P, F64P, F64P, I32P, I32P, F64P]
+        _lib.orc_bc_batch.argtypes = [C.c_int64, I32P, I32P, I32P, C.c_int64, F64P, I32P]
+        _lib.orc_free.argtypes = [C.c_void_p]
+        _lib.orc_num_threads.restype = C.c_int
+    return _lib
+
+
+def _p(a, t=I32P):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def mxm_masked(m, bn, Arp, Acol, Brp, Bcol, Mrp, Mcol, Aval=None, Bval=None, dtype=np.int64):
+    """C<M> = A*B (include/matrix_la.h:715-749); values aligned with Mcol."""
+    Arp, Acol, Brp, Bcol, Mrp, Mcol = map(_i32, (Arp, Acol, Brp, Bcol, Mrp, Mcol))
+    out = np.zeros(len(Mcol), dtype=np.float64 if dtype == np.float64 else np.int64)
+    if dtype == np.float64:
+        Av = None if Aval is None else np.ascontiguousarray(Aval, dtype=np.float64)
+        Bv = None if Bval is None else np.ascontiguousarray(Bval, dtype=np.float64)
+        rc = lib().orc_mxm_masked_f64(m, bn, _p(Arp), _p(Acol), _p(Av, F64P), _p(Brp), _p(Bcol), _p(Bv, F64P),
+                                      _p(Mrp), _p(Mcol), _p(out, F64P))
+    else:
+        Av, Bv = _i32(Aval), _i32(Bval)
+        rc = lib().orc_mxm_masked_i64(m, bn, _p(Arp), _p(Acol), _p(Av), _p(Brp), _p(Bcol), _p(Bv), _p(Mrp),
+                                      _p(Mcol), _p(out, I64P))
+    assert rc == 0, rc
+    return out
+
+
+def wedges(g, b=None) -> int:
+    b = g if b is None else b
+    return lib().orc_wedges(g.n, _p(_i32(g.rowptr)), _p(_i32(g.col)), _p(_i32(b.rowptr)))
+
+
+def tc(g) -> int:
+    """triangle_counting_masked_sandia, src/main.cpp:65-86."""
+    return lib().orc_tc_sandia(g.n, _p(_i32(g.rowptr)), _p(_i32(g.col)))
+
+
+def tc_lu(g) -> int:
+    return lib().orc_tc_lu(g.n, _p(_i32(g.rowptr)), _p(_i32(g.col)))
+
+
+def tc_vertex(g) -> np.ndarray:
+    out = np.zeros(g.n, dtype=np.int64)
+    assert lib().orc_tc_vertex(g.n, _p(_i32(g.rowptr)), _p(_i32(g.col)), _p(out, I64P)) == 0
+    return out
+
+
+def ktruss(g, k: int):
+    """k_truss, src/main.cpp:89-146: (rowptr, col, iterations)."""
+    rp, col, nnz, it = I32P(), I32P(), C.c_int64(), C.c_int()
+    assert lib().orc_ktruss(g.n, _p(_i32(g.rowptr)), _p(_i32(g.col)), k, C.byref(rp), C.byref(col), C.byref(nnz),
+                            C.byref(it)) == 0
+    r = np.ctypeslib.as_array(rp, shape=(g.n + 1,)).copy()
+    c = np.ctypeslib.as_array(col, shape=(max(nnz.value, 1),))[: nnz.value].copy()
+    lib().orc_free(C.cast(rp, C.c_void_p))
+    lib().orc_free(C.cast(col, C.c_void_p))
+    return r, c, it.value
+
+
+def truss_decomposition(g, max_passes: int = 100000):
+    """(truss[nnz], kmax, passes, pass_nnz, pass_wedges)."""
+    truss = np.zeros(max(g.nnz, 1), dtype=np.int32)
+    kmax = C.c_int32()
+    passes = C.c_int64()
+    pn = np.zeros(max_passes, dtype=np.int64)
+    pw = np.zeros(max_passes, dtype=np.int64)
+    assert lib().orc_truss_decomposition(g.n, _p(_i32(g.rowptr)), _p(_i32(g.col)), _p(truss), C.byref(kmax),
+                                         C.byref(passes), _p(pn, I64P), _p(pw, I64P), max_passes) == 0
+    np_ = min(passes.value, max_passes)
+    return truss[: g.nnz], kmax.value, passes.value, pn[:np_], pw[:np_]
+
+
+def mxm_cmask(m, bn, Arp, Acol, Brp, Bcol, Mrp, Mcol, Aval=None, Bval=None):
+    """C<!M> = A*B (include/matrix_la.h:800-891): (rowptr, col, val)."""
+    Arp, Acol, Brp, Bcol, Mrp, Mcol = map(_i32, (Arp, Acol, Brp, Bcol, Mrp, Mcol))
+    Av = None if Aval is None else np.ascontiguousarray(Aval, dtype=np.float64)
+    Bv = None if Bval is None else np.ascontiguousarray(Bval, dtype=np.float64)
+    rp, col, val, nnz = I32P(), I32P(), F64P(), C.c_int64()
+    assert lib().orc_mxm_cmask_f64(m, bn, _p(Arp), _p(Acol), _p(Av, F64P), _p(Brp), _p(Bcol), _p(Bv, F64P),
+                                   _p(Mrp), _p(Mcol), C.byref(rp), C.byref(col), C.byref(val), C.byref(nnz)) == 0
+    nz = nnz.value
+    r = np.ctypeslib.as_array(rp, shape=(m + 1,)).copy()
+    c = np.ctypeslib.as_array(col, shape=(max(nz, 1),))[:nz].copy()
+    v = np.ctypeslib.as_array(val, shape=(max(nz, 1),))[:nz].copy()
+    for p in (rp, col, val):
+        lib().orc_free(C.cast(p, C.c_void_p))
+    return r, c, v
+
+
+def mxm_spd(m, ncols, Arp, Acol, B, Mrp, Mcol, Aval=None):
+    """mxmm_spd (include/matrix_la.h:239-270) with FP64 values."""
+    Arp, Acol, Mrp, Mcol = map(_i32, (Arp, Acol, Mrp, Mcol))
+    Av = None if Aval is None else np.ascontiguousarray(Aval, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    out = np.zeros((m, ncols), dtype=np.float64)
+    lib().orc_mxm_spd_f64(m, ncols, _p(Arp), _p(Acol), _p(Av, F64P), _p(B, F64P), _p(Mrp), _p(Mcol),
+                          _p(out, F64P))
+    return out
+
+
+def bc(g, sources):
+    """FP64 restatement of betweenness_centrality_batch (include/betweenness_centrality.h:125-237)."""
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    out = np.zeros(g.n, dtype=np.float64)
+    depth = C.c_int32()
+    rc = lib().orc_bc_batch(g.n, _p(_i32(g.rowptr)), _p(_i32(g.col)), _p(src), len(src), _p(out, F64P),
+                            C.byref(depth))
+    assert rc == 0, rc
+    return out
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+# ---- the compiled reference (oracle/_ref), present where it was built ------
+def ref_available() -> bool:
+    return REF.exists()
+
+
+def ref_run(*args: str) -> dict:
+    out = subprocess.run([str(REF), *args], capture_output=True, text=True, check=True)
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def ref_truss(graph_path: str, nnz: int) -> tuple[np.ndarray, dict]:
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "truss.bin")
+        info = ref_run("ktall", graph_path, out)
+        return np.fromfile(out, dtype=np.int32, count=nnz), info
+
+
+def ref_bc(graph_path: str, batch: int, n: int) -> np.ndarray:
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "bc.bin")
+        ref_run("bc", graph_path, str(batch), out)
+        return np.fromfile(out, dtype=np.float32, count=n)

@@ -1,0 +1,345 @@
+"""Pins of the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each function of the oracle is checked against (a) the reference's own outputs
+recorded in tests/golden/golden.json by tests/golden/make_golden.py (the
+harness compiled from the reference headers), (b) closed forms, and (c)
+independent library routines (scipy, networkx), chosen so that a dropped term,
+a wrong index or a transposed operand fails at least one of them.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_DIR
+
+
+# ---------------------------------------------------------------- helpers
+def complete(G, n):
+    u, v = zip(*itertools.combinations(range(n), 2))
+    return G.CSR.from_edges(n, u, v)
+
+
+def path(G, n):
+    return G.CSR.from_edges(n, range(n - 1), range(1, n))
+
+
+def cycle(G, n):
+    return G.CSR.from_edges(n, range(n), [(i + 1) % n for i in range(n)])
+
+
+def star(G, n):
+    return G.CSR.from_edges(n, [0] * (n - 1), range(1, n))
+
+
+def wheel(G, n):  # hub 0 + cycle over 1..n-1
+    u = [0] * (n - 1) + list(range(1, n))
+    v = list(range(1, n)) + [1 + (i % (n - 1)) for i in range(1, n)]
+    return G.CSR.from_edges(n, u, v)
+
+
+def random_tree(G, n, seed):
+    rng = np.random.default_rng(seed)
+    parent = [int(rng.integers(0, i)) for i in range(1, n)]
+    return G.CSR.from_edges(n, range(1, n), parent)
+
+
+def dense(g):
+    D = np.zeros((g.n, g.n), dtype=np.int64)
+    for i in range(g.n):
+        D[i, g.col[g.rowptr[i]:g.rowptr[i + 1]]] = 1
+    return D
+
+
+# ---------------------------------------------------------------- masked SpGEMM
+def test_mxm_masked_complete_graph_closed_form(G, oracle):
+    """K_n: C<L> = L*L has C(i,j) = i-j-1 (the number of k with j<k<i)."""
+    n = 12
+    L = complete(G, n).lower()
+    C = oracle.mxm_masked(n, n, L.rowptr, L.col, L.rowptr, L.col, L.rowptr, L.col)
+    rows = np.repeat(np.arange(n), np.diff(L.rowptr))
+    assert np.array_equal(C, rows - L.col - 1)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mxm_masked_brute_force_small(G, oracle, seed):
+    """Exhaustive dense product on tiny random operands with values, rectangular shapes."""
+    rng = np.random.default_rng(seed)
+    m, k, n = rng.integers(1, 9, size=3)
+    A = (rng.random((m, k)) < 0.5) * rng.integers(-3, 4, size=(m, k))
+    B = (rng.random((k, n)) < 0.5) * rng.integers(-3, 4, size=(k, n))
+    Mk = rng.random((m, n)) < 0.5
+
+    def csr(D):
+        rp = np.concatenate([[0], np.cumsum((D != 0).sum(1))]).astype(np.int32)
+        r, c = np.nonzero(D)
+        return rp, c.astype(np.int32), D[r, c]
+    Arp, Acol, Aval = csr(A)
+    Brp, Bcol, Bval = csr(B)
+    Mrp, Mcol, _ = csr(Mk.astype(np.int64))
+    got = oracle.mxm_masked(m, n, Arp, Acol, Brp, Bcol, Mrp, Mcol, Aval=Aval, Bval=Bval)
+    r, c = np.nonzero(Mk)
+    assert np.array_equal(got, (A @ B)[r, c])
+    gotf = oracle.mxm_masked(m, n, Arp, Acol, Brp, Bcol, Mrp, Mcol, Aval=Aval * 0.5, Bval=Bval, dtype=np.float64)
+    assert np.allclose(gotf, 0.5 * (A @ B)[r, c], rtol=0, atol=1e-12)
+
+
+def test_mxm_masked_scipy_rmat(G, oracle):
+    """C<A> = A*A on an R-MAT graph against scipy, explicit zeros included."""
+    g = G.rmat(11, 8, seed=3)
+    C = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
+    S = g.to_scipy()
+    P = (S @ S).tocsr()
+    rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
+    exp = np.asarray(P[rows, g.col]).ravel()
+    assert np.array_equal(C, exp)
+    assert (C == 0).any()  # explicit zeros are part of the result
+
+
+def test_mxm_matches_reference_mxm(G, oracle, small_graphs, tmp_path):
+    """C<A> = A*A on G43 against the reference's own mxmm_msa (oracle/_ref), when built here."""
+    if not oracle.ref_available():
+        pytest.skip("reference harness not built on this host")
+    g = small_graphs["G43.mtx"][1]
+    out = tmp_path / "c.bin"
+    oracle.ref_run("mxm", str(GOLDEN_DIR / "graphs" / "G43.mtx"), str(out))
+    ref = np.fromfile(out, dtype=np.int32)
+    C = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
+    assert np.array_equal(C, ref)
+
+
+def test_wedges(G, oracle):
+    g = G.rmat(10, 8, seed=5)
+    assert oracle.wedges(g) == int((g.degrees() ** 2).sum())
+
+
+# ---------------------------------------------------------------- triangle counting
+def test_tc_reference_fixtures(small_graphs, oracle, golden):
+    for name, (g, gs) in small_graphs.items():
+        rec = golden["graphs"][name]
+        assert rec["tc_msa"] == rec["tc_mca"]
+        assert oracle.tc(gs) == rec["tc_msa"], name
+
+
+def test_tc_survey_pins(small_graphs, oracle):
+    # values the reference printed in the survey run
+    assert oracle.tc(small_graphs["G43.mtx"][1]) == 1392
+    assert oracle.tc(small_graphs["yeast_30NN.mtx.gz"][1]) == 180175
+    assert oracle.tc(small_graphs["iris_dataset_30NN.mtx"][1]) == 25972
+    assert oracle.tc(small_graphs["cage3.mtx"][1]) == 3
+
+
+def test_tc_closed_forms(G, oracle):
+    for n in (3, 4, 7, 13):
+        assert oracle.tc(complete(G, n)) == n * (n - 1) * (n - 2) // 6
+    for n in (4, 5, 9):
+        assert oracle.tc(cycle(G, n)) == 0
+    for n in (5, 8, 11):
+        assert oracle.tc(wheel(G, n)) == n - 1
+    assert oracle.tc(random_tree(G, 40, 1)) == 0
+    assert oracle.tc(G.CSR.from_edges(5, [], [])) == 0
+
+
+def test_tc_formulations_agree(G, oracle):
+    g = G.rmat(12, 8, seed=7)
+    t = oracle.tc(g)
+    assert oracle.tc_lu(g) == t
+    pv = oracle.tc_vertex(g)
+    assert pv.sum() == 3 * t
+    S = g.to_scipy()
+    diag3 = np.asarray((S @ S).multiply(S).sum(axis=1)).ravel()  # diag(A^3)
+    assert np.array_equal(pv, diag3 // 2)
+
+
+def test_tc_generated_pins(G, oracle, golden):
+    rec = golden["generated"]["er_1000_9990_s1"]
+    er = G.erdos_renyi(**rec["cfg"])
+    assert er.sha256() == rec["csr_sha256"]
+    assert oracle.tc(er) == rec["tc"]
+    assert er.lower().wedges() == rec["wedges_LL"]
+
+
+# ---------------------------------------------------------------- k-truss
+def test_ktruss_g43_sizes(small_graphs, oracle, golden):
+    """Per-k sizes and iteration counts of the reference's k_truss (from scratch per k)."""
+    gs = small_graphs["G43.mtx"][1]
+    for k, nnz, iters in golden["graphs"]["G43.mtx"]["truss"]["per_k"]:
+        rp, col, it = oracle.ktruss(gs, k)
+        assert rp[-1] == nnz and it == iters, k
+    assert oracle.ktruss(gs, 3)[0][-1] == 6818  # survey pin
+    assert oracle.ktruss(gs, 4)[0][-1] == 36
+
+
+@pytest.mark.parametrize("name", ["G43.mtx", "iris_dataset_30NN.mtx", "yeast_30NN.mtx.gz", "cage3.mtx"])
+def test_truss_decomposition_reference(small_graphs, oracle, golden, name):
+    """Incremental truss decomposition == the reference's per-k loop, entry by entry."""
+    gs = small_graphs[name][1]
+    rec = golden["graphs"][name]["truss"]
+    truss, kmax, passes, _, _ = oracle.truss_decomposition(gs)
+    ref = np.load(GOLDEN_DIR / rec["truss_npy"])
+    assert np.array_equal(truss, ref)
+    assert kmax == rec["kmax"]
+
+
+def test_truss_closed_forms(G, oracle):
+    for n in (3, 5, 8):
+        truss, kmax, *_ = oracle.truss_decomposition(complete(G, n))
+        assert (truss == n).all() and kmax == n
+    truss, kmax, *_ = oracle.truss_decomposition(random_tree(G, 30, 2))
+    assert (truss == 2).all() and kmax == 2
+    # two K4 sharing a vertex plus a pendant edge
+    u = [0, 0, 0, 1, 1, 2, 3, 3, 3, 4, 4, 5, 6]
+    v = [1, 2, 3, 2, 3, 3, 4, 5, 6, 5, 6, 6, 7]
+    g = G.CSR.from_edges(8, u, v)
+    truss, kmax, *_ = oracle.truss_decomposition(g)
+    rows = np.repeat(np.arange(8), np.diff(g.rowptr))
+    pend = ((rows == 6) & (g.col == 7)) | ((rows == 7) & (g.col == 6))
+    assert (truss[pend] == 2).all() and (truss[~pend] == 4).all()
+
+
+def test_truss_networkx(G, oracle):
+    nx = pytest.importorskip("networkx")
+    g = G.rmat(10, 8, seed=11)
+    truss, kmax, *_ = oracle.truss_decomposition(g)
+    X = nx.Graph()
+    rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
+    X.add_edges_from(zip(rows.tolist(), g.col.tolist()))
+    for k in range(3, kmax + 2):
+        T = nx.k_truss(X, k)
+        assert 2 * T.number_of_edges() == int((truss >= k).sum()), k
+
+
+def test_truss_invariant_support_sum(G, oracle):
+    """sum over directed entries of the 3-truss support = 6 T (every triangle counted 6x)."""
+    g = G.rmat(10, 8, seed=13)
+    S = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
+    assert S.sum() == 6 * oracle.tc(g)
+
+
+# ---------------------------------------------------------------- cmask / spd
+def test_cmask_scipy(G, oracle):
+    g = G.rmat(9, 8, seed=17)
+    rng = np.random.default_rng(0)
+    ns = 7
+    # B: random sparse n x ns with FP64 values, M: random sparse n x ns
+    B = (rng.random((g.n, ns)) < 0.1) * rng.random((g.n, ns))
+    M = rng.random((g.n, ns)) < 0.3
+
+    def csr(D):
+        rp = np.concatenate([[0], np.cumsum((D != 0).sum(1))]).astype(np.int32)
+        r, c = np.nonzero(D)
+        return rp, c.astype(np.int32), D[r, c]
+    Brp, Bcol, Bval = csr(B)
+    Mrp, Mcol, _ = csr(M.astype(float))
+    rp, col, val = oracle.mxm_cmask(g.n, ns, g.rowptr, g.col, Brp, Bcol, Mrp, Mcol, Bval=Bval)
+    P = dense(g).astype(float) @ B
+    Pat = (dense(g) @ (B != 0)) > 0
+    want = Pat & ~M
+    r, c = np.nonzero(want)
+    assert np.array_equal(rp, np.concatenate([[0], np.cumsum(want.sum(1))]))
+    assert np.array_equal(col, c)
+    assert np.allclose(val, P[r, c], rtol=1e-14, atol=0)
+
+
+def test_spd_dense(G, oracle):
+    g = G.rmat(8, 8, seed=19)
+    rng = np.random.default_rng(1)
+    ncols = 5
+    B = rng.random((g.n, ncols))
+    M = rng.random((g.n, ncols)) < 0.4
+    Mrp = np.concatenate([[0], np.cumsum(M.sum(1))]).astype(np.int32)
+    Mcol = np.nonzero(M)[1].astype(np.int32)
+    got = oracle.mxm_spd(g.n, ncols, g.rowptr, g.col, B, Mrp, Mcol)
+    assert np.allclose(got, (dense(g) @ B) * M, rtol=1e-14, atol=0)
+
+
+# ---------------------------------------------------------------- betweenness centrality
+def test_bc_closed_forms(G, oracle):
+    n = 9
+    s = star(G, n)
+    bc = oracle.bc(s, list(range(1, n)))
+    assert bc[0] == pytest.approx((n - 1) * (n - 2)) and np.allclose(bc[1:], 0)
+    p = path(G, n)
+    bc = oracle.bc(p, list(range(n)))
+    v = np.arange(n)
+    assert np.allclose(bc, 2 * v * (n - 1 - v))
+    assert np.allclose(oracle.bc(complete(G, 7), list(range(7))), 0)
+
+
+def test_bc_networkx(G, oracle, small_graphs):
+    nx = pytest.importorskip("networkx")
+    g = small_graphs["G43.mtx"][0]
+    X = nx.Graph()
+    X.add_nodes_from(range(g.n))
+    rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
+    X.add_edges_from(zip(rows.tolist(), g.col.tolist()))
+    rng = np.random.default_rng(5)
+    src = rng.choice(g.n, 24, replace=False)
+    bc = oracle.bc(g, src)
+    ref = nx.betweenness_centrality_subset(X, sources=src.tolist(), targets=list(range(g.n)), normalized=False)
+    ref = 2 * np.array([ref[v] for v in range(g.n)])
+    assert np.allclose(bc, ref, rtol=1e-10, atol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["G43.mtx", "iris_dataset_30NN.mtx", "yeast_30NN.mtx.gz"])
+def test_bc_reference_float32(small_graphs, oracle, golden, name):
+    """FP64 restatement vs the reference's float32 betweenness_centrality_batch
+    (sources 0..15): equal up to float32 rounding and its cancellation (defect D5)."""
+    g = small_graphs[name][0]
+    rec = golden["graphs"][name]["bc_ref_f32"]
+    ref = np.load(GOLDEN_DIR / rec["npy"]).astype(np.float64)
+    bc = oracle.bc(g, np.arange(rec["batch"]))
+    assert np.allclose(bc, ref, rtol=1e-3, atol=2e-3 * max(1.0, np.abs(ref).max()) * 1e-2)
+
+
+def test_bc_brute_force_tiny(G, oracle):
+    """Exhaustive shortest-path enumeration (BFS path counts) on a tiny random graph."""
+    g = G.erdos_renyi(14, 24, seed=3)
+    D = dense(g)
+    n = g.n
+    INF = 10 ** 9
+    dist = np.full((n, n), INF)
+    sig = np.zeros((n, n))
+    for s in range(n):
+        dist[s, s] = 0
+        sig[s, s] = 1
+        frontier = [s]
+        while frontier:
+            nxt = []
+            for u in frontier:
+                for w in np.nonzero(D[u])[0]:
+                    if dist[s, w] == INF:
+                        dist[s, w] = dist[s, u] + 1
+                        nxt.append(w)
+                    if dist[s, w] == dist[s, u] + 1:
+                        sig[s, w] += sig[s, u]
+            frontier = nxt
+    src = [0, 3, 5, 9]
+    want = np.zeros(n)
+    for s in src:
+        for t in range(n):
+            if t == s or dist[s, t] >= INF:
+                continue
+            for v in range(n):
+                if v in (s, t) or dist[s, v] >= INF or dist[v, t] >= INF:
+                    continue
+                if dist[s, v] + dist[v, t] == dist[s, t]:
+                    want[v] += sig[s, v] * sig[v, t] / sig[s, t]
+    assert np.allclose(oracle.bc(g, src), want, rtol=1e-12, atol=1e-12)
+
+
+def test_planted_mistake_is_caught(G, oracle):
+    """A plausible oracle mistake -- masking C<L> = L*L with U instead of L (transposed
+    mask) -- changes the K_n closed form, so the closed-form pin discriminates."""
+    n = 8
+    K = complete(G, n)
+    L = K.lower()
+    rows = np.repeat(np.arange(n), np.diff(L.rowptr))
+    # transposed operand: L * L^T masked by L is not the sandia product
+    Lt_rp = np.zeros(n + 1, dtype=np.int32)
+    cnt = np.bincount(L.col, minlength=n)
+    np.cumsum(cnt, out=Lt_rp[1:])
+    order = np.lexsort((rows, L.col))
+    Lt_col = rows[order].astype(np.int32)
+    wrong = oracle.mxm_masked(n, n, L.rowptr, L.col, Lt_rp, Lt_col, L.rowptr, L.col)
+    assert not np.array_equal(wrong, rows - L.col - 1)
