@@ -160,6 +160,43 @@ class Graph:
         return bc
 
 
+class Batch:
+    """Many small graphs resident together; triangle counts in one persistent launch."""
+
+    def __init__(self, graphs: list[CSR]):
+        N.init()
+        self.graphs = graphs
+        self._views = (N.GaCsr * max(len(graphs), 1))()
+        for i, g in enumerate(graphs):
+            self._views[i] = g.ga()
+        self._h = C.c_void_p()
+        N.check(N.galib().ga_batch_create(len(graphs), self._views, C.byref(self._h)))
+        self.threads = 0
+
+    def close(self) -> None:
+        if self._h:
+            N.check(N.galib().ga_batch_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def triangle_counts(self, threads: int = 0) -> np.ndarray:
+        out = np.zeros(max(len(self.graphs), 1), dtype=np.int64)
+        N.check(N.galib().ga_batch_triangle_count(self._h, threads, out.ctypes.data_as(N.I64P)))
+        return out[: len(self.graphs)]
+
+    def autotune(self) -> tuple[int, dict]:
+        best = C.c_int()
+        ms = np.zeros(3, dtype=np.float64)
+        N.check(N.galib().ga_batch_autotune(self._h, C.byref(best), ms.ctypes.data_as(N.F64P)))
+        self.threads = best.value
+        return best.value, {64: ms[0], 128: ms[1], 256: ms[2]}
+
+
 def prof_enable(on: bool = True) -> None:
     N.init()
     N.check(N.galib().ga_prof_enable(1 if on else 0))

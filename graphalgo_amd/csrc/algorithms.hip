@@ -219,6 +219,7 @@ struct ga_graph {
   bool have_L = false;
   TrussWork tw;
   MxmWork mw;
+  BcWork bw;
   DevBuf sums, scan_tmp;
 };
 
@@ -402,6 +403,13 @@ ga_status ga_graph_truss(ga_graph* g, int32_t* truss, int32_t* kmax, ga_truss_st
     GA_HIP(hipStreamSynchronize(s));
     if (kmax) *kmax = km;
     if (stats) *stats = st;
+  });
+}
+
+ga_status ga_graph_bc(ga_graph* g, const int32_t* sources, int64_t nsources, double* bc) {
+  return guarded([&] {
+    GA_REQUIRE(g && sources && bc, GA_ERR_ARG, "null argument");
+    bc_batch_dev(g->A.view(), sources, nsources, bc, g->bw, ctx().stream);
   });
 }
 

@@ -173,10 +173,20 @@ struct MxmWork {
 void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,
                     unsigned long long* sums, const int32_t* row_group, MxmWork& w, hipStream_t s);
 
+// Batched sum-mode product over many graphs stored back to back (graph-local
+// column indices): sums[g] += sum of C<L_g> = L_g * L_g.  threads = 64, 128
+// or 256 (workgroup size = A entries per work item).
+void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t* gbase, int64_t max_cols,
+                         int threads, unsigned long long* sums, MxmWork& w, hipStream_t s);
+
+// Batched betweenness centrality workspace and driver (bc.hip).
+struct BcWork {
+  DevBuf depth, sigma, q, bc, flag, levels, cnt, off, scan_tmp, src;
+};
+void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h, BcWork& w, hipStream_t s);
+
 // Strict lower triangle of a square pattern with sorted rows, on device.
 void lower_triangle_dev(const DCsr& A, OwnedCsr& L, DevBuf& tmp, hipStream_t s);
-// Transpose of a pattern on device (rows of the result sorted).
-void transpose_dev(const DCsr& A, OwnedCsr& T, DevBuf& tmp, hipStream_t s);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20) {

@@ -27,22 +27,24 @@
 
 namespace ga {
 
-constexpr int MXM_THREADS = 256;
-constexpr int MXM_TCAP = 256;   // A entries per item (== MXM_THREADS: one per thread in the scan)
-constexpr int MXM_MCAP = 2048;  // mask entries staged in LDS
 constexpr int64_t MXM_BITMAP_MAX_COLS = int64_t(1) << 18;  // 32 KB bitmap
 
-static_assert(MXM_TCAP == MXM_THREADS, "entry scan assumes one entry per thread");
-
-template <typename T, bool PAT_A, bool PAT_B, int MODE>
-__global__ __launch_bounds__(MXM_THREADS) void k_mxm_items(
+// THREADS = workgroup size = A entries per item (one entry per thread in the
+// entry scan); MCAP = mask entries staged in LDS.  BATCHED: the operands are
+// many graphs stored back to back with graph-local column indices; row i
+// belongs to graph row_group[i] whose rows start at gbase[row_group[i]].
+template <typename T, bool PAT_A, bool PAT_B, int MODE, int THREADS, bool BATCHED>
+__global__ __launch_bounds__(THREADS) void k_mxm_items(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const T* __restrict__ Aval,
     const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol, const T* __restrict__ Bval,
     const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol, T* __restrict__ Cval,
     unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
-    const int32_t* __restrict__ item_row, const int32_t* __restrict__ item_t0,
+    const int32_t* __restrict__ gbase, const int32_t* __restrict__ item_row, const int32_t* __restrict__ item_t0,
     const int32_t* __restrict__ n_items_ptr, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ wedge_counter, int bitmap_words) {
+  constexpr int MXM_THREADS = THREADS;
+  constexpr int MXM_TCAP = THREADS;
+  constexpr int MXM_MCAP = 8 * THREADS;
   extern __shared__ uint32_t s_bitmap[];
   __shared__ int32_t s_mcol[MXM_MCAP];
   __shared__ T s_acc[MODE == 0 ? MXM_MCAP : 1];
@@ -75,6 +77,7 @@ __global__ __launch_bounds__(MXM_THREADS) void k_mxm_items(
     const int ms = Mrp[i];
     const int mnz = Mrp[i + 1] - ms;
     const bool local = mnz <= MXM_MCAP;
+    const int bbase = BATCHED ? gbase[row_group[i]] : 0;
 
     // stage the mask row
     for (int p = tid; p < mnz; p += MXM_THREADS) {
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(MXM_THREADS) void k_mxm_items(
     const int ne = t1 - t0;
     int len = 0;
     if (tid < ne) {
-      const int k = Acol[t0 + tid];
+      const int k = bbase + Acol[t0 + tid];
       const int b0 = Brp[k];
       len = Brp[k + 1] - b0;
       s_ebeg[tid] = b0;
@@ -185,30 +188,33 @@ __global__ __launch_bounds__(MXM_THREADS) void k_mxm_items(
 }
 
 __global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp, int64_t m,
-                             int32_t* __restrict__ cnt) {
+                             int tcap, int32_t* __restrict__ cnt) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     const int a = Arp[i + 1] - Arp[i];
     const int mz = Mrp[i + 1] - Mrp[i];
-    cnt[i] = (a > 0 && mz > 0) ? (a + MXM_TCAP - 1) / MXM_TCAP : 0;
+    cnt[i] = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
   }
 }
 
 __global__ void k_item_fill(const int32_t* __restrict__ Arp, const int32_t* __restrict__ cnt,
-                            const int32_t* __restrict__ start, int64_t m, int32_t* __restrict__ item_row,
+                            const int32_t* __restrict__ start, int64_t m, int tcap, int32_t* __restrict__ item_row,
                             int32_t* __restrict__ item_t0) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     const int c = cnt[i], s0 = start[i], a0 = Arp[i];
     for (int q = 0; q < c; ++q) {
       item_row[s0 + q] = (int32_t)i;
-      item_t0[s0 + q] = a0 + q * MXM_TCAP;
+      item_t0[s0 + q] = a0 + q * tcap;
     }
   }
 }
 
-template <typename T, bool PA, bool PB, int MODE>
+template <typename T, bool PA, bool PB, int MODE, int THREADS, bool BATCHED>
 static void launch_items(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
-                         const int32_t* row_group, MxmWork& w, int bitmap_words, hipStream_t s) {
-  auto kern = k_mxm_items<T, PA, PB, MODE>;
+                         const int32_t* row_group, const int32_t* gbase, MxmWork& w, int bitmap_words,
+                         hipStream_t s) {
+  constexpr int MXM_THREADS = THREADS;
+  constexpr int MXM_TCAP = THREADS;
+  auto kern = k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED>;
   const size_t dyn = sizeof(uint32_t) * (size_t)bitmap_words;
   int occ = 0;
   GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, MXM_THREADS, dyn));
@@ -236,7 +242,7 @@ static void launch_items(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval
   }
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(MXM_THREADS), dyn, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col,
-                     static_cast<T*>(Cval), sums, row_group, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
+                     static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
                      (const int32_t*)(w.start.as<int32_t>() + A.nrows), w.counter.as<unsigned int>(),
                      c.prof ? c.prof_dev.as<unsigned long long>() : nullptr, bitmap_words);
   GA_HIP(hipGetLastError());
@@ -254,10 +260,49 @@ template <typename T, int MODE>
 static void dispatch_pat(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
                          const int32_t* row_group, MxmWork& w, int bm, hipStream_t s) {
   const bool pa = A.val == nullptr, pb = B.val == nullptr;
-  if (pa && pb) launch_items<T, true, true, MODE>(A, B, M, Cval, sums, row_group, w, bm, s);
-  else if (pa) launch_items<T, true, false, MODE>(A, B, M, Cval, sums, row_group, w, bm, s);
-  else if (pb) launch_items<T, false, true, MODE>(A, B, M, Cval, sums, row_group, w, bm, s);
-  else launch_items<T, false, false, MODE>(A, B, M, Cval, sums, row_group, w, bm, s);
+  if (pa && pb) launch_items<T, true, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
+  else if (pa) launch_items<T, true, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
+  else if (pb) launch_items<T, false, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
+  else launch_items<T, false, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
+}
+
+static void build_items(const DCsr& A, const DCsr& M, int tcap, MxmWork& w, hipStream_t s) {
+  const int64_t m = A.nrows;
+  const int64_t max_items = m + ceil_div(A.nnz, tcap) + 1;
+  GA_REQUIRE(max_items < INT32_MAX, GA_ERR_LIMIT, "too many work items");
+  w.cnt.ensure(sizeof(int32_t) * m);
+  w.start.ensure(sizeof(int32_t) * (m + 1));
+  w.item_row.ensure(sizeof(int32_t) * max_items);
+  w.item_t0.ensure(sizeof(int32_t) * max_items);
+  w.counter.ensure(256);
+  fill_u32(w.counter.p, 0u, 1, s);
+  hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, m, tcap,
+                     w.cnt.as<int32_t>());
+  GA_HIP(hipGetLastError());
+  scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
+  hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
+                     (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, tcap,
+                     w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
+  GA_HIP(hipGetLastError());
+}
+
+void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t* gbase, int64_t max_cols,
+                         int threads, unsigned long long* sums, MxmWork& w, hipStream_t s) {
+  if (L.nrows == 0 || L.nnz == 0) return;
+  build_items(L, L, threads, w, s);
+  const int bm = (max_cols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(max_cols, 32) : 0;
+  switch (threads) {
+    case 64:
+      launch_items<int64_t, true, true, 1, 64, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s);
+      break;
+    case 128:
+      launch_items<int64_t, true, true, 1, 128, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s);
+      break;
+    case 256:
+      launch_items<int64_t, true, true, 1, 256, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s);
+      break;
+    default: throw Error{GA_ERR_ARG, "workgroup size must be 64, 128 or 256"};
+  }
 }
 
 void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,
@@ -272,22 +317,7 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
     else fill_u64(Cval, 0ull, M.nnz, s);
   }
   if (m == 0 || A.nnz == 0 || M.nnz == 0) return;
-  const int64_t max_items = m + ceil_div(A.nnz, MXM_TCAP) + 1;
-  GA_REQUIRE(max_items < INT32_MAX, GA_ERR_LIMIT, "too many work items");
-  w.cnt.ensure(sizeof(int32_t) * m);
-  w.start.ensure(sizeof(int32_t) * (m + 1));
-  w.item_row.ensure(sizeof(int32_t) * max_items);
-  w.item_t0.ensure(sizeof(int32_t) * max_items);
-  w.counter.ensure(256);
-  fill_u32(w.counter.p, 0u, 1, s);
-  hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, m,
-                     w.cnt.as<int32_t>());
-  GA_HIP(hipGetLastError());
-  scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
-  hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
-                     (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m,
-                     w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
-  GA_HIP(hipGetLastError());
+  build_items(A, M, 256, w, s);
   const int bm = (M.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(M.ncols, 32) : 0;
   if (mode == 1) {
     dispatch_pat<int64_t, 1>(A, B, M, nullptr, sums, row_group, w, bm, s);

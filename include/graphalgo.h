@@ -140,6 +140,23 @@ ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles);
 ga_status ga_graph_truss(ga_graph* g, int32_t* truss, int32_t* kmax, ga_truss_stats* stats);
 ga_status ga_graph_bc(ga_graph* g, const int32_t* sources, int64_t nsources, double* bc);
 
+/* ---- batched triangle counting ----------------------------------------- */
+/* Many small graphs resident together and counted in one persistent launch.
+ * The reference counts one graph per call (triangle_counting_masked_sandia,
+ * src/main.cpp:65-86); a batch replaces ngraphs such calls. */
+typedef struct ga_batch ga_batch;
+/* graphs[g]: symmetric pattern, sorted rows.  Uploads the strict lower
+ * triangles back to back (graph-local column indices). */
+ga_status ga_batch_create(int64_t ngraphs, const ga_csr* graphs, ga_batch** out);
+ga_status ga_batch_destroy(ga_batch* b);
+/* counts[g] = triangles of graph g.  threads: workgroup width 64, 128 or 256
+ * lanes (A entries per work item), 0 = the autotuned choice (256 before
+ * tuning). */
+ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts);
+/* Times the three widths on the resident batch, keeps the fastest;
+ * ms_per_candidate (may be NULL) receives 3 times for 64, 128, 256. */
+ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candidate);
+
 /* ---- kernel timing ----------------------------------------------------- */
 /* When enabled, the library brackets every launch of the masked-SpGEMM
  * kernel with HIP events on the stream it runs on and accumulates launch
