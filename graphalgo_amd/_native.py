@@ -42,7 +42,7 @@ class GaTrussStats(C.Structure):
 class GaKernelProf(C.Structure):
     _fields_ = [
         ("launches", C.c_int64), ("kernel_ms", C.c_double),
-        ("wedges", C.c_int64), ("bytes_alg", C.c_int64),
+        ("wedges", C.c_int64), ("bytes_alg", C.c_int64), ("wedges_processed", C.c_int64),
     ]
 
 
@@ -81,6 +81,7 @@ GA_SIGNATURES = {
     "ga_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), F64P]),
     "ga_prof_enable": (C.c_int, [C.c_int]),
     "ga_prof_read": (C.c_int, [C.POINTER(GaKernelProf), C.c_int]),
+    "ga_stream_copy": (C.c_int, [C.c_int64, C.c_int, F64P]),
 }
 
 GAH_SIGNATURES = {

@@ -159,7 +159,7 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
       if (all_k && nnz == 0) break;
       const DCsr C{n, n, nnz, tw.cur[cur].rp.as<int32_t>(), tw.cur[cur].col.as<int32_t>(), nullptr};
       if (!valid) {
-        mxm_masked_dev(C, C, C, GA_INT32, 0, tw.sup.p, nullptr, nullptr, tw.mw, s);
+        support_symmetric_dev(C, tw.sup.as<int32_t>(), tw.mw, s);
         ++passes;
         valid = true;
       }
@@ -214,6 +214,7 @@ using namespace ga;
 
 struct ga_graph {
   int device = -1;
+  bool symmetric = false;
   OwnedCsr A;
   OwnedCsr L;
   bool have_L = false;
@@ -240,6 +241,7 @@ ga_status ga_triangle_count(const ga_csr* A, int64_t* triangles) {
     sums.ensure(64);
     fill_u64(sums.p, 0ull, 1, s);
     MxmWork w;
+    w.max_mask_row = host_max_row(A);
     const DCsr vL = L.view();
     mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
     unsigned long long t = 0;
@@ -267,6 +269,7 @@ ga_status ga_triangle_count_lu(const ga_csr* A, int64_t* triangles) {
     sums.ensure(64);
     fill_u64(sums.p, 0ull, 1, s);
     MxmWork w;
+    w.max_mask_row = host_max_row(A);
     mxm_masked_dev(L.view(), U.view(), vA, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
     unsigned long long t = 0;
     GA_HIP(hipMemcpyAsync(&t, sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
@@ -292,6 +295,7 @@ ga_status ga_triangle_count_vertex(const ga_csr* A, int64_t* per_vertex) {
     fill_u64(sums.p, 0ull, n, s);
     hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, groups.as<int32_t>(), n);
     MxmWork w;
+    w.max_mask_row = host_max_row(A);
     mxm_masked_dev(vA, vA, vA, GA_INT64, 1, nullptr, sums.as<unsigned long long>(),
                    (const int32_t*)groups.as<int32_t>(), w, s);
     if (n) GA_HIP(hipMemcpyAsync(per_vertex, sums.p, 8 * n, hipMemcpyDeviceToHost, s));
@@ -304,6 +308,7 @@ ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_co
                     int32_t* iterations) {
   return guarded([&] {
     validate_host_csr(A, "A");
+    GA_REQUIRE(host_csr_symmetric(A), GA_ERR_ARG, "k-truss needs a symmetric pattern");
     GA_REQUIRE(k >= 3, GA_ERR_ARG, "k must be >= 3");
     GA_REQUIRE(out_rowptr && out_nnz && (A->nnz == 0 || out_col), GA_ERR_ARG, "null output");
     hipStream_t s = ctx().stream;
@@ -312,6 +317,7 @@ ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_co
     DCsr vA = dA.view();
     vA.val = nullptr;
     TrussWork tw;
+    tw.mw.max_mask_row = host_max_row(A);
     int cur = 0;
     int64_t nnz = 0;
     ga_truss_stats st{};
@@ -327,6 +333,7 @@ ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_co
 ga_status ga_truss_decomposition(const ga_csr* A, int32_t* truss, int32_t* kmax, ga_truss_stats* stats) {
   return guarded([&] {
     validate_host_csr(A, "A");
+    GA_REQUIRE(host_csr_symmetric(A), GA_ERR_ARG, "truss decomposition needs a symmetric pattern");
     GA_REQUIRE(A->nnz == 0 || truss != nullptr, GA_ERR_ARG, "null output");
     hipStream_t s = ctx().stream;
     OwnedCsr dA;
@@ -334,6 +341,7 @@ ga_status ga_truss_decomposition(const ga_csr* A, int32_t* truss, int32_t* kmax,
     DCsr vA = dA.view();
     vA.val = nullptr;
     TrussWork tw;
+    tw.mw.max_mask_row = host_max_row(A);
     int32_t km = 2;
     ga_truss_stats st{};
     truss_loop(vA, true, 0, tw, s, nullptr, nullptr, &km, &st);
@@ -353,6 +361,9 @@ ga_status ga_graph_create(const ga_csr* A, ga_graph** out) {
     auto* g = new ga_graph();
     try {
       g->device = c.device;
+      g->symmetric = host_csr_symmetric(A);
+      g->tw.mw.max_mask_row = host_max_row(A);
+      g->mw.max_mask_row = g->tw.mw.max_mask_row;
       upload_csr(A, GA_INT32, g->A, c.stream);
       g->A.val.release();
       GA_HIP(hipStreamSynchronize(c.stream));
@@ -394,6 +405,7 @@ ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles) {
 ga_status ga_graph_truss(ga_graph* g, int32_t* truss, int32_t* kmax, ga_truss_stats* stats) {
   return guarded([&] {
     GA_REQUIRE(g != nullptr, GA_ERR_ARG, "null graph");
+    GA_REQUIRE(g->symmetric, GA_ERR_ARG, "truss decomposition needs a symmetric pattern");
     hipStream_t s = ctx().stream;
     int32_t km = 2;
     ga_truss_stats st{};

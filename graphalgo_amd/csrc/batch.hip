@@ -62,6 +62,7 @@ ga_status ga_batch_create(int64_t ngraphs, const ga_csr* graphs, ga_batch** out)
       b->ngraphs = ngraphs;
       b->rows = rows;
       b->max_n = max_n;
+      b->mw.max_mask_row = max_n;
       ga_csr h{rows, max_n, nnz, rp.data(), col.data(), nullptr};
       upload_csr(&h, GA_INT32, b->L, c.stream);
       b->row_group.ensure(sizeof(int32_t) * grp.size());
@@ -91,7 +92,8 @@ ga_status ga_batch_destroy(ga_batch* b) {
 static void batch_run(ga_batch* b, int threads, hipStream_t s) {
   fill_u64(b->sums.p, 0ull, b->ngraphs, s);
   const DCsr L = b->L.view();
-  mxm_batched_sum_dev(L, b->row_group.as<int32_t>(), b->gbase.as<int32_t>(), b->max_n, threads,
+  b->mw.threads = threads;
+  mxm_batched_sum_dev(L, b->row_group.as<int32_t>(), b->gbase.as<int32_t>(), b->max_n,
                       b->sums.as<unsigned long long>(), b->mw, s);
 }
 

@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -99,13 +100,17 @@ struct Context {
   int num_cu = 0;
   int64_t hbm_bytes = 0;
   std::string name;
+  // row binning default: rows with at most this many products (and a mask
+  // of at most 128 entries) go to the wavefront-per-row kernel; GA_WAVE_FLOPS
+  // overrides it at ga_init
+  long long wave_flops = 65536;
   // kernel timing
   bool prof = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
   std::vector<hipEvent_t> event_pool;
   int64_t prof_launches = 0;
   int64_t prof_bytes_host = 0;  // host-known part of the algorithmic bytes
-  DevBuf prof_dev;              // [0] = wedges processed (u64)
+  DevBuf prof_dev;              // [0] = wedges processed, [1] = algorithmic wedges (u64)
 };
 
 Context& ctx();  // throws GA_ERR_STATE when ga_init has not run
@@ -127,6 +132,14 @@ struct OwnedCsr {
 
 size_t dtype_size(int dtype);
 void validate_host_csr(const ga_csr* A, const char* name);
+// Longest row of a host CSR.
+inline int64_t host_max_row(const ga_csr* A) {
+  int64_t mx = 0;
+  for (int64_t i = 0; i < A->nrows; ++i) mx = std::max<int64_t>(mx, A->rowptr[i + 1] - A->rowptr[i]);
+  return mx;
+}
+// True when the (validated) pattern equals its transpose.
+bool host_csr_symmetric(const ga_csr* A);
 void upload_csr(const ga_csr* A, int dtype, OwnedCsr& out, hipStream_t s);
 
 // ---- device primitives --------------------------------------------------
@@ -164,7 +177,12 @@ void fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s);
 
 // Workspace of the masked SpGEMM (row items and counters).
 struct MxmWork {
-  DevBuf cnt, start, item_row, item_t0, scan_tmp, counter;
+  DevBuf cnt, start, item_row, item_t0, scan_tmp, counter, mend;
+  DevBuf flag, rstart, rows;  // wavefront-per-row bin
+  long long wave_flops = -1;     // binning threshold (products per row) for the wavefront kernel;
+                                 // -1 = the context default
+  int threads = 256;             // workgroup width of the item kernel (64, 128 or 256)
+  int64_t max_mask_row = INT32_MAX;  // longest mask row (host-known bound); < 65536 enables the rank lookup
 };
 
 // C<M> = A*B (structural mask).  mode 0: per-entry values into Cval (dtype);
@@ -173,11 +191,15 @@ struct MxmWork {
 void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,
                     unsigned long long* sums, const int32_t* row_group, MxmWork& w, hipStream_t s);
 
+// S<C> = C*C (int32 per entry) for a symmetric loop-free pattern C, computed
+// on the strict lower half and mirrored (the k-truss support pass).
+void support_symmetric_dev(const DCsr& C, int32_t* S, MxmWork& w, hipStream_t s);
+
 // Batched sum-mode product over many graphs stored back to back (graph-local
 // column indices): sums[g] += sum of C<L_g> = L_g * L_g.  threads = 64, 128
 // or 256 (workgroup size = A entries per work item).
 void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t* gbase, int64_t max_cols,
-                         int threads, unsigned long long* sums, MxmWork& w, hipStream_t s);
+                         unsigned long long* sums, MxmWork& w, hipStream_t s);
 
 // Batched betweenness centrality workspace and driver (bc.hip).
 struct BcWork {

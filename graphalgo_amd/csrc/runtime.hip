@@ -3,6 +3,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 
 namespace ga {
@@ -49,6 +50,19 @@ void validate_host_csr(const ga_csr* A, const char* name) {
                  nm + ": row " + std::to_string(i) + " not sorted / has duplicates");
     }
   }
+}
+
+bool host_csr_symmetric(const ga_csr* A) {
+  if (A->nrows != A->ncols) return false;
+  for (int64_t i = 0; i < A->nrows; ++i)
+    for (int32_t t = A->rowptr[i]; t < A->rowptr[i + 1]; ++t) {
+      const int32_t j = A->col[t];
+      const int32_t* b = A->col + A->rowptr[j];
+      const int32_t* e = A->col + A->rowptr[j + 1];
+      const int32_t* p = std::lower_bound(b, e, (int32_t)i);
+      if (p == e || *p != i) return false;
+    }
+  return true;
 }
 
 void upload_csr(const ga_csr* A, int dtype, OwnedCsr& out, hipStream_t s) {
@@ -225,6 +239,14 @@ void lower_triangle_dev(const DCsr& A, OwnedCsr& L, DevBuf& tmp, hipStream_t s) 
   GA_HIP(hipGetLastError());
 }
 
+// Plain streaming copy (16 B per lane per iteration): the calibration
+// reference for HBM byte counters and the achievable-bandwidth probe.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     int64_t n16) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace ga
 
 using namespace ga;
@@ -249,6 +271,7 @@ ga_status ga_init(int device) {
     c->hbm_bytes = (int64_t)prop.totalGlobalMem;
     c->name = prop.gcnArchName;
     GA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (const char* wf = std::getenv("GA_WAVE_FLOPS")) c->wave_flops = std::atoll(wf);
     c->prof_dev.ensure(64);
     GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
     GA_HIP(hipStreamSynchronize(c->stream));
@@ -278,6 +301,33 @@ ga_status ga_check_csr(const ga_csr* A) {
 
 void ga_free_host(void* p) { std::free(p); }
 
+ga_status ga_stream_copy(int64_t bytes, int reps, double* ms_per_copy) {
+  return guarded([&] {
+    GA_REQUIRE(bytes > 0 && bytes % 16 == 0 && reps > 0, GA_ERR_ARG, "bytes must be a positive multiple of 16");
+    Context& c = ctx();
+    DevBuf a, b;
+    a.ensure(bytes);
+    b.ensure(bytes);
+    fill_u32(a.p, 0x01020304u, bytes / 4, c.stream);
+    const int64_t n16 = bytes / 16;
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n16, 256), (int64_t)c.num_cu * 32);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c.stream, (const uint4*)a.p, (uint4*)b.p, n16);
+    hipEvent_t e0, e1;
+    GA_HIP(hipEventCreate(&e0));
+    GA_HIP(hipEventCreate(&e1));
+    GA_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c.stream, (const uint4*)a.p, (uint4*)b.p, n16);
+    GA_HIP(hipEventRecord(e1, c.stream));
+    GA_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+  });
+}
+
 ga_status ga_prof_enable(int on) {
   return guarded([&] { ctx().prof = on != 0; });
 }
@@ -293,12 +343,13 @@ ga_status ga_prof_read(ga_kernel_prof* out, int reset) {
       GA_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
       ms += t;
     }
-    unsigned long long w = 0;
-    GA_HIP(hipMemcpy(&w, c.prof_dev.p, sizeof(w), hipMemcpyDeviceToHost));
+    unsigned long long w[2] = {0, 0};
+    GA_HIP(hipMemcpy(w, c.prof_dev.p, sizeof(w), hipMemcpyDeviceToHost));
     out->launches = c.prof_launches;
     out->kernel_ms = ms;
-    out->wedges = (int64_t)w;
-    out->bytes_alg = c.prof_bytes_host + 4 * (int64_t)w;
+    out->wedges = (int64_t)w[1];
+    out->wedges_processed = (int64_t)w[0];
+    out->bytes_alg = c.prof_bytes_host + 4 * (int64_t)w[1];
     if (reset) {
       for (auto& pr : c.prof_events) {
         c.event_pool.push_back(pr.first);

@@ -28,12 +28,14 @@
 namespace ga {
 
 constexpr int64_t MXM_BITMAP_MAX_COLS = int64_t(1) << 18;  // 32 KB bitmap
+constexpr int MXM_UNROLL = 4;                                // gathers in flight per lane
+constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up to 2^17 columns
 
 // THREADS = workgroup size = A entries per item (one entry per thread in the
 // entry scan); MCAP = mask entries staged in LDS.  BATCHED: the operands are
 // many graphs stored back to back with graph-local column indices; row i
 // belongs to graph row_group[i] whose rows start at gbase[row_group[i]].
-template <typename T, bool PAT_A, bool PAT_B, int MODE, int THREADS, bool BATCHED>
+template <typename T, bool PAT_A, bool PAT_B, int MODE, int THREADS, bool BATCHED, bool RANK>
 __global__ __launch_bounds__(THREADS) void k_mxm_items(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const T* __restrict__ Aval,
     const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol, const T* __restrict__ Bval,
@@ -41,12 +43,16 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ item_row, const int32_t* __restrict__ item_t0,
     const int32_t* __restrict__ n_items_ptr, unsigned int* __restrict__ work_counter,
-    unsigned long long* __restrict__ wedge_counter, int bitmap_words) {
+    unsigned long long* __restrict__ wedge_counter, int bitmap_words, const int32_t* __restrict__ Mend,
+    int trunc, int chunk) {
   constexpr int MXM_THREADS = THREADS;
   constexpr int MXM_TCAP = THREADS;
-  constexpr int MXM_MCAP = 8 * THREADS;
+  constexpr int MXM_MCAP = 4 * THREADS;
+  // dynamic LDS: the n-bit mask bitmap, then (RANK) per bitmap word the
+  // position in the mask row of the word's first mask column
   extern __shared__ uint32_t s_bitmap[];
-  __shared__ int32_t s_mcol[MXM_MCAP];
+  uint16_t* s_wpre = reinterpret_cast<uint16_t*>(s_bitmap + bitmap_words);
+  __shared__ int32_t s_mcol[RANK ? 1 : MXM_MCAP];
   __shared__ T s_acc[MODE == 0 ? MXM_MCAP : 1];
   __shared__ int32_t s_ebeg[MXM_TCAP];
   __shared__ long long s_eoff[MXM_TCAP + 1];
@@ -57,13 +63,22 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = tid >> 6;
-  const bool use_bm = bitmap_words > 0;
+  const bool use_bm = RANK || bitmap_words > 0;
   for (int q = tid; q < bitmap_words; q += MXM_THREADS) s_bitmap[q] = 0u;
   const int n_items = *n_items_ptr;
+  // items are claimed `chunk` at a time (one atomic per chunk keeps the shared
+  // counter off the critical path); thread 0 holds the claimed range
+  int next = 0, end = 0;
 
   while (true) {
     __syncthreads();  // previous item fully retired (LDS reuse)
-    if (tid == 0) s_item = (int)atomicAdd(work_counter, 1u);
+    if (tid == 0) {
+      if (next >= end) {
+        next = (int)atomicAdd(work_counter, (unsigned)chunk);
+        end = min(next + chunk, n_items);
+      }
+      s_item = next < end ? next++ : n_items;
+    }
     __syncthreads();
     const int q = s_item;
     if (q >= n_items) break;
@@ -75,17 +90,20 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     const int t1 = min(t0 + MXM_TCAP, a_end);
     const bool multi = (a_end - a_beg) > MXM_TCAP;
     const int ms = Mrp[i];
-    const int mnz = Mrp[i + 1] - ms;
+    const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // Mend: only a prefix of M's row is the mask
     const bool local = mnz <= MXM_MCAP;
     const int bbase = BATCHED ? gbase[row_group[i]] : 0;
 
-    // stage the mask row
+    // stage the mask row: bitmap bits (and the per-word rank base, or the
+    // sorted column list for the binary-search variant)
     for (int p = tid; p < mnz; p += MXM_THREADS) {
       const int c = Mcol[ms + p];
-      if (local) {
+      if (RANK) {
+        if (p == 0 || (Mcol[ms + p - 1] >> 5) != (c >> 5)) s_wpre[c >> 5] = (uint16_t)p;
+      } else if (local) {
         s_mcol[p] = c;
-        if (MODE == 0) s_acc[p] = T(0);
       }
+      if (MODE == 0 && local) s_acc[p] = T(0);
       if (use_bm) atomicOr(&s_bitmap[c >> 5], 1u << (c & 31));
     }
     // stage the item's entries: B row start and length, A value
@@ -94,7 +112,20 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     if (tid < ne) {
       const int k = bbase + Acol[t0 + tid];
       const int b0 = Brp[k];
-      len = Brp[k + 1] - b0;
+      int b1 = Brp[k + 1];
+      if (trunc && mnz > 0) {
+        // products beyond the last mask column are discarded by the mask:
+        // stop B's row at the upper bound of that column
+        const int maxc = Mcol[ms + mnz - 1];
+        int lo = b0, hi = b1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (Bcol[mid] <= maxc) lo = mid + 1;
+          else hi = mid;
+        }
+        b1 = lo;
+      }
+      len = b1 - b0;
       s_ebeg[tid] = b0;
       if (!PAT_A) s_aval[tid] = Aval[t0 + tid];
     }
@@ -111,45 +142,73 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     if (tid == 0) s_eoff[ne] = total;
     __syncthreads();
 
-    // stride over the flattened wedge space of the item
+    // stride over the flattened wedge space of the item, MXM_UNROLL wedges
+    // per lane per trip: the B-column loads of one trip are independent, so
+    // each lane keeps MXM_UNROLL gathers in flight before it consumes them
     long long lsum = 0;
     int e = 0;
-    for (long long w = tid; w < total; w += MXM_THREADS) {
-      if (s_eoff[e + 1] <= w) {  // advance to the entry holding wedge w
-        int lo = e + 1, hi = ne - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_eoff[mid] <= w) lo = mid;
-          else hi = mid - 1;
+    for (long long w0 = tid; w0 < total; w0 += (long long)MXM_THREADS * MXM_UNROLL) {
+      int cc[MXM_UNROLL], jj[MXM_UNROLL], ee[MXM_UNROLL];
+#pragma unroll
+      for (int u = 0; u < MXM_UNROLL; ++u) {
+        const long long w = w0 + (long long)u * MXM_THREADS;
+        cc[u] = -1;
+        jj[u] = 0;
+        ee[u] = 0;
+        if (w < total) {
+          if (s_eoff[e + 1] <= w) {  // advance to the entry holding wedge w
+            int lo = e + 1, hi = ne - 1;
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (s_eoff[mid] <= w) lo = mid;
+              else hi = mid - 1;
+            }
+            e = lo;
+          }
+          ee[u] = e;
+          jj[u] = s_ebeg[e] + (int)(w - s_eoff[e]);
+          cc[u] = Bcol[jj[u]];
         }
-        e = lo;
       }
-      const int j = s_ebeg[e] + (int)(w - s_eoff[e]);
-      const int c = Bcol[j];
-      if (use_bm && !((s_bitmap[c >> 5] >> (c & 31)) & 1u)) continue;
-      int lo = 0, hi = mnz;
-      bool found;
-      if (local) {
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_mcol[mid] < c) lo = mid + 1;
-          else hi = mid;
+#pragma unroll
+      for (int u = 0; u < MXM_UNROLL; ++u) {
+        const int c = cc[u];
+        if (c < 0) continue;
+        int p;
+        if (RANK) {
+          // exact membership and mask position from the bitmap word:
+          // position = rank base of the word + set bits below column c
+          const uint32_t word = s_bitmap[c >> 5];
+          const uint32_t bit = 1u << (c & 31);
+          if (!(word & bit)) continue;
+          p = (int)s_wpre[c >> 5] + __popc(word & (bit - 1u));
+        } else {
+          if (use_bm && !((s_bitmap[c >> 5] >> (c & 31)) & 1u)) continue;
+          int lo = 0, hi = mnz;
+          bool found;
+          if (local) {
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (s_mcol[mid] < c) lo = mid + 1;
+              else hi = mid;
+            }
+            found = lo < mnz && s_mcol[lo] == c;
+          } else {
+            const int32_t* mrow = Mcol + ms;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (mrow[mid] < c) lo = mid + 1;
+              else hi = mid;
+            }
+            found = lo < mnz && mrow[lo] == c;
+          }
+          if (!found) continue;
+          p = lo;
         }
-        found = lo < mnz && s_mcol[lo] == c;
-      } else {
-        const int32_t* mrow = Mcol + ms;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (mrow[mid] < c) lo = mid + 1;
-          else hi = mid;
-        }
-        found = lo < mnz && mrow[lo] == c;
-      }
-      if (found) {
-        const T v = (PAT_A ? T(1) : s_aval[e]) * (PAT_B ? T(1) : Bval[j]);
+        const T v = (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]);
         if (MODE == 1) lsum += (long long)v;
-        else if (local) atomic_add_val(&s_acc[lo], v);
-        else atomic_add_val(&Cval[ms + lo], v);
+        else if (local) atomic_add_val(&s_acc[p], v);
+        else atomic_add_val(&Cval[ms + p], v);
       }
     }
     __syncthreads();
@@ -179,7 +238,7 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     }
     if (use_bm) {
       for (int p = tid; p < mnz; p += MXM_THREADS) {
-        const int c = local ? s_mcol[p] : Mcol[ms + p];
+        const int c = (!RANK && local) ? s_mcol[p] : Mcol[ms + p];
         s_bitmap[c >> 5] = 0u;
       }
     }
@@ -187,11 +246,190 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
   }
 }
 
-__global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp, int64_t m,
-                             int tcap, int32_t* __restrict__ cnt) {
+// ---- short rows: one wavefront per row -------------------------------------
+// For matrices with at most WR_MAXCOLS columns (the small graphs of a batch),
+// a row whose mask has at most WR_MCAP entries and whose product count is
+// below the binning threshold is owned by a single wavefront: the mask row is
+// staged as a per-wave LDS bitmap plus per-word rank bases, B rows are
+// gathered over the flattened wedge space of up to 64 A entries at a time,
+// and hits accumulate into a per-wave mask-indexed accumulator written out
+// once.  No workgroup barrier is involved, so the four waves of a workgroup
+// run four independent rows and overlap their dependent-load chains.
+constexpr int WR_WAVES = 4;
+constexpr int WR_MCAP = 128;
+constexpr int WR_MAXCOLS = 4096;
+constexpr int WR_BMW = WR_MAXCOLS / 32;
+constexpr int WR_UNROLL = 4;
+
+template <typename T, bool PAT_A, bool PAT_B, int MODE, bool BATCHED>
+__global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
+    const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const T* __restrict__ Aval,
+    const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol, const T* __restrict__ Bval,
+    const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol, const int32_t* __restrict__ Mend,
+    T* __restrict__ Cval, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
+    const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
+    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
+  __shared__ uint32_t s_bm[WR_WAVES][WR_BMW];
+  __shared__ uint16_t s_wpre[WR_WAVES][WR_BMW];
+  __shared__ T s_acc[WR_WAVES][MODE == 0 ? WR_MCAP : 1];
+  __shared__ int32_t s_ebeg[WR_WAVES][64];
+  __shared__ int32_t s_eoff[WR_WAVES][65];
+  __shared__ T s_aval[WR_WAVES][PAT_A ? 1 : 64];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  uint32_t* bm = s_bm[wv];
+  uint16_t* wpre = s_wpre[wv];
+  int32_t* ebeg = s_ebeg[wv];
+  int32_t* eoff = s_eoff[wv];
+  for (int q = lane; q < WR_BMW; q += 64) bm[q] = 0u;
+  const int n_rows = *n_rows_ptr;
+  unsigned long long wedges = 0;
+  int next = 0, end = 0;  // rows claimed `chunk` at a time (lane 0's range)
+  while (true) {
+    int r = 0;
+    if (lane == 0) {
+      if (next >= end) {
+        next = (int)atomicAdd(counter, (unsigned)chunk);
+        end = min(next + chunk, n_rows);
+      }
+      r = next < end ? next++ : n_rows;
+    }
+    r = __shfl(r, 0, 64);
+    if (r >= n_rows) break;
+    const int i = rows[r];
+    const int ms = Mrp[i];
+    const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // <= WR_MCAP by binning
+    for (int p = lane; p < mnz; p += 64) {
+      const int c = Mcol[ms + p];
+      if (p == 0 || (Mcol[ms + p - 1] >> 5) != (c >> 5)) wpre[c >> 5] = (uint16_t)p;
+      atomicOr(&bm[c >> 5], 1u << (c & 31));
+      if (MODE == 0) s_acc[wv][p] = T(0);
+    }
+    const int maxc = (trunc && mnz > 0) ? Mcol[ms + mnz - 1] : 0x7FFFFFFF;
+    const int bbase = BATCHED ? gbase[row_group[i]] : 0;
+    __threadfence_block();
+    long long lsum = 0;
+    const int a_end = Arp[i + 1];
+    for (int t0 = Arp[i]; t0 < a_end; t0 += 64) {
+      const int ne = min(64, a_end - t0);
+      int len = 0;
+      if (lane < ne) {
+        const int k = bbase + Acol[t0 + lane];
+        const int b0 = Brp[k];
+        int b1 = Brp[k + 1];
+        if (trunc) {
+          int lo = b0, hi = b1;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Bcol[mid] <= maxc) lo = mid + 1;
+            else hi = mid;
+          }
+          b1 = lo;
+        }
+        len = b1 - b0;
+        ebeg[lane] = b0;
+        if (!PAT_A) s_aval[wv][lane] = Aval[t0 + lane];
+      }
+      const int incl = wave_incl_scan(len);
+      if (lane < ne) eoff[lane] = incl - len;
+      const int total = __shfl(incl, 63, 64);
+      if (lane == 0) eoff[ne] = total;
+      __threadfence_block();
+      if (lane == 0) wedges += (unsigned long long)total;
+      int e = 0;
+      for (int w0 = lane; w0 < total; w0 += 64 * WR_UNROLL) {
+        int cc[WR_UNROLL], jj[WR_UNROLL], ee[WR_UNROLL];
+#pragma unroll
+        for (int u = 0; u < WR_UNROLL; ++u) {
+          const int w = w0 + u * 64;
+          cc[u] = -1;
+          jj[u] = 0;
+          ee[u] = 0;
+          if (w < total) {
+            if (eoff[e + 1] <= w) {
+              int lo = e + 1, hi = ne - 1;
+              while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (eoff[mid] <= w) lo = mid;
+                else hi = mid - 1;
+              }
+              e = lo;
+            }
+            ee[u] = e;
+            jj[u] = ebeg[e] + (w - eoff[e]);
+            cc[u] = Bcol[jj[u]];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < WR_UNROLL; ++u) {
+          const int c = cc[u];
+          if (c < 0) continue;
+          const uint32_t word = bm[c >> 5];
+          const uint32_t bit = 1u << (c & 31);
+          if (!(word & bit)) continue;
+          const T v = (PAT_A ? T(1) : s_aval[wv][ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]);
+          if (MODE == 1) lsum += (long long)v;
+          else atomic_add_val(&s_acc[wv][(int)wpre[c >> 5] + __popc(word & (bit - 1u))], v);
+        }
+      }
+      __threadfence_block();  // entry slots are rewritten by the next chunk
+    }
+    if (MODE == 0) {
+      for (int p = lane; p < mnz; p += 64) Cval[ms + p] = s_acc[wv][p];
+    } else {
+      const long long tot = wave_reduce_sum(lsum);
+      if (lane == 0 && tot) atomicAdd(&sums[row_group ? row_group[i] : 0], (unsigned long long)tot);
+    }
+    for (int p = lane; p < mnz; p += 64) bm[Mcol[ms + p] >> 5] = 0u;
+    __threadfence_block();
+  }
+  if (wedge_counter) {
+    wedges = wave_reduce_sum(wedges);
+    if (lane == 0 && wedges) atomicAdd(wedge_counter, wedges);
+  }
+}
+
+// Row binning: rows with an empty A row or empty mask get no work; rows
+// whose mask fits WR_MCAP and whose (untruncated) product count is at most
+// wave_flops go to the wavefront kernel; the rest are split into items of
+// tcap A entries for the workgroup kernel.
+__global__ void k_bin_rows(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol,
+                           const int32_t* __restrict__ Brp, const int32_t* __restrict__ Mrp,
+                           const int32_t* __restrict__ Mend, const int32_t* __restrict__ row_group,
+                           const int32_t* __restrict__ gbase, int64_t m, int tcap, long long wave_flops,
+                           int32_t* __restrict__ short_flag, int32_t* __restrict__ item_cnt) {
+  // wave_flops < 0: no wavefront bin (matrix too wide for the per-wave bitmap)
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < m; i += (int64_t)gridDim.x * wpb) {
+    const int ab = Arp[i], ae = Arp[i + 1];
+    const int mz = (Mend ? Mend[i] : Mrp[i + 1]) - Mrp[i];
+    const int bbase = gbase ? gbase[row_group[i]] : 0;
+    long long f = 0;
+    if (mz > 0 && mz <= WR_MCAP)
+      for (int t = ab + lane; t < ae; t += 64) f += Brp[bbase + Acol[t] + 1] - Brp[bbase + Acol[t]];
+    f = wave_reduce_sum(f);
+    if (lane == 0) {
+      const int a = ae - ab;
+      const bool none = a == 0 || mz == 0;
+      const bool shrt = !none && wave_flops >= 0 && mz <= WR_MCAP && f <= wave_flops;
+      short_flag[i] = shrt ? 1 : 0;
+      item_cnt[i] = (none || shrt) ? 0 : (a + tcap - 1) / tcap;
+    }
+  }
+}
+
+__global__ void k_emit_rows(const int32_t* __restrict__ flag, const int32_t* __restrict__ off, int64_t m,
+                            int32_t* __restrict__ rows) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    if (flag[i]) rows[off[i]] = (int32_t)i;
+}
+
+__global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp,
+                             const int32_t* __restrict__ Mend, int64_t m, int tcap, int32_t* __restrict__ cnt) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     const int a = Arp[i + 1] - Arp[i];
-    const int mz = Mrp[i + 1] - Mrp[i];
+    const int mz = (Mend ? Mend[i] : Mrp[i + 1]) - Mrp[i];
     cnt[i] = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
   }
 }
@@ -208,43 +446,89 @@ __global__ void k_item_fill(const int32_t* __restrict__ Arp, const int32_t* __re
   }
 }
 
+static hipEvent_t take_event(Context& c) {
+  hipEvent_t ev;
+  if (!c.event_pool.empty()) {
+    ev = c.event_pool.back();
+    c.event_pool.pop_back();
+  } else {
+    GA_HIP(hipEventCreate(&ev));
+  }
+  return ev;
+}
+
+// Bins the rows, then runs the wavefront kernel on the short rows and the
+// workgroup kernel on the items of the long rows.  With profiling on, the
+// pair is bracketed by one event pair (one "launch" of the masked product).
 template <typename T, bool PA, bool PB, int MODE, int THREADS, bool BATCHED>
-static void launch_items(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
-                         const int32_t* row_group, const int32_t* gbase, MxmWork& w, int bitmap_words,
-                         hipStream_t s) {
-  constexpr int MXM_THREADS = THREADS;
-  constexpr int MXM_TCAP = THREADS;
-  auto kern = k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED>;
-  const size_t dyn = sizeof(uint32_t) * (size_t)bitmap_words;
-  int occ = 0;
-  GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, MXM_THREADS, dyn));
-  if (occ < 1) occ = 1;
+static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
+                          const int32_t* row_group, const int32_t* gbase, MxmWork& w, int bitmap_words,
+                          hipStream_t s, const int32_t* Mend, int trunc) {
+  constexpr int TCAP = THREADS;
+  const int64_t m = A.nrows;
+  const int64_t max_items = m + ceil_div(A.nnz, TCAP) + 1;
+  GA_REQUIRE(max_items < INT32_MAX, GA_ERR_LIMIT, "too many work items");
+  w.cnt.ensure(sizeof(int32_t) * m);
+  w.start.ensure(sizeof(int32_t) * (m + 1));
+  w.flag.ensure(sizeof(int32_t) * m);
+  w.rstart.ensure(sizeof(int32_t) * (m + 1));
+  w.rows.ensure(sizeof(int32_t) * m);
+  w.item_row.ensure(sizeof(int32_t) * max_items);
+  w.item_t0.ensure(sizeof(int32_t) * max_items);
+  w.counter.ensure(256);
+  fill_u32(w.counter.p, 0u, 2, s);
+  const long long wave_flops =
+      (M.ncols <= WR_MAXCOLS) ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops) : -1;
+  hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, A.rp, A.col, B.rp, M.rp, Mend,
+                     row_group, BATCHED ? gbase : nullptr, m, TCAP, wave_flops, w.flag.as<int32_t>(),
+                     w.cnt.as<int32_t>());
+  GA_HIP(hipGetLastError());
+  scan_exclusive_i32(w.flag.as<int32_t>(), w.rstart.as<int32_t>(), m, w.scan_tmp, s);
+  hipLaunchKernelGGL(k_emit_rows, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, (const int32_t*)w.flag.as<int32_t>(),
+                     (const int32_t*)w.rstart.as<int32_t>(), m, w.rows.as<int32_t>());
+  scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
+  hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
+                     (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, TCAP,
+                     w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
+  GA_HIP(hipGetLastError());
+
+  auto krows = k_mxm_rows<T, PA, PB, MODE, BATCHED>;
+  // rank lookup when the bitmap is at most 16 KB and every mask row fits
+  // the 16-bit word rank bases
+  const bool rank = bitmap_words > 0 && bitmap_words <= MXM_RANK_MAX_WORDS && w.max_mask_row < 65536;
+  auto kitems = rank ? k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, true>
+                     : k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, false>;
+  const size_t dyn = sizeof(uint32_t) * (size_t)bitmap_words + (rank ? sizeof(uint16_t) * ((bitmap_words + 1) & ~1) : 0);
+  int occ_r = 0, occ_i = 0;
+  GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, krows, 64 * WR_WAVES, 0));
+  GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_i, kitems, THREADS, dyn));
   Context& c = ctx();
-  const int64_t max_items = A.nrows + ceil_div(A.nnz, MXM_TCAP) + 1;
-  int64_t grid = (int64_t)c.num_cu * occ;
-  if (grid > max_items) grid = max_items;
-  if (grid < 1) grid = 1;
+  const int64_t grid_r = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ_r, 1),
+                                                               ceil_div(m, WR_WAVES)));
+  const int64_t grid_i = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ_i, 1), max_items));
+  // claim sizes: short rows about 8 claims per resident wave (at most 32
+  // rows per claim); items one at a time -- they are long and uneven, and
+  // claiming several at once leaves a tail of heavy items on few workgroups
+  const int chunk_r = (int)std::max<int64_t>(1, std::min<int64_t>(32, m / (grid_r * WR_WAVES * 8)));
+  const int chunk_i = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c.prof) {
-    auto take = [&]() {
-      hipEvent_t ev;
-      if (!c.event_pool.empty()) {
-        ev = c.event_pool.back();
-        c.event_pool.pop_back();
-      } else {
-        GA_HIP(hipEventCreate(&ev));
-      }
-      return ev;
-    };
-    e0 = take();
-    e1 = take();
+    e0 = take_event(c);
+    e1 = take_event(c);
     GA_HIP(hipEventRecord(e0, s));
   }
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(MXM_THREADS), dyn, s, A.rp, A.col,
+  unsigned long long* wc = c.prof ? c.prof_dev.as<unsigned long long>() : nullptr;
+  hipLaunchKernelGGL(krows, dim3((unsigned)grid_r), dim3(64 * WR_WAVES), 0, s, A.rp, A.col,
+                     static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col, Mend,
+                     static_cast<T*>(Cval), sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
+                     (const int32_t*)(w.rstart.as<int32_t>() + m), w.counter.as<unsigned int>() + 1, wc, trunc,
+                     chunk_r);
+  GA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(kitems, dim3((unsigned)grid_i), dim3(THREADS), dyn, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col,
                      static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
-                     (const int32_t*)(w.start.as<int32_t>() + A.nrows), w.counter.as<unsigned int>(),
-                     c.prof ? c.prof_dev.as<unsigned long long>() : nullptr, bitmap_words);
+                     (const int32_t*)(w.start.as<int32_t>() + m), w.counter.as<unsigned int>(), wc, bitmap_words,
+                     Mend, trunc, chunk_i);
   GA_HIP(hipGetLastError());
   if (c.prof) {
     GA_HIP(hipEventRecord(e1, s));
@@ -260,49 +544,102 @@ template <typename T, int MODE>
 static void dispatch_pat(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
                          const int32_t* row_group, MxmWork& w, int bm, hipStream_t s) {
   const bool pa = A.val == nullptr, pb = B.val == nullptr;
-  if (pa && pb) launch_items<T, true, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
-  else if (pa) launch_items<T, true, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
-  else if (pb) launch_items<T, false, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
-  else launch_items<T, false, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s);
+  if (pa && pb) launch_binned<T, true, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+  else if (pa) launch_binned<T, true, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+  else if (pb) launch_binned<T, false, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+  else launch_binned<T, false, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
 }
 
-static void build_items(const DCsr& A, const DCsr& M, int tcap, MxmWork& w, hipStream_t s) {
-  const int64_t m = A.nrows;
-  const int64_t max_items = m + ceil_div(A.nnz, tcap) + 1;
-  GA_REQUIRE(max_items < INT32_MAX, GA_ERR_LIMIT, "too many work items");
-  w.cnt.ensure(sizeof(int32_t) * m);
-  w.start.ensure(sizeof(int32_t) * (m + 1));
-  w.item_row.ensure(sizeof(int32_t) * max_items);
-  w.item_t0.ensure(sizeof(int32_t) * max_items);
-  w.counter.ensure(256);
-  fill_u32(w.counter.p, 0u, 1, s);
-  hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, m, tcap,
-                     w.cnt.as<int32_t>());
-  GA_HIP(hipGetLastError());
-  scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
-  hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
-                     (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, tcap,
-                     w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
-  GA_HIP(hipGetLastError());
+__global__ void k_count_wedges(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol,
+                               const int32_t* __restrict__ Brp, int64_t m, unsigned long long* __restrict__ out,
+                               const int32_t* __restrict__ row_group = nullptr,
+                               const int32_t* __restrict__ gbase = nullptr) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  unsigned long long acc = 0;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < m; i += (int64_t)gridDim.x * wpb) {
+    const int bb = gbase ? gbase[row_group[i]] : 0;
+    for (int t = Arp[i] + lane; t < Arp[i + 1]; t += 64) acc += Brp[bb + Acol[t] + 1] - Brp[bb + Acol[t]];
+  }
+  acc = wave_reduce_sum(acc);
+  if (lane == 0 && acc) atomicAdd(out, acc);
 }
 
 void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t* gbase, int64_t max_cols,
-                         int threads, unsigned long long* sums, MxmWork& w, hipStream_t s) {
+                         unsigned long long* sums, MxmWork& w, hipStream_t s) {
   if (L.nrows == 0 || L.nnz == 0) return;
-  build_items(L, L, threads, w, s);
   const int bm = (max_cols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(max_cols, 32) : 0;
-  switch (threads) {
+  if (ctx().prof)
+    hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(L.nrows, 4, 4096)), dim3(256), 0, s, L.rp, L.col, L.rp,
+                       L.nrows, ctx().prof_dev.as<unsigned long long>() + 1, row_group, gbase);
+  switch (w.threads) {
     case 64:
-      launch_items<int64_t, true, true, 1, 64, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s);
+      launch_binned<int64_t, true, true, 1, 64, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s, nullptr, 0);
       break;
     case 128:
-      launch_items<int64_t, true, true, 1, 128, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s);
+      launch_binned<int64_t, true, true, 1, 128, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s, nullptr, 0);
       break;
     case 256:
-      launch_items<int64_t, true, true, 1, 256, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s);
+      launch_binned<int64_t, true, true, 1, 256, true>(L, L, L, nullptr, sums, row_group, gbase, w, bm, s, nullptr, 0);
       break;
     default: throw Error{GA_ERR_ARG, "workgroup size must be 64, 128 or 256"};
   }
+}
+
+// ---- symmetric support: S<C> = C*C for a symmetric pattern C ---------------
+// S is symmetric, so only the strict lower half of the mask is computed
+// (mask prefix Mend[i] = first entry of row i with column >= i, B rows cut
+// at the last lower mask column) and the values are mirrored to the upper
+// half.  Same values as the full product, about half the wedges.
+__global__ void k_row_split(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                            int32_t* __restrict__ split) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = rp[i], hi = rp[i + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (col[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    split[i] = lo;
+  }
+}
+
+__global__ void k_mirror_lower(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                               const int32_t* __restrict__ split, int64_t n, int32_t* __restrict__ S) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
+    const int b = rp[i], e = split[i];
+    for (int t = b + lane; t < e; t += 64) {
+      const int j = col[t];  // j < i: entry (j, i) lies in row j's upper part
+      int lo = split[j], hi = rp[j + 1];
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < i) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < rp[j + 1] && col[lo] == i) S[lo] = S[t];
+    }
+  }
+}
+
+void support_symmetric_dev(const DCsr& C, int32_t* S, MxmWork& w, hipStream_t s) {
+  const int64_t n = C.nrows;
+  GA_REQUIRE(C.nrows == C.ncols, GA_ERR_DIM, "symmetric support needs a square matrix");
+  if (C.nnz > 0) fill_u32(S, 0u, C.nnz, s);
+  if (n == 0 || C.nnz == 0) return;
+  w.mend.ensure(sizeof(int32_t) * n);
+  int32_t* split = w.mend.as<int32_t>();
+  hipLaunchKernelGGL(k_row_split, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, C.rp, C.col, n, split);
+  const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : 0;
+  Context& c = ctx();
+  if (c.prof)
+    hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
+                       c.prof_dev.as<unsigned long long>() + 1);
+  launch_binned<int32_t, true, true, 0, 256, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, split, 1);
+  hipLaunchKernelGGL(k_mirror_lower, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, C.rp, C.col,
+                     (const int32_t*)split, n, S);
+  GA_HIP(hipGetLastError());
 }
 
 void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,
@@ -317,7 +654,9 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
     else fill_u64(Cval, 0ull, M.nnz, s);
   }
   if (m == 0 || A.nnz == 0 || M.nnz == 0) return;
-  build_items(A, M, 256, w, s);
+  if (ctx().prof)
+    hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(m, 4, 4096)), dim3(256), 0, s, A.rp, A.col, B.rp, m,
+                       ctx().prof_dev.as<unsigned long long>() + 1);
   const int bm = (M.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(M.ncols, 32) : 0;
   if (mode == 1) {
     dispatch_pat<int64_t, 1>(A, B, M, nullptr, sums, row_group, w, bm, s);
@@ -354,6 +693,7 @@ extern "C" ga_status ga_mxm_masked(const ga_csr* A, const ga_csr* B, const ga_cs
     const size_t es = dtype_size(dtype);
     dC.ensure(es * std::max<int64_t>(M->nnz, 1));
     MxmWork w;
+    w.max_mask_row = host_max_row(M);
     DCsr vM = dM.view();
     vM.val = nullptr;
     mxm_masked_dev(dA.view(), dB.view(), vM, dtype, 0, dC.p, nullptr, nullptr, w, s);

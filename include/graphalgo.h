@@ -118,6 +118,8 @@ typedef struct {
   double total_ms;        /* wall time of the whole decomposition */
 } ga_truss_stats;
 
+/* (ga_ktruss, ga_truss_decomposition and ga_graph_truss reject a pattern that
+ * is not symmetric with GA_ERR_ARG: the support pass relies on S = S^T.) */
 /* Truss number of every stored entry of the symmetric pattern A: the largest
  * k whose k-truss holds the entry, 2 if none (the caller-side loop over k of
  * the reference's k_truss).  truss: A.nnz int32 (host).  stats may be NULL. */
@@ -164,10 +166,16 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candi
 typedef struct {
   int64_t launches;
   double kernel_ms;
-  int64_t wedges;
-  int64_t bytes_alg;
+  int64_t wedges;            /* algorithmic: sum_{(i,k) in A} nnz(B_k) per launch, summed */
+  int64_t bytes_alg;         /* algorithmic bytes of the launches (full Gustavson product) */
+  int64_t wedges_processed;  /* products the kernel actually formed (mask-pruned) */
 } ga_kernel_prof;
 ga_status ga_prof_enable(int on);
+/* Diagnostic: a plain device copy of `bytes` (multiple of 16) repeated reps
+ * times; *ms_per_copy = GPU time per copy.  Used as the calibration reference
+ * of HBM byte counters and as the achievable-bandwidth probe.  No reference
+ * counterpart. */
+ga_status ga_stream_copy(int64_t bytes, int reps, double* ms_per_copy);
 ga_status ga_prof_read(ga_kernel_prof* out, int reset);
 
 #ifdef __cplusplus
