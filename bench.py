@@ -46,16 +46,44 @@ def _cpu_cores() -> int:
         return os.cpu_count() or 1
 
 
-def _roofline(prof: dict) -> dict:
-    sec = prof["kernel_ms"] / 1e3
-    achieved = prof["bytes_alg"] / sec / 1e9 if sec > 0 else 0.0
+def _measure(step, steps: int, warmup: int):
+    """Runs `warmup` untimed steps, then `steps` timed steps with HIP events
+    around every masked-SpGEMM launch, then one extra untimed step with the
+    algorithmic byte accounting on (every step issues the same launches).
+    Returns (seconds per step, last step result, timing, accounting)."""
+    from graphalgo_amd import api
+    for _ in range(warmup):
+        step()
+    _sync()
+    api.prof_enable(api.PROF_EVENTS)
+    api.prof_read(reset=True)
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = step()
+    _sync()
+    dt = (time.perf_counter() - t0) / steps
+    timing = api.prof_read(reset=True)
+    api.prof_enable(api.PROF_BYTES)
+    step()
+    acct = api.prof_read(reset=True)
+    api.prof_enable(0)
+    timing["steps"] = steps
+    return dt, out, timing, acct
+
+
+def _roofline(timing: dict, acct: dict, kernel: str) -> dict:
+    steps = max(timing.get("steps", 1), 1)
+    launches = max(timing["launches"] // steps, 1)
+    kernel_s = timing["kernel_ms"] / steps / 1e3
+    achieved = acct["bytes_alg"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "kernel": "k_mxm_items", "launches": prof["launches"],
-        "avg_launch_us": round(prof["kernel_ms"] * 1e3 / max(prof["launches"], 1), 2),
-        "bytes_alg_per_launch": int(prof["bytes_alg"] / max(prof["launches"], 1)),
-        "wedges": prof["wedges"],
+        "kernel": kernel, "launches_per_step": launches,
+        "avg_launch_us": round(kernel_s * 1e6 / launches, 2),
+        "bytes_alg_per_launch": int(acct["bytes_alg"] / max(acct["accounted_launches"], 1)),
+        "wedges_per_step": acct["wedges"], "wedges_processed_per_step": acct["wedges_processed"],
     }
 
 
@@ -64,20 +92,7 @@ def bench_ktruss(args) -> dict:
     g = G.rmat(16, 16, seed=1)
     m = g.m_undirected
     R = api.Graph(g)
-    for _ in range(args.warmup):
-        R.truss(want_values=False)
-    _sync()
-    api.prof_enable(True)
-    api.prof_read(reset=True)
-    t0 = time.perf_counter()
-    kmax = 0
-    stats = {}
-    for _ in range(args.steps):
-        _, kmax, stats = R.truss(want_values=False)
-    _sync()
-    dt = (time.perf_counter() - t0) / args.steps
-    prof = api.prof_read(reset=True)
-    api.prof_enable(False)
+    dt, (_, kmax, stats), timing, acct = _measure(lambda: R.truss(want_values=False), args.steps, args.warmup)
     R.close()
     # per-step averages for the roofline (every step repeats the same launches)
     out = {
@@ -86,7 +101,7 @@ def bench_ktruss(args) -> dict:
                    "graph": "rmat(scale=16, edge_factor=16, a=.57, b=.19, c=.19, seed=1)",
                    "n": g.n, "m_undirected": m, "nnz": g.nnz, "kmax": kmax,
                    "support_passes": stats.get("passes"), "filter_steps": stats.get("steps")},
-        "roofline": _roofline(prof),
+        "roofline": _roofline(timing, acct, "k_mxm_items (symmetric support pass)"),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_ktruss(g)
@@ -109,22 +124,12 @@ def bench_tc(args) -> dict:
     from graphalgo_amd import api, graphs as G
     g = G.rmat(16, 16, seed=1)
     R = api.Graph(g)
-    for _ in range(args.warmup):
-        R.triangle_count()
-    _sync()
-    api.prof_enable(True)
-    api.prof_read(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t = R.triangle_count()
-    dt = (time.perf_counter() - t0) / args.steps
-    prof = api.prof_read(reset=True)
-    api.prof_enable(False)
+    dt, t, timing, acct = _measure(R.triangle_count, args.steps, args.warmup)
     R.close()
     out = {"value": g.m_undirected / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": "triangle count C<L>=L*L on R-MAT scale 16", "n": g.n,
                       "m_undirected": g.m_undirected, "triangles": t},
-           "roofline": _roofline(prof)}
+           "roofline": _roofline(timing, acct, "k_mxm_items (C<L> = L*L, sum mode)")}
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle as O
@@ -166,17 +171,7 @@ def bench_batch(args) -> dict:
     m_total = sum(g.m_undirected for g in graphs)
     B = api.Batch(graphs)
     best, tune_ms = B.autotune()
-    for _ in range(args.warmup):
-        B.triangle_counts()
-    _sync()
-    api.prof_enable(True)
-    api.prof_read(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        counts = B.triangle_counts()
-    dt = (time.perf_counter() - t0) / args.steps
-    prof = api.prof_read(reset=True)
-    api.prof_enable(False)
+    dt, counts, timing, acct = _measure(B.triangle_counts, args.steps, args.warmup)
     # plain path: every graph resident, one triangle-count call per graph
     handles = [api.Graph(g) for g in graphs]
     for h in handles[:64]:
@@ -188,8 +183,7 @@ def bench_batch(args) -> dict:
         h.close()
     B.close()
     assert plain == counts.tolist(), "batched and per-graph counts differ"
-    roof = _roofline(prof)
-    roof["kernel"] = "k_mxm_rows + k_mxm_items (batched)"
+    roof = _roofline(timing, acct, "k_mxm_rows + k_mxm_items (batched)")
     out = {"value": m_total / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": f"batched triangle count over {count} kNN graphs (k=30) in one launch",
                       "graphs": count, "m_total": m_total, "triangles_total": int(counts.sum()),

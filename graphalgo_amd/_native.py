@@ -43,6 +43,7 @@ class GaKernelProf(C.Structure):
     _fields_ = [
         ("launches", C.c_int64), ("kernel_ms", C.c_double),
         ("wedges", C.c_int64), ("bytes_alg", C.c_int64), ("wedges_processed", C.c_int64),
+        ("accounted_launches", C.c_int64),
     ]
 
 

@@ -197,9 +197,13 @@ class Batch:
         return best.value, {64: ms[0], 128: ms[1], 256: ms[2]}
 
 
-def prof_enable(on: bool = True) -> None:
+PROF_EVENTS = 1  # HIP events around every masked-SpGEMM launch
+PROF_BYTES = 2   # algorithmic wedge / byte accounting (extra kernels: not for timed runs)
+
+
+def prof_enable(flags: int = PROF_EVENTS) -> None:
     N.init()
-    N.check(N.galib().ga_prof_enable(1 if on else 0))
+    N.check(N.galib().ga_prof_enable(int(flags)))
 
 
 def prof_read(reset: bool = True) -> dict:

@@ -84,7 +84,8 @@ __global__ void k_filter_count(const int32_t* __restrict__ rp, const int32_t* __
 __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                  const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t n,
                                  int thr, const int32_t* __restrict__ nrp, int32_t* __restrict__ ncol,
-                                 int32_t* __restrict__ neid) {
+                                 int32_t* __restrict__ neid, int32_t* __restrict__ nold,
+                                 int32_t* __restrict__ newpos) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t wpb = blockDim.x >> 6;
@@ -94,13 +95,25 @@ __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* 
     for (int j = b + lane; j - lane < e; j += 64) {
       const bool keep = j < e && sup[j] >= thr;
       const unsigned long long m = __ballot(keep);
+      const int r = dst + __popcll(m & lt);
       if (keep) {
-        const int r = dst + __popcll(m & lt);
         ncol[r] = col[j];
         neid[r] = eid[j];
+        nold[r] = j;
       }
+      if (j < e) newpos[j] = keep ? r : -1;
       dst += __popcll(m);
     }
+  }
+}
+
+// The transposed entry of a survivor survives too (equal supports), so the
+// mirror index carries over through the old -> new position map.
+__global__ void k_mirror_update(const int32_t* __restrict__ mirror_old, const int32_t* __restrict__ nold,
+                                const int32_t* __restrict__ newpos, int64_t nnz, int32_t* __restrict__ mirror_new) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nnz; r += (int64_t)gridDim.x * blockDim.x) {
+    const int mo = mirror_old[nold[r]];
+    mirror_new[r] = mo >= 0 ? newpos[mo] : -1;
   }
 }
 
@@ -116,8 +129,8 @@ __global__ void k_iota(int32_t* __restrict__ p, int64_t n) {
 
 struct TrussWork {
   OwnedCsr cur[2];
-  DevBuf eid[2];
-  DevBuf sup, cnt, scan_tmp, truss, kept;
+  DevBuf eid[2], mirror[2];
+  DevBuf sup, cnt, scan_tmp, truss, kept, nold, newpos;
   MxmWork mw;
 };
 
@@ -135,13 +148,18 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
     tw.cur[b].rp.ensure(sizeof(int32_t) * (n + 1));
     tw.cur[b].col.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
     tw.eid[b].ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+    tw.mirror[b].ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
   }
+  tw.nold.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+  tw.newpos.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
   tw.sup.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
   tw.cnt.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
   tw.kept.ensure(64);
   GA_HIP(hipMemcpyAsync(tw.cur[0].rp.p, A.rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s));
   if (nnz0) GA_HIP(hipMemcpyAsync(tw.cur[0].col.p, A.col, sizeof(int32_t) * nnz0, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz0, 256, 8192)), dim3(256), 0, s, tw.eid[0].as<int32_t>(), nnz0);
+  mirror_index_dev(DCsr{n, n, nnz0, tw.cur[0].rp.as<int32_t>(), tw.cur[0].col.as<int32_t>(), nullptr},
+                   tw.mirror[0].as<int32_t>(), s);
   if (all_k) {
     tw.truss.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
     fill_u32(tw.truss.p, 2u, nnz0, s);
@@ -159,7 +177,7 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
       if (all_k && nnz == 0) break;
       const DCsr C{n, n, nnz, tw.cur[cur].rp.as<int32_t>(), tw.cur[cur].col.as<int32_t>(), nullptr};
       if (!valid) {
-        support_symmetric_dev(C, tw.sup.as<int32_t>(), tw.mw, s);
+        support_symmetric_dev(C, tw.mirror[cur].as<int32_t>(), tw.sup.as<int32_t>(), tw.mw, s);
         ++passes;
         valid = true;
       }
@@ -185,7 +203,11 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
       }
       hipLaunchKernelGGL(k_filter_scatter, dim3(wgrid), dim3(256), 0, s, C.rp, C.col,
                          (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)tw.sup.as<int32_t>(), n, thr,
-                         (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>());
+                         (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>(),
+                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>());
+      hipLaunchKernelGGL(k_mirror_update, dim3(grid_for(kept, 256, 8192)), dim3(256), 0, s,
+                         (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
+                         (const int32_t*)tw.newpos.as<int32_t>(), kept, tw.mirror[cur ^ 1].as<int32_t>());
       GA_HIP(hipGetLastError());
       cur ^= 1;
       nnz = kept;

@@ -104,8 +104,16 @@ struct Context {
   // of at most 128 entries) go to the wavefront-per-row kernel; GA_WAVE_FLOPS
   // overrides it at ga_init
   long long wave_flops = 65536;
-  // kernel timing
-  bool prof = false;
+  // item list construction when the wavefront bin is off: false = scanned
+  // list of the non-empty rows' items; true = rows are their own first item
+  // and only extra chunks are listed (GA_DIRECT_ITEMS=1), which pays a claim
+  // for every empty row and loses on peeled graphs
+  bool direct_items = false;
+  // kernel timing (events around the masked-product launches) and
+  // algorithmic byte accounting (extra counting kernels; kept out of timed runs)
+  bool prof_events_on = false;
+  bool prof_bytes = false;
+  int64_t prof_bytes_launches = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
   std::vector<hipEvent_t> event_pool;
   int64_t prof_launches = 0;
@@ -193,7 +201,10 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
 
 // S<C> = C*C (int32 per entry) for a symmetric loop-free pattern C, computed
 // on the strict lower half and mirrored (the k-truss support pass).
-void support_symmetric_dev(const DCsr& C, int32_t* S, MxmWork& w, hipStream_t s);
+// mirror[t] = position of entry t's transposed entry; kept valid through
+// compaction by the k-truss loop.
+void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s);
+void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s);
 
 // Batched sum-mode product over many graphs stored back to back (graph-local
 // column indices): sums[g] += sum of C<L_g> = L_g * L_g.  threads = 64, 128

@@ -272,6 +272,7 @@ ga_status ga_init(int device) {
     c->name = prop.gcnArchName;
     GA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* wf = std::getenv("GA_WAVE_FLOPS")) c->wave_flops = std::atoll(wf);
+    if (const char* di = std::getenv("GA_DIRECT_ITEMS")) c->direct_items = std::atoi(di) != 0;
     c->prof_dev.ensure(64);
     GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
     GA_HIP(hipStreamSynchronize(c->stream));
@@ -328,8 +329,11 @@ ga_status ga_stream_copy(int64_t bytes, int reps, double* ms_per_copy) {
   });
 }
 
-ga_status ga_prof_enable(int on) {
-  return guarded([&] { ctx().prof = on != 0; });
+ga_status ga_prof_enable(int flags) {
+  return guarded([&] {
+    ctx().prof_events_on = (flags & 1) != 0;
+    ctx().prof_bytes = (flags & 2) != 0;
+  });
 }
 
 ga_status ga_prof_read(ga_kernel_prof* out, int reset) {
@@ -346,6 +350,7 @@ ga_status ga_prof_read(ga_kernel_prof* out, int reset) {
     unsigned long long w[2] = {0, 0};
     GA_HIP(hipMemcpy(w, c.prof_dev.p, sizeof(w), hipMemcpyDeviceToHost));
     out->launches = c.prof_launches;
+    out->accounted_launches = c.prof_bytes_launches;
     out->kernel_ms = ms;
     out->wedges = (int64_t)w[1];
     out->wedges_processed = (int64_t)w[0];
@@ -357,6 +362,7 @@ ga_status ga_prof_read(ga_kernel_prof* out, int reset) {
       }
       c.prof_events.clear();
       c.prof_launches = 0;
+      c.prof_bytes_launches = 0;
       c.prof_bytes_host = 0;
       GA_HIP(hipMemsetAsync(c.prof_dev.p, 0, 64, c.stream));
       GA_HIP(hipStreamSynchronize(c.stream));

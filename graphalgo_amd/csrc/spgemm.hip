@@ -30,6 +30,7 @@ namespace ga {
 constexpr int64_t MXM_BITMAP_MAX_COLS = int64_t(1) << 18;  // 32 KB bitmap
 constexpr int MXM_UNROLL = 4;                                // gathers in flight per lane
 constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up to 2^17 columns
+constexpr int MXM_NSEG = 32;                                 // claim segments (one counter line each)
 
 // THREADS = workgroup size = A entries per item (one entry per thread in the
 // entry scan); MCAP = mask entries staged in LDS.  BATCHED: the operands are
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ item_row, const int32_t* __restrict__ item_t0,
     const int32_t* __restrict__ n_items_ptr, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ wedge_counter, int bitmap_words, const int32_t* __restrict__ Mend,
-    int trunc, int chunk) {
+    int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror) {
   constexpr int MXM_THREADS = THREADS;
   constexpr int MXM_TCAP = THREADS;
   constexpr int MXM_MCAP = 4 * THREADS;
@@ -65,34 +66,60 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
   const int wv = tid >> 6;
   const bool use_bm = RANK || bitmap_words > 0;
   for (int q = tid; q < bitmap_words; q += MXM_THREADS) s_bitmap[q] = 0u;
-  const int n_items = *n_items_ptr;
-  // items are claimed `chunk` at a time (one atomic per chunk keeps the shared
-  // counter off the critical path); thread 0 holds the claimed range
-  int next = 0, end = 0;
+  // direct mode: items 0..n_list-1 are the listed extra chunks of long rows
+  // (the heaviest work goes first), then item n_list + r is row r's first
+  // chunk for r < direct_rows
+  const int n_list = *n_items_ptr;
+  const int n_items = direct_rows + n_list;
+  // Segmented claiming: the item range is cut into MXM_NSEG segments, each
+  // with its own counter on its own 128-byte line, so claims do not serialise
+  // on one address.  A workgroup drains its home segment, then moves on to
+  // the next ones (work stealing); `chunk` items are claimed per atomic.
+  const int seg_len = (n_items + MXM_NSEG - 1) / MXM_NSEG;
+  int seg = blockIdx.x % MXM_NSEG, tries = 0;
+  int next = 0, end = 0;  // thread 0's claimed range
 
   while (true) {
     __syncthreads();  // previous item fully retired (LDS reuse)
     if (tid == 0) {
-      if (next >= end) {
-        next = (int)atomicAdd(work_counter, (unsigned)chunk);
-        end = min(next + chunk, n_items);
+      int item = n_items;
+      if (next < end) {
+        item = next++;
+      } else {
+        while (tries < MXM_NSEG) {
+          const int lo = seg * seg_len;
+          const int hi = min(lo + seg_len, n_items);
+          if (lo < hi) {
+            const int k = (int)atomicAdd(&work_counter[seg * 32], (unsigned)chunk);
+            if (lo + k < hi) {
+              item = lo + k;
+              next = item + 1;
+              end = min(lo + k + chunk, hi);
+              break;
+            }
+          }
+          seg = (seg + 1) % MXM_NSEG;
+          ++tries;
+        }
       }
-      s_item = next < end ? next++ : n_items;
+      s_item = item;
     }
     __syncthreads();
     const int q = s_item;
     if (q >= n_items) break;
 
-    const int i = item_row[q];
-    const int t0 = item_t0[q];
+    const bool listed = q < n_list;
+    const int i = listed ? item_row[q] : q - n_list;
     const int a_beg = Arp[i];
     const int a_end = Arp[i + 1];
+    const int t0 = listed ? item_t0[q] : a_beg;
     const int t1 = min(t0 + MXM_TCAP, a_end);
     const bool multi = (a_end - a_beg) > MXM_TCAP;
     const int ms = Mrp[i];
     const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // Mend: only a prefix of M's row is the mask
     const bool local = mnz <= MXM_MCAP;
     const int bbase = BATCHED ? gbase[row_group[i]] : 0;
+    if (a_end == a_beg || mnz <= 0) continue;  // direct items include empty rows
 
     // stage the mask row: bitmap bits (and the per-word rank base, or the
     // sorted column list for the binary-search variant)
@@ -208,19 +235,32 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
         const T v = (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]);
         if (MODE == 1) lsum += (long long)v;
         else if (local) atomic_add_val(&s_acc[p], v);
-        else atomic_add_val(&Cval[ms + p], v);
+        else {
+          atomic_add_val(&Cval[ms + p], v);
+          if (mirror) {
+            const int mt = mirror[ms + p];
+            if (mt >= 0) atomic_add_val(&Cval[mt], v);
+          }
+        }
       }
     }
     __syncthreads();
 
     if (MODE == 0) {
       if (local) {
+        // mirror: C is symmetric and only its lower half is computed; each
+        // value is also written to the transposed entry mirror[ms + p]
         for (int p = tid; p < mnz; p += MXM_THREADS) {
           const T v = s_acc[p];
+          const int mt = mirror ? mirror[ms + p] : -1;  // negative = no transposed entry
           if (multi) {
-            if (v != T(0)) atomic_add_val(&Cval[ms + p], v);
+            if (v != T(0)) {
+              atomic_add_val(&Cval[ms + p], v);
+              if (mt >= 0) atomic_add_val(&Cval[mt], v);
+            }
           } else {
             Cval[ms + p] = v;
+            if (mt >= 0) Cval[mt] = v;
           }
         }
       }
@@ -268,7 +308,8 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
     const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol, const int32_t* __restrict__ Mend,
     T* __restrict__ Cval, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
-    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
+    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk,
+    const int32_t* __restrict__ mirror) {
   __shared__ uint32_t s_bm[WR_WAVES][WR_BMW];
   __shared__ uint16_t s_wpre[WR_WAVES][WR_BMW];
   __shared__ T s_acc[WR_WAVES][MODE == 0 ? WR_MCAP : 1];
@@ -375,7 +416,14 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
       __threadfence_block();  // entry slots are rewritten by the next chunk
     }
     if (MODE == 0) {
-      for (int p = lane; p < mnz; p += 64) Cval[ms + p] = s_acc[wv][p];
+      for (int p = lane; p < mnz; p += 64) {
+        const T v = s_acc[wv][p];
+        Cval[ms + p] = v;
+        if (mirror) {  // symmetric C: also the transposed entry (negative = none)
+          const int mt = mirror[ms + p];
+          if (mt >= 0) Cval[mt] = v;
+        }
+      }
     } else {
       const long long tot = wave_reduce_sum(lsum);
       if (lane == 0 && tot) atomicAdd(&sums[row_group ? row_group[i] : 0], (unsigned long long)tot);
@@ -434,6 +482,25 @@ __global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __r
   }
 }
 
+// Extra items of rows longer than one item (chunks 1..), appended in any
+// order through an atomic cursor; chunk 0 of every row is a direct item.
+__global__ void k_extra_items(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp,
+                              const int32_t* __restrict__ Mend, int64_t m, int tcap,
+                              int32_t* __restrict__ item_row, int32_t* __restrict__ item_t0,
+                              int32_t* __restrict__ n_extra) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = Arp[i + 1] - Arp[i];
+    const int mz = (Mend ? Mend[i] : Mrp[i + 1]) - Mrp[i];
+    if (a <= tcap || mz <= 0) continue;
+    const int c = (a + tcap - 1) / tcap - 1;
+    const int base = atomicAdd(n_extra, c);
+    for (int q = 0; q < c; ++q) {
+      item_row[base + q] = (int32_t)i;
+      item_t0[base + q] = Arp[i] + (q + 1) * tcap;
+    }
+  }
+}
+
 __global__ void k_item_fill(const int32_t* __restrict__ Arp, const int32_t* __restrict__ cnt,
                             const int32_t* __restrict__ start, int64_t m, int tcap, int32_t* __restrict__ item_row,
                             int32_t* __restrict__ item_t0) {
@@ -463,7 +530,7 @@ static hipEvent_t take_event(Context& c) {
 template <typename T, bool PA, bool PB, int MODE, int THREADS, bool BATCHED>
 static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
                           const int32_t* row_group, const int32_t* gbase, MxmWork& w, int bitmap_words,
-                          hipStream_t s, const int32_t* Mend, int trunc) {
+                          hipStream_t s, const int32_t* Mend, int trunc, const int32_t* mirror = nullptr) {
   constexpr int TCAP = THREADS;
   const int64_t m = A.nrows;
   const int64_t max_items = m + ceil_div(A.nnz, TCAP) + 1;
@@ -475,22 +542,48 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   w.rows.ensure(sizeof(int32_t) * m);
   w.item_row.ensure(sizeof(int32_t) * max_items);
   w.item_t0.ensure(sizeof(int32_t) * max_items);
-  w.counter.ensure(256);
-  fill_u32(w.counter.p, 0u, 2, s);
+  // counters: MXM_NSEG item-segment lines (32 u32 each), then the row
+  // counter and the extra-item cursor on a line of their own
+  const int ctr_words = MXM_NSEG * 32 + 32;
+  w.counter.ensure(sizeof(uint32_t) * ctr_words);
+  fill_u32(w.counter.p, 0u, ctr_words, s);
   const long long wave_flops =
       (M.ncols <= WR_MAXCOLS) ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops) : -1;
-  hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, A.rp, A.col, B.rp, M.rp, Mend,
-                     row_group, BATCHED ? gbase : nullptr, m, TCAP, wave_flops, w.flag.as<int32_t>(),
-                     w.cnt.as<int32_t>());
-  GA_HIP(hipGetLastError());
-  scan_exclusive_i32(w.flag.as<int32_t>(), w.rstart.as<int32_t>(), m, w.scan_tmp, s);
-  hipLaunchKernelGGL(k_emit_rows, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, (const int32_t*)w.flag.as<int32_t>(),
-                     (const int32_t*)w.rstart.as<int32_t>(), m, w.rows.as<int32_t>());
-  scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
-  hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
-                     (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, TCAP,
-                     w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
-  GA_HIP(hipGetLastError());
+  // direct mode (no wavefront bin): items 0..m-1 are the rows' first chunks and
+  // only the extra chunks of long rows are listed -- two small launches
+  // instead of binning, two scans and two compactions
+  const bool direct = wave_flops < 0 && ctx().direct_items;
+  int32_t* n_extra = reinterpret_cast<int32_t*>(w.counter.as<unsigned int>() + MXM_NSEG * 32 + 1);
+  if (direct) {
+    hipLaunchKernelGGL(k_extra_items, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
+                       w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(), n_extra);
+    GA_HIP(hipGetLastError());
+  } else if (wave_flops < 0) {
+    // no wavefront bin: items of every non-empty row, listed in row order
+    hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
+                       w.cnt.as<int32_t>());
+    GA_HIP(hipGetLastError());
+    scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
+    hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
+                       (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, TCAP,
+                       w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
+    GA_HIP(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, A.rp, A.col, B.rp, M.rp, Mend,
+                       row_group, BATCHED ? gbase : nullptr, m, TCAP, wave_flops, w.flag.as<int32_t>(),
+                       w.cnt.as<int32_t>());
+    GA_HIP(hipGetLastError());
+    scan_exclusive_i32(w.flag.as<int32_t>(), w.rstart.as<int32_t>(), m, w.scan_tmp, s);
+    hipLaunchKernelGGL(k_emit_rows, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s,
+                       (const int32_t*)w.flag.as<int32_t>(), (const int32_t*)w.rstart.as<int32_t>(), m,
+                       w.rows.as<int32_t>());
+    scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
+    hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
+                       (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, TCAP,
+                       w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
+    GA_HIP(hipGetLastError());
+  }
+  const bool wave_on = wave_flops >= 0;
 
   auto krows = k_mxm_rows<T, PA, PB, MODE, BATCHED>;
   // rank lookup when the bitmap is at most 16 KB and every mask row fits
@@ -512,31 +605,36 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   const int chunk_r = (int)std::max<int64_t>(1, std::min<int64_t>(32, m / (grid_r * WR_WAVES * 8)));
   const int chunk_i = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c.prof) {
+  if (c.prof_events_on) {
     e0 = take_event(c);
     e1 = take_event(c);
     GA_HIP(hipEventRecord(e0, s));
   }
-  unsigned long long* wc = c.prof ? c.prof_dev.as<unsigned long long>() : nullptr;
-  hipLaunchKernelGGL(krows, dim3((unsigned)grid_r), dim3(64 * WR_WAVES), 0, s, A.rp, A.col,
+  unsigned long long* wc = c.prof_bytes ? c.prof_dev.as<unsigned long long>() : nullptr;
+  if (wave_on)
+    hipLaunchKernelGGL(krows, dim3((unsigned)grid_r), dim3(64 * WR_WAVES), 0, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col, Mend,
                      static_cast<T*>(Cval), sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
-                     (const int32_t*)(w.rstart.as<int32_t>() + m), w.counter.as<unsigned int>() + 1, wc, trunc,
-                     chunk_r);
+                     (const int32_t*)(w.rstart.as<int32_t>() + m), w.counter.as<unsigned int>() + MXM_NSEG * 32, wc,
+                     trunc, chunk_r, mirror);
   GA_HIP(hipGetLastError());
   hipLaunchKernelGGL(kitems, dim3((unsigned)grid_i), dim3(THREADS), dyn, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col,
                      static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
-                     (const int32_t*)(w.start.as<int32_t>() + m), w.counter.as<unsigned int>(), wc, bitmap_words,
-                     Mend, trunc, chunk_i);
+                     direct ? (const int32_t*)n_extra : (const int32_t*)(w.start.as<int32_t>() + m),
+                     w.counter.as<unsigned int>(), wc, bitmap_words, Mend, trunc, chunk_i, direct ? (int)m : 0,
+                     mirror);
   GA_HIP(hipGetLastError());
-  if (c.prof) {
+  if (c.prof_events_on) {
     GA_HIP(hipEventRecord(e1, s));
     c.prof_events.emplace_back(e0, e1);
     c.prof_launches += 1;
+  }
+  if (c.prof_bytes) {
     // host-known algorithmic bytes: A and M streamed, B row bounds, output
     const int64_t out = MODE == 0 ? (int64_t)sizeof(T) * M.nnz : 8;
     c.prof_bytes_host += 4 * (A.nrows + 1) + 4 * A.nnz + 8 * A.nnz + 4 * (M.nrows + 1) + 4 * M.nnz + out;
+    c.prof_bytes_launches += 1;
   }
 }
 
@@ -569,7 +667,7 @@ void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t*
                          unsigned long long* sums, MxmWork& w, hipStream_t s) {
   if (L.nrows == 0 || L.nnz == 0) return;
   const int bm = (max_cols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(max_cols, 32) : 0;
-  if (ctx().prof)
+  if (ctx().prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(L.nrows, 4, 4096)), dim3(256), 0, s, L.rp, L.col, L.rp,
                        L.nrows, ctx().prof_dev.as<unsigned long long>() + 1, row_group, gbase);
   switch (w.threads) {
@@ -604,26 +702,33 @@ __global__ void k_row_split(const int32_t* __restrict__ rp, const int32_t* __res
   }
 }
 
-__global__ void k_mirror_lower(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                               const int32_t* __restrict__ split, int64_t n, int32_t* __restrict__ S) {
+// mirror[t] = position of the transposed entry of entry t (symmetric pattern)
+__global__ void k_mirror_index(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                               int32_t* __restrict__ mirror) {
   const int lane = threadIdx.x & 63;
   const int64_t wpb = blockDim.x >> 6;
   for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
-    const int b = rp[i], e = split[i];
-    for (int t = b + lane; t < e; t += 64) {
-      const int j = col[t];  // j < i: entry (j, i) lies in row j's upper part
-      int lo = split[j], hi = rp[j + 1];
+    for (int t = rp[i] + lane; t < rp[i + 1]; t += 64) {
+      const int j = col[t];
+      int lo = rp[j], hi = rp[j + 1];
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (col[mid] < i) lo = mid + 1;
         else hi = mid;
       }
-      if (lo < rp[j + 1] && col[lo] == i) S[lo] = S[t];
+      mirror[t] = (lo < rp[j + 1] && col[lo] == i) ? lo : -1;
     }
   }
 }
 
-void support_symmetric_dev(const DCsr& C, int32_t* S, MxmWork& w, hipStream_t s) {
+void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s) {
+  if (C.nrows == 0 || C.nnz == 0) return;
+  hipLaunchKernelGGL(k_mirror_index, dim3(grid_for(C.nrows, 4, 16384)), dim3(256), 0, s, C.rp, C.col, C.nrows,
+                     mirror);
+  GA_HIP(hipGetLastError());
+}
+
+void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s) {
   const int64_t n = C.nrows;
   GA_REQUIRE(C.nrows == C.ncols, GA_ERR_DIM, "symmetric support needs a square matrix");
   if (C.nnz > 0) fill_u32(S, 0u, C.nnz, s);
@@ -633,13 +738,11 @@ void support_symmetric_dev(const DCsr& C, int32_t* S, MxmWork& w, hipStream_t s)
   hipLaunchKernelGGL(k_row_split, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, C.rp, C.col, n, split);
   const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : 0;
   Context& c = ctx();
-  if (c.prof)
+  if (c.prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
                        c.prof_dev.as<unsigned long long>() + 1);
-  launch_binned<int32_t, true, true, 0, 256, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, split, 1);
-  hipLaunchKernelGGL(k_mirror_lower, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, C.rp, C.col,
-                     (const int32_t*)split, n, S);
-  GA_HIP(hipGetLastError());
+  launch_binned<int32_t, true, true, 0, 256, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, split, 1,
+                                                     mirror);
 }
 
 void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,
@@ -654,7 +757,7 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
     else fill_u64(Cval, 0ull, M.nnz, s);
   }
   if (m == 0 || A.nnz == 0 || M.nnz == 0) return;
-  if (ctx().prof)
+  if (ctx().prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(m, 4, 4096)), dim3(256), 0, s, A.rp, A.col, B.rp, m,
                        ctx().prof_dev.as<unsigned long long>() + 1);
   const int bm = (M.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(M.ncols, 32) : 0;

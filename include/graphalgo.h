@@ -160,17 +160,19 @@ ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts);
 ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candidate);
 
 /* ---- kernel timing ----------------------------------------------------- */
-/* When enabled, the library brackets every launch of the masked-SpGEMM
- * kernel with HIP events on the stream it runs on and accumulates launch
- * count, GPU milliseconds, wedges and algorithmic bytes. */
+/* flags bit 0: bracket every launch of the masked-SpGEMM kernels with HIP
+ * events on the stream they run on (launch count, GPU milliseconds);
+ * bit 1: account the algorithmic wedges and bytes of every launch (runs an
+ * extra counting kernel per launch, so keep it out of timed runs). */
 typedef struct {
   int64_t launches;
   double kernel_ms;
   int64_t wedges;            /* algorithmic: sum_{(i,k) in A} nnz(B_k) per launch, summed */
   int64_t bytes_alg;         /* algorithmic bytes of the launches (full Gustavson product) */
   int64_t wedges_processed;  /* products the kernel actually formed (mask-pruned) */
+  int64_t accounted_launches;/* launches covered by wedges / bytes_alg */
 } ga_kernel_prof;
-ga_status ga_prof_enable(int on);
+ga_status ga_prof_enable(int flags);
 /* Diagnostic: a plain device copy of `bytes` (multiple of 16) repeated reps
  * times; *ms_per_copy = GPU time per copy.  Used as the calibration reference
  * of HBM byte counters and as the achievable-bandwidth probe.  No reference
