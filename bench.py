@@ -72,14 +72,25 @@ def _measure(step, steps: int, warmup: int):
     return dt, out, timing, acct
 
 
-def _roofline(timing: dict, acct: dict, kernel: str) -> dict:
+def _traffic(key: str):
+    """HBM bytes per launch measured offline with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (scaled by the copy-kernel calibration), as recorded in profiles/traffic.json."""
+    try:
+        rec = json.loads((ROOT / "profiles" / "traffic.json").read_text())[key]
+        return rec["traffic_bytes_per_launch"], rec["source"]
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
+def _roofline(timing: dict, acct: dict, kernel: str, traffic_key: str | None = None) -> dict:
     steps = max(timing.get("steps", 1), 1)
     launches = max(timing["launches"] // steps, 1)
     kernel_s = timing["kernel_ms"] / steps / 1e3
     achieved = acct["bytes_alg"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    traffic, source = _traffic(traffic_key) if traffic_key else (None, None)
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
         "kernel": kernel, "launches_per_step": launches,
         "avg_launch_us": round(kernel_s * 1e6 / launches, 2),
         "bytes_alg_per_launch": int(acct["bytes_alg"] / max(acct["accounted_launches"], 1)),
@@ -101,7 +112,7 @@ def bench_ktruss(args) -> dict:
                    "graph": "rmat(scale=16, edge_factor=16, a=.57, b=.19, c=.19, seed=1)",
                    "n": g.n, "m_undirected": m, "nnz": g.nnz, "kmax": kmax,
                    "support_passes": stats.get("passes"), "filter_steps": stats.get("steps")},
-        "roofline": _roofline(timing, acct, "k_mxm_items (symmetric support pass)"),
+        "roofline": _roofline(timing, acct, "k_mxm_items (symmetric support pass)", "k_mxm_items/truss_rmat16"),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_ktruss(g)
@@ -183,7 +194,7 @@ def bench_batch(args) -> dict:
         h.close()
     B.close()
     assert plain == counts.tolist(), "batched and per-graph counts differ"
-    roof = _roofline(timing, acct, "k_mxm_rows + k_mxm_items (batched)")
+    roof = _roofline(timing, acct, "k_mxm_rows + k_mxm_items (batched)", "k_mxm_rows/batched_tc_knn4096")
     out = {"value": m_total / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": f"batched triangle count over {count} kNN graphs (k=30) in one launch",
                       "graphs": count, "m_total": m_total, "triangles_total": int(counts.sum()),
