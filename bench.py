@@ -217,7 +217,48 @@ def bench_batch(args) -> dict:
     return out
 
 
-WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "copy": bench_copy, "batch": bench_batch}
+def pick_sources(g, count: int, seed: int) -> np.ndarray:
+    """`count` distinct non-isolated vertices, ordered by a SplitMix64 hash of
+    (seed, vertex): a seeded uniform choice that does not depend on numpy's RNG."""
+    v = np.nonzero(g.degrees() > 0)[0].astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = v + np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return v[np.argsort(z, kind="stable")[:count]].astype(np.int32)
+
+
+def bench_bc(args) -> dict:
+    """Batched betweenness centrality on R-MAT scale 18 (edge factor 16), 256 seeded
+    random non-isolated sources as one 256-column pass, FP64 sigma and delta."""
+    from graphalgo_amd import api, graphs as G
+    g = G.rmat(18, 16, seed=1)
+    src = pick_sources(g, 256, seed=7)
+    R = api.Graph(g)
+    dt, bc, timing, acct = _measure(lambda: R.bc(src), args.steps, args.warmup)
+    R.close()
+    m = g.m_undirected
+    roof = _roofline(timing, acct, "k_bc_forward_items + k_bc_backward_items (per level)")
+    roof["launches_note"] = "one event pair per forward level and one around the backward sweep"
+    out = {"value": m * len(src) / dt, "unit": "edges*sources/s", "ms_per_step": dt * 1e3, "dtype": "f64",
+           "config": {"workload": "batched BC, 256 sources, R-MAT scale 18", "n": g.n, "m_undirected": m,
+                      "sources": len(src), "source_seed": 7, "bc_sum": float(bc.sum())},
+           "roofline": roof}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        sample = src[:32]
+        t0 = time.perf_counter()
+        O.bc(g, sample)
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": m * len(sample) / dtc, "unit": "edges*sources/s",
+                               "cores": O.num_threads(), "kind": "port",
+                               "sample": f"oracle FP64 batched BC with the first 32 of the 256 sources ({dtc:.1f} s)"}
+    return out
+
+
+WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "copy": bench_copy, "batch": bench_batch, "bc": bench_bc}
 
 
 def main() -> None:

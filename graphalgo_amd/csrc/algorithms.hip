@@ -263,7 +263,7 @@ ga_status ga_triangle_count(const ga_csr* A, int64_t* triangles) {
     sums.ensure(64);
     fill_u64(sums.p, 0ull, 1, s);
     MxmWork w;
-    w.max_mask_row = host_max_row(A);
+    w.max_mask_row = w.max_b_row = host_max_row(A);
     const DCsr vL = L.view();
     mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
     unsigned long long t = 0;
@@ -291,7 +291,7 @@ ga_status ga_triangle_count_lu(const ga_csr* A, int64_t* triangles) {
     sums.ensure(64);
     fill_u64(sums.p, 0ull, 1, s);
     MxmWork w;
-    w.max_mask_row = host_max_row(A);
+    w.max_mask_row = w.max_b_row = host_max_row(A);
     mxm_masked_dev(L.view(), U.view(), vA, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
     unsigned long long t = 0;
     GA_HIP(hipMemcpyAsync(&t, sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
@@ -317,7 +317,7 @@ ga_status ga_triangle_count_vertex(const ga_csr* A, int64_t* per_vertex) {
     fill_u64(sums.p, 0ull, n, s);
     hipLaunchKernelGGL(k_iota, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, groups.as<int32_t>(), n);
     MxmWork w;
-    w.max_mask_row = host_max_row(A);
+    w.max_mask_row = w.max_b_row = host_max_row(A);
     mxm_masked_dev(vA, vA, vA, GA_INT64, 1, nullptr, sums.as<unsigned long long>(),
                    (const int32_t*)groups.as<int32_t>(), w, s);
     if (n) GA_HIP(hipMemcpyAsync(per_vertex, sums.p, 8 * n, hipMemcpyDeviceToHost, s));
@@ -339,7 +339,7 @@ ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_co
     DCsr vA = dA.view();
     vA.val = nullptr;
     TrussWork tw;
-    tw.mw.max_mask_row = host_max_row(A);
+    tw.mw.max_mask_row = tw.mw.max_b_row = host_max_row(A);
     int cur = 0;
     int64_t nnz = 0;
     ga_truss_stats st{};
@@ -363,7 +363,7 @@ ga_status ga_truss_decomposition(const ga_csr* A, int32_t* truss, int32_t* kmax,
     DCsr vA = dA.view();
     vA.val = nullptr;
     TrussWork tw;
-    tw.mw.max_mask_row = host_max_row(A);
+    tw.mw.max_mask_row = tw.mw.max_b_row = host_max_row(A);
     int32_t km = 2;
     ga_truss_stats st{};
     truss_loop(vA, true, 0, tw, s, nullptr, nullptr, &km, &st);
@@ -384,8 +384,8 @@ ga_status ga_graph_create(const ga_csr* A, ga_graph** out) {
     try {
       g->device = c.device;
       g->symmetric = host_csr_symmetric(A);
-      g->tw.mw.max_mask_row = host_max_row(A);
-      g->mw.max_mask_row = g->tw.mw.max_mask_row;
+      g->tw.mw.max_mask_row = g->tw.mw.max_b_row = host_max_row(A);
+      g->mw.max_mask_row = g->mw.max_b_row = g->tw.mw.max_mask_row;
       upload_csr(A, GA_INT32, g->A, c.stream);
       g->A.val.release();
       GA_HIP(hipStreamSynchronize(c.stream));

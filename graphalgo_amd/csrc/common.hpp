@@ -118,7 +118,8 @@ struct Context {
   std::vector<hipEvent_t> event_pool;
   int64_t prof_launches = 0;
   int64_t prof_bytes_host = 0;  // host-known part of the algorithmic bytes
-  DevBuf prof_dev;              // [0] = wedges processed, [1] = algorithmic wedges (u64)
+  DevBuf prof_dev;  // u64: [0] wedges processed, [1] algorithmic wedges (4-byte gathers),
+                    // [2] algorithmic bytes reported by BC, [3] BC wedges
 };
 
 Context& ctx();  // throws GA_ERR_STATE when ga_init has not run
@@ -191,6 +192,7 @@ struct MxmWork {
                                  // -1 = the context default
   int threads = 256;             // workgroup width of the item kernel (64, 128 or 256)
   int64_t max_mask_row = INT32_MAX;  // longest mask row (host-known bound); < 65536 enables the rank lookup
+  int64_t max_b_row = INT32_MAX;     // longest B row (host-known bound; B.nnz is used when tighter)
 };
 
 // C<M> = A*B (structural mask).  mode 0: per-entry values into Cval (dtype);
@@ -215,6 +217,7 @@ void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t*
 // Batched betweenness centrality workspace and driver (bc.hip).
 struct BcWork {
   DevBuf depth, sigma, q, bc, flag, levels, cnt, off, scan_tmp, src;
+  DevBuf icnt, hcnt, istart, hstart, it_v, it_t0, it_slot, partial;  // neighbour-chunked items
 };
 void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h, BcWork& w, hipStream_t s);
 

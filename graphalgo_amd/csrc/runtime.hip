@@ -347,14 +347,17 @@ ga_status ga_prof_read(ga_kernel_prof* out, int reset) {
       GA_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
       ms += t;
     }
-    unsigned long long w[2] = {0, 0};
+    unsigned long long w[4] = {0, 0, 0, 0};
     GA_HIP(hipMemcpy(w, c.prof_dev.p, sizeof(w), hipMemcpyDeviceToHost));
     out->launches = c.prof_launches;
     out->accounted_launches = c.prof_bytes_launches;
     out->kernel_ms = ms;
-    out->wedges = (int64_t)w[1];
+    out->wedges = (int64_t)(w[1] + w[3]);
     out->wedges_processed = (int64_t)w[0];
-    out->bytes_alg = c.prof_bytes_host + 4 * (int64_t)w[1];
+    // [1]: wedges of products with 4-byte gathers; [2]: algorithmic bytes
+    // reported directly by kernels with another per-wedge cost (BC), whose
+    // wedges are counted in [3]
+    out->bytes_alg = c.prof_bytes_host + 4 * (int64_t)w[1] + (int64_t)w[2];
     if (reset) {
       for (auto& pr : c.prof_events) {
         c.event_pool.push_back(pr.first);

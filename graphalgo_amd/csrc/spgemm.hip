@@ -55,8 +55,10 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
   uint16_t* s_wpre = reinterpret_cast<uint16_t*>(s_bitmap + bitmap_words);
   __shared__ int32_t s_mcol[RANK ? 1 : MXM_MCAP];
   __shared__ T s_acc[MODE == 0 ? MXM_MCAP : 1];
-  __shared__ int32_t s_ebeg[MXM_TCAP];
-  __shared__ long long s_eoff[MXM_TCAP + 1];
+  // entry e of the item: wedges [s_eoff[e], s_eoff[e+1]) live at B.col[s_jb[e] + w]
+  // (the host guarantees an item's wedge count fits int32)
+  __shared__ int32_t s_jb[MXM_TCAP];
+  __shared__ int32_t s_eoff[MXM_TCAP + 1];
   __shared__ T s_aval[PAT_A ? 1 : MXM_TCAP];
   __shared__ long long s_wsum[MXM_THREADS / 64];
   __shared__ int s_item;
@@ -135,10 +137,10 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     }
     // stage the item's entries: B row start and length, A value
     const int ne = t1 - t0;
-    int len = 0;
+    int len = 0, b0 = 0;
     if (tid < ne) {
       const int k = bbase + Acol[t0 + tid];
-      const int b0 = Brp[k];
+      b0 = Brp[k];
       int b1 = Brp[k + 1];
       if (trunc && mnz > 0) {
         // products beyond the last mask column are discarded by the mask:
@@ -153,19 +155,22 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
         b1 = lo;
       }
       len = b1 - b0;
-      s_ebeg[tid] = b0;
       if (!PAT_A) s_aval[tid] = Aval[t0 + tid];
     }
-    const long long incl = wave_incl_scan((long long)len);
+    const int incl = wave_incl_scan(len);
     if (lane == 63) s_wsum[wv] = incl;
     __syncthreads();
-    long long base = 0, total = 0;
+    int base = 0, total = 0;
 #pragma unroll
     for (int k = 0; k < MXM_THREADS / 64; ++k) {
-      if (k < wv) base += s_wsum[k];
-      total += s_wsum[k];
+      if (k < wv) base += (int)s_wsum[k];
+      total += (int)s_wsum[k];
     }
-    if (tid < ne) s_eoff[tid] = base + incl - len;
+    if (tid < ne) {
+      const int off = base + incl - len;
+      s_eoff[tid] = off;
+      s_jb[tid] = b0 - off;
+    }
     if (tid == 0) s_eoff[ne] = total;
     __syncthreads();
 
@@ -173,17 +178,21 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     // per lane per trip: the B-column loads of one trip are independent, so
     // each lane keeps MXM_UNROLL gathers in flight before it consumes them
     long long lsum = 0;
+    // current entry e, its jbase and the first wedge of the next entry are
+    // kept in registers: the common case costs one compare per wedge
     int e = 0;
-    for (long long w0 = tid; w0 < total; w0 += (long long)MXM_THREADS * MXM_UNROLL) {
+    int jb = ne > 0 ? s_jb[0] : 0;
+    int nb = ne > 0 ? s_eoff[1] : 0;
+    for (int w0 = tid; w0 < total; w0 += MXM_THREADS * MXM_UNROLL) {
       int cc[MXM_UNROLL], jj[MXM_UNROLL], ee[MXM_UNROLL];
 #pragma unroll
       for (int u = 0; u < MXM_UNROLL; ++u) {
-        const long long w = w0 + (long long)u * MXM_THREADS;
+        const int w = w0 + u * MXM_THREADS;
         cc[u] = -1;
         jj[u] = 0;
         ee[u] = 0;
         if (w < total) {
-          if (s_eoff[e + 1] <= w) {  // advance to the entry holding wedge w
+          if (w >= nb) {  // advance to the entry holding wedge w
             int lo = e + 1, hi = ne - 1;
             while (lo < hi) {
               const int mid = (lo + hi + 1) >> 1;
@@ -191,9 +200,11 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
               else hi = mid - 1;
             }
             e = lo;
+            jb = s_jb[e];
+            nb = s_eoff[e + 1];
           }
           ee[u] = e;
-          jj[u] = s_ebeg[e] + (int)(w - s_eoff[e]);
+          jj[u] = jb + w;
           cc[u] = Bcol[jj[u]];
         }
       }
@@ -284,7 +295,7 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     }
     if (tid == 0 && wedge_counter) atomicAdd(wedge_counter, (unsigned long long)total);
   }
-}
+}  // k_mxm_items
 
 // ---- short rows: one wavefront per row -------------------------------------
 // For matrices with at most WR_MAXCOLS columns (the small graphs of a batch),
@@ -313,14 +324,14 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
   __shared__ uint32_t s_bm[WR_WAVES][WR_BMW];
   __shared__ uint16_t s_wpre[WR_WAVES][WR_BMW];
   __shared__ T s_acc[WR_WAVES][MODE == 0 ? WR_MCAP : 1];
-  __shared__ int32_t s_ebeg[WR_WAVES][64];
+  __shared__ int32_t s_jb[WR_WAVES][64];
   __shared__ int32_t s_eoff[WR_WAVES][65];
   __shared__ T s_aval[WR_WAVES][PAT_A ? 1 : 64];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   uint32_t* bm = s_bm[wv];
   uint16_t* wpre = s_wpre[wv];
-  int32_t* ebeg = s_ebeg[wv];
+  int32_t* jbase = s_jb[wv];
   int32_t* eoff = s_eoff[wv];
   for (int q = lane; q < WR_BMW; q += 64) bm[q] = 0u;
   const int n_rows = *n_rows_ptr;
@@ -353,10 +364,10 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
     const int a_end = Arp[i + 1];
     for (int t0 = Arp[i]; t0 < a_end; t0 += 64) {
       const int ne = min(64, a_end - t0);
-      int len = 0;
+      int len = 0, b0 = 0;
       if (lane < ne) {
         const int k = bbase + Acol[t0 + lane];
-        const int b0 = Brp[k];
+        b0 = Brp[k];
         int b1 = Brp[k + 1];
         if (trunc) {
           int lo = b0, hi = b1;
@@ -368,16 +379,20 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
           b1 = lo;
         }
         len = b1 - b0;
-        ebeg[lane] = b0;
         if (!PAT_A) s_aval[wv][lane] = Aval[t0 + lane];
       }
       const int incl = wave_incl_scan(len);
-      if (lane < ne) eoff[lane] = incl - len;
+      if (lane < ne) {
+        eoff[lane] = incl - len;
+        jbase[lane] = b0 - (incl - len);
+      }
       const int total = __shfl(incl, 63, 64);
       if (lane == 0) eoff[ne] = total;
       __threadfence_block();
       if (lane == 0) wedges += (unsigned long long)total;
       int e = 0;
+      int jb = jbase[0];
+      int nb = eoff[1];
       for (int w0 = lane; w0 < total; w0 += 64 * WR_UNROLL) {
         int cc[WR_UNROLL], jj[WR_UNROLL], ee[WR_UNROLL];
 #pragma unroll
@@ -387,7 +402,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
           jj[u] = 0;
           ee[u] = 0;
           if (w < total) {
-            if (eoff[e + 1] <= w) {
+            if (w >= nb) {
               int lo = e + 1, hi = ne - 1;
               while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
@@ -395,9 +410,11 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
                 else hi = mid - 1;
               }
               e = lo;
+              jb = jbase[e];
+              nb = eoff[e + 1];
             }
             ee[u] = e;
-            jj[u] = ebeg[e] + (w - eoff[e]);
+            jj[u] = jb + w;
             cc[u] = Bcol[jj[u]];
           }
         }
@@ -534,6 +551,9 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   constexpr int TCAP = THREADS;
   const int64_t m = A.nrows;
   const int64_t max_items = m + ceil_div(A.nnz, TCAP) + 1;
+  // an item's wedge count (<= TCAP entries x longest B row) must fit int32
+  GA_REQUIRE(std::min<int64_t>(w.max_b_row, B.nnz) * TCAP < INT32_MAX, GA_ERR_LIMIT,
+             "B rows too long for 32-bit wedge indexing");
   GA_REQUIRE(max_items < INT32_MAX, GA_ERR_LIMIT, "too many work items");
   w.cnt.ensure(sizeof(int32_t) * m);
   w.start.ensure(sizeof(int32_t) * (m + 1));
@@ -797,6 +817,7 @@ extern "C" ga_status ga_mxm_masked(const ga_csr* A, const ga_csr* B, const ga_cs
     dC.ensure(es * std::max<int64_t>(M->nnz, 1));
     MxmWork w;
     w.max_mask_row = host_max_row(M);
+    w.max_b_row = host_max_row(B);
     DCsr vM = dM.view();
     vM.val = nullptr;
     mxm_masked_dev(dA.view(), dB.view(), vM, dtype, 0, dC.p, nullptr, nullptr, w, s);
