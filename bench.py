@@ -217,24 +217,12 @@ def bench_batch(args) -> dict:
     return out
 
 
-def pick_sources(g, count: int, seed: int) -> np.ndarray:
-    """`count` distinct non-isolated vertices, ordered by a SplitMix64 hash of
-    (seed, vertex): a seeded uniform choice that does not depend on numpy's RNG."""
-    v = np.nonzero(g.degrees() > 0)[0].astype(np.uint64)
-    with np.errstate(over="ignore"):
-        z = v + np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    return v[np.argsort(z, kind="stable")[:count]].astype(np.int32)
-
-
 def bench_bc(args) -> dict:
     """Batched betweenness centrality on R-MAT scale 18 (edge factor 16), 256 seeded
     random non-isolated sources as one 256-column pass, FP64 sigma and delta."""
     from graphalgo_amd import api, graphs as G
     g = G.rmat(18, 16, seed=1)
-    src = pick_sources(g, 256, seed=7)
+    src = G.pick_sources(g, 256, seed=7)
     R = api.Graph(g)
     dt, bc, timing, acct = _measure(lambda: R.bc(src), args.steps, args.warmup)
     R.close()
@@ -258,7 +246,125 @@ def bench_bc(args) -> dict:
     return out
 
 
-WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "copy": bench_copy, "batch": bench_batch, "bc": bench_bc}
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_serve(args) -> dict:
+    """Served query stream: Poisson arrivals at a fixed QPS mixing 60% triangle
+    count, 25% k-truss (one k in 3..16 per query) and 15% 64-source BC over 11
+    resident R-MAT graphs (scales 10..20, edge factors 8..32), admitted by HBM
+    footprint into a fixed device arena.  The same stream is replayed on the
+    plain path (one query at a time through the per-graph calls).  The QPS is
+    set from the measured capacities (all queries at t = 0) unless --qps."""
+    from graphalgo_amd import api, serving as S
+    cfg = S.pool_configs(1)
+    t0 = time.perf_counter()
+    pool = S.make_pool(cfg)
+    _log(f"pool generated in {time.perf_counter() - t0:.1f} s")
+    srv = api.Server(hbm_budget=args.budget_gb << 30, slots=args.slots, max_batch=args.max_batch)
+    for g in pool:
+        srv.add_graph(g)
+    graphs = [api.Graph(g) for g in pool]
+    nq = args.queries
+    # warm-up: every graph and kind once on both paths (code objects, first touch)
+    warm, _ = S.make_stream(pool, 3 * len(pool), 1.0, seed=99)
+    srv.replay(warm, np.zeros(len(warm)))
+    api.plain_replay(graphs, warm, np.zeros(len(warm)))
+    _log("warm-up done")
+    sat, _ = S.make_stream(pool, nq, 1.0, seed=2)
+    z = np.zeros(nq)
+    r_sat = srv.replay(sat, z)
+    cap_served = S.latency_summary(r_sat, z)["throughput_qps"]
+    _log(f"served capacity {cap_served:.1f} queries/s")
+    r_sat_plain = api.plain_replay(graphs, sat, z)
+    cap_plain = S.latency_summary(r_sat_plain, z)["throughput_qps"]
+    _log(f"plain capacity {cap_plain:.1f} queries/s")
+    # the same saturated stream without batching: concurrency alone
+    nb = api.Server(hbm_budget=args.budget_gb << 30, slots=args.slots, max_batch=1)
+    for g in pool:
+        nb.add_graph(g)
+    cap_nobatch = S.latency_summary(nb.replay(sat, z), z)["throughput_qps"]
+    nb.close()
+    _log(f"served capacity without batching {cap_nobatch:.1f} queries/s")
+    busy = {}
+    for q, r in zip(sat, r_sat_plain):
+        key = f"{q['kind']}@{cfg[q['graph']]['scale']}"
+        busy[key] = busy.get(key, 0.0) + r["t_end_ms"] - r["t_start_ms"]
+    top = sorted(busy.items(), key=lambda kv: -kv[1])[:8]
+    qps = args.qps or round(0.8 * cap_served, 1)
+    specs, arr = S.make_stream(pool, nq, qps, seed=3)
+    served, lat = [], []
+    t_served = 0.0
+    for step in range(args.steps):
+        t0 = time.perf_counter()
+        res = srv.replay(specs, arr)
+        t_served += time.perf_counter() - t0
+        served.append(S.latency_summary(res, arr))
+        _log(f"served replay {step}: p50 {served[-1]['p50_ms']:.2f} ms p99 {served[-1]['p99_ms']:.2f} ms")
+    plain_res = api.plain_replay(graphs, specs, arr)
+    plain = S.latency_summary(plain_res, arr)
+    _log(f"plain replay: p50 {plain['p50_ms']:.2f} ms p99 {plain['p99_ms']:.2f} ms")
+    for q, a, b in zip(specs, res, plain_res):
+        if q["kind"] == "bc":
+            assert np.allclose(a["bc"], b["bc"], rtol=1e-9), "served and plain BC differ"
+        else:
+            assert a["value"] == b["value"], "served and plain results differ"
+    st = srv.stats()
+    srv.close()
+    for h in graphs:
+        h.close()
+    last = served[-1]
+    kinds = {k: sum(q["kind"] == k for q in specs) for k, _ in S.MIX}
+    out = {"value": last["throughput_qps"], "unit": "queries/s", "ms_per_step": 1e3 * t_served / args.steps,
+           "dtype": "int32+f64",
+           "config": {"workload": "served query stream over 11 resident R-MAT graphs (scale 10..20, edge factor "
+                                  "8..32): Poisson arrivals, 60% TC / 25% k-truss / 15% 64-source BC",
+                      "pool": [(c["scale"], c["edge_factor"]) for c in cfg], "queries": nq, "kinds": kinds,
+                      "qps_offered": qps, "slots": args.slots, "arena_gb": args.budget_gb, "stream_seed": 3},
+           "roofline": None,
+           "served": {**{k: round(v, 3) for k, v in last.items()},
+                      "p50_ms_all_steps": [round(x["p50_ms"], 3) for x in served],
+                      "p99_ms_all_steps": [round(x["p99_ms"], 3) for x in served],
+                      "capacity_qps": round(cap_served, 2), "capacity_qps_without_batching": round(cap_nobatch, 2),
+                      "runs": st["batches"], "queries_completed": st["completed"],
+                      "peak_inflight": st["peak_inflight"],
+                      "peak_arena_bytes": st["peak_bytes"], "resident_bytes": st["resident_bytes"]},
+           "plain_path": {**{k: round(v, 3) for k, v in plain.items()}, "capacity_qps": round(cap_plain, 2),
+                          "busy_ms_top_kind_at_scale": {k: round(v, 1) for k, v in top}},
+           "served_vs_plain": {"throughput": last["throughput_qps"] / plain["throughput_qps"],
+                               "capacity": cap_served / cap_plain,
+                               "p50_latency": plain["p50_ms"] / last["p50_ms"],
+                               "p99_latency": plain["p99_ms"] / last["p99_ms"]},
+           "roofline_note": "queries of every kind run concurrently on 'slots' streams; per-kernel roofline "
+                            "is reported by the ktruss, tc, bc and batch workloads"}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        done = 0
+        for q in specs:
+            g = pool[q["graph"]]
+            if g.n > (1 << 15):
+                continue
+            if q["kind"] == "tc":
+                O.tc(g)
+            elif q["kind"] == "ktruss":
+                O.ktruss(g, q["k"])
+            else:
+                O.bc(g, q["sources"])
+            done += 1
+            if time.perf_counter() - t0 > 15.0:
+                break
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / dtc, "unit": "queries/s", "cores": O.num_threads(), "kind": "port",
+                               "sample": f"oracle on the stream's first {done} queries over pool graphs of scale "
+                                         f"<= 15, back to back ({dtc:.1f} s)"}
+    return out
+
+
+WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "copy": bench_copy, "batch": bench_batch, "bc": bench_bc,
+             "serve": bench_serve}
 
 
 def main() -> None:
@@ -268,6 +374,11 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ktruss", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--queries", type=int, default=400, help="serve: queries per stream")
+    ap.add_argument("--qps", type=float, default=0.0, help="serve: offered load (0 = 0.8 x served capacity)")
+    ap.add_argument("--slots", type=int, default=4, help="serve: queries in flight")
+    ap.add_argument("--budget-gb", type=int, default=96, help="serve: device arena size")
+    ap.add_argument("--max-batch", type=int, default=16, help="serve: queued queries one run may take")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "

@@ -47,6 +47,36 @@ class GaKernelProf(C.Structure):
     ]
 
 
+class GaServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("slots", C.c_int32), ("max_batch", C.c_int32)]
+
+
+GA_Q_TRIANGLES, GA_Q_KTRUSS, GA_Q_BC = 0, 1, 2
+
+
+class GaQuery(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("graph", C.c_int32), ("k", C.c_int32), ("nsources", C.c_int32),
+        ("sources", I32P), ("bc", F64P),
+    ]
+
+
+class GaQueryResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("passes", C.c_int32), ("value", C.c_int64), ("footprint", C.c_int64),
+        ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double), ("t_end_ms", C.c_double),
+        ("slot", C.c_int32), ("batched", C.c_int32),
+    ]
+
+
+class GaServerStats(C.Structure):
+    _fields_ = [
+        ("arena_bytes", C.c_int64), ("resident_bytes", C.c_int64), ("peak_bytes", C.c_int64),
+        ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+        ("batches", C.c_int64), ("peak_inflight", C.c_int32), ("slots", C.c_int32),
+    ]
+
+
 class GraphAlgoError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{GA_ERRORS.get(code, code)}: {msg}")
@@ -76,6 +106,17 @@ GA_SIGNATURES = {
     "ga_graph_triangle_count": (C.c_int, [C.c_void_p, I64P]),
     "ga_graph_truss": (C.c_int, [C.c_void_p, I32P, I32P, C.POINTER(GaTrussStats)]),
     "ga_graph_bc": (C.c_int, [C.c_void_p, I32P, C.c_int64, F64P]),
+    "ga_graph_ktruss": (C.c_int, [C.c_void_p, C.c_int, I64P, I32P]),
+    "ga_server_create": (C.c_int, [C.POINTER(GaServerConfig), C.POINTER(C.c_void_p)]),
+    "ga_server_destroy": (C.c_int, [C.c_void_p]),
+    "ga_server_add_graph": (C.c_int, [C.c_void_p, C.POINTER(GaCsr), I32P]),
+    "ga_server_query_footprint": (C.c_int, [C.c_void_p, C.POINTER(GaQuery), I64P]),
+    "ga_server_submit": (C.c_int, [C.c_void_p, C.POINTER(GaQuery), I64P]),
+    "ga_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(GaQueryResult)]),
+    "ga_server_stats_read": (C.c_int, [C.c_void_p, C.POINTER(GaServerStats)]),
+    "ga_server_replay": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(GaQuery), F64P, C.POINTER(GaQueryResult)]),
+    "ga_plain_replay": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p), C.c_int64, C.POINTER(GaQuery), F64P,
+                                  C.POINTER(GaQueryResult)]),
     "ga_batch_create": (C.c_int, [C.c_int64, C.POINTER(GaCsr), C.POINTER(C.c_void_p)]),
     "ga_batch_destroy": (C.c_int, [C.c_void_p]),
     "ga_batch_triangle_count": (C.c_int, [C.c_void_p, C.c_int, I64P]),

@@ -159,6 +159,13 @@ class Graph:
         N.check(N.galib().ga_graph_bc(self._h, src.ctypes.data_as(N.I32P), len(src), bc.ctypes.data_as(N.F64P)))
         return bc
 
+    def ktruss(self, k: int) -> tuple[int, int]:
+        """k-truss for one k: (stored entries of the k-truss, support passes)."""
+        nnz = C.c_int64()
+        passes = C.c_int32()
+        N.check(N.galib().ga_graph_ktruss(self._h, int(k), C.byref(nnz), C.byref(passes)))
+        return nnz.value, passes.value
+
 
 class Batch:
     """Many small graphs resident together; triangle counts in one persistent launch."""
@@ -195,6 +202,122 @@ class Batch:
         N.check(N.galib().ga_batch_autotune(self._h, C.byref(best), ms.ctypes.data_as(N.F64P)))
         self.threads = best.value
         return best.value, {64: ms[0], 128: ms[1], 256: ms[2]}
+
+
+_KINDS = {"tc": N.GA_Q_TRIANGLES, "ktruss": N.GA_Q_KTRUSS, "bc": N.GA_Q_BC}
+
+
+class _QueryArray:
+    """ctypes array of ga_query built from dicts {"kind": "tc"|"ktruss"|"bc",
+    "graph": id, "k": int, "sources": ints}; owns the source and output arrays."""
+
+    def __init__(self, specs: list[dict], n_of_graph):
+        self.specs = specs
+        self.arr = (N.GaQuery * max(len(specs), 1))()
+        self.src: list[np.ndarray | None] = []
+        self.bc: list[np.ndarray | None] = []
+        for i, q in enumerate(specs):
+            kind = _KINDS[q["kind"]]
+            self.arr[i].kind = kind
+            self.arr[i].graph = int(q["graph"])
+            self.arr[i].k = int(q.get("k", 0))
+            src = bc = None
+            if kind == N.GA_Q_BC:
+                src = np.ascontiguousarray(q["sources"], dtype=np.int32)
+                bc = np.zeros(max(n_of_graph(q["graph"]), 1), dtype=np.float64)
+                self.arr[i].nsources = len(src)
+                self.arr[i].sources = src.ctypes.data_as(N.I32P)
+                self.arr[i].bc = bc.ctypes.data_as(N.F64P)
+            self.src.append(src)
+            self.bc.append(bc)
+
+    def result(self, i: int, r: N.GaQueryResult) -> dict:
+        d = {f: getattr(r, f) for f, _ in N.GaQueryResult._fields_}
+        d["kind"] = self.specs[i]["kind"]
+        d["graph"] = int(self.specs[i]["graph"])
+        if self.bc[i] is not None:
+            d["bc"] = self.bc[i]
+        return d
+
+
+class Server:
+    """Request loop over graphs resident in HBM (ga_server_*): one device arena
+    of hbm_budget bytes holds the graphs and every admitted query's workspace;
+    `slots` workers run queries concurrently, each on a stream of its own;
+    queued k-truss / BC queries on the same graph share a run (up to
+    max_batch; 1 = no batching, 0 = the library default)."""
+
+    def __init__(self, hbm_budget: int, slots: int = 4, max_batch: int = 0):
+        N.init()
+        cfg = N.GaServerConfig(int(hbm_budget), int(slots), int(max_batch))
+        self._h = C.c_void_p()
+        N.check(N.galib().ga_server_create(C.byref(cfg), C.byref(self._h)))
+        self.graphs: list[CSR] = []
+        self._pending: dict[int, tuple[_QueryArray, int]] = {}
+
+    def close(self) -> None:
+        if self._h:
+            N.check(N.galib().ga_server_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_graph(self, A: CSR) -> int:
+        gid = C.c_int32()
+        g = A.ga()
+        N.check(N.galib().ga_server_add_graph(self._h, C.byref(g), C.byref(gid)))
+        self.graphs.append(A)
+        return gid.value
+
+    def _n(self, gid) -> int:
+        return self.graphs[int(gid)].n if 0 <= int(gid) < len(self.graphs) else 1
+
+    def footprint(self, spec: dict) -> int:
+        qa = _QueryArray([spec], self._n)
+        b = C.c_int64()
+        N.check(N.galib().ga_server_query_footprint(self._h, C.byref(qa.arr[0]), C.byref(b)))
+        return b.value
+
+    def submit(self, spec: dict) -> int:
+        qa = _QueryArray([spec], self._n)
+        t = C.c_int64()
+        N.check(N.galib().ga_server_submit(self._h, C.byref(qa.arr[0]), C.byref(t)))
+        self._pending[t.value] = (qa, 0)
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        qa, i = self._pending.pop(ticket)
+        r = N.GaQueryResult()
+        N.check(N.galib().ga_server_wait(self._h, int(ticket), C.byref(r)))
+        return qa.result(i, r)
+
+    def replay(self, specs: list[dict], arrival_ms) -> list[dict]:
+        """Submits query i at arrival_ms[i] after the call starts, returns every result."""
+        qa = _QueryArray(specs, self._n)
+        arr = np.ascontiguousarray(arrival_ms, dtype=np.float64)
+        res = (N.GaQueryResult * max(len(specs), 1))()
+        N.check(N.galib().ga_server_replay(self._h, len(specs), qa.arr, arr.ctypes.data_as(N.F64P), res))
+        return [qa.result(i, res[i]) for i in range(len(specs))]
+
+    def stats(self) -> dict:
+        st = N.GaServerStats()
+        N.check(N.galib().ga_server_stats_read(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in N.GaServerStats._fields_}
+
+
+def plain_replay(graphs: list[Graph], specs: list[dict], arrival_ms) -> list[dict]:
+    """The same stream on the one-graph-per-call path: one query at a time in
+    arrival order on the library stream (ga_plain_replay)."""
+    qa = _QueryArray(specs, lambda g: graphs[int(g)].A.n)
+    hs = (C.c_void_p * max(len(graphs), 1))(*[g._h.value for g in graphs])
+    arr = np.ascontiguousarray(arrival_ms, dtype=np.float64)
+    res = (N.GaQueryResult * max(len(specs), 1))()
+    N.check(N.galib().ga_plain_replay(len(graphs), hs, len(specs), qa.arr, arr.ctypes.data_as(N.F64P), res))
+    return [qa.result(i, res[i]) for i in range(len(specs))]
 
 
 PROF_EVENTS = 1  # HIP events around every masked-SpGEMM launch

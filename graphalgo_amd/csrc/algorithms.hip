@@ -66,8 +66,9 @@ static void upper_triangle_dev(const DCsr& A, OwnedCsr& U, DevBuf& tmp, hipStrea
 }
 
 // ---- filter / compaction (wavefront per row) -------------------------------
+// cnt[i] = entries of row i with support >= thr; dirty[i] = row i loses some
 __global__ void k_filter_count(const int32_t* __restrict__ rp, const int32_t* __restrict__ sup, int64_t n, int thr,
-                               int32_t* __restrict__ cnt) {
+                               int32_t* __restrict__ cnt, uint8_t* __restrict__ dirty) {
   const int lane = threadIdx.x & 63;
   const int64_t wpb = blockDim.x >> 6;
   for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
@@ -77,7 +78,10 @@ __global__ void k_filter_count(const int32_t* __restrict__ rp, const int32_t* __
       const bool keep = j < e && sup[j] >= thr;
       c += __popcll(__ballot(keep));
     }
-    if (lane == 0) cnt[i] = c;
+    if (lane == 0) {
+      cnt[i] = c;
+      dirty[i] = c < e - b ? 1 : 0;
+    }
   }
 }
 
@@ -85,7 +89,7 @@ __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* 
                                  const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t n,
                                  int thr, const int32_t* __restrict__ nrp, int32_t* __restrict__ ncol,
                                  int32_t* __restrict__ neid, int32_t* __restrict__ nold,
-                                 int32_t* __restrict__ newpos) {
+                                 int32_t* __restrict__ newpos, int32_t* __restrict__ nsup) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t wpb = blockDim.x >> 6;
@@ -100,6 +104,7 @@ __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* 
         ncol[r] = col[j];
         neid[r] = eid[j];
         nold[r] = j;
+        nsup[r] = sup[j];  // carried over; the next pass recomputes the affected ones
       }
       if (j < e) newpos[j] = keep ? r : -1;
       dst += __popcll(m);
@@ -127,18 +132,15 @@ __global__ void k_iota(int32_t* __restrict__ p, int64_t n) {
     p[e] = (int32_t)e;
 }
 
-struct TrussWork {
-  OwnedCsr cur[2];
-  DevBuf eid[2], mirror[2];
-  DevBuf sup, cnt, scan_tmp, truss, kept, nold, newpos;
-  MxmWork mw;
-};
-
 // Runs the peeling loop on the device-resident graph A.  all_k: truss
 // decomposition (truss[] per entry of A); otherwise k-truss for one k, and
 // the surviving pattern is left in tw.cur[*cur_out].
-static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hipStream_t s, int* cur_out,
-                       int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st) {
+// k-truss mode takes an ascending list of k: the loop peels at ks[0] until
+// stable, records the surviving entries and passes so far, and continues
+// from that truss to ks[1] (the k2-truss lies inside the k1-truss), ...
+static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s,
+                       int* cur_out, int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st,
+                       int64_t* nnz_at = nullptr, int32_t* passes_at = nullptr) {
   auto t_start = std::chrono::steady_clock::now();
   const int64_t n = A.nrows, nnz0 = A.nnz;
   GA_REQUIRE(A.nrows == A.ncols, GA_ERR_DIM, "k-truss needs a square matrix");
@@ -149,11 +151,12 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
     tw.cur[b].col.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
     tw.eid[b].ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
     tw.mirror[b].ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+    tw.sup[b].ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
   }
   tw.nold.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
   tw.newpos.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
-  tw.sup.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
   tw.cnt.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
+  tw.dirty.ensure(std::max<int64_t>(n, 1));
   tw.kept.ensure(64);
   GA_HIP(hipMemcpyAsync(tw.cur[0].rp.p, A.rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s));
   if (nnz0) GA_HIP(hipMemcpyAsync(tw.cur[0].col.p, A.col, sizeof(int32_t) * nnz0, hipMemcpyDeviceToDevice, s));
@@ -166,25 +169,32 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
   }
   int cur = 0;
   int64_t nnz = nnz0;
-  int k = all_k ? 3 : k_fixed;
+  GA_REQUIRE(all_k || (ks && nk >= 1), GA_ERR_ARG, "no k given");
+  int kidx = 0;
+  int k = all_k ? 3 : ks[0];
   bool valid = false;
   int64_t passes = 0, steps = 0, wedges_host = 0;
   int32_t kmax = 2;
-  int32_t* kept_h = nullptr;
-  GA_HIP(hipHostMalloc((void**)&kept_h, 64, hipHostMallocDefault));
-  try {
+  int32_t* kept_h = static_cast<int32_t*>(pinned_scratch());
+  {
     while (true) {
       if (all_k && nnz == 0) break;
       const DCsr C{n, n, nnz, tw.cur[cur].rp.as<int32_t>(), tw.cur[cur].col.as<int32_t>(), nullptr};
+      int32_t* sup = tw.sup[cur].as<int32_t>();
       if (!valid) {
-        support_symmetric_dev(C, tw.mirror[cur].as<int32_t>(), tw.sup.as<int32_t>(), tw.mw, s);
+        // first pass: every support; later passes: only the entries with an
+        // endpoint that lost entries in the last compaction (the rest carry over)
+        if (passes == 0 || !ctx().selective_support)
+          support_symmetric_dev(C, tw.mirror[cur].as<int32_t>(), sup, tw.mw, s);
+        else
+          support_selected_dev(C, tw.mirror[cur].as<int32_t>(), tw.dirty.as<uint8_t>(), sup, tw.sw, tw.mw, s);
         ++passes;
         valid = true;
       }
       const int thr = k - 2;
       const unsigned wgrid = grid_for(n, 4, 16384);
-      hipLaunchKernelGGL(k_filter_count, dim3(wgrid), dim3(256), 0, s, C.rp, (const int32_t*)tw.sup.as<int32_t>(), n,
-                         thr, tw.cnt.as<int32_t>());
+      hipLaunchKernelGGL(k_filter_count, dim3(wgrid), dim3(256), 0, s, C.rp, (const int32_t*)sup, n, thr,
+                         tw.cnt.as<int32_t>(), tw.dirty.as<uint8_t>());
       GA_HIP(hipGetLastError());
       int32_t* nrp = tw.cur[cur ^ 1].rp.as<int32_t>();
       scan_exclusive_i32(tw.cnt.as<int32_t>(), nrp, n, tw.scan_tmp, s);
@@ -193,7 +203,13 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
       const int64_t kept = *kept_h;
       ++steps;
       if (kept == nnz) {
-        if (!all_k) break;
+        if (!all_k) {
+          if (nnz_at) nnz_at[kidx] = nnz;
+          if (passes_at) passes_at[kidx] = (int32_t)passes;
+          if (++kidx == nk) break;
+          k = ks[kidx];
+          continue;
+        }
         hipLaunchKernelGGL(k_truss_assign, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s,
                            (const int32_t*)tw.eid[cur].as<int32_t>(), nnz, k, tw.truss.as<int32_t>());
         GA_HIP(hipGetLastError());
@@ -202,9 +218,9 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
         continue;
       }
       hipLaunchKernelGGL(k_filter_scatter, dim3(wgrid), dim3(256), 0, s, C.rp, C.col,
-                         (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)tw.sup.as<int32_t>(), n, thr,
+                         (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, n, thr,
                          (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>(),
-                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>());
+                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(), tw.sup[cur ^ 1].as<int32_t>());
       hipLaunchKernelGGL(k_mirror_update, dim3(grid_for(kept, 256, 8192)), dim3(256), 0, s,
                          (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
                          (const int32_t*)tw.newpos.as<int32_t>(), kept, tw.mirror[cur ^ 1].as<int32_t>());
@@ -213,11 +229,7 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
       nnz = kept;
       valid = false;
     }
-  } catch (...) {
-    (void)hipHostFree(kept_h);
-    throw;
   }
-  GA_HIP(hipHostFree(kept_h));
   GA_HIP(hipStreamSynchronize(s));
   if (cur_out) *cur_out = cur;
   if (nnz_out) *nnz_out = nnz;
@@ -228,6 +240,42 @@ static void truss_loop(const DCsr& A, bool all_k, int k_fixed, TrussWork& tw, hi
     st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     (void)wedges_host;
   }
+}
+
+void ktruss_dev(const DCsr& A, int k, TrussWork& tw, hipStream_t s, int64_t* nnz_out, int32_t* passes_out) {
+  GA_REQUIRE(k >= 3, GA_ERR_ARG, "k must be >= 3");
+  int cur = 0;
+  int64_t nnz = 0;
+  ga_truss_stats st{};
+  const int32_t kk = k;
+  truss_loop(A, false, &kk, 1, tw, s, &cur, &nnz, nullptr, &st);
+  if (nnz_out) *nnz_out = nnz;
+  if (passes_out) *passes_out = (int32_t)st.passes;
+}
+
+void ktruss_multi_dev(const DCsr& A, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s, int64_t* nnz_at,
+                      int32_t* passes_at) {
+  GA_REQUIRE(nk >= 1, GA_ERR_ARG, "no k given");
+  for (int i = 0; i < nk; ++i) {
+    GA_REQUIRE(ks[i] >= 3, GA_ERR_ARG, "k must be >= 3");
+    GA_REQUIRE(i == 0 || ks[i] > ks[i - 1], GA_ERR_ARG, "k list must be ascending and distinct");
+  }
+  ga_truss_stats st{};
+  truss_loop(A, false, ks, nk, tw, s, nullptr, nullptr, nullptr, &st, nnz_at, passes_at);
+}
+
+// Every buffer truss_loop sizes for one k (no per-entry truss array), at the
+// sizes of the first pass, which are the largest.
+size_t ktruss_workspace_bytes(int64_t n, int64_t nnz) {
+  const size_t e = lease_bytes(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
+  size_t b = 2 * (lease_bytes(sizeof(int32_t) * (n + 1)) + 4 * e);  // cur rp/col, eid, mirror, sup
+  b += 2 * e;                                                       // nold, newpos
+  b += lease_bytes(sizeof(int32_t) * std::max<int64_t>(n, 1)) + lease_bytes(std::max<int64_t>(n, 1));
+  b += lease_bytes(64);
+  b += scan_tmp_bytes(n);
+  b += mxm_workspace_bytes(n, nnz, true);
+  b += sel_workspace_bytes(n, nnz);
+  return b;
 }
 
 }  // namespace ga
@@ -343,7 +391,8 @@ ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_co
     int cur = 0;
     int64_t nnz = 0;
     ga_truss_stats st{};
-    truss_loop(vA, false, k, tw, s, &cur, &nnz, nullptr, &st);
+    const int32_t kk = k;
+    truss_loop(vA, false, &kk, 1, tw, s, &cur, &nnz, nullptr, &st);
     GA_HIP(hipMemcpyAsync(out_rowptr, tw.cur[cur].rp.p, sizeof(int32_t) * (A->nrows + 1), hipMemcpyDeviceToHost, s));
     if (nnz) GA_HIP(hipMemcpyAsync(out_col, tw.cur[cur].col.p, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
@@ -366,7 +415,7 @@ ga_status ga_truss_decomposition(const ga_csr* A, int32_t* truss, int32_t* kmax,
     tw.mw.max_mask_row = tw.mw.max_b_row = host_max_row(A);
     int32_t km = 2;
     ga_truss_stats st{};
-    truss_loop(vA, true, 0, tw, s, nullptr, nullptr, &km, &st);
+    truss_loop(vA, true, nullptr, 0, tw, s, nullptr, nullptr, &km, &st);
     if (A->nnz) GA_HIP(hipMemcpyAsync(truss, tw.truss.p, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
     if (kmax) *kmax = km;
@@ -431,12 +480,20 @@ ga_status ga_graph_truss(ga_graph* g, int32_t* truss, int32_t* kmax, ga_truss_st
     hipStream_t s = ctx().stream;
     int32_t km = 2;
     ga_truss_stats st{};
-    truss_loop(g->A.view(), true, 0, g->tw, s, nullptr, nullptr, &km, &st);
+    truss_loop(g->A.view(), true, nullptr, 0, g->tw, s, nullptr, nullptr, &km, &st);
     if (truss && g->A.nnz)
       GA_HIP(hipMemcpyAsync(truss, g->tw.truss.p, sizeof(int32_t) * g->A.nnz, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
     if (kmax) *kmax = km;
     if (stats) *stats = st;
+  });
+}
+
+ga_status ga_graph_ktruss(ga_graph* g, int k, int64_t* nnz, int32_t* passes) {
+  return guarded([&] {
+    GA_REQUIRE(g && nnz, GA_ERR_ARG, "null argument");
+    GA_REQUIRE(g->symmetric, GA_ERR_ARG, "k-truss needs a symmetric pattern");
+    ktruss_dev(g->A.view(), k, g->tw, ctx().stream, nnz, passes);
   });
 }
 

@@ -46,13 +46,15 @@ __global__ void k_bc_init(uint16_t* __restrict__ depth, double* __restrict__ sig
   }
 }
 
-__global__ void k_bc_sources(const int32_t* __restrict__ src, int ns, int nsp, uint16_t* __restrict__ depth,
-                             double* __restrict__ sigma, int32_t* __restrict__ level0) {
+// source j sits on lane lane[j]; its vertex is flagged for the level-0 list
+// (a vertex that is a source on several lanes is listed once)
+__global__ void k_bc_sources(const int32_t* __restrict__ src, const int32_t* __restrict__ lane, int ns, int nsp,
+                             uint16_t* __restrict__ depth, double* __restrict__ sigma, uint8_t* __restrict__ flag) {
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ns; j += gridDim.x * blockDim.x) {
-    const int64_t idx = (int64_t)src[j] * nsp + j;
+    const int64_t idx = (int64_t)src[j] * nsp + lane[j];
     depth[idx] = 0;
     sigma[idx] = 1.0;
-    level0[j] = src[j];
+    flag[src[j]] = 1;
   }
 }
 
@@ -79,6 +81,18 @@ __global__ void k_flag_emit(uint8_t* __restrict__ flag, int64_t n, const int32_t
   }
 }
 
+// flag[v] = 1 when some lane of vertex v is at depth k
+__global__ __launch_bounds__(256) void k_level_flags(const uint16_t* __restrict__ depth, int64_t n, int nsp, int k,
+                                                     uint8_t* __restrict__ flag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += (int64_t)gridDim.x * wpb) {
+    bool hit = false;
+    for (int c = lane; c < nsp; c += 64) hit |= depth[v * nsp + c] == (uint16_t)k;
+    if (__ballot(hit) != 0ull && lane == 0) flag[v] = 1;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_bc_last(const int32_t* __restrict__ front, int nfront, int nsp, int k,
                                                  const uint16_t* __restrict__ depth,
                                                  const double* __restrict__ sigma, double* __restrict__ q) {
@@ -99,6 +113,7 @@ __global__ __launch_bounds__(256) void k_bc_last(const int32_t* __restrict__ fro
 // least one item (a backward vertex without successors still needs q).
 constexpr int BC_CH = 256;
 constexpr int BC_UNROLL = 4;
+constexpr int BC_FLAG_CHUNK = 64;  // vertices per thread when flags become a list
 
 __global__ void k_bc_item_count(const int32_t* __restrict__ front, int nf, const int32_t* __restrict__ rp,
                                 int32_t* __restrict__ cnt, int32_t* __restrict__ hcnt) {
@@ -177,7 +192,7 @@ __global__ __launch_bounds__(256) void k_bc_backward_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
     int nsp, int k, const uint16_t* __restrict__ depth, const double* __restrict__ sigma, double* __restrict__ q,
-    double* __restrict__ bc, double* __restrict__ partial) {
+    double* __restrict__ bc, double* __restrict__ partial, int64_t bc_stride) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const int n_items = *n_items_ptr;
@@ -207,14 +222,21 @@ __global__ __launch_bounds__(256) void k_bc_backward_items(
       }
       if (slot >= 0) {
         if (act) partial[(int64_t)slot * nsp + c + lane] = acc;
-      } else if (act) {
-        const double sg = sigma[vi];
-        const double delta = sg * acc;
-        q[vi] = (1.0 + delta) / sg;
-        part += delta;
+      } else {
+        if (act) {
+          const double sg = sigma[vi];
+          const double delta = sg * acc;
+          q[vi] = (1.0 + delta) / sg;
+          part += delta;
+        }
+        if (bc_stride) {  // one score vector per 64-lane group
+          part = wave_reduce_sum(part);
+          if (lane == 0) bc[(c >> 6) * bc_stride + v] += part;
+          part = 0.0;
+        }
       }
     }
-    if (slot < 0) {
+    if (slot < 0 && !bc_stride) {
       part = wave_reduce_sum(part);
       if (lane == 0) bc[v] += part;
     }
@@ -226,7 +248,8 @@ __global__ __launch_bounds__(256) void k_bc_combine(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ hstart, int nsp, int k,
                                                     const uint16_t* __restrict__ depth,
                                                     const double* __restrict__ sigma, double* __restrict__ q,
-                                                    double* __restrict__ bc, const double* __restrict__ partial) {
+                                                    double* __restrict__ bc, const double* __restrict__ partial,
+                                                    int64_t bc_stride) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const uint16_t dk = (uint16_t)k;
@@ -239,25 +262,30 @@ __global__ __launch_bounds__(256) void k_bc_combine(const int32_t* __restrict__ 
     double part = 0.0;
     for (int c = 0; c < nsp; c += 64) {
       const int64_t vi = (int64_t)v * nsp + c + lane;
-      if (depth[vi] != dk) continue;
-      double acc = 0.0;
-      for (int j = 0; j < nc; ++j) acc += partial[(int64_t)(slot0 + j) * nsp + c + lane];
-      const double sg = sigma[vi];
-      const double delta = sg * acc;
-      q[vi] = (1.0 + delta) / sg;
-      part += delta;
+      if (depth[vi] == dk) {
+        double acc = 0.0;
+        for (int j = 0; j < nc; ++j) acc += partial[(int64_t)(slot0 + j) * nsp + c + lane];
+        const double sg = sigma[vi];
+        const double delta = sg * acc;
+        q[vi] = (1.0 + delta) / sg;
+        part += delta;
+      }
+      if (bc_stride) {  // one score vector per 64-lane group
+        part = wave_reduce_sum(part);
+        if (lane == 0) bc[(c >> 6) * bc_stride + v] += part;
+        part = 0.0;
+      }
     }
-    part = wave_reduce_sum(part);
-    if (lane == 0) bc[v] += part;
+    if (!bc_stride) {
+      part = wave_reduce_sum(part);
+      if (lane == 0) bc[v] += part;
+    }
   }
 }
 
-// item list of a frontier (device scans; the item count stays on the device)
+// item list of a frontier (device scans; the item count stays on the device);
+// the buffers are sized for nf = n by bc_batch_dev
 static void bc_items(const DCsr& A, const int32_t* front, int nf, BcWork& w, hipStream_t s) {
-  w.icnt.ensure(sizeof(int32_t) * std::max(nf, 1));
-  w.hcnt.ensure(sizeof(int32_t) * std::max(nf, 1));
-  w.istart.ensure(sizeof(int32_t) * (nf + 1));
-  w.hstart.ensure(sizeof(int32_t) * (nf + 1));
   hipLaunchKernelGGL(k_bc_item_count, dim3(grid_for(nf, 256, 8192)), dim3(256), 0, s, front, nf, A.rp,
                      w.icnt.as<int32_t>(), w.hcnt.as<int32_t>());
   scan_exclusive_i32(w.icnt.as<int32_t>(), w.istart.as<int32_t>(), nf, w.scan_tmp, s);
@@ -304,10 +332,8 @@ static hipEvent_t bc_event(Context& c) {
 
 static void frontier_from_flags(uint8_t* flag, int64_t n, BcWork& w, int32_t* out, int32_t* count_h,
                                 hipStream_t s) {
-  const int chunk = 64;
+  const int chunk = BC_FLAG_CHUNK;
   const int64_t ng = ceil_div(n, chunk);
-  w.cnt.ensure(sizeof(int32_t) * std::max<int64_t>(ng, 1));
-  w.off.ensure(sizeof(int32_t) * (ng + 1));
   hipLaunchKernelGGL(k_flag_count, dim3(grid_for(ng, 256)), dim3(256), 0, s, flag, n, w.cnt.as<int32_t>(), chunk);
   scan_exclusive_i32(w.cnt.as<int32_t>(), w.off.as<int32_t>(), ng, w.scan_tmp, s);
   hipLaunchKernelGGL(k_flag_emit, dim3(grid_for(ng, 256)), dim3(256), 0, s, flag, n,
@@ -316,135 +342,222 @@ static void frontier_from_flags(uint8_t* flag, int64_t n, BcWork& w, int32_t* ou
   GA_HIP(hipStreamSynchronize(s));
 }
 
-// A: device pattern (out-edges); src: host array of distinct sources.
+// Sizes of every BcWork buffer for an n-vertex graph with nnz entries, ns
+// sources on nsp lanes and `outs` score vectors.  Two frontier lists of at
+// most n vertices each (the backward sweep rebuilds its lists from depths).
+struct BcSizes {
+  int64_t nsp, total, item_cap, partial, levels, ng, outs;
+};
+static BcSizes bc_sizes(int64_t n, int64_t nnz, int64_t ns, int64_t nsp, int64_t outs) {
+  BcSizes z;
+  z.nsp = nsp;
+  z.total = n * z.nsp;
+  z.item_cap = n + ceil_div(nnz, BC_CH) + 1;
+  z.partial = (2 * ceil_div(nnz, BC_CH) + 1) * z.nsp;
+  z.levels = 2 * std::max<int64_t>(n, 1);
+  z.ng = ceil_div(std::max<int64_t>(n, 1), BC_FLAG_CHUNK);
+  z.outs = outs;
+  return z;
+}
+
+static void bc_reserve(BcWork& w, int64_t n, const BcSizes& z, int64_t ns) {
+  const int64_t n1 = std::max<int64_t>(n, 1);
+  w.depth.ensure(sizeof(uint16_t) * z.total);
+  w.sigma.ensure(sizeof(double) * z.total);
+  w.q.ensure(sizeof(double) * z.total);
+  w.bc.ensure(sizeof(double) * n1 * z.outs);
+  w.flag.ensure(4 * ceil_div(n1, 4));
+  w.src.ensure(sizeof(int32_t) * 2 * ns);  // vertices, then lanes
+  w.it_v.ensure(sizeof(int32_t) * z.item_cap);
+  w.it_t0.ensure(sizeof(int32_t) * z.item_cap);
+  w.it_slot.ensure(sizeof(int32_t) * z.item_cap);
+  w.partial.ensure(sizeof(double) * z.partial);
+  w.levels.ensure(sizeof(int32_t) * z.levels);
+  w.cnt.ensure(sizeof(int32_t) * z.ng);
+  w.off.ensure(sizeof(int32_t) * (z.ng + 1));
+  w.icnt.ensure(sizeof(int32_t) * n1);
+  w.hcnt.ensure(sizeof(int32_t) * n1);
+  w.istart.ensure(sizeof(int32_t) * (n1 + 1));
+  w.hstart.ensure(sizeof(int32_t) * (n1 + 1));
+  // largest scan: a level list (at most n entries) or the flag groups
+  w.scan_tmp.ensure(scan_tmp_bytes(std::max(n1, z.ng)));
+}
+
+static size_t bc_bytes(int64_t n, const BcSizes& z, int64_t ns) {
+  const int64_t n1 = std::max<int64_t>(n, 1);
+  size_t b = lease_bytes(sizeof(uint16_t) * z.total) + 2 * lease_bytes(sizeof(double) * z.total);
+  b += lease_bytes(sizeof(double) * n1 * z.outs) + lease_bytes(4 * ceil_div(n1, 4)) +
+       lease_bytes(sizeof(int32_t) * 2 * ns);
+  b += 3 * lease_bytes(sizeof(int32_t) * z.item_cap) + lease_bytes(sizeof(double) * z.partial);
+  b += lease_bytes(sizeof(int32_t) * z.levels);
+  b += lease_bytes(sizeof(int32_t) * z.ng) + lease_bytes(sizeof(int32_t) * (z.ng + 1));
+  b += 2 * lease_bytes(sizeof(int32_t) * n1) + 2 * lease_bytes(sizeof(int32_t) * (n1 + 1));
+  b += lease_bytes(scan_tmp_bytes(std::max(n1, z.ng)));
+  return b;
+}
+
+size_t bc_workspace_bytes(int64_t n, int64_t nnz, int64_t ns) {
+  return bc_bytes(n, bc_sizes(n, nnz, ns, ceil_div(ns, 64) * 64, 1), ns);
+}
+
+size_t bc_groups_workspace_bytes(int64_t n, int64_t nnz, const std::vector<int64_t>& group_sizes) {
+  int64_t ns = 0;
+  for (int64_t g : group_sizes) ns += g;
+  const int64_t G = (int64_t)group_sizes.size();
+  return bc_bytes(n, bc_sizes(n, nnz, ns, 64 * G, G), ns);
+}
+
+// Runs the multi-source pass.  Source j is vertex src_h[j] on lane lane_h[j]
+// (distinct lanes; nsp lanes in all).  outs == 1: bc_h[v] sums every lane;
+// outs == nsp / 64: lane group g (lanes 64g .. 64g+63) sums into bc_h[g*n + v].
+static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp, int64_t outs,
+                   double* bc_h, BcWork& w, hipStream_t s);
+
+// A: device pattern (out-edges); src: host array of distinct sources, on
+// lanes 0..ns-1.
 void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h, BcWork& w, hipStream_t s) {
   const int64_t n = A.nrows;
   GA_REQUIRE(A.nrows == A.ncols, GA_ERR_DIM, "betweenness centrality needs a square matrix");
   GA_REQUIRE(ns >= 1 && ns <= 4096, GA_ERR_LIMIT, "between 1 and 4096 sources per batch");
-  {
+  std::vector<char> seen(n, 0);
+  std::vector<int32_t> lane(ns);
+  for (int64_t j = 0; j < ns; ++j) {
+    GA_REQUIRE(src_h[j] >= 0 && src_h[j] < n, GA_ERR_ARG, "source out of range");
+    GA_REQUIRE(!seen[src_h[j]], GA_ERR_ARG, "duplicate source");
+    seen[src_h[j]] = 1;
+    lane[j] = (int32_t)j;
+  }
+  bc_run(A, src_h, lane.data(), ns, ceil_div(ns, 64) * 64, 1, bc_h, w, s);
+}
+
+// Several independent source sets in one pass: set g (at most 64 distinct
+// sources) takes lane group g and its scores go to bc_out[g] (n doubles).
+// Every (vertex, lane) value and every per-vertex sum is formed exactly as
+// in a pass of that set alone, so the scores are bit-identical to it.
+void bc_groups_dev(const DCsr& A, const std::vector<std::vector<int32_t>>& sets, double* const* bc_out, BcWork& w,
+                   hipStream_t s) {
+  const int64_t n = A.nrows;
+  GA_REQUIRE(A.nrows == A.ncols, GA_ERR_DIM, "betweenness centrality needs a square matrix");
+  const int64_t G = (int64_t)sets.size();
+  GA_REQUIRE(G >= 1 && G <= 64, GA_ERR_LIMIT, "between 1 and 64 source sets per pass");
+  std::vector<int32_t> src, lane;
+  for (int64_t g = 0; g < G; ++g) {
+    GA_REQUIRE(!sets[g].empty() && sets[g].size() <= 64, GA_ERR_LIMIT, "1 to 64 sources per set");
     std::vector<char> seen(n, 0);
-    for (int64_t j = 0; j < ns; ++j) {
-      GA_REQUIRE(src_h[j] >= 0 && src_h[j] < n, GA_ERR_ARG, "source out of range");
-      GA_REQUIRE(!seen[src_h[j]], GA_ERR_ARG, "duplicate source");
-      seen[src_h[j]] = 1;
+    for (size_t i = 0; i < sets[g].size(); ++i) {
+      const int32_t v = sets[g][i];
+      GA_REQUIRE(v >= 0 && v < n, GA_ERR_ARG, "source out of range");
+      GA_REQUIRE(!seen[v], GA_ERR_ARG, "duplicate source");
+      seen[v] = 1;
+      src.push_back(v);
+      lane.push_back((int32_t)(64 * g + i));
     }
   }
-  const int nsp = (int)ceil_div(ns, 64) * 64;
-  const int64_t total = n * nsp;
-  w.depth.ensure(sizeof(uint16_t) * total);
-  w.sigma.ensure(sizeof(double) * total);
-  w.q.ensure(sizeof(double) * total);
-  w.bc.ensure(sizeof(double) * std::max<int64_t>(n, 1));
-  w.flag.ensure(4 * ceil_div(std::max<int64_t>(n, 1), 4));
-  w.src.ensure(sizeof(int32_t) * ns);
-  const int64_t item_cap = n + ceil_div(A.nnz, BC_CH) + 1;
-  w.it_v.ensure(sizeof(int32_t) * item_cap);
-  w.it_t0.ensure(sizeof(int32_t) * item_cap);
-  w.it_slot.ensure(sizeof(int32_t) * item_cap);
-  w.partial.ensure(sizeof(double) * (size_t)(2 * ceil_div(A.nnz, BC_CH) + 1) * nsp);
+  std::vector<double> bc((size_t)(G * std::max<int64_t>(n, 1)));
+  bc_run(A, src.data(), lane.data(), (int64_t)src.size(), 64 * G, G, bc.data(), w, s);
+  for (int64_t g = 0; g < G; ++g)
+    if (n) std::memcpy(bc_out[g], bc.data() + g * n, sizeof(double) * n);
+}
+
+static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp_, int64_t outs,
+                   double* bc_h, BcWork& w, hipStream_t s) {
+  const int64_t n = A.nrows;
+  const BcSizes z = bc_sizes(n, A.nnz, ns, nsp_, outs);
+  const int nsp = (int)z.nsp;
+  const int64_t total = z.total;
+  const int64_t bc_stride = outs > 1 ? n : 0;
+  bc_reserve(w, n, z, ns);
   GA_HIP(hipMemcpyAsync(w.src.p, src_h, sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  GA_HIP(hipMemcpyAsync(w.src.as<int32_t>() + ns, lane_h, sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_bc_init, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, w.depth.as<uint16_t>(),
                      w.sigma.as<double>(), total);
-  fill_u64(w.bc.p, 0ull, n, s);
+  fill_u64(w.bc.p, 0ull, n * outs, s);
   fill_u32(w.flag.p, 0u, ceil_div(std::max<int64_t>(n, 1), 4), s);
-  // level lists, grown as the BFS proceeds
-  std::vector<int64_t> loff{0, ns};
-  std::vector<int32_t> lcnt{(int32_t)ns};
-  int64_t cap = std::max<int64_t>(4 * n, ns + n);
-  w.levels.ensure(sizeof(int32_t) * cap);
   hipLaunchKernelGGL(k_bc_sources, dim3(grid_for(ns, 256)), dim3(256), 0, s, (const int32_t*)w.src.as<int32_t>(),
-                     (int)ns, nsp, w.depth.as<uint16_t>(), w.sigma.as<double>(), w.levels.as<int32_t>());
+                     (const int32_t*)(w.src.as<int32_t>() + ns), (int)ns, nsp, w.depth.as<uint16_t>(),
+                     w.sigma.as<double>(), w.flag.as<uint8_t>());
   GA_HIP(hipGetLastError());
-  int32_t* count_h = nullptr;
-  GA_HIP(hipHostMalloc((void**)&count_h, 64, hipHostMallocDefault));
-  try {
-    int d = 0;
-    while (true) {
-      GA_REQUIRE(d < BC_MAX_LEVELS - 1, GA_ERR_LIMIT, "BFS deeper than 65533 levels");
-      const int32_t nf = lcnt[d];
-      const int32_t* front = w.levels.as<int32_t>() + loff[d];
-      Context& cx = ctx();
-      hipEvent_t fe0 = nullptr, fe1 = nullptr;
-      if (cx.prof_events_on) {
-        fe0 = bc_event(cx);
-        fe1 = bc_event(cx);
-        GA_HIP(hipEventRecord(fe0, s));
-      }
-      bc_items(A, front, nf, w, s);
-      hipLaunchKernelGGL(k_bc_forward_items, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s, A.rp,
-                         A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
-                         (const int32_t*)(w.istart.as<int32_t>() + nf), nsp, d, w.depth.as<uint16_t>(),
-                         w.sigma.as<double>(), w.flag.as<uint8_t>());
-      GA_HIP(hipGetLastError());
-      if (cx.prof_events_on) {
-        GA_HIP(hipEventRecord(fe1, s));
-        cx.prof_events.emplace_back(fe0, fe1);
-        cx.prof_launches += 1;
-      }
-      if (loff[d + 1] + n > cap) {  // make room for the next list (at most n vertices)
-        int64_t ncap = std::max<int64_t>(2 * cap, loff[d + 1] + n);
-        DevBuf nb;
-        nb.ensure(sizeof(int32_t) * ncap);
-        GA_HIP(hipMemcpyAsync(nb.p, w.levels.p, sizeof(int32_t) * loff[d + 1], hipMemcpyDeviceToDevice, s));
-        GA_HIP(hipStreamSynchronize(s));
-        w.levels = std::move(nb);
-        cap = ncap;
-      }
-      frontier_from_flags(w.flag.as<uint8_t>(), n, w, w.levels.as<int32_t>() + loff[d + 1], count_h, s);
-      const int32_t nn = *count_h;
-      if (nn == 0) break;
-      lcnt.push_back(nn);
-      loff.push_back(loff[d + 1] + nn);
-      ++d;
-    }
-    // d = deepest level that holds vertices
-    const int dmax = d;
-    Context& cx = ctx();
-    if (cx.prof_bytes) {
-      hipLaunchKernelGGL(k_bc_account, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
-                         (const uint16_t*)w.depth.as<uint16_t>(), n, nsp, cx.prof_dev.as<unsigned long long>());
-      cx.prof_bytes_host += (int64_t)(dmax + 1) * (4 * (n + 1) + 4 * A.nnz);
-      cx.prof_bytes_launches += 2 * dmax;
-    }
-    hipEvent_t be0 = nullptr, be1 = nullptr;
+  int32_t* count_h = static_cast<int32_t*>(pinned_scratch());
+  // forward frontier lists ping-pong between the two halves of w.levels;
+  // level 0 = the source vertices
+  int32_t* lists[2] = {w.levels.as<int32_t>(), w.levels.as<int32_t>() + std::max<int64_t>(n, 1)};
+  frontier_from_flags(w.flag.as<uint8_t>(), n, w, lists[0], count_h, s);
+  int32_t nf = *count_h;
+  int d = 0;
+  Context& cx = ctx();
+  while (true) {
+    GA_REQUIRE(d < BC_MAX_LEVELS - 1, GA_ERR_LIMIT, "BFS deeper than 65533 levels");
+    const int32_t* front = lists[d & 1];
+    hipEvent_t fe0 = nullptr, fe1 = nullptr;
     if (cx.prof_events_on) {
-      be0 = bc_event(cx);
-      be1 = bc_event(cx);
-      GA_HIP(hipEventRecord(be0, s));
+      fe0 = bc_event(cx);
+      fe1 = bc_event(cx);
+      GA_HIP(hipEventRecord(fe0, s));
     }
-    if (dmax >= 1) {
-      hipLaunchKernelGGL(k_bc_last, dim3(grid_for(lcnt[dmax], 4, 65536)), dim3(256), 0, s,
-                         (const int32_t*)(w.levels.as<int32_t>() + loff[dmax]), lcnt[dmax], nsp, dmax,
-                         (const uint16_t*)w.depth.as<uint16_t>(), (const double*)w.sigma.as<double>(),
-                         w.q.as<double>());
-      for (int k = dmax - 1; k >= 1; --k) {
-        const int32_t* front = w.levels.as<int32_t>() + loff[k];
-        const int nf = lcnt[k];
-        bc_items(A, front, nf, w, s);
-        hipLaunchKernelGGL(k_bc_backward_items, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s,
-                           A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
-                           (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nf),
-                           nsp, k, (const uint16_t*)w.depth.as<uint16_t>(), (const double*)w.sigma.as<double>(),
-                           w.q.as<double>(), w.bc.as<double>(), w.partial.as<double>());
-        hipLaunchKernelGGL(k_bc_combine, dim3(grid_for(nf, 4, 65536)), dim3(256), 0, s, front, nf, A.rp,
-                           (const int32_t*)w.hstart.as<int32_t>(), nsp, k, (const uint16_t*)w.depth.as<uint16_t>(),
-                           (const double*)w.sigma.as<double>(), w.q.as<double>(), w.bc.as<double>(),
-                           (const double*)w.partial.as<double>());
-        GA_HIP(hipGetLastError());
-      }
-    }
+    bc_items(A, front, nf, w, s);
+    hipLaunchKernelGGL(k_bc_forward_items, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s, A.rp,
+                       A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+                       (const int32_t*)(w.istart.as<int32_t>() + nf), nsp, d, w.depth.as<uint16_t>(),
+                       w.sigma.as<double>(), w.flag.as<uint8_t>());
+    GA_HIP(hipGetLastError());
     if (cx.prof_events_on) {
-      GA_HIP(hipEventRecord(be1, s));
-      cx.prof_events.emplace_back(be0, be1);
+      GA_HIP(hipEventRecord(fe1, s));
+      cx.prof_events.emplace_back(fe0, fe1);
       cx.prof_launches += 1;
     }
-    if (n) GA_HIP(hipMemcpyAsync(bc_h, w.bc.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    GA_HIP(hipStreamSynchronize(s));
-  } catch (...) {
-    (void)hipHostFree(count_h);
-    throw;
+    frontier_from_flags(w.flag.as<uint8_t>(), n, w, lists[(d + 1) & 1], count_h, s);
+    const int32_t nn = *count_h;
+    if (nn == 0) break;
+    nf = nn;
+    ++d;
   }
-  GA_HIP(hipHostFree(count_h));
+  // d = deepest level that holds vertices; its list is lists[d & 1] (nf entries)
+  const int dmax = d;
+  if (cx.prof_bytes) {
+    hipLaunchKernelGGL(k_bc_account, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
+                       (const uint16_t*)w.depth.as<uint16_t>(), n, nsp, cx.prof_dev.as<unsigned long long>());
+    cx.prof_bytes_host += (int64_t)(dmax + 1) * (4 * (n + 1) + 4 * A.nnz);
+    cx.prof_bytes_launches += 2 * dmax;
+  }
+  hipEvent_t be0 = nullptr, be1 = nullptr;
+  if (cx.prof_events_on) {
+    be0 = bc_event(cx);
+    be1 = bc_event(cx);
+    GA_HIP(hipEventRecord(be0, s));
+  }
+  if (dmax >= 1) {
+    hipLaunchKernelGGL(k_bc_last, dim3(grid_for(nf, 4, 65536)), dim3(256), 0, s, (const int32_t*)lists[dmax & 1], nf,
+                       nsp, dmax, (const uint16_t*)w.depth.as<uint16_t>(), (const double*)w.sigma.as<double>(),
+                       w.q.as<double>());
+    GA_HIP(hipGetLastError());
+    for (int k = dmax - 1; k >= 1; --k) {
+      // the level-k list again, from the depths (vertices with a lane at depth k)
+      hipLaunchKernelGGL(k_level_flags, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s,
+                         (const uint16_t*)w.depth.as<uint16_t>(), n, nsp, k, w.flag.as<uint8_t>());
+      int32_t* front = lists[0];
+      frontier_from_flags(w.flag.as<uint8_t>(), n, w, front, count_h, s);
+      const int32_t nk = *count_h;
+      if (nk == 0) continue;
+      bc_items(A, front, nk, w, s);
+      hipLaunchKernelGGL(k_bc_backward_items, dim3(grid_for(nk + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s,
+                         A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+                         (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nk), nsp,
+                         k, (const uint16_t*)w.depth.as<uint16_t>(), (const double*)w.sigma.as<double>(),
+                         w.q.as<double>(), w.bc.as<double>(), w.partial.as<double>(), bc_stride);
+      hipLaunchKernelGGL(k_bc_combine, dim3(grid_for(nk, 4, 65536)), dim3(256), 0, s, (const int32_t*)front, nk, A.rp,
+                         (const int32_t*)w.hstart.as<int32_t>(), nsp, k, (const uint16_t*)w.depth.as<uint16_t>(),
+                         (const double*)w.sigma.as<double>(), w.q.as<double>(), w.bc.as<double>(),
+                         (const double*)w.partial.as<double>(), bc_stride);
+      GA_HIP(hipGetLastError());
+    }
+  }
+  if (cx.prof_events_on) {
+    GA_HIP(hipEventRecord(be1, s));
+    cx.prof_events.emplace_back(be0, be1);
+    cx.prof_launches += 1;
+  }
+  if (n) GA_HIP(hipMemcpyAsync(bc_h, w.bc.p, sizeof(double) * n * outs, hipMemcpyDeviceToHost, s));
+  GA_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace ga

@@ -57,31 +57,56 @@ int guarded(F&& f) {
 }
 
 // Grow-only device allocation owned by a workspace or handle.
+// A query's device workspace inside a server arena: one contiguous block,
+// handed out front to back.  While a worker thread runs a query it points
+// t_lease at the block, and every DevBuf sized on that thread takes its
+// memory from it instead of hipMalloc (no allocation, and no implicit device
+// synchronisation of hipFree, while other queries run).  The block size is
+// the query's footprint; running past it is an error, never a silent
+// allocation.
+struct Lease {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t b);
+};
+extern thread_local Lease* t_lease;
+// bytes a buffer of b bytes occupies in a lease (256-byte aligned blocks)
+inline size_t lease_bytes(size_t b) { return (b + 255) & ~size_t(255); }
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool own = false;  // allocated with hipMalloc (else a lease block or empty)
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), own(o.own) {
+    o.p = nullptr; o.bytes = 0; o.own = false;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
-      p = o.p; bytes = o.bytes;
-      o.p = nullptr; o.bytes = 0;
+      p = o.p; bytes = o.bytes; own = o.own;
+      o.p = nullptr; o.bytes = 0; o.own = false;
     }
     return *this;
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    own = false;
   }
   void ensure(size_t b) {
     if (b <= bytes) return;
     release();
     if (b == 0) return;
+    if (t_lease) {
+      p = t_lease->take(b);
+      bytes = b;
+      return;
+    }
     hipError_t e = hipMalloc(&p, b);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -89,10 +114,16 @@ struct DevBuf {
       throw Error{GA_ERR_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed"};
     }
     bytes = b;
+    own = true;
   }
   template <typename T>
   T* as() const { return static_cast<T*>(p); }
 };
+
+// 256 bytes of pinned host memory per thread, for the small device-to-host
+// reads that steer the host loops (allocated on first use, kept for the
+// thread's lifetime: no per-call hipHostMalloc / hipHostFree).
+void* pinned_scratch();
 
 struct Context {
   int device = -1;
@@ -109,6 +140,12 @@ struct Context {
   // and only extra chunks are listed (GA_DIRECT_ITEMS=1), which pays a claim
   // for every empty row and loses on peeled graphs
   bool direct_items = false;
+  // GA_SELECTIVE_SUPPORT=1: k-truss passes after the first recompute only
+  // the supports of entries with an endpoint that lost entries.  Off by
+  // default: on R-MAT 16 it forms 19% fewer products but the pass time is
+  // set by per-item costs, not products, and the selection launches cost
+  // more than the products save (510 vs 493 ms per decomposition).
+  bool selective_support = false;
   // kernel timing (events around the masked-product launches) and
   // algorithmic byte accounting (extra counting kernels; kept out of timed runs)
   bool prof_events_on = false;
@@ -207,6 +244,14 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
 // compaction by the k-truss loop.
 void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s);
 void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s);
+// Support pass over the affected entries only (entries with an endpoint in a
+// row flagged dirty); the other entries of S keep their values.
+struct SelWork {
+  DevBuf cnt, mrp, mcol, opos, scan_tmp;
+};
+void support_selected_dev(const DCsr& C, const int32_t* mirror, const uint8_t* dirty, int32_t* S, SelWork& sw,
+                          MxmWork& w, hipStream_t s);
+size_t sel_workspace_bytes(int64_t n, int64_t nnz);
 
 // Batched sum-mode product over many graphs stored back to back (graph-local
 // column indices): sums[g] += sum of C<L_g> = L_g * L_g.  threads = 64, 128
@@ -214,12 +259,42 @@ void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s);
 void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t* gbase, int64_t max_cols,
                          unsigned long long* sums, MxmWork& w, hipStream_t s);
 
+// Device bytes of the masked-product workspace for an m-row A with nnz_a
+// entries (every buffer launch_binned sizes, plus the symmetric mask prefix
+// when with_mend), as taken from a lease.
+size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend);
+size_t scan_tmp_bytes(int64_t n);
+
+// k-truss workspace (algorithms.hip): ping-pong CSR, edge ids, mirror index.
+struct TrussWork {
+  OwnedCsr cur[2];
+  DevBuf eid[2], mirror[2], sup[2];
+  DevBuf cnt, scan_tmp, truss, kept, nold, newpos, dirty;
+  MxmWork mw;
+  SelWork sw;
+};
+// k-truss for one k of a symmetric loop-free pattern on the device; returns
+// the surviving entry count (both triangles) and the support passes run.
+void ktruss_dev(const DCsr& A, int k, TrussWork& tw, hipStream_t s, int64_t* nnz_out, int32_t* passes_out);
+// Several k at once (ascending, distinct): one peeling run that stops at each
+// k in turn; nnz_at[i] / passes_at[i] = entries of the ks[i]-truss and the
+// support passes run up to it.
+void ktruss_multi_dev(const DCsr& A, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s, int64_t* nnz_at,
+                      int32_t* passes_at);
+size_t ktruss_workspace_bytes(int64_t n, int64_t nnz);
+
 // Batched betweenness centrality workspace and driver (bc.hip).
 struct BcWork {
   DevBuf depth, sigma, q, bc, flag, levels, cnt, off, scan_tmp, src;
   DevBuf icnt, hcnt, istart, hstart, it_v, it_t0, it_slot, partial;  // neighbour-chunked items
 };
 void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h, BcWork& w, hipStream_t s);
+size_t bc_workspace_bytes(int64_t n, int64_t nnz, int64_t ns);
+// Independent source sets (each at most 64 sources) in one pass, one score
+// vector per set, bit-identical to passes of the sets alone.
+void bc_groups_dev(const DCsr& A, const std::vector<std::vector<int32_t>>& sets, double* const* bc_out, BcWork& w,
+                   hipStream_t s);
+size_t bc_groups_workspace_bytes(int64_t n, int64_t nnz, const std::vector<int64_t>& group_sizes);
 
 // Strict lower triangle of a square pattern with sorted rows, on device.
 void lower_triangle_dev(const DCsr& A, OwnedCsr& L, DevBuf& tmp, hipStream_t s);

@@ -16,6 +16,33 @@ Context* g_ctx = nullptr;
 
 void set_error(int code, const std::string& msg) { t_err = "[" + std::to_string(code) + "] " + msg; }
 
+thread_local Lease* t_lease = nullptr;
+
+void* Lease::take(size_t b) {
+  const size_t a = lease_bytes(b);
+  if (used + a > cap)
+    throw Error{GA_ERR_LIMIT, "workspace needs more than the query's footprint of " + std::to_string(cap) +
+                                  " bytes (" + std::to_string(used + a) + " requested)"};
+  void* p = base + used;
+  used += a;
+  return p;
+}
+
+namespace {
+struct PinnedScratch {
+  void* p = nullptr;
+  ~PinnedScratch() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+thread_local PinnedScratch t_pinned;
+}  // namespace
+
+void* pinned_scratch() {
+  if (!t_pinned.p) GA_HIP(hipHostMalloc(&t_pinned.p, 256, hipHostMallocDefault));
+  return t_pinned.p;
+}
+
 Context& ctx() {
   GA_REQUIRE(g_ctx != nullptr, GA_ERR_STATE, "ga_init has not been called");
   GA_HIP(hipSetDevice(g_ctx->device));
@@ -179,6 +206,8 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tile_apply(const int32_t* __res
   }
 }
 
+size_t scan_tmp_bytes(int64_t n) { return n <= 0 ? 0 : lease_bytes(sizeof(long long) * ceil_div(n, SCAN_TILE)); }
+
 void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf& tmp, hipStream_t s) {
   if (n <= 0) {
     fill_u32(out, 0u, 1, s);
@@ -273,6 +302,7 @@ ga_status ga_init(int device) {
     GA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* wf = std::getenv("GA_WAVE_FLOPS")) c->wave_flops = std::atoll(wf);
     if (const char* di = std::getenv("GA_DIRECT_ITEMS")) c->direct_items = std::atoi(di) != 0;
+    if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT")) c->selective_support = std::atoi(ss) != 0;
     c->prof_dev.ensure(64);
     GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
     GA_HIP(hipStreamSynchronize(c->stream));

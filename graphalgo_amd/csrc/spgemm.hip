@@ -45,7 +45,7 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ item_row, const int32_t* __restrict__ item_t0,
     const int32_t* __restrict__ n_items_ptr, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ wedge_counter, int bitmap_words, const int32_t* __restrict__ Mend,
-    int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror) {
+    int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror, const int32_t* __restrict__ opos) {
   constexpr int MXM_THREADS = THREADS;
   constexpr int MXM_TCAP = THREADS;
   constexpr int MXM_MCAP = 4 * THREADS;
@@ -247,9 +247,10 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
         if (MODE == 1) lsum += (long long)v;
         else if (local) atomic_add_val(&s_acc[p], v);
         else {
-          atomic_add_val(&Cval[ms + p], v);
+          const int o = opos ? opos[ms + p] : ms + p;
+          atomic_add_val(&Cval[o], v);
           if (mirror) {
-            const int mt = mirror[ms + p];
+            const int mt = mirror[o];
             if (mt >= 0) atomic_add_val(&Cval[mt], v);
           }
         }
@@ -261,16 +262,19 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
       if (local) {
         // mirror: C is symmetric and only its lower half is computed; each
         // value is also written to the transposed entry mirror[ms + p]
+        // opos: the mask is a selection of C's entries; entry p of the mask
+        // row is C entry opos[ms + p]
         for (int p = tid; p < mnz; p += MXM_THREADS) {
           const T v = s_acc[p];
-          const int mt = mirror ? mirror[ms + p] : -1;  // negative = no transposed entry
+          const int o = opos ? opos[ms + p] : ms + p;
+          const int mt = mirror ? mirror[o] : -1;  // negative = no transposed entry
           if (multi) {
             if (v != T(0)) {
-              atomic_add_val(&Cval[ms + p], v);
+              atomic_add_val(&Cval[o], v);
               if (mt >= 0) atomic_add_val(&Cval[mt], v);
             }
           } else {
-            Cval[ms + p] = v;
+            Cval[o] = v;
             if (mt >= 0) Cval[mt] = v;
           }
         }
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
     T* __restrict__ Cval, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
     unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk,
-    const int32_t* __restrict__ mirror) {
+    const int32_t* __restrict__ mirror, const int32_t* __restrict__ opos) {
   __shared__ uint32_t s_bm[WR_WAVES][WR_BMW];
   __shared__ uint16_t s_wpre[WR_WAVES][WR_BMW];
   __shared__ T s_acc[WR_WAVES][MODE == 0 ? WR_MCAP : 1];
@@ -435,9 +439,10 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
     if (MODE == 0) {
       for (int p = lane; p < mnz; p += 64) {
         const T v = s_acc[wv][p];
-        Cval[ms + p] = v;
+        const int o = opos ? opos[ms + p] : ms + p;  // selected mask: C entry of mask entry p
+        Cval[o] = v;
         if (mirror) {  // symmetric C: also the transposed entry (negative = none)
-          const int mt = mirror[ms + p];
+          const int mt = mirror[o];
           if (mt >= 0) Cval[mt] = v;
         }
       }
@@ -547,7 +552,8 @@ static hipEvent_t take_event(Context& c) {
 template <typename T, bool PA, bool PB, int MODE, int THREADS, bool BATCHED>
 static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
                           const int32_t* row_group, const int32_t* gbase, MxmWork& w, int bitmap_words,
-                          hipStream_t s, const int32_t* Mend, int trunc, const int32_t* mirror = nullptr) {
+                          hipStream_t s, const int32_t* Mend, int trunc, const int32_t* mirror = nullptr,
+                          const int32_t* opos = nullptr) {
   constexpr int TCAP = THREADS;
   const int64_t m = A.nrows;
   const int64_t max_items = m + ceil_div(A.nnz, TCAP) + 1;
@@ -636,14 +642,14 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col, Mend,
                      static_cast<T*>(Cval), sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
                      (const int32_t*)(w.rstart.as<int32_t>() + m), w.counter.as<unsigned int>() + MXM_NSEG * 32, wc,
-                     trunc, chunk_r, mirror);
+                     trunc, chunk_r, mirror, opos);
   GA_HIP(hipGetLastError());
   hipLaunchKernelGGL(kitems, dim3((unsigned)grid_i), dim3(THREADS), dyn, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col,
                      static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
                      direct ? (const int32_t*)n_extra : (const int32_t*)(w.start.as<int32_t>() + m),
                      w.counter.as<unsigned int>(), wc, bitmap_words, Mend, trunc, chunk_i, direct ? (int)m : 0,
-                     mirror);
+                     mirror, opos);
   GA_HIP(hipGetLastError());
   if (c.prof_events_on) {
     GA_HIP(hipEventRecord(e1, s));
@@ -748,6 +754,17 @@ void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s) {
   GA_HIP(hipGetLastError());
 }
 
+size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend) {
+  // launch_binned's buffers at the 64-lane item width (the most items)
+  const int64_t max_items = m + ceil_div(nnz_a, 64) + 1;
+  size_t b = 3 * lease_bytes(sizeof(int32_t) * m) + 2 * lease_bytes(sizeof(int32_t) * (m + 1));
+  b += 2 * lease_bytes(sizeof(int32_t) * max_items);
+  b += lease_bytes(sizeof(uint32_t) * (MXM_NSEG * 32 + 32));
+  b += scan_tmp_bytes(m);
+  if (with_mend) b += lease_bytes(sizeof(int32_t) * m);
+  return b;
+}
+
 void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s) {
   const int64_t n = C.nrows;
   GA_REQUIRE(C.nrows == C.ncols, GA_ERR_DIM, "symmetric support needs a square matrix");
@@ -763,6 +780,101 @@ void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, Mxm
                        c.prof_dev.as<unsigned long long>() + 1);
   launch_binned<int32_t, true, true, 0, 256, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, split, 1,
                                                      mirror);
+}
+
+// ---- support of the affected entries only ----------------------------------
+// After a compaction only the entries with a "dirty" endpoint (a row that
+// lost entries) can change support: S(i,j) = |N(i) & N(j)| is unchanged when
+// neither N(i) nor N(j) changed.  Each affected undirected edge {i,j}, i > j,
+// is computed once, by row i as a lower entry, as in the full symmetric pass:
+// row i's mask is the part of its lower half with a dirty endpoint, so B rows
+// are cut at the largest such column (never later than in the full pass).
+// (Giving the edges of a clean row to its dirty partner's upper half forms
+// fewer products, but puts the long rows of hub vertices on the slow global
+// accumulator and runs slower.)  The selection becomes a mask CSR M'
+// (columns + the C position of each entry), the other supports stay as
+// carried over from the previous pass, and the product writes each value to
+// its C entry and the transposed one.
+__global__ void k_sel_count(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                            const uint8_t* __restrict__ dirty, int64_t n, int32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
+    const bool di = dirty[i] != 0;
+    const int b = rp[i], e = rp[i + 1];
+    int c = 0;
+    for (int t = b + lane; t < e; t += 64) {
+      const int j = col[t];
+      c += (j < i && (di || dirty[j])) ? 1 : 0;
+    }
+    c = wave_reduce_sum(c);
+    if (lane == 0) cnt[i] = c;
+  }
+}
+
+__global__ void k_sel_fill(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                           const uint8_t* __restrict__ dirty, const int32_t* __restrict__ mirror, int64_t n,
+                           const int32_t* __restrict__ mrp, int32_t* __restrict__ mcol, int32_t* __restrict__ opos,
+                           int32_t* __restrict__ S) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
+    int dst = mrp[i];
+    if (mrp[i + 1] == dst) continue;
+    const bool di = dirty[i] != 0;
+    const int b = rp[i], e = rp[i + 1];
+    for (int t = b + lane; t - lane < e; t += 64) {
+      bool sel = false;
+      int j = 0;
+      if (t < e) {
+        j = col[t];
+        sel = j < i && (di || dirty[j]);
+      }
+      const unsigned long long m = __ballot(sel);
+      if (sel) {
+        const int r = dst + __popcll(m & lt);
+        mcol[r] = j;
+        opos[r] = t;
+        S[t] = 0;  // accumulated into (multi-item rows) or overwritten
+        const int mt = mirror[t];
+        if (mt >= 0) S[mt] = 0;
+      }
+      dst += __popcll(m);
+    }
+  }
+}
+
+void support_selected_dev(const DCsr& C, const int32_t* mirror, const uint8_t* dirty, int32_t* S, SelWork& sw,
+                          MxmWork& w, hipStream_t s) {
+  const int64_t n = C.nrows;
+  GA_REQUIRE(C.nrows == C.ncols, GA_ERR_DIM, "symmetric support needs a square matrix");
+  if (n == 0 || C.nnz == 0) return;
+  sw.cnt.ensure(sizeof(int32_t) * n);
+  sw.mrp.ensure(sizeof(int32_t) * (n + 1));
+  sw.mcol.ensure(sizeof(int32_t) * C.nnz);
+  sw.opos.ensure(sizeof(int32_t) * C.nnz);
+  hipLaunchKernelGGL(k_sel_count, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, C.rp, C.col, dirty, n,
+                     sw.cnt.as<int32_t>());
+  scan_exclusive_i32(sw.cnt.as<int32_t>(), sw.mrp.as<int32_t>(), n, sw.scan_tmp, s);
+  hipLaunchKernelGGL(k_sel_fill, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, C.rp, C.col, dirty, mirror, n,
+                     (const int32_t*)sw.mrp.as<int32_t>(), sw.mcol.as<int32_t>(), sw.opos.as<int32_t>(), S);
+  GA_HIP(hipGetLastError());
+  // M'.nnz is only a bound here (the exact count stays on the device); the
+  // algorithmic accounting is the reference's full product, M = C
+  const DCsr M{n, n, C.nnz, sw.mrp.as<int32_t>(), sw.mcol.as<int32_t>(), nullptr};
+  const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : 0;
+  Context& c = ctx();
+  if (c.prof_bytes)
+    hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
+                       c.prof_dev.as<unsigned long long>() + 1);
+  launch_binned<int32_t, true, true, 0, 256, false>(C, C, M, S, nullptr, nullptr, nullptr, w, bm, s, nullptr, 1,
+                                                     mirror, sw.opos.as<int32_t>());
+}
+
+size_t sel_workspace_bytes(int64_t n, int64_t nnz) {
+  return lease_bytes(sizeof(int32_t) * n) + lease_bytes(sizeof(int32_t) * (n + 1)) +
+         2 * lease_bytes(sizeof(int32_t) * nnz) + scan_tmp_bytes(n);
 }
 
 void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,

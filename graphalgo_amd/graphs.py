@@ -126,6 +126,25 @@ def rmat(scale: int, edge_factor: int, seed: int, abc=RMAT_ABCD) -> CSR:
     return _take(rp, col, n.value, nnz.value)
 
 
+def splitmix64(x) -> np.ndarray:
+    """SplitMix64 finaliser (vectorised, uint64 wrap-around)."""
+    z = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def pick_sources(g: CSR, count: int, seed: int) -> np.ndarray:
+    """`count` distinct non-isolated vertices, ordered by a SplitMix64 hash of
+    (seed, vertex): a seeded uniform choice that does not depend on numpy's RNG."""
+    v = np.nonzero(g.degrees() > 0)[0].astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = splitmix64(v + np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15) - np.uint64(0x9E3779B97F4A7C15))
+    return v[np.argsort(z, kind="stable")[:count]].astype(np.int32)
+
+
 def knn_params(base_seed: int, g: int) -> tuple[int, int, int]:
     n, d, c = C.c_int64(), C.c_int(), C.c_int()
     hostlib().gah_knn_params(base_seed, g, C.byref(n), C.byref(d), C.byref(c))

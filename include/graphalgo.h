@@ -141,6 +141,10 @@ ga_status ga_graph_destroy(ga_graph* g);
 ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles);
 ga_status ga_graph_truss(ga_graph* g, int32_t* truss, int32_t* kmax, ga_truss_stats* stats);
 ga_status ga_graph_bc(ga_graph* g, const int32_t* sources, int64_t nsources, double* bc);
+/* k-truss for one k of the resident graph (k_truss, src/main.cpp:89-146, on
+ * a graph already in HBM): *nnz = stored entries of the k-truss (both
+ * triangles, the reference's C.nz), *passes = support passes. */
+ga_status ga_graph_ktruss(ga_graph* g, int k, int64_t* nnz, int32_t* passes);
 
 /* ---- batched triangle counting ----------------------------------------- */
 /* Many small graphs resident together and counted in one persistent launch.
@@ -158,6 +162,84 @@ ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts);
 /* Times the three widths on the resident batch, keeps the fastest;
  * ms_per_candidate (may be NULL) receives 3 times for 64, 128, 256. */
 ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candidate);
+
+/* ---- serving ----------------------------------------------------------- */
+/* A request loop over graphs resident in HBM.  The reference has no serving
+ * layer: each query is the reference's own call (triangle_counting_masked_
+ * sandia src/main.cpp:65-86, k_truss :89-146, betweenness_centrality_batch
+ * include/betweenness_centrality.h:125-237) on a graph that stays uploaded.
+ *
+ * The server owns one device arena of hbm_budget bytes, reserved once.
+ * Resident graphs and every admitted query's workspace are blocks of that
+ * arena; nothing is allocated while queries run.  A query's footprint is the
+ * exact size of the workspace its path uses (ga_server_query_footprint).
+ * Queries are admitted in arrival order when one of `slots` workers is free
+ * and the arena holds a block of the footprint; each worker runs its query
+ * on a HIP stream of its own, so queries of every kind and size overlap on
+ * the device and a new query starts as soon as a slot and memory free up.
+ * When a worker admits a k-truss or BC query it also takes the queued
+ * queries of the same kind on the same graph (up to max_batch): k-truss
+ * queries share one peeling run that stops at each requested k in ascending
+ * order; BC queries of at most 64 sources share one multi-source pass, one
+ * 64-lane group per query.  Results do not depend on what runs beside or
+ * with a query (BC scores are bit-identical to the query run alone). */
+typedef struct ga_server ga_server;
+typedef struct {
+  int64_t hbm_budget; /* bytes of the device arena (graphs + query workspaces) */
+  int32_t slots;      /* runs in flight at most (one stream each), 1..64 */
+  int32_t max_batch;  /* queued queries one run may take (0 = 16, 1 = no batching) */
+} ga_server_config;
+enum { GA_Q_TRIANGLES = 0, GA_Q_KTRUSS = 1, GA_Q_BC = 2 };
+typedef struct {
+  int32_t kind;            /* GA_Q_* */
+  int32_t graph;           /* id from ga_server_add_graph */
+  int32_t k;               /* GA_Q_KTRUSS: k >= 3 */
+  int32_t nsources;        /* GA_Q_BC: 1..4096 distinct sources */
+  const int32_t* sources;  /* GA_Q_BC: copied at submit */
+  double* bc;              /* GA_Q_BC: n doubles, written before the query completes */
+} ga_query;
+typedef struct {
+  int32_t status;      /* ga_status of the query */
+  int32_t passes;      /* GA_Q_KTRUSS: support passes of its run up to its k */
+  int64_t value;       /* triangles | k-truss entries (both triangles) | 0 for BC */
+  int64_t footprint;   /* workspace bytes the query held */
+  double t_submit_ms;  /* submit, admission, completion (ms; server clock, */
+  double t_start_ms;   /* or the replay's clock for ga_*_replay) */
+  double t_end_ms;
+  int32_t slot;        /* worker that ran it (-1 if rejected) */
+  int32_t batched;     /* queries that shared its run (1 = alone) */
+} ga_query_result;
+typedef struct {
+  int64_t arena_bytes, resident_bytes; /* arena size; bytes held by graphs */
+  int64_t peak_bytes;                  /* peak of graphs + leased workspaces */
+  int64_t submitted, completed, rejected;
+  int64_t batches;                     /* runs started (queries / batches = mean batch) */
+  int32_t peak_inflight, slots;
+} ga_server_stats;
+
+ga_status ga_server_create(const ga_server_config* cfg, ga_server** out);
+/* Waits for every submitted query, then frees the arena. */
+ga_status ga_server_destroy(ga_server* srv);
+/* Symmetric pattern, sorted rows; kept in the arena with its strict lower
+ * triangle.  GA_ERR_OOM when the arena cannot hold it. */
+ga_status ga_server_add_graph(ga_server* srv, const ga_csr* A, int32_t* graph_id);
+ga_status ga_server_query_footprint(ga_server* srv, const ga_query* q, int64_t* bytes);
+/* Queues the query and returns at once.  GA_ERR_OOM when its footprint can
+ * never fit beside the resident graphs; argument errors are reported here. */
+ga_status ga_server_submit(ga_server* srv, const ga_query* q, int64_t* ticket);
+/* Blocks until the ticket completes; returns the query's status. */
+ga_status ga_server_wait(ga_server* srv, int64_t ticket, ga_query_result* out);
+ga_status ga_server_stats_read(ga_server* srv, ga_server_stats* out);
+/* Replays a query stream: query i is submitted at arrival_ms[i] after the
+ * call starts (sorted ascending), then every result is collected; times in
+ * out[] are ms since the call started. */
+ga_status ga_server_replay(ga_server* srv, int64_t nq, const ga_query* q, const double* arrival_ms,
+                           ga_query_result* out);
+/* The same stream on the plain path: one query at a time in arrival order on
+ * the library stream, through ga_graph_triangle_count / ga_graph_ktruss /
+ * ga_graph_bc on graphs[q.graph] (the one-graph-per-call baseline). */
+ga_status ga_plain_replay(int64_t ngraphs, ga_graph* const* graphs, int64_t nq, const ga_query* q,
+                          const double* arrival_ms, ga_query_result* out);
 
 /* ---- kernel timing ----------------------------------------------------- */
 /* flags bit 0: bracket every launch of the masked-SpGEMM kernels with HIP

@@ -1,0 +1,182 @@
+"""GPU tests of the serving loop: every query's result is the same in a
+concurrent mix as alone (TC and k-truss bit-exact, BC within 1e-9 relative),
+admission by HBM footprint, clean rejection, and the plain replay."""
+import numpy as np
+import pytest
+
+from graphalgo_amd import serving as S
+
+pytestmark = pytest.mark.gpu
+
+BC_RTOL = 1e-9
+GA_ERR_ARG, GA_ERR_OOM = 1, 4
+
+
+@pytest.fixture(scope="module")
+def pool(G, small_graphs):
+    gs = S.make_pool(S.pool_configs(5, scales=range(9, 14)))
+    gs.append(small_graphs["G43.mtx"][1])
+    gs.append(small_graphs["yeast_30NN.mtx.gz"][1])
+    return gs
+
+
+def _alone(api, graphs, q):
+    g = graphs[q["graph"]]
+    if q["kind"] == "tc":
+        return g.triangle_count()
+    if q["kind"] == "ktruss":
+        return g.ktruss(q["k"])[0]
+    return g.bc(q["sources"])
+
+
+def _same(q, got, want):
+    if q["kind"] == "bc":
+        scale = max(1.0, float(np.abs(want).max()))
+        return np.allclose(got["bc"], want, rtol=BC_RTOL, atol=BC_RTOL * scale * 1e-3)
+    return got["value"] == want
+
+
+def test_mix_equals_alone(gpu, pool, oracle):
+    specs, _ = S.make_stream(pool, 90, 1000.0, seed=11)
+    srv = gpu.Server(hbm_budget=8 << 30, slots=4)
+    for g in pool:
+        srv.add_graph(g)
+    print("case serve mix: 90 queries, all arriving at once, 4 slots", flush=True)
+    res = srv.replay(specs, np.zeros(len(specs)))
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(specs) and st["rejected"] == 0
+    assert st["peak_inflight"] > 1  # queries did run side by side
+    assert all(r["status"] == 0 for r in res)
+    alone = [gpu.Graph(g) for g in pool]
+    for q, r in zip(specs, res):
+        assert r["t_submit_ms"] <= r["t_start_ms"] <= r["t_end_ms"]
+        assert _same(q, r, _alone(gpu, alone, q)), q
+    # a few against the oracle directly
+    for q, r in list(zip(specs, res))[:12]:
+        g = pool[q["graph"]]
+        if q["kind"] == "tc":
+            assert r["value"] == oracle.tc(g)
+        elif q["kind"] == "ktruss":
+            assert r["value"] == len(oracle.ktruss(g, q["k"])[1])
+        else:
+            want = oracle.bc(g, q["sources"])
+            assert np.allclose(r["bc"], want, rtol=BC_RTOL, atol=BC_RTOL * max(1.0, want.max()) * 1e-3)
+    for g in alone:
+        g.close()
+
+
+def test_poisson_replay_and_plain_path_agree(gpu, pool):
+    specs, arr = S.make_stream(pool, 60, 400.0, seed=12)
+    srv = gpu.Server(hbm_budget=8 << 30, slots=3)
+    for g in pool:
+        srv.add_graph(g)
+    print("case serve poisson: 60 queries at 400 QPS", flush=True)
+    res = srv.replay(specs, arr)
+    srv.close()
+    graphs = [gpu.Graph(g) for g in pool]
+    plain = gpu.plain_replay(graphs, specs, arr)
+    for q, a, b, t in zip(specs, res, plain, arr):
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["t_submit_ms"] >= t - 1e-6 and b["t_start_ms"] >= t - 1e-6
+        if q["kind"] == "bc":
+            assert np.allclose(a["bc"], b["bc"], rtol=BC_RTOL)
+        else:
+            assert a["value"] == b["value"]
+            if q["kind"] == "ktruss" and a["batched"] == 1:
+                assert a["passes"] == b["passes"]
+    s = S.latency_summary(res, arr)
+    assert s["p99_ms"] >= s["p50_ms"] > 0
+    for g in graphs:
+        g.close()
+
+
+def test_batched_runs_equal_alone(gpu, G):
+    """Queued BC queries on one graph share a multi-source pass (one 64-lane
+    group each, sources may repeat across queries) and queued k-truss
+    queries share one peeling run; every result equals the query alone, BC
+    bit for bit."""
+    g = G.rmat(13, 16, seed=4)
+    srv = gpu.Server(hbm_budget=4 << 30, slots=1, max_batch=16)
+    gid = srv.add_graph(g)
+    rng = np.random.default_rng(5)
+    specs = []
+    for i in range(10):
+        specs.append({"kind": "bc", "graph": gid, "sources": G.pick_sources(g, int(rng.integers(1, 65)), i % 4)})
+        specs.append({"kind": "ktruss", "graph": gid, "k": int(rng.integers(3, 12))})
+    specs.append({"kind": "ktruss", "graph": gid, "k": specs[1]["k"]})  # a repeated k
+    print("case serve batching: 10 BC + 11 k-truss queries on one graph, 1 slot", flush=True)
+    res = srv.replay(specs, np.zeros(len(specs)))
+    st = srv.stats()
+    srv.close()
+    assert max(r["batched"] for r in res) > 1 and st["batches"] < len(specs)
+    alone = gpu.Graph(g)
+    for q, r in zip(specs, res):
+        if q["kind"] == "bc":
+            assert np.array_equal(r["bc"], alone.bc(q["sources"])), q
+        else:
+            assert r["value"] == alone.ktruss(q["k"])[0], q
+    alone.close()
+
+
+def test_admission_by_footprint(gpu, G):
+    g = G.rmat(12, 16, seed=9)
+    probe = gpu.Server(hbm_budget=1 << 30, slots=1, max_batch=1)
+    gid = probe.add_graph(g)
+    resident = probe.stats()["resident_bytes"]
+    bcq = {"kind": "bc", "graph": gid, "sources": G.pick_sources(g, 64, 1)}
+    fp_bc = probe.footprint(bcq)
+    fp_tc = probe.footprint({"kind": "tc", "graph": gid})
+    fp_kt = probe.footprint({"kind": "ktruss", "graph": gid, "k": 4})
+    probe.close()
+    assert fp_bc > fp_kt > fp_tc > 0
+    # room for one BC workspace at a time beside the graph (4 KiB arena pages)
+    budget = resident + fp_bc + fp_bc // 2 + (64 << 10)
+    srv = gpu.Server(hbm_budget=budget, slots=4, max_batch=1)
+    gid = srv.add_graph(g)
+    specs = [{"kind": "bc", "graph": gid, "sources": G.pick_sources(g, 64, s)} for s in range(6)]
+    print("case serve admission: 6 BC queries, arena for one", flush=True)
+    res = srv.replay(specs, np.zeros(6))
+    st = srv.stats()
+    assert st["peak_inflight"] == 1 and st["completed"] == 6
+    assert st["peak_bytes"] <= st["arena_bytes"]
+    alone = gpu.Graph(g)
+    for q, r in zip(specs, res):
+        assert r["footprint"] == fp_bc
+        assert _same(q, r, alone.bc(q["sources"]))
+    # a query that can never fit is refused at submit, the server keeps serving
+    big = {"kind": "bc", "graph": gid, "sources": G.pick_sources(g, 1024, 3)}
+    with pytest.raises(Exception) as e:
+        srv.submit(big)
+    assert getattr(e.value, "code", None) == GA_ERR_OOM
+    t = srv.submit({"kind": "tc", "graph": gid})
+    assert srv.wait(t)["value"] == alone.triangle_count()
+    assert srv.stats()["rejected"] == 1
+    srv.close()
+    alone.close()
+
+
+def test_submit_validation(gpu, G):
+    g = G.rmat(9, 8, seed=2)
+    srv = gpu.Server(hbm_budget=256 << 20, slots=2)
+    gid = srv.add_graph(g)
+    bad = [
+        {"kind": "tc", "graph": gid + 1},
+        {"kind": "ktruss", "graph": gid, "k": 2},
+        {"kind": "bc", "graph": gid, "sources": np.array([1, 1], dtype=np.int32)},
+        {"kind": "bc", "graph": gid, "sources": np.array([g.n], dtype=np.int32)},
+    ]
+    for q in bad:
+        with pytest.raises(Exception) as e:
+            srv.submit(q)
+        assert getattr(e.value, "code", None) == GA_ERR_ARG, q
+    t = srv.submit({"kind": "ktruss", "graph": gid, "k": 3})
+    r = srv.wait(t)
+    assert r["status"] == 0 and r["value"] > 0
+    srv.close()
+    asym = G.CSR(3, np.array([0, 1, 1, 1], dtype=np.int32), np.array([1], dtype=np.int32))
+    srv = gpu.Server(hbm_budget=64 << 20, slots=1)
+    with pytest.raises(Exception) as e:
+        srv.add_graph(asym)
+    assert getattr(e.value, "code", None) == GA_ERR_ARG
+    srv.close()
