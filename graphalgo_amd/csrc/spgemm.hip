@@ -24,11 +24,19 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace ga {
 
 constexpr int64_t MXM_BITMAP_MAX_COLS = int64_t(1) << 18;  // 32 KB bitmap
-constexpr int MXM_UNROLL = 4;                                // gathers in flight per lane
+#ifndef GA_MXM_G
+#define GA_MXM_G 16
+#endif
+#ifndef GA_MXM_K
+#define GA_MXM_K 8
+#endif
+constexpr int MXM_G = GA_MXM_G;  // lanes that read adjacent B columns (one 64-byte run)
+constexpr int MXM_K = GA_MXM_K;  // wedges per lane per trip (gathers in flight)
 constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up to 2^17 columns
 constexpr int MXM_NSEG = 32;                                 // claim segments (one counter line each)
 
@@ -174,88 +182,115 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     if (tid == 0) s_eoff[ne] = total;
     __syncthreads();
 
-    // stride over the flattened wedge space of the item, MXM_UNROLL wedges
-    // per lane per trip: the B-column loads of one trip are independent, so
-    // each lane keeps MXM_UNROLL gathers in flight before it consumes them
+    // Wedge mapping: lanes form groups of MXM_G; group g of a trip owns the
+    // MXM_G * MXM_K consecutive wedges starting at base + g * MXM_G * MXM_K,
+    // lane r of the group takes every MXM_G-th of them.  The MXM_G lanes of a
+    // group read adjacent B columns (32-byte runs), and a lane's wedges are
+    // MXM_G apart, so its entry advances by a short linear walk; the entry of
+    // the first wedge of a trip is found by one binary search (from the
+    // lane's current entry) per MXM_K wedges.
     long long lsum = 0;
-    // current entry e, its jbase and the first wedge of the next entry are
-    // kept in registers: the common case costs one compare per wedge
-    int e = 0;
-    int jb = ne > 0 ? s_jb[0] : 0;
-    int nb = ne > 0 ? s_eoff[1] : 0;
-    for (int w0 = tid; w0 < total; w0 += MXM_THREADS * MXM_UNROLL) {
-      int cc[MXM_UNROLL], jj[MXM_UNROLL], ee[MXM_UNROLL];
+    const int grp = tid / MXM_G, rl = tid % MXM_G;
+    // the loop is instantiated twice: mask row staged in LDS (sorted list and
+    // LDS accumulator) or not (search and accumulate in global memory), so
+    // the hot loop has no branch on it
+    auto wedge_loop = [&](auto local_c) {
+      constexpr bool LOCAL = decltype(local_c)::value;
+      int e = 0;
+      for (int base = 0; base < total; base += MXM_THREADS * MXM_K) {
+        const int p0 = base + grp * (MXM_G * MXM_K) + rl;
+        if (p0 >= total) break;  // later trips start even further out
+        {  // entry holding wedge p0: largest e' >= e with s_eoff[e'] <= p0
+          int lo = e, hi = ne - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_eoff[mid] <= p0) lo = mid;
+            else hi = mid - 1;
+          }
+          e = lo;
+        }
+        int jb = s_jb[e];
+        int nb = s_eoff[e + 1];
+        int cc[MXM_K], jj[MXM_K], ee[MXM_K];
 #pragma unroll
-      for (int u = 0; u < MXM_UNROLL; ++u) {
-        const int w = w0 + u * MXM_THREADS;
-        cc[u] = -1;
-        jj[u] = 0;
-        ee[u] = 0;
-        if (w < total) {
-          if (w >= nb) {  // advance to the entry holding wedge w
-            int lo = e + 1, hi = ne - 1;
-            while (lo < hi) {
-              const int mid = (lo + hi + 1) >> 1;
-              if (s_eoff[mid] <= w) lo = mid;
-              else hi = mid - 1;
-            }
-            e = lo;
-            jb = s_jb[e];
+        for (int u = 0; u < MXM_K; ++u) {
+          // wedges past the end re-read the last one (in bounds) and are dropped
+          const bool ok = p0 + u * MXM_G < total;
+          const int w = ok ? p0 + u * MXM_G : total - 1;
+          while (w >= nb) {  // next non-empty entry (zero-length entries are skipped)
+            ++e;
             nb = s_eoff[e + 1];
+            jb = s_jb[e];
           }
           ee[u] = e;
           jj[u] = jb + w;
-          cc[u] = Bcol[jj[u]];
+          const int c = Bcol[(unsigned)jj[u]];
+          cc[u] = ok ? c : -1;
         }
-      }
-#pragma unroll
-      for (int u = 0; u < MXM_UNROLL; ++u) {
-        const int c = cc[u];
-        if (c < 0) continue;
-        int p;
-        if (RANK) {
-          // exact membership and mask position from the bitmap word:
-          // position = rank base of the word + set bits below column c
-          const uint32_t word = s_bitmap[c >> 5];
-          const uint32_t bit = 1u << (c & 31);
-          if (!(word & bit)) continue;
-          p = (int)s_wpre[c >> 5] + __popc(word & (bit - 1u));
-        } else {
-          if (use_bm && !((s_bitmap[c >> 5] >> (c & 31)) & 1u)) continue;
-          int lo = 0, hi = mnz;
-          bool found;
-          if (local) {
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (s_mcol[mid] < c) lo = mid + 1;
-              else hi = mid;
-            }
-            found = lo < mnz && s_mcol[lo] == c;
+        auto accumulate = [&](int p, T v) {
+          if (MODE == 1) {
+            lsum += (long long)v;
+          } else if (LOCAL) {
+            atomic_add_val(&s_acc[p], v);
           } else {
-            const int32_t* mrow = Mcol + ms;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (mrow[mid] < c) lo = mid + 1;
-              else hi = mid;
+            const int o = opos ? opos[ms + p] : ms + p;
+            atomic_add_val(&Cval[o], v);
+            if (mirror) {
+              const int mt = mirror[o];
+              if (mt >= 0) atomic_add_val(&Cval[mt], v);
             }
-            found = lo < mnz && mrow[lo] == c;
           }
-          if (!found) continue;
-          p = lo;
-        }
-        const T v = (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]);
-        if (MODE == 1) lsum += (long long)v;
-        else if (local) atomic_add_val(&s_acc[p], v);
-        else {
-          const int o = opos ? opos[ms + p] : ms + p;
-          atomic_add_val(&Cval[o], v);
-          if (mirror) {
-            const int mt = mirror[o];
-            if (mt >= 0) atomic_add_val(&Cval[mt], v);
+        };
+        if (RANK) {
+          // all bitmap words, then all rank bases, then the hits: the LDS
+          // reads of the MXM_K wedges are in flight together
+          uint32_t wd[MXM_K];
+          int wi[MXM_K], pre[MXM_K];
+#pragma unroll
+          for (int u = 0; u < MXM_K; ++u) {
+            wi[u] = cc[u] >= 0 ? (cc[u] >> 5) : 0;
+            wd[u] = s_bitmap[wi[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < MXM_K; ++u) pre[u] = s_wpre[wi[u]];
+#pragma unroll
+          for (int u = 0; u < MXM_K; ++u) {
+            const uint32_t bit = 1u << (cc[u] & 31);
+            if (cc[u] >= 0 && (wd[u] & bit))
+              accumulate(pre[u] + __popc(wd[u] & (bit - 1u)),
+                         (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < MXM_K; ++u) {
+            const int c = cc[u];
+            if (c < 0) continue;
+            if (use_bm && !((s_bitmap[c >> 5] >> (c & 31)) & 1u)) continue;
+            int lo = 0, hi = mnz;
+            bool found;
+            if (LOCAL) {
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_mcol[mid] < c) lo = mid + 1;
+                else hi = mid;
+              }
+              found = lo < mnz && s_mcol[lo] == c;
+            } else {
+              const int32_t* mrow = Mcol + ms;
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (mrow[mid] < c) lo = mid + 1;
+                else hi = mid;
+              }
+              found = lo < mnz && mrow[lo] == c;
+            }
+            if (found) accumulate(lo, (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
           }
         }
       }
-    }
+    };
+    if (local) wedge_loop(std::true_type{});
+    else wedge_loop(std::false_type{});
     __syncthreads();
 
     if (MODE == 0) {
@@ -314,7 +349,6 @@ constexpr int WR_WAVES = 4;
 constexpr int WR_MCAP = 128;
 constexpr int WR_MAXCOLS = 4096;
 constexpr int WR_BMW = WR_MAXCOLS / 32;
-constexpr int WR_UNROLL = 4;
 
 template <typename T, bool PAT_A, bool PAT_B, int MODE, bool BATCHED>
 __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
@@ -394,44 +428,55 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
       if (lane == 0) eoff[ne] = total;
       __threadfence_block();
       if (lane == 0) wedges += (unsigned long long)total;
+      // same wedge mapping as the item kernel: groups of MXM_G lanes, MXM_K
+      // wedges per lane per trip, one binary search per trip plus a walk
+      const int grp = lane / MXM_G, rl = lane % MXM_G;
       int e = 0;
-      int jb = jbase[0];
-      int nb = eoff[1];
-      for (int w0 = lane; w0 < total; w0 += 64 * WR_UNROLL) {
-        int cc[WR_UNROLL], jj[WR_UNROLL], ee[WR_UNROLL];
-#pragma unroll
-        for (int u = 0; u < WR_UNROLL; ++u) {
-          const int w = w0 + u * 64;
-          cc[u] = -1;
-          jj[u] = 0;
-          ee[u] = 0;
-          if (w < total) {
-            if (w >= nb) {
-              int lo = e + 1, hi = ne - 1;
-              while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (eoff[mid] <= w) lo = mid;
-                else hi = mid - 1;
-              }
-              e = lo;
-              jb = jbase[e];
-              nb = eoff[e + 1];
-            }
-            ee[u] = e;
-            jj[u] = jb + w;
-            cc[u] = Bcol[jj[u]];
+      for (int base = 0; base < total; base += 64 * MXM_K) {
+        const int p0 = base + grp * (MXM_G * MXM_K) + rl;
+        if (p0 >= total) break;
+        {
+          int lo = e, hi = ne - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (eoff[mid] <= p0) lo = mid;
+            else hi = mid - 1;
           }
+          e = lo;
+        }
+        int jb = jbase[e];
+        int nb = eoff[e + 1];
+        int cc[MXM_K], jj[MXM_K], ee[MXM_K];
+#pragma unroll
+        for (int u = 0; u < MXM_K; ++u) {
+          const bool ok = p0 + u * MXM_G < total;
+          const int w = ok ? p0 + u * MXM_G : total - 1;
+          while (w >= nb) {
+            ++e;
+            nb = eoff[e + 1];
+            jb = jbase[e];
+          }
+          ee[u] = e;
+          jj[u] = jb + w;
+          const int c = Bcol[(unsigned)jj[u]];
+          cc[u] = ok ? c : -1;
+        }
+        uint32_t wd[MXM_K];
+        int wi[MXM_K], pre[MXM_K];
+#pragma unroll
+        for (int u = 0; u < MXM_K; ++u) {
+          wi[u] = cc[u] >= 0 ? (cc[u] >> 5) : 0;
+          wd[u] = bm[wi[u]];
         }
 #pragma unroll
-        for (int u = 0; u < WR_UNROLL; ++u) {
-          const int c = cc[u];
-          if (c < 0) continue;
-          const uint32_t word = bm[c >> 5];
-          const uint32_t bit = 1u << (c & 31);
-          if (!(word & bit)) continue;
+        for (int u = 0; u < MXM_K; ++u) pre[u] = wpre[wi[u]];
+#pragma unroll
+        for (int u = 0; u < MXM_K; ++u) {
+          const uint32_t bit = 1u << (cc[u] & 31);
+          if (cc[u] < 0 || !(wd[u] & bit)) continue;
           const T v = (PAT_A ? T(1) : s_aval[wv][ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]);
           if (MODE == 1) lsum += (long long)v;
-          else atomic_add_val(&s_acc[wv][(int)wpre[c >> 5] + __popc(word & (bit - 1u))], v);
+          else atomic_add_val(&s_acc[wv][pre[u] + __popc(wd[u] & (bit - 1u))], v);
         }
       }
       __threadfence_block();  // entry slots are rewritten by the next chunk
