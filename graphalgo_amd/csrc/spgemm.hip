@@ -38,6 +38,12 @@ constexpr int64_t MXM_BITMAP_MAX_COLS = int64_t(1) << 18;  // 32 KB bitmap
 constexpr int MXM_G = GA_MXM_G;  // lanes that read adjacent B columns (one 64-byte run)
 constexpr int MXM_K = GA_MXM_K;  // wedges per lane per trip (gathers in flight)
 constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up to 2^17 columns
+// Items of the support pass and of pattern products: 512 A entries and an
+// LDS mask accumulator of 4096 entries.  On R-MAT 16 the support kernel takes
+// 344 us per pass at 256 / 1024, 233 us at 512 / 2048 and 212 us at
+// 512 / 4096: mask rows beyond the LDS capacity accumulate with global
+// atomics, and fewer, larger items pay the per-item staging less often.
+constexpr int MXM_WIDE = 512;
 constexpr int MXM_NSEG = 32;                                 // claim segments (one counter line each)
 
 // THREADS = workgroup size = A entries per item (one entry per thread in the
@@ -56,7 +62,7 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror, const int32_t* __restrict__ opos) {
   constexpr int MXM_THREADS = THREADS;
   constexpr int MXM_TCAP = THREADS;
-  constexpr int MXM_MCAP = 4 * THREADS;
+  constexpr int MXM_MCAP = 8 * THREADS;  // mask rows up to this long accumulate in LDS
   // dynamic LDS: the n-bit mask bitmap, then (RANK) per bitmap word the
   // position in the mask row of the word's first mask column
   extern __shared__ uint32_t s_bitmap[];
@@ -713,10 +719,13 @@ template <typename T, int MODE>
 static void dispatch_pat(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
                          const int32_t* row_group, MxmWork& w, int bm, hipStream_t s) {
   const bool pa = A.val == nullptr, pb = B.val == nullptr;
-  if (pa && pb) launch_binned<T, true, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
-  else if (pa) launch_binned<T, true, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
-  else if (pb) launch_binned<T, false, true, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
-  else launch_binned<T, false, false, MODE, 256, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+  // pattern products with 4-byte (or no) accumulators use the wide items;
+  // 8-byte valued accumulators keep 256 to bound the LDS
+  constexpr int TH = (MODE == 1 || sizeof(T) == 4) ? MXM_WIDE : 256;
+  if (pa && pb) launch_binned<T, true, true, MODE, TH, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+  else if (pa) launch_binned<T, true, false, MODE, TH, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+  else if (pb) launch_binned<T, false, true, MODE, TH, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+  else launch_binned<T, false, false, MODE, TH, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
 }
 
 __global__ void k_count_wedges(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol,
@@ -823,8 +832,8 @@ void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, Mxm
   if (c.prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
                        c.prof_dev.as<unsigned long long>() + 1);
-  launch_binned<int32_t, true, true, 0, 256, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, split, 1,
-                                                     mirror);
+  launch_binned<int32_t, true, true, 0, MXM_WIDE, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, split,
+                                                          1, mirror);
 }
 
 // ---- support of the affected entries only ----------------------------------
@@ -913,8 +922,8 @@ void support_selected_dev(const DCsr& C, const int32_t* mirror, const uint8_t* d
   if (c.prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
                        c.prof_dev.as<unsigned long long>() + 1);
-  launch_binned<int32_t, true, true, 0, 256, false>(C, C, M, S, nullptr, nullptr, nullptr, w, bm, s, nullptr, 1,
-                                                     mirror, sw.opos.as<int32_t>());
+  launch_binned<int32_t, true, true, 0, MXM_WIDE, false>(C, C, M, S, nullptr, nullptr, nullptr, w, bm, s, nullptr,
+                                                          1, mirror, sw.opos.as<int32_t>());
 }
 
 size_t sel_workspace_bytes(int64_t n, int64_t nnz) {
