@@ -45,6 +45,7 @@ constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up t
 // atomics, and fewer, larger items pay the per-item staging less often.
 constexpr int MXM_WIDE = 512;
 constexpr int MXM_NSEG = 32;                                 // claim segments (one counter line each)
+constexpr int MXM_CTR_WORDS = 2 * MXM_NSEG * 32 + 32;         // item segments, row segments, cursor
 
 // THREADS = workgroup size = A entries per item (one entry per thread in the
 // entry scan); MCAP = mask entries staged in LDS.  BATCHED: the operands are
@@ -380,15 +381,36 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
   for (int q = lane; q < WR_BMW; q += 64) bm[q] = 0u;
   const int n_rows = *n_rows_ptr;
   unsigned long long wedges = 0;
-  int next = 0, end = 0;  // rows claimed `chunk` at a time (lane 0's range)
+  // Rows are claimed `chunk` at a time from MXM_NSEG segments of the row list
+  // (one counter line each).  A wave starts in the segment of its workgroup
+  // (blockIdx % MXM_NSEG: workgroups are dealt round-robin to the XCDs, so a
+  // segment is drained by one XCD and the rows of neighbouring rows -- the
+  // same graph of a batch -- share that XCD's L2), then steals from the next.
+  const int seg_len = (n_rows + MXM_NSEG - 1) / MXM_NSEG;
+  int seg = blockIdx.x % MXM_NSEG, tries = 0;
+  int next = 0, end = 0;  // lane 0's claimed range
   while (true) {
-    int r = 0;
+    int r = n_rows;
     if (lane == 0) {
-      if (next >= end) {
-        next = (int)atomicAdd(counter, (unsigned)chunk);
-        end = min(next + chunk, n_rows);
+      if (next < end) {
+        r = next++;
+      } else {
+        while (tries < MXM_NSEG) {
+          const int lo = seg * seg_len;
+          const int hi = min(lo + seg_len, n_rows);
+          if (lo < hi) {
+            const int k = (int)atomicAdd(&counter[seg * 32], (unsigned)chunk);
+            if (lo + k < hi) {
+              r = lo + k;
+              next = r + 1;
+              end = min(lo + k + chunk, hi);
+              break;
+            }
+          }
+          seg = (seg + 1) % MXM_NSEG;
+          ++tries;
+        }
       }
-      r = next < end ? next++ : n_rows;
     }
     r = __shfl(r, 0, 64);
     if (r >= n_rows) break;
@@ -619,9 +641,9 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   w.rows.ensure(sizeof(int32_t) * m);
   w.item_row.ensure(sizeof(int32_t) * max_items);
   w.item_t0.ensure(sizeof(int32_t) * max_items);
-  // counters: MXM_NSEG item-segment lines (32 u32 each), then the row
-  // counter and the extra-item cursor on a line of their own
-  const int ctr_words = MXM_NSEG * 32 + 32;
+  // counters: MXM_NSEG item-segment lines (32 u32 each), MXM_NSEG row-segment
+  // lines, then the extra-item cursor on a line of its own
+  const int ctr_words = MXM_CTR_WORDS;
   w.counter.ensure(sizeof(uint32_t) * ctr_words);
   fill_u32(w.counter.p, 0u, ctr_words, s);
   const long long wave_flops =
@@ -630,7 +652,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // only the extra chunks of long rows are listed -- two small launches
   // instead of binning, two scans and two compactions
   const bool direct = wave_flops < 0 && ctx().direct_items;
-  int32_t* n_extra = reinterpret_cast<int32_t*>(w.counter.as<unsigned int>() + MXM_NSEG * 32 + 1);
+  int32_t* n_extra = reinterpret_cast<int32_t*>(w.counter.as<unsigned int>() + 2 * MXM_NSEG * 32 + 1);
   if (direct) {
     hipLaunchKernelGGL(k_extra_items, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
                        w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(), n_extra);
@@ -813,7 +835,7 @@ size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend) {
   const int64_t max_items = m + ceil_div(nnz_a, 64) + 1;
   size_t b = 3 * lease_bytes(sizeof(int32_t) * m) + 2 * lease_bytes(sizeof(int32_t) * (m + 1));
   b += 2 * lease_bytes(sizeof(int32_t) * max_items);
-  b += lease_bytes(sizeof(uint32_t) * (MXM_NSEG * 32 + 32));
+  b += lease_bytes(sizeof(uint32_t) * MXM_CTR_WORDS);
   b += scan_tmp_bytes(m);
   if (with_mend) b += lease_bytes(sizeof(int32_t) * m);
   return b;
