@@ -227,7 +227,7 @@ def bench_bc(args) -> dict:
     dt, bc, timing, acct = _measure(lambda: R.bc(src), args.steps, args.warmup)
     R.close()
     m = g.m_undirected
-    roof = _roofline(timing, acct, "k_bc_forward_items + k_bc_backward_items (per level)")
+    roof = _roofline(timing, acct, "k_bc_forward_items + k_bc_backward_items (per level)", "k_bc/bc_rmat18")
     roof["launches_note"] = "one event pair per forward level and one around the backward sweep"
     out = {"value": m * len(src) / dt, "unit": "edges*sources/s", "ms_per_step": dt * 1e3, "dtype": "f64",
            "config": {"workload": "batched BC, 256 sources, R-MAT scale 18", "n": g.n, "m_undirected": m,
