@@ -36,20 +36,26 @@
 
 namespace ga {
 
-constexpr uint16_t BC_UNVISITED = 0xFFFF;
-constexpr int BC_MAX_LEVELS = 0xFFFE;
+// Depths are stored in DT = uint8_t (one byte per (vertex, lane): half the
+// bytes of the dominant gathers) and the pass is redone with uint16_t in the
+// rare case of a BFS deeper than 253 levels.  The all-ones value marks an
+// unvisited pair.
+template <typename DT>
+__host__ __device__ constexpr DT bc_unvisited() { return (DT)~DT(0); }
 
-__global__ void k_bc_init(uint16_t* __restrict__ depth, double* __restrict__ sigma, int64_t total) {
+template <typename DT>
+__global__ void k_bc_init(DT* __restrict__ depth, double* __restrict__ sigma, int64_t total) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    depth[i] = BC_UNVISITED;
+    depth[i] = bc_unvisited<DT>();
     sigma[i] = 0.0;
   }
 }
 
 // source j sits on lane lane[j]; its vertex is flagged for the level-0 list
 // (a vertex that is a source on several lanes is listed once)
+template <typename DT>
 __global__ void k_bc_sources(const int32_t* __restrict__ src, const int32_t* __restrict__ lane, int ns, int nsp,
-                             uint16_t* __restrict__ depth, double* __restrict__ sigma, uint8_t* __restrict__ flag) {
+                             DT* __restrict__ depth, double* __restrict__ sigma, uint8_t* __restrict__ flag) {
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ns; j += gridDim.x * blockDim.x) {
     const int64_t idx = (int64_t)src[j] * nsp + lane[j];
     depth[idx] = 0;
@@ -82,19 +88,21 @@ __global__ void k_flag_emit(uint8_t* __restrict__ flag, int64_t n, const int32_t
 }
 
 // flag[v] = 1 when some lane of vertex v is at depth k
-__global__ __launch_bounds__(256) void k_level_flags(const uint16_t* __restrict__ depth, int64_t n, int nsp, int k,
+template <typename DT>
+__global__ __launch_bounds__(256) void k_level_flags(const DT* __restrict__ depth, int64_t n, int nsp, int k,
                                                      uint8_t* __restrict__ flag) {
   const int lane = threadIdx.x & 63;
   const int64_t wpb = blockDim.x >> 6;
   for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += (int64_t)gridDim.x * wpb) {
     bool hit = false;
-    for (int c = lane; c < nsp; c += 64) hit |= depth[v * nsp + c] == (uint16_t)k;
+    for (int c = lane; c < nsp; c += 64) hit |= depth[v * nsp + c] == (DT)k;
     if (__ballot(hit) != 0ull && lane == 0) flag[v] = 1;
   }
 }
 
+template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_last(const int32_t* __restrict__ front, int nfront, int nsp, int k,
-                                                 const uint16_t* __restrict__ depth,
+                                                 const DT* __restrict__ depth,
                                                  const double* __restrict__ sigma, double* __restrict__ q) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
@@ -102,7 +110,7 @@ __global__ __launch_bounds__(256) void k_bc_last(const int32_t* __restrict__ fro
     const int v = front[f];
     for (int c = 0; c < nsp; c += 64) {
       const int64_t vi = (int64_t)v * nsp + c + lane;
-      if (depth[vi] == (uint16_t)k) q[vi] = 1.0 / sigma[vi];
+      if (depth[vi] == (DT)k) q[vi] = 1.0 / sigma[vi];
     }
   }
 }
@@ -145,14 +153,15 @@ __global__ void k_bc_item_fill(const int32_t* __restrict__ front, int nf, const 
 
 // forward, level d -> d+1: one wavefront per item (vertex u, <= BC_CH
 // neighbours), lanes = 64 sources; BC_UNROLL neighbour states in flight
+template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_forward_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ n_items_ptr, int nsp, int d,
-    uint16_t* __restrict__ depth, double* __restrict__ sigma, uint8_t* __restrict__ next_flag) {
+    DT* __restrict__ depth, double* __restrict__ sigma, uint8_t* __restrict__ next_flag) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const int n_items = *n_items_ptr;
-  const uint16_t dd = (uint16_t)d, dn = (uint16_t)(d + 1);
+  const DT dd = (DT)d, dn = (DT)(d + 1);
   for (int q = blockIdx.x * wpb + (threadIdx.x >> 6); q < n_items; q += gridDim.x * wpb) {
     const int u = it_v[q];
     const int t0 = it_t0[q];
@@ -164,17 +173,17 @@ __global__ __launch_bounds__(256) void k_bc_forward_items(
       const double su = act ? sigma[ui] : 0.0;
       for (int t = t0; t < t1; t += BC_UNROLL) {
         int vv[BC_UNROLL];
-        uint16_t dv[BC_UNROLL];
+        DT dv[BC_UNROLL];
 #pragma unroll
         for (int k = 0; k < BC_UNROLL; ++k) vv[k] = (t + k < t1) ? col[t + k] : -1;
 #pragma unroll
         for (int k = 0; k < BC_UNROLL; ++k)
-          dv[k] = (act && vv[k] >= 0) ? depth[(int64_t)vv[k] * nsp + c + lane] : (uint16_t)0;
+          dv[k] = (act && vv[k] >= 0) ? depth[(int64_t)vv[k] * nsp + c + lane] : (DT)0;
 #pragma unroll
         for (int k = 0; k < BC_UNROLL; ++k) {
-          if (act && vv[k] >= 0 && (dv[k] == BC_UNVISITED || dv[k] == dn)) {
+          if (act && vv[k] >= 0 && (dv[k] == bc_unvisited<DT>() || dv[k] == dn)) {
             const int64_t vi = (int64_t)vv[k] * nsp + c + lane;
-            if (dv[k] == BC_UNVISITED) depth[vi] = dn;
+            if (dv[k] == bc_unvisited<DT>()) depth[vi] = dn;
             atomicAdd(&sigma[vi], su);
             next_flag[vv[k]] = 1;
           }
@@ -188,15 +197,16 @@ __global__ __launch_bounds__(256) void k_bc_forward_items(
 // within the item.  Single-item vertices finish delta, q and bc here; the
 // items of a hub vertex write per-chunk partial sums that k_bc_combine adds
 // in chunk order (a fixed order: results are reproducible run to run).
+template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_backward_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
-    int nsp, int k, const uint16_t* __restrict__ depth, const double* __restrict__ sigma, double* __restrict__ q,
+    int nsp, int k, const DT* __restrict__ depth, const double* __restrict__ sigma, double* __restrict__ q,
     double* __restrict__ bc, double* __restrict__ partial, int64_t bc_stride) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const int n_items = *n_items_ptr;
-  const uint16_t dk = (uint16_t)k, dn = (uint16_t)(k + 1);
+  const DT dk = (DT)k, dn = (DT)(k + 1);
   for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < n_items; it += gridDim.x * wpb) {
     const int v = it_v[it];
     const int t0 = it_t0[it];
@@ -210,12 +220,12 @@ __global__ __launch_bounds__(256) void k_bc_backward_items(
       double acc = 0.0;
       for (int t = t0; t < t1; t += BC_UNROLL) {
         int ww[BC_UNROLL];
-        uint16_t dw[BC_UNROLL];
+        DT dw[BC_UNROLL];
 #pragma unroll
         for (int j = 0; j < BC_UNROLL; ++j) ww[j] = (t + j < t1) ? col[t + j] : -1;
 #pragma unroll
         for (int j = 0; j < BC_UNROLL; ++j)
-          dw[j] = (act && ww[j] >= 0) ? depth[(int64_t)ww[j] * nsp + c + lane] : (uint16_t)0;
+          dw[j] = (act && ww[j] >= 0) ? depth[(int64_t)ww[j] * nsp + c + lane] : (DT)0;
 #pragma unroll
         for (int j = 0; j < BC_UNROLL; ++j)
           if (act && ww[j] >= 0 && dw[j] == dn) acc += q[(int64_t)ww[j] * nsp + c + lane];
@@ -243,16 +253,17 @@ __global__ __launch_bounds__(256) void k_bc_backward_items(
   }
 }
 
+template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_combine(const int32_t* __restrict__ front, int nf,
                                                     const int32_t* __restrict__ rp,
                                                     const int32_t* __restrict__ hstart, int nsp, int k,
-                                                    const uint16_t* __restrict__ depth,
+                                                    const DT* __restrict__ depth,
                                                     const double* __restrict__ sigma, double* __restrict__ q,
                                                     double* __restrict__ bc, const double* __restrict__ partial,
                                                     int64_t bc_stride) {
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
-  const uint16_t dk = (uint16_t)k;
+  const DT dk = (DT)k;
   for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nf; f += gridDim.x * wpb) {
     const int v = front[f];
     const int deg = rp[v + 1] - rp[v];
@@ -299,14 +310,15 @@ static void bc_items(const DCsr& A, const int32_t* front, int nf, BcWork& w, hip
 // Algorithmic accounting of one batch: reached (vertex, source) pairs P and
 // wedges W = sum over reached pairs of deg(v); bytes = 24 W + 12 P (forward
 // and backward: source index + FP64 sigma / W per wedge; sigma / delta per pair).
-__global__ void k_bc_account(const int32_t* __restrict__ rp, const uint16_t* __restrict__ depth, int64_t n, int nsp,
+template <typename DT>
+__global__ void k_bc_account(const int32_t* __restrict__ rp, const DT* __restrict__ depth, int64_t n, int nsp,
                              unsigned long long* __restrict__ prof) {
   const int lane = threadIdx.x & 63;
   const int64_t wpb = blockDim.x >> 6;
   unsigned long long wedges = 0, pairs = 0;
   for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += (int64_t)gridDim.x * wpb) {
     int cnt = 0;
-    for (int c = lane; c < nsp; c += 64) cnt += depth[v * nsp + c] != BC_UNVISITED;
+    for (int c = lane; c < nsp; c += 64) cnt += depth[v * nsp + c] != bc_unvisited<DT>();
     cnt = wave_reduce_sum(cnt);
     if (lane == 0) {
       wedges += (unsigned long long)cnt * (unsigned long long)(rp[v + 1] - rp[v]);
@@ -362,7 +374,7 @@ static BcSizes bc_sizes(int64_t n, int64_t nnz, int64_t ns, int64_t nsp, int64_t
 
 static void bc_reserve(BcWork& w, int64_t n, const BcSizes& z, int64_t ns) {
   const int64_t n1 = std::max<int64_t>(n, 1);
-  w.depth.ensure(sizeof(uint16_t) * z.total);
+  w.depth.ensure(sizeof(uint16_t) * z.total);  // room for either depth width
   w.sigma.ensure(sizeof(double) * z.total);
   w.q.ensure(sizeof(double) * z.total);
   w.bc.ensure(sizeof(double) * n1 * z.outs);
@@ -410,8 +422,14 @@ size_t bc_groups_workspace_bytes(int64_t n, int64_t nnz, const std::vector<int64
 // Runs the multi-source pass.  Source j is vertex src_h[j] on lane lane_h[j]
 // (distinct lanes; nsp lanes in all).  outs == 1: bc_h[v] sums every lane;
 // outs == nsp / 64: lane group g (lanes 64g .. 64g+63) sums into bc_h[g*n + v].
-static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp, int64_t outs,
+template <typename DT>
+static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp, int64_t outs,
                    double* bc_h, BcWork& w, hipStream_t s);
+static void bc_run_any(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp,
+                       int64_t outs, double* bc_h, BcWork& w, hipStream_t s) {
+  if (!bc_run<uint8_t>(A, src_h, lane_h, ns, nsp, outs, bc_h, w, s))
+    (void)bc_run<uint16_t>(A, src_h, lane_h, ns, nsp, outs, bc_h, w, s);
+}
 
 // A: device pattern (out-edges); src: host array of distinct sources, on
 // lanes 0..ns-1.
@@ -427,7 +445,7 @@ void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h,
     seen[src_h[j]] = 1;
     lane[j] = (int32_t)j;
   }
-  bc_run(A, src_h, lane.data(), ns, ceil_div(ns, 64) * 64, 1, bc_h, w, s);
+  bc_run_any(A, src_h, lane.data(), ns, ceil_div(ns, 64) * 64, 1, bc_h, w, s);
 }
 
 // Several independent source sets in one pass: set g (at most 64 distinct
@@ -454,12 +472,14 @@ void bc_groups_dev(const DCsr& A, const std::vector<std::vector<int32_t>>& sets,
     }
   }
   std::vector<double> bc((size_t)(G * std::max<int64_t>(n, 1)));
-  bc_run(A, src.data(), lane.data(), (int64_t)src.size(), 64 * G, G, bc.data(), w, s);
+  bc_run_any(A, src.data(), lane.data(), (int64_t)src.size(), 64 * G, G, bc.data(), w, s);
   for (int64_t g = 0; g < G; ++g)
     if (n) std::memcpy(bc_out[g], bc.data() + g * n, sizeof(double) * n);
 }
 
-static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp_, int64_t outs,
+// false: the BFS needs more levels than DT can hold (the pass is abandoned)
+template <typename DT>
+static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp_, int64_t outs,
                    double* bc_h, BcWork& w, hipStream_t s) {
   const int64_t n = A.nrows;
   const BcSizes z = bc_sizes(n, A.nnz, ns, nsp_, outs);
@@ -469,12 +489,12 @@ static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
   bc_reserve(w, n, z, ns);
   GA_HIP(hipMemcpyAsync(w.src.p, src_h, sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
   GA_HIP(hipMemcpyAsync(w.src.as<int32_t>() + ns, lane_h, sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_bc_init, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, w.depth.as<uint16_t>(),
+  hipLaunchKernelGGL(k_bc_init<DT>, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, w.depth.as<DT>(),
                      w.sigma.as<double>(), total);
   fill_u64(w.bc.p, 0ull, n * outs, s);
   fill_u32(w.flag.p, 0u, ceil_div(std::max<int64_t>(n, 1), 4), s);
-  hipLaunchKernelGGL(k_bc_sources, dim3(grid_for(ns, 256)), dim3(256), 0, s, (const int32_t*)w.src.as<int32_t>(),
-                     (const int32_t*)(w.src.as<int32_t>() + ns), (int)ns, nsp, w.depth.as<uint16_t>(),
+  hipLaunchKernelGGL(k_bc_sources<DT>, dim3(grid_for(ns, 256)), dim3(256), 0, s, (const int32_t*)w.src.as<int32_t>(),
+                     (const int32_t*)(w.src.as<int32_t>() + ns), (int)ns, nsp, w.depth.as<DT>(),
                      w.sigma.as<double>(), w.flag.as<uint8_t>());
   GA_HIP(hipGetLastError());
   int32_t* count_h = static_cast<int32_t*>(pinned_scratch());
@@ -486,7 +506,12 @@ static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
   int d = 0;
   Context& cx = ctx();
   while (true) {
-    GA_REQUIRE(d < BC_MAX_LEVELS - 1, GA_ERR_LIMIT, "BFS deeper than 65533 levels");
+    // level d + 1 must stay below the unvisited marker
+    if (d + 1 >= (int)bc_unvisited<DT>()) {
+      GA_REQUIRE(sizeof(DT) < sizeof(uint16_t), GA_ERR_LIMIT, "BFS deeper than 65533 levels");
+      GA_HIP(hipStreamSynchronize(s));
+      return false;
+    }
     const int32_t* front = lists[d & 1];
     hipEvent_t fe0 = nullptr, fe1 = nullptr;
     if (cx.prof_events_on) {
@@ -495,9 +520,9 @@ static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
       GA_HIP(hipEventRecord(fe0, s));
     }
     bc_items(A, front, nf, w, s);
-    hipLaunchKernelGGL(k_bc_forward_items, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s, A.rp,
+    hipLaunchKernelGGL(k_bc_forward_items<DT>, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s, A.rp,
                        A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
-                       (const int32_t*)(w.istart.as<int32_t>() + nf), nsp, d, w.depth.as<uint16_t>(),
+                       (const int32_t*)(w.istart.as<int32_t>() + nf), nsp, d, w.depth.as<DT>(),
                        w.sigma.as<double>(), w.flag.as<uint8_t>());
     GA_HIP(hipGetLastError());
     if (cx.prof_events_on) {
@@ -514,8 +539,8 @@ static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
   // d = deepest level that holds vertices; its list is lists[d & 1] (nf entries)
   const int dmax = d;
   if (cx.prof_bytes) {
-    hipLaunchKernelGGL(k_bc_account, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
-                       (const uint16_t*)w.depth.as<uint16_t>(), n, nsp, cx.prof_dev.as<unsigned long long>());
+    hipLaunchKernelGGL(k_bc_account<DT>, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
+                       (const DT*)w.depth.as<DT>(), n, nsp, cx.prof_dev.as<unsigned long long>());
     cx.prof_bytes_host += (int64_t)(dmax + 1) * (4 * (n + 1) + 4 * A.nnz);
     cx.prof_bytes_launches += 2 * dmax;
   }
@@ -526,26 +551,26 @@ static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
     GA_HIP(hipEventRecord(be0, s));
   }
   if (dmax >= 1) {
-    hipLaunchKernelGGL(k_bc_last, dim3(grid_for(nf, 4, 65536)), dim3(256), 0, s, (const int32_t*)lists[dmax & 1], nf,
-                       nsp, dmax, (const uint16_t*)w.depth.as<uint16_t>(), (const double*)w.sigma.as<double>(),
+    hipLaunchKernelGGL(k_bc_last<DT>, dim3(grid_for(nf, 4, 65536)), dim3(256), 0, s, (const int32_t*)lists[dmax & 1], nf,
+                       nsp, dmax, (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(),
                        w.q.as<double>());
     GA_HIP(hipGetLastError());
     for (int k = dmax - 1; k >= 1; --k) {
       // the level-k list again, from the depths (vertices with a lane at depth k)
-      hipLaunchKernelGGL(k_level_flags, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s,
-                         (const uint16_t*)w.depth.as<uint16_t>(), n, nsp, k, w.flag.as<uint8_t>());
+      hipLaunchKernelGGL(k_level_flags<DT>, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s,
+                         (const DT*)w.depth.as<DT>(), n, nsp, k, w.flag.as<uint8_t>());
       int32_t* front = lists[0];
       frontier_from_flags(w.flag.as<uint8_t>(), n, w, front, count_h, s);
       const int32_t nk = *count_h;
       if (nk == 0) continue;
       bc_items(A, front, nk, w, s);
-      hipLaunchKernelGGL(k_bc_backward_items, dim3(grid_for(nk + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_bc_backward_items<DT>, dim3(grid_for(nk + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s,
                          A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
                          (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nk), nsp,
-                         k, (const uint16_t*)w.depth.as<uint16_t>(), (const double*)w.sigma.as<double>(),
+                         k, (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(),
                          w.q.as<double>(), w.bc.as<double>(), w.partial.as<double>(), bc_stride);
-      hipLaunchKernelGGL(k_bc_combine, dim3(grid_for(nk, 4, 65536)), dim3(256), 0, s, (const int32_t*)front, nk, A.rp,
-                         (const int32_t*)w.hstart.as<int32_t>(), nsp, k, (const uint16_t*)w.depth.as<uint16_t>(),
+      hipLaunchKernelGGL(k_bc_combine<DT>, dim3(grid_for(nk, 4, 65536)), dim3(256), 0, s, (const int32_t*)front, nk, A.rp,
+                         (const int32_t*)w.hstart.as<int32_t>(), nsp, k, (const DT*)w.depth.as<DT>(),
                          (const double*)w.sigma.as<double>(), w.q.as<double>(), w.bc.as<double>(),
                          (const double*)w.partial.as<double>(), bc_stride);
       GA_HIP(hipGetLastError());
@@ -558,6 +583,7 @@ static void bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
   }
   if (n) GA_HIP(hipMemcpyAsync(bc_h, w.bc.p, sizeof(double) * n * outs, hipMemcpyDeviceToHost, s));
   GA_HIP(hipStreamSynchronize(s));
+  return true;
 }
 
 }  // namespace ga

@@ -29,6 +29,22 @@ def test_bc_fixtures_vs_oracle(gpu, small_graphs, oracle, name):
         assert _bc_close(got, oracle.bc(g, src))
 
 
+def test_bc_deep_bfs_widens_depths(gpu, G):
+    """A path of 600 vertices has BFS levels beyond the 8-bit depth range: the
+    pass is redone with 16-bit depths and still meets the closed form."""
+    n = 600
+    path = G.CSR.from_edges(n, range(n - 1), range(1, n))
+    print("case bc path600", flush=True)
+    src = np.array([0, n // 3, n - 1], dtype=np.int32)
+    bc = gpu.betweenness_centrality_batch(path, src)
+    v = np.arange(n, dtype=np.float64)
+    # each source s adds (number of vertices beyond v, seen from s) for v strictly between
+    want = np.zeros(n)
+    for s_ in src:
+        want += np.where(v > s_, n - 1 - v, 0) + np.where(v < s_, v, 0)
+    assert np.allclose(bc, want, rtol=1e-12)
+
+
 def test_bc_closed_forms(gpu, G):
     n = 70  # more sources than one 64-lane chunk
     star = G.CSR.from_edges(n, [0] * (n - 1), range(1, n))
