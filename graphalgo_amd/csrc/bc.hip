@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_bc_last(const int32_t* __restrict__ fro
 // hub adjacency lists are spread over many wavefronts; every vertex has at
 // least one item (a backward vertex without successors still needs q).
 constexpr int BC_CH = 256;
-constexpr int BC_UNROLL = 4;
+constexpr int BC_UNROLL = 8;
 constexpr int BC_FLAG_CHUNK = 64;  // vertices per thread when flags become a list
 
 __global__ void k_bc_item_count(const int32_t* __restrict__ front, int nf, const int32_t* __restrict__ rp,
@@ -152,83 +152,100 @@ __global__ void k_bc_item_fill(const int32_t* __restrict__ front, int nf, const 
 }
 
 // forward, level d -> d+1: one wavefront per item (vertex u, <= BC_CH
-// neighbours), lanes = 64 sources; BC_UNROLL neighbour states in flight
+// neighbours), lanes = 64 sources.  The item's neighbour ids are staged in
+// LDS once (one coalesced load) and reused for every 64-lane chunk; BC_UNROLL
+// neighbour depths are in flight per lane.
 template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_forward_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ n_items_ptr, int nsp, int d,
     DT* __restrict__ depth, double* __restrict__ sigma, uint8_t* __restrict__ next_flag) {
+  __shared__ int32_t s_nb[4][BC_CH];
   const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
   const int n_items = *n_items_ptr;
   const DT dd = (DT)d, dn = (DT)(d + 1);
-  for (int q = blockIdx.x * wpb + (threadIdx.x >> 6); q < n_items; q += gridDim.x * wpb) {
+  int32_t* nb = s_nb[wv];
+  for (int q = blockIdx.x * wpb + wv; q < n_items; q += gridDim.x * wpb) {
     const int u = it_v[q];
     const int t0 = it_t0[q];
-    const int t1 = min(t0 + BC_CH, rp[u + 1]);
+    const int cnt = min(BC_CH, rp[u + 1] - t0);
+    for (int t = lane; t < cnt; t += 64) nb[t] = col[t0 + t];
+    __threadfence_block();
     for (int c = 0; c < nsp; c += 64) {
       const int64_t ui = (int64_t)u * nsp + c + lane;
       const bool act = depth[ui] == dd;
       if (__ballot(act) == 0ull) continue;
       const double su = act ? sigma[ui] : 0.0;
-      for (int t = t0; t < t1; t += BC_UNROLL) {
+      for (int t = 0; t < cnt; t += BC_UNROLL) {
         int vv[BC_UNROLL];
         DT dv[BC_UNROLL];
 #pragma unroll
-        for (int k = 0; k < BC_UNROLL; ++k) vv[k] = (t + k < t1) ? col[t + k] : -1;
+        for (int j = 0; j < BC_UNROLL; ++j) vv[j] = (t + j < cnt) ? nb[t + j] : -1;
 #pragma unroll
-        for (int k = 0; k < BC_UNROLL; ++k)
-          dv[k] = (act && vv[k] >= 0) ? depth[(int64_t)vv[k] * nsp + c + lane] : (DT)0;
+        for (int j = 0; j < BC_UNROLL; ++j)
+          dv[j] = (act && vv[j] >= 0) ? depth[(int64_t)vv[j] * nsp + c + lane] : (DT)0;
 #pragma unroll
-        for (int k = 0; k < BC_UNROLL; ++k) {
-          if (act && vv[k] >= 0 && (dv[k] == bc_unvisited<DT>() || dv[k] == dn)) {
-            const int64_t vi = (int64_t)vv[k] * nsp + c + lane;
-            if (dv[k] == bc_unvisited<DT>()) depth[vi] = dn;
+        for (int j = 0; j < BC_UNROLL; ++j) {
+          if (act && vv[j] >= 0 && (dv[j] == bc_unvisited<DT>() || dv[j] == dn)) {
+            const int64_t vi = (int64_t)vv[j] * nsp + c + lane;
+            if (dv[j] == bc_unvisited<DT>()) depth[vi] = dn;
             atomicAdd(&sigma[vi], su);
-            next_flag[vv[k]] = 1;
+            next_flag[vv[j]] = 1;
           }
         }
       }
     }
+    __threadfence_block();  // the neighbour slots are rewritten by the next item
   }
 }
 
 // backward, level k: one wavefront per item; successors summed in CSR order
-// within the item.  Single-item vertices finish delta, q and bc here; the
-// items of a hub vertex write per-chunk partial sums that k_bc_combine adds
-// in chunk order (a fixed order: results are reproducible run to run).
+// within the item (a non-successor adds +0.0, which leaves the sum unchanged).
+// Single-item vertices finish delta, q and bc here; the items of a hub vertex
+// write per-chunk partial sums that k_bc_combine adds in chunk order (a fixed
+// order: results are reproducible run to run).
 template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_backward_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
     int nsp, int k, const DT* __restrict__ depth, const double* __restrict__ sigma, double* __restrict__ q,
     double* __restrict__ bc, double* __restrict__ partial, int64_t bc_stride) {
+  __shared__ int32_t s_nb[4][BC_CH];
   const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
   const int n_items = *n_items_ptr;
   const DT dk = (DT)k, dn = (DT)(k + 1);
-  for (int it = blockIdx.x * wpb + (threadIdx.x >> 6); it < n_items; it += gridDim.x * wpb) {
+  int32_t* nb = s_nb[wv];
+  for (int it = blockIdx.x * wpb + wv; it < n_items; it += gridDim.x * wpb) {
     const int v = it_v[it];
     const int t0 = it_t0[it];
-    const int t1 = min(t0 + BC_CH, rp[v + 1]);
+    const int cnt = min(BC_CH, rp[v + 1] - t0);
     const int slot = it_slot[it];
+    for (int t = lane; t < cnt; t += 64) nb[t] = col[t0 + t];
+    __threadfence_block();
     double part = 0.0;
     for (int c = 0; c < nsp; c += 64) {
       const int64_t vi = (int64_t)v * nsp + c + lane;
       const bool act = depth[vi] == dk;
       if (__ballot(act) == 0ull) continue;
       double acc = 0.0;
-      for (int t = t0; t < t1; t += BC_UNROLL) {
+      for (int t = 0; t < cnt; t += BC_UNROLL) {
         int ww[BC_UNROLL];
         DT dw[BC_UNROLL];
+        double qw[BC_UNROLL];
 #pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) ww[j] = (t + j < t1) ? col[t + j] : -1;
+        for (int j = 0; j < BC_UNROLL; ++j) ww[j] = (t + j < cnt) ? nb[t + j] : -1;
 #pragma unroll
         for (int j = 0; j < BC_UNROLL; ++j)
           dw[j] = (act && ww[j] >= 0) ? depth[(int64_t)ww[j] * nsp + c + lane] : (DT)0;
 #pragma unroll
         for (int j = 0; j < BC_UNROLL; ++j)
-          if (act && ww[j] >= 0 && dw[j] == dn) acc += q[(int64_t)ww[j] * nsp + c + lane];
+          qw[j] = (act && ww[j] >= 0 && dw[j] == dn) ? q[(int64_t)ww[j] * nsp + c + lane] : 0.0;
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) acc += qw[j];
       }
       if (slot >= 0) {
         if (act) partial[(int64_t)slot * nsp + c + lane] = acc;
@@ -250,6 +267,7 @@ __global__ __launch_bounds__(256) void k_bc_backward_items(
       part = wave_reduce_sum(part);
       if (lane == 0) bc[v] += part;
     }
+    __threadfence_block();  // the neighbour slots are rewritten by the next item
   }
 }
 
