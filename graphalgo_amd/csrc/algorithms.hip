@@ -104,7 +104,7 @@ __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* 
         ncol[r] = col[j];
         neid[r] = eid[j];
         nold[r] = j;
-        nsup[r] = sup[j];  // carried over; the next pass recomputes the affected ones
+        if (nsup) nsup[r] = sup[j];  // carried over; the next pass recomputes the affected ones
       }
       if (j < e) newpos[j] = keep ? r : -1;
       dst += __popcll(m);
@@ -220,7 +220,8 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       hipLaunchKernelGGL(k_filter_scatter, dim3(wgrid), dim3(256), 0, s, C.rp, C.col,
                          (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, n, thr,
                          (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>(),
-                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(), tw.sup[cur ^ 1].as<int32_t>());
+                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(),
+                         ctx().selective_support ? tw.sup[cur ^ 1].as<int32_t>() : nullptr);
       hipLaunchKernelGGL(k_mirror_update, dim3(grid_for(kept, 256, 8192)), dim3(256), 0, s,
                          (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
                          (const int32_t*)tw.newpos.as<int32_t>(), kept, tw.mirror[cur ^ 1].as<int32_t>());

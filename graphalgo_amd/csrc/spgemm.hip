@@ -122,8 +122,12 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
       s_item = item;
     }
     __syncthreads();
-    const int q = s_item;
-    if (q >= n_items) break;
+    if (s_item >= n_items) break;
+    // listed items are taken from the last row to the first: generators and
+    // most graph files label hubs first, so the rows whose lower-half cut is
+    // loosest (the heaviest items of the support pass) come last in row order
+    // and would otherwise form the tail of the launch (4% on R-MAT 16)
+    const int q = direct_rows ? s_item : n_items - 1 - s_item;
 
     const bool listed = q < n_list;
     const int i = listed ? item_row[q] : q - n_list;
