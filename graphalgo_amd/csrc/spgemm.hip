@@ -95,6 +95,25 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
   const int seg_len = (n_items + MXM_NSEG - 1) / MXM_NSEG;
   int seg = blockIdx.x % MXM_NSEG, tries = 0;
   int next = 0, end = 0;  // thread 0's claimed range
+  // sum mode: per-thread partial sums run on across the items of one target
+  // (one graph of a batch, or the single total) and are reduced and added
+  // with one atomic when the target changes and at exit, not once per item
+  long long lsum = 0;
+  int cur_g = -1;
+  auto flush_sum = [&]() {
+    const long long r = wave_reduce_sum(lsum);
+    __syncthreads();
+    if (lane == 0) s_wsum[wv] = r;
+    __syncthreads();
+    if (tid == 0) {
+      long long t = 0;
+#pragma unroll
+      for (int k = 0; k < MXM_THREADS / 64; ++k) t += s_wsum[k];
+      if (t) atomicAdd(&sums[cur_g], (unsigned long long)t);
+    }
+    __syncthreads();  // s_wsum is reused by the next item's entry scan
+    lsum = 0;
+  };
 
   while (true) {
     __syncthreads();  // previous item fully retired (LDS reuse)
@@ -122,7 +141,10 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
       s_item = item;
     }
     __syncthreads();
-    if (s_item >= n_items) break;
+    if (s_item >= n_items) {
+      if (MODE == 1 && cur_g >= 0) flush_sum();
+      break;
+    }
     // listed items are taken from the last row to the first: generators and
     // most graph files label hubs first, so the rows whose lower-half cut is
     // loosest (the heaviest items of the support pass) come last in row order
@@ -141,6 +163,11 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     const bool local = mnz <= MXM_MCAP;
     const int bbase = BATCHED ? gbase[row_group[i]] : 0;
     if (a_end == a_beg || mnz <= 0) continue;  // direct items include empty rows
+    if (MODE == 1) {
+      const int g_item = row_group ? row_group[i] : 0;  // uniform across the workgroup
+      if (cur_g >= 0 && g_item != cur_g) flush_sum();
+      cur_g = g_item;
+    }
 
     // stage the mask row: bitmap bits (and the per-word rank base, or the
     // sorted column list for the binary-search variant)
@@ -200,7 +227,6 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     // MXM_G apart, so its entry advances by a short linear walk; the entry of
     // the first wedge of a trip is found by one binary search (from the
     // lane's current entry) per MXM_K wedges.
-    long long lsum = 0;
     const int grp = tid / MXM_G, rl = tid % MXM_G;
     // the loop is instantiated twice: mask row staged in LDS (sorted list and
     // LDS accumulator) or not (search and accumulate in global memory), so
@@ -325,17 +351,6 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
           }
         }
       }
-    } else {
-      const long long r = wave_reduce_sum(lsum);
-      __syncthreads();
-      if (lane == 0) s_wsum[wv] = r;
-      __syncthreads();
-      if (tid == 0) {
-        long long t = 0;
-#pragma unroll
-        for (int k = 0; k < MXM_THREADS / 64; ++k) t += s_wsum[k];
-        if (t) atomicAdd(&sums[row_group ? row_group[i] : 0], (unsigned long long)t);
-      }
     }
     if (use_bm) {
       for (int p = tid; p < mnz; p += MXM_THREADS) {
@@ -393,6 +408,8 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
   const int seg_len = (n_rows + MXM_NSEG - 1) / MXM_NSEG;
   int seg = blockIdx.x % MXM_NSEG, tries = 0;
   int next = 0, end = 0;  // lane 0's claimed range
+  long long wave_sum = 0;  // sum mode: this wave's running total for graph wave_g
+  int wave_g = 0;
   while (true) {
     int r = n_rows;
     if (lane == 0) {
@@ -417,7 +434,10 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
       }
     }
     r = __shfl(r, 0, 64);
-    if (r >= n_rows) break;
+    if (r >= n_rows) {
+      if (MODE == 1 && lane == 0 && wave_sum) atomicAdd(&sums[wave_g], (unsigned long long)wave_sum);
+      break;
+    }
     const int i = rows[r];
     const int ms = Mrp[i];
     const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // <= WR_MCAP by binning
@@ -524,8 +544,16 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
         }
       }
     } else {
+      // the wave's total runs on while its rows belong to one graph and is
+      // added once per run of rows (rows are claimed in chunks, in order)
       const long long tot = wave_reduce_sum(lsum);
-      if (lane == 0 && tot) atomicAdd(&sums[row_group ? row_group[i] : 0], (unsigned long long)tot);
+      const int g_row = row_group ? row_group[i] : 0;
+      if (g_row != wave_g) {
+        if (lane == 0 && wave_sum) atomicAdd(&sums[wave_g], (unsigned long long)wave_sum);
+        wave_sum = 0;
+        wave_g = g_row;
+      }
+      wave_sum += tot;
     }
     for (int p = lane; p < mnz; p += 64) bm[Mcol[ms + p] >> 5] = 0u;
     __threadfence_block();
