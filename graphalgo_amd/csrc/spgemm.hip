@@ -45,7 +45,11 @@ constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up t
 // atomics, and fewer, larger items pay the per-item staging less often.
 constexpr int MXM_WIDE = 512;
 constexpr int MXM_NSEG = 32;                                 // claim segments (one counter line each)
-constexpr int MXM_CTR_WORDS = 2 * MXM_NSEG * 32 + 32;         // item segments, row segments, cursor
+#ifndef GA_MXM_CTR_STRIDE
+#define GA_MXM_CTR_STRIDE 32
+#endif
+constexpr int MXM_CTR_STRIDE = GA_MXM_CTR_STRIDE;            // u32 words between segment counters
+constexpr int MXM_CTR_WORDS = 2 * MXM_NSEG * MXM_CTR_STRIDE + 32;  // item segments, row segments, cursor
 
 // THREADS = workgroup size = A entries per item (one entry per thread in the
 // entry scan); MCAP = mask entries staged in LDS.  BATCHED: the operands are
@@ -93,7 +97,9 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
   // on one address.  A workgroup drains its home segment, then moves on to
   // the next ones (work stealing); `chunk` items are claimed per atomic.
   const int seg_len = (n_items + MXM_NSEG - 1) / MXM_NSEG;
-  int seg = blockIdx.x % MXM_NSEG, tries = 0;
+  int seg = blockIdx.x % MXM_NSEG;
+  bool drained = false;   // wavefront 0: no segment has items left
+  int tries_seq = 0;      // sum mode: segments found drained in order
   int next = 0, end = 0;  // thread 0's claimed range
   // sum mode: per-thread partial sums run on across the items of one target
   // (one graph of a batch, or the single total) and are reduced and added
@@ -117,28 +123,94 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
 
   while (true) {
     __syncthreads();  // previous item fully retired (LDS reuse)
-    if (tid == 0) {
+    if (MODE == 1) {
+      // sum mode (many short items): thread 0 walks the segments in order
+      if (tid == 0) {
+        int item = n_items;
+        if (next < end) {
+          item = next++;
+        } else {
+          while (!drained) {
+            const int lo = seg * seg_len;
+            const int hi = min(lo + seg_len, n_items);
+            if (lo < hi) {
+              const int k = (int)atomicAdd(&work_counter[seg * MXM_CTR_STRIDE], (unsigned)chunk);
+              if (lo + k < hi) {
+                item = lo + k;
+                next = item + 1;
+                end = min(lo + k + chunk, hi);
+                break;
+              }
+            }
+            seg = (seg + 1) % MXM_NSEG;
+            if (++tries_seq >= MXM_NSEG) drained = true;
+          }
+        }
+        s_item = item;
+      }
+    } else if (wv == 0) {
+      // value mode (the support pass): wavefront 0 claims, the current
+      // segment first (one atomic); when it is drained, the 32 segment
+      // counters are read at once (one lane each) and an atomic goes to the
+      // next segment with items left, so the end of the launch costs two
+      // round trips per claim, not one per segment (4% on the support pass;
+      // the sum-mode triangle count measured slower with it, so it keeps
+      // the ordered walk above)
       int item = n_items;
-      if (next < end) {
-        item = next++;
-      } else {
-        while (tries < MXM_NSEG) {
+      int have = 0;
+      if (lane == 0) {
+        if (next < end) {
+          item = next++;
+          have = 1;
+        } else if (!drained) {
           const int lo = seg * seg_len;
           const int hi = min(lo + seg_len, n_items);
           if (lo < hi) {
-            const int k = (int)atomicAdd(&work_counter[seg * 32], (unsigned)chunk);
+            const int k = (int)atomicAdd(&work_counter[seg * MXM_CTR_STRIDE], (unsigned)chunk);
             if (lo + k < hi) {
               item = lo + k;
               next = item + 1;
               end = min(lo + k + chunk, hi);
-              break;
+              have = 1;
             }
           }
-          seg = (seg + 1) % MXM_NSEG;
-          ++tries;
         }
       }
-      s_item = item;
+      have = __shfl(have, 0, 64);
+      while (!have && !drained) {
+        const int sc = __shfl(seg, 0, 64);
+        bool avail = false;
+        if (lane < MXM_NSEG) {
+          const int sj = (sc + 1 + lane) % MXM_NSEG;
+          const int lo = sj * seg_len;
+          const int hi = min(lo + seg_len, n_items);
+          if (lo < hi) {
+            const unsigned taken =
+                __hip_atomic_load(&work_counter[sj * MXM_CTR_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            avail = lo + (long long)taken < hi;
+          }
+        }
+        const unsigned long long m = __ballot(avail);
+        if (m == 0ull) {
+          drained = true;  // every segment handed out (wave-uniform)
+          break;
+        }
+        const int sj = (sc + 1 + (__ffsll((long long)m) - 1)) % MXM_NSEG;
+        if (lane == 0) {
+          seg = sj;
+          const int lo = sj * seg_len;
+          const int hi = min(lo + seg_len, n_items);
+          const int k = (int)atomicAdd(&work_counter[sj * MXM_CTR_STRIDE], (unsigned)chunk);
+          if (lo + k < hi) {
+            item = lo + k;
+            next = item + 1;
+            end = min(lo + k + chunk, hi);
+            have = 1;
+          }
+        }
+        have = __shfl(have, 0, 64);
+      }
+      if (lane == 0) s_item = item;
     }
     __syncthreads();
     if (s_item >= n_items) {
@@ -420,7 +492,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
           const int lo = seg * seg_len;
           const int hi = min(lo + seg_len, n_rows);
           if (lo < hi) {
-            const int k = (int)atomicAdd(&counter[seg * 32], (unsigned)chunk);
+            const int k = (int)atomicAdd(&counter[seg * MXM_CTR_STRIDE], (unsigned)chunk);
             if (lo + k < hi) {
               r = lo + k;
               next = r + 1;
@@ -684,7 +756,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // only the extra chunks of long rows are listed -- two small launches
   // instead of binning, two scans and two compactions
   const bool direct = wave_flops < 0 && ctx().direct_items;
-  int32_t* n_extra = reinterpret_cast<int32_t*>(w.counter.as<unsigned int>() + 2 * MXM_NSEG * 32 + 1);
+  int32_t* n_extra = reinterpret_cast<int32_t*>(w.counter.as<unsigned int>() + 2 * MXM_NSEG * MXM_CTR_STRIDE + 1);
   if (direct) {
     hipLaunchKernelGGL(k_extra_items, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
                        w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(), n_extra);
@@ -746,7 +818,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     hipLaunchKernelGGL(krows, dim3((unsigned)grid_r), dim3(64 * WR_WAVES), 0, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col, Mend,
                      static_cast<T*>(Cval), sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
-                     (const int32_t*)(w.rstart.as<int32_t>() + m), w.counter.as<unsigned int>() + MXM_NSEG * 32, wc,
+                     (const int32_t*)(w.rstart.as<int32_t>() + m), w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc,
                      trunc, chunk_r, mirror, opos);
   GA_HIP(hipGetLastError());
   hipLaunchKernelGGL(kitems, dim3((unsigned)grid_i), dim3(THREADS), dyn, s, A.rp, A.col,
