@@ -158,6 +158,8 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   tw.cnt.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
   tw.dirty.ensure(std::max<int64_t>(n, 1));
   tw.kept.ensure(64);
+  tw.mw.balance = true;  // the support passes split rows by the previous pass's work
+  tw.mw.rwork_have = false;
   GA_HIP(hipMemcpyAsync(tw.cur[0].rp.p, A.rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s));
   if (nnz0) GA_HIP(hipMemcpyAsync(tw.cur[0].col.p, A.col, sizeof(int32_t) * nnz0, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz0, 256, 8192)), dim3(256), 0, s, tw.eid[0].as<int32_t>(), nnz0);

@@ -146,6 +146,9 @@ struct Context {
   // set by per-item costs, not products, and the selection launches cost
   // more than the products save (510 vs 493 ms per decomposition).
   bool selective_support = false;
+  // target wedges per work item when rows are split by their measured work
+  // (GA_ITEM_WEDGES overrides it at ga_init)
+  long long item_wedges = 32768;
   // kernel timing (events around the masked-product launches) and
   // algorithmic byte accounting (extra counting kernels; kept out of timed runs)
   bool prof_events_on = false;
@@ -230,6 +233,14 @@ struct MxmWork {
   int threads = 256;             // workgroup width of the item kernel (64, 128 or 256)
   int64_t max_mask_row = INT32_MAX;  // longest mask row (host-known bound); < 65536 enables the rank lookup
   int64_t max_b_row = INT32_MAX;     // longest B row (host-known bound; B.nnz is used when tighter)
+  // Work balance over repeated products of one matrix (the k-truss support
+  // passes): each launch records the wedges of every row, and the next
+  // launch splits rows into items of about item_wedges wedges (at most
+  // MXM_SPLIT_MAX times the entry-based item count).
+  bool balance = false;
+  DevBuf rwork[2];
+  int rwork_cur = 0;
+  bool rwork_have = false;
 };
 
 // C<M> = A*B (structural mask).  mode 0: per-entry values into Cval (dtype);

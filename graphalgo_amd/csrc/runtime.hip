@@ -303,6 +303,7 @@ ga_status ga_init(int device) {
     if (const char* wf = std::getenv("GA_WAVE_FLOPS")) c->wave_flops = std::atoll(wf);
     if (const char* di = std::getenv("GA_DIRECT_ITEMS")) c->direct_items = std::atoi(di) != 0;
     if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT")) c->selective_support = std::atoi(ss) != 0;
+    if (const char* iw = std::getenv("GA_ITEM_WEDGES")) c->item_wedges = std::max(1LL, std::atoll(iw));
     c->prof_dev.ensure(64);
     GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
     GA_HIP(hipStreamSynchronize(c->stream));

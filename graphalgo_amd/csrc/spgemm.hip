@@ -44,6 +44,7 @@ constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up t
 // 512 / 4096: mask rows beyond the LDS capacity accumulate with global
 // atomics, and fewer, larger items pay the per-item staging less often.
 constexpr int MXM_WIDE = 512;
+constexpr int MXM_SPLIT_MAX = 8;  // a row gets at most this many times its entry-based item count
 constexpr int MXM_NSEG = 32;                                 // claim segments (one counter line each)
 #ifndef GA_MXM_CTR_STRIDE
 #define GA_MXM_CTR_STRIDE 32
@@ -64,7 +65,8 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ item_row, const int32_t* __restrict__ item_t0,
     const int32_t* __restrict__ n_items_ptr, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ wedge_counter, int bitmap_words, const int32_t* __restrict__ Mend,
-    int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror, const int32_t* __restrict__ opos) {
+    int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror, const int32_t* __restrict__ opos,
+    unsigned long long* __restrict__ row_work) {
   constexpr int MXM_THREADS = THREADS;
   constexpr int MXM_TCAP = THREADS;
   constexpr int MXM_MCAP = 8 * THREADS;  // mask rows up to this long accumulate in LDS
@@ -228,8 +230,11 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     const int a_beg = Arp[i];
     const int a_end = Arp[i + 1];
     const int t0 = listed ? item_t0[q] : a_beg;
-    const int t1 = min(t0 + MXM_TCAP, a_end);
-    const bool multi = (a_end - a_beg) > MXM_TCAP;
+    // a listed item ends where the next item of its row starts; a direct
+    // item (or the last of a row) takes up to MXM_TCAP entries
+    const int t1 = (listed && !direct_rows && q + 1 < n_list && item_row[q + 1] == i) ? item_t0[q + 1]
+                                                                                      : min(t0 + MXM_TCAP, a_end);
+    const bool multi = t0 > a_beg || t1 < a_end;  // the row is shared with other items
     const int ms = Mrp[i];
     const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // Mend: only a prefix of M's row is the mask
     const bool local = mnz <= MXM_MCAP;
@@ -289,7 +294,10 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
       s_eoff[tid] = off;
       s_jb[tid] = b0 - off;
     }
-    if (tid == 0) s_eoff[ne] = total;
+    if (tid == 0) {
+      s_eoff[ne] = total;
+      if (row_work) atomicAdd(&row_work[i], (unsigned long long)total);
+    }
     __syncthreads();
 
     // Wedge mapping: lanes form groups of MXM_G; group g of a trip owns the
@@ -672,12 +680,22 @@ __global__ void k_emit_rows(const int32_t* __restrict__ flag, const int32_t* __r
     if (flag[i]) rows[off[i]] = (int32_t)i;
 }
 
+// Items per row: ceil(entries / tcap), or -- when the previous launch's
+// wedges per row are known -- enough items of about `target` wedges each
+// (entries split evenly), capped at MXM_SPLIT_MAX times the entry-based count
+// and at one entry per item.
 __global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp,
-                             const int32_t* __restrict__ Mend, int64_t m, int tcap, int32_t* __restrict__ cnt) {
+                             const int32_t* __restrict__ Mend, int64_t m, int tcap, int32_t* __restrict__ cnt,
+                             const unsigned long long* __restrict__ work_prev, long long target) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     const int a = Arp[i + 1] - Arp[i];
     const int mz = (Mend ? Mend[i] : Mrp[i + 1]) - Mrp[i];
-    cnt[i] = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
+    int c = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
+    if (c > 0 && work_prev) {
+      const long long byw = (long long)((work_prev[i] + (unsigned long long)target - 1) / (unsigned long long)target);
+      c = (int)min<long long>(max<long long>(c, byw), min<long long>((long long)c * MXM_SPLIT_MAX, a));
+    }
+    cnt[i] = c;
   }
 }
 
@@ -700,14 +718,18 @@ __global__ void k_extra_items(const int32_t* __restrict__ Arp, const int32_t* __
   }
 }
 
+// Item q of a row with c items starts at entry a0 + floor(q * a / c): at most
+// ceil(a / c) <= tcap entries each; an item ends where the row's next one
+// starts (or at the row end).
 __global__ void k_item_fill(const int32_t* __restrict__ Arp, const int32_t* __restrict__ cnt,
-                            const int32_t* __restrict__ start, int64_t m, int tcap, int32_t* __restrict__ item_row,
+                            const int32_t* __restrict__ start, int64_t m, int32_t* __restrict__ item_row,
                             int32_t* __restrict__ item_t0) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
     const int c = cnt[i], s0 = start[i], a0 = Arp[i];
+    const int64_t a = Arp[i + 1] - a0;
     for (int q = 0; q < c; ++q) {
       item_row[s0 + q] = (int32_t)i;
-      item_t0[s0 + q] = a0 + q * tcap;
+      item_t0[s0 + q] = a0 + (int32_t)((int64_t)q * a / c);
     }
   }
 }
@@ -733,7 +755,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
                           const int32_t* opos = nullptr) {
   constexpr int TCAP = THREADS;
   const int64_t m = A.nrows;
-  const int64_t max_items = m + ceil_div(A.nnz, TCAP) + 1;
+  const int64_t max_items = (w.balance ? MXM_SPLIT_MAX : 1) * (m + ceil_div(A.nnz, TCAP)) + 1;
   // an item's wedge count (<= TCAP entries x longest B row) must fit int32
   GA_REQUIRE(std::min<int64_t>(w.max_b_row, B.nnz) * TCAP < INT32_MAX, GA_ERR_LIMIT,
              "B rows too long for 32-bit wedge indexing");
@@ -756,6 +778,19 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // only the extra chunks of long rows are listed -- two small launches
   // instead of binning, two scans and two compactions
   const bool direct = wave_flops < 0 && ctx().direct_items;
+  // work-balanced items (listed path of repeated products): the previous
+  // launch's wedges per row set this launch's split
+  unsigned long long* rw_cur = nullptr;
+  const unsigned long long* rw_prev = nullptr;
+  if (w.balance && wave_flops < 0 && !direct) {
+    w.rwork[0].ensure(sizeof(unsigned long long) * m);
+    w.rwork[1].ensure(sizeof(unsigned long long) * m);
+    rw_cur = w.rwork[w.rwork_cur].as<unsigned long long>();
+    rw_prev = w.rwork_have ? w.rwork[w.rwork_cur ^ 1].as<unsigned long long>() : nullptr;
+    fill_u64(rw_cur, 0ull, m, s);
+    w.rwork_cur ^= 1;
+    w.rwork_have = true;
+  }
   int32_t* n_extra = reinterpret_cast<int32_t*>(w.counter.as<unsigned int>() + 2 * MXM_NSEG * MXM_CTR_STRIDE + 1);
   if (direct) {
     hipLaunchKernelGGL(k_extra_items, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
@@ -764,11 +799,11 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   } else if (wave_flops < 0) {
     // no wavefront bin: items of every non-empty row, listed in row order
     hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
-                       w.cnt.as<int32_t>());
+                       w.cnt.as<int32_t>(), rw_prev, ctx().item_wedges);
     GA_HIP(hipGetLastError());
     scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
     hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
-                       (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, TCAP,
+                       (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m,
                        w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
     GA_HIP(hipGetLastError());
   } else {
@@ -782,7 +817,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
                        w.rows.as<int32_t>());
     scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
     hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
-                       (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m, TCAP,
+                       (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m,
                        w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
     GA_HIP(hipGetLastError());
   }
@@ -826,7 +861,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
                      static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
                      direct ? (const int32_t*)n_extra : (const int32_t*)(w.start.as<int32_t>() + m),
                      w.counter.as<unsigned int>(), wc, bitmap_words, Mend, trunc, chunk_i, direct ? (int)m : 0,
-                     mirror, opos);
+                     mirror, opos, rw_cur);
   GA_HIP(hipGetLastError());
   if (c.prof_events_on) {
     GA_HIP(hipEventRecord(e1, s));
@@ -936,12 +971,13 @@ void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s) {
 
 size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend) {
   // launch_binned's buffers at the 64-lane item width (the most items)
-  const int64_t max_items = m + ceil_div(nnz_a, 64) + 1;
+  // with_mend: the symmetric support pass, which splits rows by measured work
+  const int64_t max_items = (with_mend ? MXM_SPLIT_MAX : 1) * (m + ceil_div(nnz_a, 64)) + 1;
   size_t b = 3 * lease_bytes(sizeof(int32_t) * m) + 2 * lease_bytes(sizeof(int32_t) * (m + 1));
   b += 2 * lease_bytes(sizeof(int32_t) * max_items);
   b += lease_bytes(sizeof(uint32_t) * MXM_CTR_WORDS);
   b += scan_tmp_bytes(m);
-  if (with_mend) b += lease_bytes(sizeof(int32_t) * m);
+  if (with_mend) b += lease_bytes(sizeof(int32_t) * m) + 2 * lease_bytes(sizeof(unsigned long long) * m);
   return b;
 }
 
