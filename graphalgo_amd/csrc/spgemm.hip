@@ -644,6 +644,171 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
   }
 }
 
+// ---- short rows of pattern sum-mode products: 16 lanes per row ------------
+// The rows of a batch of small graphs (triangle counts) have a few dozen
+// entries and a few hundred products, so a row is a chain of dependent loads
+// (row bounds, mask, entries, B bounds, gathers) with little work.  Here a
+// 16-lane group owns a row and the four groups of a wavefront run four rows
+// at once, overlapping their chains.  Same per-row structure as k_mxm_rows:
+// an exact per-group bitmap + rank bases (<= WR_MAXCOLS columns), entries
+// in chunks of 16 with a 16-lane scan, 16-lane wedge runs with a short walk.
+constexpr int SR_SUB = 16;
+constexpr int SR_GROUPS = WR_WAVES * (64 / SR_SUB);  // groups per workgroup
+constexpr int SR_K = 8;                               // wedges per lane per trip
+
+template <bool BATCHED>
+__global__ __launch_bounds__(64 * WR_WAVES) void k_sum_subrows(
+    const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const int32_t* __restrict__ Brp,
+    const int32_t* __restrict__ Bcol, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol,
+    const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
+    const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
+    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
+  __shared__ uint32_t s_bm[SR_GROUPS][WR_BMW];
+  __shared__ uint16_t s_wpre[SR_GROUPS][WR_BMW];
+  __shared__ int32_t s_jb[SR_GROUPS][SR_SUB];
+  __shared__ int32_t s_eoff[SR_GROUPS][SR_SUB + 1];
+  const int sg = threadIdx.x / SR_SUB;  // group within the workgroup
+  const int sl = threadIdx.x % SR_SUB;  // lane within the group
+  uint32_t* bm = s_bm[sg];
+  uint16_t* wpre = s_wpre[sg];
+  int32_t* jbase = s_jb[sg];
+  int32_t* eoff = s_eoff[sg];
+  for (int q = sl; q < WR_BMW; q += SR_SUB) bm[q] = 0u;
+  const int n_rows = *n_rows_ptr;
+  // segmented claiming as in k_mxm_rows, by each group's lane 0
+  const int seg_len = (n_rows + MXM_NSEG - 1) / MXM_NSEG;
+  int seg = blockIdx.x % MXM_NSEG, tries = 0;
+  int next = 0, end = 0;
+  long long lsum = 0;  // this lane's hits for graph cur_g
+  int cur_g = -1;
+  unsigned long long wedges = 0;
+  auto flush = [&]() {  // group-uniform
+    long long t = lsum;
+#pragma unroll
+    for (int d = SR_SUB / 2; d >= 1; d >>= 1) t += __shfl_xor(t, d, SR_SUB);
+    if (sl == 0 && t) atomicAdd(&sums[cur_g], (unsigned long long)t);
+    lsum = 0;
+  };
+  while (true) {
+    int r = n_rows;
+    if (sl == 0) {
+      if (next < end) {
+        r = next++;
+      } else {
+        while (tries < MXM_NSEG) {
+          const int lo = seg * seg_len;
+          const int hi = min(lo + seg_len, n_rows);
+          if (lo < hi) {
+            const int k = (int)atomicAdd(&counter[seg * MXM_CTR_STRIDE], (unsigned)chunk);
+            if (lo + k < hi) {
+              r = lo + k;
+              next = r + 1;
+              end = min(lo + k + chunk, hi);
+              break;
+            }
+          }
+          seg = (seg + 1) % MXM_NSEG;
+          ++tries;
+        }
+      }
+    }
+    r = __shfl(r, 0, SR_SUB);
+    if (r >= n_rows) break;
+    const int i = rows[r];
+    const int ms = Mrp[i];
+    const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // <= WR_MCAP by binning
+    const int g_row = row_group ? row_group[i] : 0;
+    if (g_row != cur_g) {
+      if (cur_g >= 0) flush();
+      cur_g = g_row;
+    }
+    for (int p = sl; p < mnz; p += SR_SUB) {
+      const int c = Mcol[ms + p];
+      if (p == 0 || (Mcol[ms + p - 1] >> 5) != (c >> 5)) wpre[c >> 5] = (uint16_t)p;
+      atomicOr(&bm[c >> 5], 1u << (c & 31));
+    }
+    const int maxc = (trunc && mnz > 0) ? Mcol[ms + mnz - 1] : 0x7FFFFFFF;
+    const int bbase = BATCHED ? gbase[row_group[i]] : 0;
+    __threadfence_block();
+    const int a_end = Arp[i + 1];
+    for (int t0 = Arp[i]; t0 < a_end; t0 += SR_SUB) {
+      const int ne = min(SR_SUB, a_end - t0);
+      int len = 0, b0 = 0;
+      if (sl < ne) {
+        const int k = bbase + Acol[t0 + sl];
+        b0 = Brp[k];
+        int b1 = Brp[k + 1];
+        if (trunc) {
+          int lo = b0, hi = b1;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Bcol[mid] <= maxc) lo = mid + 1;
+            else hi = mid;
+          }
+          b1 = lo;
+        }
+        len = b1 - b0;
+      }
+      int incl = len;
+#pragma unroll
+      for (int d = 1; d < SR_SUB; d <<= 1) {
+        const int t = __shfl_up(incl, d, SR_SUB);
+        if (sl >= d) incl += t;
+      }
+      if (sl < ne) {
+        eoff[sl] = incl - len;
+        jbase[sl] = b0 - (incl - len);
+      }
+      const int total = __shfl(incl, SR_SUB - 1, SR_SUB);
+      if (sl == 0) {
+        eoff[ne] = total;
+        wedges += (unsigned long long)total;
+      }
+      __threadfence_block();
+      int e = 0;
+      for (int base = 0; base < total; base += SR_SUB * SR_K) {
+        const int p0 = base + sl;
+        if (p0 >= total) break;
+        {
+          int lo = e, hi = ne - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (eoff[mid] <= p0) lo = mid;
+            else hi = mid - 1;
+          }
+          e = lo;
+        }
+        int jb = jbase[e];
+        int nb = eoff[e + 1];
+        int cc[SR_K];
+#pragma unroll
+        for (int u = 0; u < SR_K; ++u) {
+          const bool ok = p0 + u * SR_SUB < total;
+          const int w = ok ? p0 + u * SR_SUB : total - 1;
+          while (w >= nb) {
+            ++e;
+            nb = eoff[e + 1];
+            jb = jbase[e];
+          }
+          const int c = Bcol[(unsigned)(jb + w)];
+          cc[u] = ok ? c : -1;
+        }
+        uint32_t wd[SR_K];
+#pragma unroll
+        for (int u = 0; u < SR_K; ++u) wd[u] = bm[cc[u] >= 0 ? (cc[u] >> 5) : 0];
+#pragma unroll
+        for (int u = 0; u < SR_K; ++u)
+          lsum += (cc[u] >= 0 && ((wd[u] >> (cc[u] & 31)) & 1u)) ? 1 : 0;
+      }
+      __threadfence_block();  // entry slots are rewritten by the next chunk
+    }
+    for (int p = sl; p < mnz; p += SR_SUB) bm[Mcol[ms + p] >> 5] = 0u;
+    __threadfence_block();
+  }
+  if (cur_g >= 0) flush();
+  if (wedge_counter && sl == 0 && wedges) atomicAdd(wedge_counter, wedges);
+}
+
 // Row binning: rows with an empty A row or empty mask get no work; rows
 // whose mask fits WR_MCAP and whose (untruncated) product count is at most
 // wave_flops go to the wavefront kernel; the rest are split into items of
@@ -849,7 +1014,19 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     GA_HIP(hipEventRecord(e0, s));
   }
   unsigned long long* wc = c.prof_bytes ? c.prof_dev.as<unsigned long long>() : nullptr;
-  if (wave_on)
+  if (wave_on && MODE == 1 && PA && PB) {
+    // pattern sum mode: 16 lanes per short row, four rows per wavefront
+    auto ksub = k_sum_subrows<BATCHED>;
+    int occ_s = 0;
+    GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, ksub, 64 * WR_WAVES, 0));
+    const int64_t grid_s = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ_s, 1),
+                                                                 ceil_div(m, SR_GROUPS)));
+    const int chunk_s = (int)std::max<int64_t>(1, std::min<int64_t>(32, m / (grid_s * SR_GROUPS * 8)));
+    hipLaunchKernelGGL(ksub, dim3((unsigned)grid_s), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
+                       M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
+                       (const int32_t*)(w.rstart.as<int32_t>() + m),
+                       w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, trunc, chunk_s);
+  } else if (wave_on)
     hipLaunchKernelGGL(krows, dim3((unsigned)grid_r), dim3(64 * WR_WAVES), 0, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col, Mend,
                      static_cast<T*>(Cval), sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
