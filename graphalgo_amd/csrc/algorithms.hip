@@ -438,6 +438,7 @@ ga_status ga_graph_create(const ga_csr* A, ga_graph** out) {
       g->symmetric = host_csr_symmetric(A);
       g->tw.mw.max_mask_row = g->tw.mw.max_b_row = host_max_row(A);
       g->mw.max_mask_row = g->mw.max_b_row = g->tw.mw.max_mask_row;
+      g->mw.reuse_plan = true;  // the triangle-count product of a resident graph never changes
       upload_csr(A, GA_INT32, g->A, c.stream);
       g->A.val.release();
       GA_HIP(hipStreamSynchronize(c.stream));

@@ -63,6 +63,7 @@ ga_status ga_batch_create(int64_t ngraphs, const ga_csr* graphs, ga_batch** out)
       b->rows = rows;
       b->max_n = max_n;
       b->mw.max_mask_row = b->mw.max_b_row = max_n;
+      b->mw.reuse_plan = true;  // the batch is immutable: bins and items are built once per width
       ga_csr h{rows, max_n, nnz, rp.data(), col.data(), nullptr};
       upload_csr(&h, GA_INT32, b->L, c.stream);
       b->row_group.ensure(sizeof(int32_t) * grp.size());

@@ -241,6 +241,14 @@ struct MxmWork {
   DevBuf rwork[2];
   int rwork_cur = 0;
   bool rwork_have = false;
+  // reuse_plan: the operands do not change between launches (a resident
+  // batch), so the row bins and item lists of the first launch are kept
+  bool reuse_plan = false;
+  bool plan_valid = false;
+  const int32_t* plan_rp = nullptr;
+  int64_t plan_m = 0, plan_nnz = 0;
+  int plan_tcap = 0;
+  long long plan_wf = 0;
 };
 
 // C<M> = A*B (structural mask).  mode 0: per-entry values into Cval (dtype);

@@ -943,6 +943,18 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // only the extra chunks of long rows are listed -- two small launches
   // instead of binning, two scans and two compactions
   const bool direct = wave_flops < 0 && ctx().direct_items;
+  // an immutable operand set (reuse_plan) keeps the row bins and item lists
+  // of its first launch; work-balanced lists change every launch
+  const bool plan_hit = w.reuse_plan && !w.balance && !direct && w.plan_valid && w.plan_rp == A.rp &&
+                        w.plan_m == m && w.plan_nnz == A.nnz && w.plan_tcap == TCAP && w.plan_wf == wave_flops;
+  if (w.reuse_plan && !w.balance && !direct) {
+    w.plan_valid = true;
+    w.plan_rp = A.rp;
+    w.plan_m = m;
+    w.plan_nnz = A.nnz;
+    w.plan_tcap = TCAP;
+    w.plan_wf = wave_flops;
+  }
   // work-balanced items (listed path of repeated products): the previous
   // launch's wedges per row set this launch's split
   unsigned long long* rw_cur = nullptr;
@@ -961,6 +973,8 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     hipLaunchKernelGGL(k_extra_items, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
                        w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(), n_extra);
     GA_HIP(hipGetLastError());
+  } else if (plan_hit) {
+    // same operands as the previous launch: its row bins and items stand
   } else if (wave_flops < 0) {
     // no wavefront bin: items of every non-empty row, listed in row order
     hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
