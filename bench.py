@@ -140,7 +140,7 @@ def bench_tc(args) -> dict:
     out = {"value": g.m_undirected / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": "triangle count C<L>=L*L on R-MAT scale 16", "n": g.n,
                       "m_undirected": g.m_undirected, "triangles": t},
-           "roofline": _roofline(timing, acct, "k_mxm_items (C<L> = L*L, sum mode)")}
+           "roofline": _roofline(timing, acct, "k_mxm_items (C<L> = L*L, sum mode)", "k_mxm_items/tc_rmat16")}
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle as O
@@ -151,6 +151,33 @@ def bench_tc(args) -> dict:
         dtc = (time.perf_counter() - t0) / reps
         out["cpu_baseline"] = {"value": g.m_undirected / dtc, "unit": "edges/s", "cores": O.num_threads(),
                                "kind": "port", "sample": f"{reps} oracle triangle counts of the same graph"}
+    return out
+
+
+def bench_tc_er(args) -> dict:
+    """Triangle count C<L> = L*L on the Erdos-Renyi graph n = 1000, m = 9990
+    (seed 1): a latency-bound single launch; value = edges/s of one call on the
+    resident graph, with the per-call latency beside it."""
+    from graphalgo_amd import api, graphs as G
+    g = G.erdos_renyi(1000, 9990, seed=1)
+    R = api.Graph(g)
+    dt, t, timing, acct = _measure(R.triangle_count, max(args.steps, 20), args.warmup + 5)
+    R.close()
+    out = {"value": g.m_undirected / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
+           "config": {"workload": "triangle count C<L>=L*L on Erdos-Renyi n=1000 m=9990 (seed 1)", "n": g.n,
+                      "m_undirected": g.m_undirected, "triangles": t, "latency_us": round(dt * 1e6, 2)},
+           "roofline": _roofline(timing, acct, "k_sum_subrows (C<L> = L*L, sum mode)")}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 2.0:
+            O.tc(g)
+            reps += 1
+        dtc = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": g.m_undirected / dtc, "unit": "edges/s", "cores": O.num_threads(),
+                               "kind": "port", "sample": f"{reps} oracle triangle counts of the same graph "
+                                                         f"({dtc * 1e6:.0f} us each)"}
     return out
 
 
@@ -181,7 +208,7 @@ def bench_batch(args) -> dict:
     gen_s = time.perf_counter() - t0
     m_total = sum(g.m_undirected for g in graphs)
     B = api.Batch(graphs)
-    best, tune_ms = B.autotune()
+    (best_w, best_c), tune_ms = B.autotune()
     dt, counts, timing, acct = _measure(B.triangle_counts, args.steps, args.warmup)
     # plain path: every graph resident, one triangle-count call per graph
     handles = [api.Graph(g) for g in graphs]
@@ -194,11 +221,12 @@ def bench_batch(args) -> dict:
         h.close()
     B.close()
     assert plain == counts.tolist(), "batched and per-graph counts differ"
-    roof = _roofline(timing, acct, "k_mxm_rows + k_mxm_items (batched)", "k_mxm_rows/batched_tc_knn4096")
+    roof = _roofline(timing, acct, "k_sum_subrows + k_mxm_items (batched)", "k_sum_subrows/batched_tc_knn4096")
     out = {"value": m_total / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": f"batched triangle count over {count} kNN graphs (k=30) in one launch",
                       "graphs": count, "m_total": m_total, "triangles_total": int(counts.sum()),
-                      "autotuned_threads": best, "autotune_ms": {str(k): round(v, 3) for k, v in tune_ms.items()},
+                      "autotuned": {"item_width": best_w, "rows_per_claim": best_c},
+                      "autotune_ms": {f"{w}x{c}": round(v, 3) for (w, c), v in tune_ms.items()},
                       "generation_s": round(gen_s, 1)},
            "roofline": roof,
            "plain_path": {"value": m_total / dt_plain, "unit": "edges/s", "ms": dt_plain * 1e3,
@@ -363,8 +391,8 @@ def bench_serve(args) -> dict:
     return out
 
 
-WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "copy": bench_copy, "batch": bench_batch, "bc": bench_bc,
-             "serve": bench_serve}
+WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "tc_er": bench_tc_er, "copy": bench_copy,
+             "batch": bench_batch, "bc": bench_bc, "serve": bench_serve}
 
 
 def main() -> None:

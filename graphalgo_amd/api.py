@@ -196,12 +196,18 @@ class Batch:
         N.check(N.galib().ga_batch_triangle_count(self._h, threads, out.ctypes.data_as(N.I64P)))
         return out[: len(self.graphs)]
 
-    def autotune(self) -> tuple[int, dict]:
-        best = C.c_int()
-        ms = np.zeros(3, dtype=np.float64)
-        N.check(N.galib().ga_batch_autotune(self._h, C.byref(best), ms.ctypes.data_as(N.F64P)))
-        self.threads = best.value
-        return best.value, {64: ms[0], 128: ms[1], 256: ms[2]}
+    TUNE_WIDTHS = (128, 256)
+    TUNE_CLAIMS = (2, 4, 8, 16)
+
+    def autotune(self) -> tuple[tuple[int, int], dict]:
+        """Times every (item width, rows per claim) pair on the resident batch
+        and keeps the fastest; returns it and the time of each pair in ms."""
+        best_t, best_c = C.c_int(), C.c_int()
+        ms = np.zeros(len(self.TUNE_WIDTHS) * len(self.TUNE_CLAIMS), dtype=np.float64)
+        N.check(N.galib().ga_batch_autotune(self._h, C.byref(best_t), C.byref(best_c), ms.ctypes.data_as(N.F64P)))
+        self.threads = best_t.value
+        pairs = [(t, c) for t in self.TUNE_WIDTHS for c in self.TUNE_CLAIMS]
+        return (best_t.value, best_c.value), dict(zip(pairs, ms.tolist()))
 
 
 _KINDS = {"tc": N.GA_Q_TRIANGLES, "ktruss": N.GA_Q_KTRUSS, "bc": N.GA_Q_BC}

@@ -317,10 +317,10 @@ ga_status ga_triangle_count(const ga_csr* A, int64_t* triangles) {
     w.max_mask_row = w.max_b_row = host_max_row(A);
     const DCsr vL = L.view();
     mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
-    unsigned long long t = 0;
-    GA_HIP(hipMemcpyAsync(&t, sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
+    auto* t = static_cast<unsigned long long*>(pinned_scratch());
+    GA_HIP(hipMemcpyAsync(t, sums.p, sizeof(*t), hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
-    *triangles = (int64_t)t;
+    *triangles = (int64_t)*t;
   });
 }
 
@@ -344,10 +344,10 @@ ga_status ga_triangle_count_lu(const ga_csr* A, int64_t* triangles) {
     MxmWork w;
     w.max_mask_row = w.max_b_row = host_max_row(A);
     mxm_masked_dev(L.view(), U.view(), vA, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
-    unsigned long long t = 0;
-    GA_HIP(hipMemcpyAsync(&t, sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
+    auto* t = static_cast<unsigned long long*>(pinned_scratch());
+    GA_HIP(hipMemcpyAsync(t, sums.p, sizeof(*t), hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
-    *triangles = (int64_t)(t >> 1);
+    *triangles = (int64_t)(*t >> 1);
   });
 }
 
@@ -470,10 +470,11 @@ ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles) {
     fill_u64(g->sums.p, 0ull, 1, s);
     const DCsr vL = g->L.view();
     mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, g->sums.as<unsigned long long>(), nullptr, g->mw, s);
-    unsigned long long t = 0;
-    GA_HIP(hipMemcpyAsync(&t, g->sums.p, sizeof(t), hipMemcpyDeviceToHost, s));
+    // the count comes back through pinned memory (a pageable copy is staged)
+    auto* t = static_cast<unsigned long long*>(pinned_scratch());
+    GA_HIP(hipMemcpyAsync(t, g->sums.p, sizeof(*t), hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
-    *triangles = (int64_t)t;
+    *triangles = (int64_t)*t;
   });
 }
 

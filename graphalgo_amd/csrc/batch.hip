@@ -8,8 +8,10 @@
 // every row of every graph, so the persistent grid never drains between
 // graphs and one launch replaces ngraphs launches of the per-graph path
 // (triangle_counting_masked_sandia, src/main.cpp:65-86, once per graph).
-// The workgroup width (64, 128 or 256 lanes = A entries per work item) is
-// chosen by timing the candidates on the resident batch (ga_batch_autotune).
+// The launch shape -- workgroup width of the long-row items (A entries per
+// work item) and rows per claim of the short-row kernel (the rows in flight
+// on an XCD, whose graphs share its L2) -- is chosen by timing the
+// candidates on the resident batch (ga_batch_autotune).
 #include "common.hpp"
 
 #include <algorithm>
@@ -24,6 +26,7 @@ struct ga_batch {
   DevBuf row_group, gbase, sums;
   MxmWork mw;
   int threads = 0;  // 0 = not tuned (256)
+  int claim = 0;    // rows per claim of the short-row kernel (0 = automatic)
 };
 
 extern "C" {
@@ -90,10 +93,11 @@ ga_status ga_batch_destroy(ga_batch* b) {
   });
 }
 
-static void batch_run(ga_batch* b, int threads, hipStream_t s) {
+static void batch_run(ga_batch* b, int threads, int claim, hipStream_t s) {
   fill_u64(b->sums.p, 0ull, b->ngraphs, s);
   const DCsr L = b->L.view();
   b->mw.threads = threads;
+  b->mw.row_chunk = claim;
   mxm_batched_sum_dev(L, b->row_group.as<int32_t>(), b->gbase.as<int32_t>(), b->max_n,
                       b->sums.as<unsigned long long>(), b->mw, s);
 }
@@ -104,29 +108,32 @@ ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts) {
     if (threads == 0) threads = b->threads ? b->threads : 256;
     GA_REQUIRE(threads == 64 || threads == 128 || threads == 256, GA_ERR_ARG, "threads must be 64, 128 or 256");
     hipStream_t s = ctx().stream;
-    batch_run(b, threads, s);
+    batch_run(b, threads, b->claim, s);
     if (counts && b->ngraphs)
       GA_HIP(hipMemcpyAsync(counts, b->sums.p, sizeof(int64_t) * b->ngraphs, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
   });
 }
 
-ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candidate) {
+ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, double* ms_per_candidate) {
   return guarded([&] {
     GA_REQUIRE(b != nullptr, GA_ERR_ARG, "null batch");
     hipStream_t s = ctx().stream;
-    const int cand[3] = {64, 128, 256};
+    const int widths[2] = {128, 256};
+    const int claims[4] = {2, 4, 8, 16};
+    static_assert(GA_BATCH_TUNE_CANDIDATES == 8, "candidate grid");
     hipEvent_t e0, e1;
     GA_HIP(hipEventCreate(&e0));
     GA_HIP(hipEventCreate(&e1));
     double best = 1e300;
-    int arg = 256;
+    int arg_t = 256, arg_c = 0;
     try {
-      for (int ci = 0; ci < 3; ++ci) {
-        batch_run(b, cand[ci], s);  // warm-up
+      for (int ci = 0; ci < GA_BATCH_TUNE_CANDIDATES; ++ci) {
+        const int th = widths[ci / 4], cl = claims[ci % 4];
+        batch_run(b, th, cl, s);  // warm-up
         GA_HIP(hipEventRecord(e0, s));
         const int reps = 3;
-        for (int r = 0; r < reps; ++r) batch_run(b, cand[ci], s);
+        for (int r = 0; r < reps; ++r) batch_run(b, th, cl, s);
         GA_HIP(hipEventRecord(e1, s));
         GA_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -135,7 +142,8 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candi
         if (ms_per_candidate) ms_per_candidate[ci] = per;
         if (per < best) {
           best = per;
-          arg = cand[ci];
+          arg_t = th;
+          arg_c = cl;
         }
       }
     } catch (...) {
@@ -145,8 +153,10 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candi
     }
     GA_HIP(hipEventDestroy(e0));
     GA_HIP(hipEventDestroy(e1));
-    b->threads = arg;
-    if (best_threads) *best_threads = arg;
+    b->threads = arg_t;
+    b->claim = arg_c;
+    if (best_threads) *best_threads = arg_t;
+    if (best_claim) *best_claim = arg_c;
   });
 }
 

@@ -6,17 +6,20 @@
 // visited set with add_nointersect and advances with the complement-masked
 // SpGEMM (AT * Front)<!Numsp>; backward it runs the masked sparse x dense
 // product A * W on the previous level's pattern.  Here the per-(vertex,
-// source) state is dense and vertex-major -- depth (uint16), sigma (FP64),
+// source) state is dense and vertex-major -- depth (8 bits, 16 past 253 levels), sigma (FP64),
 // q = (1 + delta) / sigma (FP64), row v holding all sources contiguously
 // (padded to a multiple of 64) -- so a wavefront owning vertex v reads the
 // state of 64 sources with one coalesced load, and the sparse structure is
 // carried by per-level vertex lists:
 //
-//  forward, level d -> d+1 (push along out-edges, the frontier product):
-//    for u in F_d, for v in N(u), for sources s with depth(u,s) = d and
-//    depth(v,s) in {unvisited, d+1}: depth(v,s) = d+1, sigma(v,s) += sigma(u,s)
-//    The complement mask is the depth test; sigma values are integers, so the
-//    FP64 atomic additions are exact in any order (below 2^53).
+//  forward, level d -> d+1 (the frontier product), in two sweeps:
+//    discovery, pushed along out-edges: for u in F_d, v in N(u), sources s with
+//      depth(u,s) = d and depth(v,s) unvisited: depth(v,s) = d+1 (the
+//      complement mask is the depth test);
+//    path counts, pulled: for v in F_{d+1}, sources s with depth(v,s) = d+1:
+//      sigma(v,s) = sum_{u in N(v), depth(u,s) = d} sigma(u,s), in CSR order.
+//    Neither sweep needs an atomic operation (the push version's FP64 atomics
+//    on sigma serialise on the rows of hub vertices).
 //  backward, level k = dmax-1 .. 1 (pull along out-edges, the A * W product):
 //    for v in F_k, for sources s with depth(v,s) = k:
 //      delta(v,s) = sigma(v,s) * sum_{w in N(v), depth(w,s) = k+1} q(w,s)
@@ -24,11 +27,11 @@
 //    and bc[v] += sum_s delta(v,s) (a fixed-order wavefront reduction).
 // The result is bc[v] = sum_{s in S} delta_s(v): unnormalised, not halved.
 //
-// Both sweeps run over work items of at most BC_CH neighbours of one level
+// Every sweep runs over work items of at most BC_CH neighbours of one level
 // vertex, one wavefront each, so the adjacency list of a hub is spread over
-// many wavefronts instead of serialising its level; the backward partial sums
-// of a hub are added in chunk order by a second pass (fixed order: the FP64
-// result is the same on every run).
+// many wavefronts instead of serialising its level; the partial sums of a hub
+// (sigma forward, delta backward) are added in chunk order by a second pass
+// (fixed order: the FP64 result is the same on every run).
 #include "common.hpp"
 
 #include <algorithm>
@@ -151,15 +154,19 @@ __global__ void k_bc_item_fill(const int32_t* __restrict__ front, int nf, const 
   }
 }
 
-// forward, level d -> d+1: one wavefront per item (vertex u, <= BC_CH
+// forward, level d -> d+1, in two steps that need no atomics.
+//
+// Discovery: one wavefront per item of a level-d vertex u (<= BC_CH
 // neighbours), lanes = 64 sources.  The item's neighbour ids are staged in
 // LDS once (one coalesced load) and reused for every 64-lane chunk; BC_UNROLL
-// neighbour depths are in flight per lane.
+// neighbour depths are in flight per lane.  A neighbour pair that is still
+// unvisited gets depth d+1 (every writer stores the same byte) and its vertex
+// is flagged for the level-(d+1) list.
 template <typename DT>
-__global__ __launch_bounds__(256) void k_bc_forward_items(
+__global__ __launch_bounds__(256) void k_bc_discover_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ n_items_ptr, int nsp, int d,
-    DT* __restrict__ depth, double* __restrict__ sigma, uint8_t* __restrict__ next_flag) {
+    DT* __restrict__ depth, uint8_t* __restrict__ next_flag) {
   __shared__ int32_t s_nb[4][BC_CH];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -174,10 +181,8 @@ __global__ __launch_bounds__(256) void k_bc_forward_items(
     for (int t = lane; t < cnt; t += 64) nb[t] = col[t0 + t];
     __threadfence_block();
     for (int c = 0; c < nsp; c += 64) {
-      const int64_t ui = (int64_t)u * nsp + c + lane;
-      const bool act = depth[ui] == dd;
+      const bool act = depth[(int64_t)u * nsp + c + lane] == dd;
       if (__ballot(act) == 0ull) continue;
-      const double su = act ? sigma[ui] : 0.0;
       for (int t = 0; t < cnt; t += BC_UNROLL) {
         int vv[BC_UNROLL];
         DT dv[BC_UNROLL];
@@ -188,16 +193,229 @@ __global__ __launch_bounds__(256) void k_bc_forward_items(
           dv[j] = (act && vv[j] >= 0) ? depth[(int64_t)vv[j] * nsp + c + lane] : (DT)0;
 #pragma unroll
         for (int j = 0; j < BC_UNROLL; ++j) {
-          if (act && vv[j] >= 0 && (dv[j] == bc_unvisited<DT>() || dv[j] == dn)) {
-            const int64_t vi = (int64_t)vv[j] * nsp + c + lane;
-            if (dv[j] == bc_unvisited<DT>()) depth[vi] = dn;
-            atomicAdd(&sigma[vi], su);
+          if (act && vv[j] >= 0 && dv[j] == bc_unvisited<DT>()) {
+            depth[(int64_t)vv[j] * nsp + c + lane] = dn;
             next_flag[vv[j]] = 1;
           }
         }
       }
     }
     __threadfence_block();  // the neighbour slots are rewritten by the next item
+  }
+}
+
+// Path counts of level d (pull along the edges, the frontier product):
+// one wavefront per item of a level-d vertex v; for the lanes with
+// depth(v, s) = d, sigma(v, s) = sum over neighbours u with depth(u, s) = d-1
+// of sigma(u, s), in CSR order.  The items of a hub vertex write per-chunk
+// partial sums that k_bc_sigma_combine adds in chunk order.  Level-d pairs are
+// written while level-(d-1) pairs are read: the two sets are disjoint.
+template <typename DT>
+__global__ __launch_bounds__(256) void k_bc_sigma_items(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
+    const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
+    int nsp, int d, const DT* __restrict__ depth, double* __restrict__ sigma, double* __restrict__ partial) {
+  __shared__ int32_t s_nb[4][BC_CH];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int n_items = *n_items_ptr;
+  const DT dd = (DT)d, dp = (DT)(d - 1);
+  int32_t* nb = s_nb[wv];
+  for (int it = blockIdx.x * wpb + wv; it < n_items; it += gridDim.x * wpb) {
+    const int v = it_v[it];
+    const int t0 = it_t0[it];
+    const int cnt = min(BC_CH, rp[v + 1] - t0);
+    const int slot = it_slot[it];
+    for (int t = lane; t < cnt; t += 64) nb[t] = col[t0 + t];
+    __threadfence_block();
+    for (int c = 0; c < nsp; c += 64) {
+      const int64_t vi = (int64_t)v * nsp + c + lane;
+      const bool act = depth[vi] == dd;
+      if (__ballot(act) == 0ull) continue;
+      double acc = 0.0;
+      for (int t = 0; t < cnt; t += BC_UNROLL) {
+        int uu[BC_UNROLL];
+        DT du[BC_UNROLL];
+        double su[BC_UNROLL];
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) uu[j] = (t + j < cnt) ? nb[t + j] : -1;
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j)
+          du[j] = (act && uu[j] >= 0) ? depth[(int64_t)uu[j] * nsp + c + lane] : (DT)0;
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j)
+          su[j] = (act && uu[j] >= 0 && du[j] == dp) ? sigma[(int64_t)uu[j] * nsp + c + lane] : 0.0;
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) acc += su[j];
+      }
+      if (act) {
+        if (slot >= 0)
+          partial[(int64_t)slot * nsp + c + lane] = acc;
+        else
+          sigma[vi] = acc;
+      }
+    }
+    __threadfence_block();  // the neighbour slots are rewritten by the next item
+  }
+}
+
+// Bottom-up step of level d -> d+1 (direction-optimising BFS): one wavefront
+// per item of a candidate vertex v (a vertex with an edge and an unvisited
+// lane); for its unvisited lanes, sigma = the sum of sigma(u, s) over
+// neighbours u with depth(u, s) = d, in CSR order -- the same terms in the
+// same order as the pull above -- and a lane with a nonzero sum (path counts
+// of reached pairs are >= 1) is at depth d+1.  It replaces the discovery push
+// from the level-d list and the pull over the level-(d+1) list when the
+// candidates have fewer edges than the frontier.  Hub items write per-chunk
+// partial sums; k_bc_bottom_up_combine decides their lanes.  Reads of
+// depth = d never see a lane written here (those become d+1).
+template <typename DT>
+__global__ __launch_bounds__(256) void k_bc_bottom_up_items(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
+    const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
+    int nsp, int d, DT* __restrict__ depth, double* __restrict__ sigma, double* __restrict__ partial,
+    uint8_t* __restrict__ next_flag) {
+  __shared__ int32_t s_nb[4][BC_CH];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int n_items = *n_items_ptr;
+  const DT dd = (DT)d, dn = (DT)(d + 1);
+  int32_t* nb = s_nb[wv];
+  for (int it = blockIdx.x * wpb + wv; it < n_items; it += gridDim.x * wpb) {
+    const int v = it_v[it];
+    const int t0 = it_t0[it];
+    const int cnt = min(BC_CH, rp[v + 1] - t0);
+    const int slot = it_slot[it];
+    for (int t = lane; t < cnt; t += 64) nb[t] = col[t0 + t];
+    __threadfence_block();
+    bool reached = false;
+    for (int c = 0; c < nsp; c += 64) {
+      const int64_t vi = (int64_t)v * nsp + c + lane;
+      const bool act = depth[vi] == bc_unvisited<DT>();
+      if (__ballot(act) == 0ull) continue;
+      double acc = 0.0;
+      for (int t = 0; t < cnt; t += BC_UNROLL) {
+        int uu[BC_UNROLL];
+        DT du[BC_UNROLL];
+        double su[BC_UNROLL];
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) uu[j] = (t + j < cnt) ? nb[t + j] : -1;
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j)
+          du[j] = (act && uu[j] >= 0) ? depth[(int64_t)uu[j] * nsp + c + lane] : (DT)0;
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j)
+          su[j] = (act && uu[j] >= 0 && du[j] == dd) ? sigma[(int64_t)uu[j] * nsp + c + lane] : 0.0;
+#pragma unroll
+        for (int j = 0; j < BC_UNROLL; ++j) acc += su[j];
+      }
+      if (slot >= 0) {
+        if (act) partial[(int64_t)slot * nsp + c + lane] = acc;
+      } else if (act && acc > 0.0) {
+        depth[vi] = dn;
+        sigma[vi] = acc;
+        reached = true;
+      }
+    }
+    if (__ballot(reached) != 0ull && lane == 0) next_flag[v] = 1;
+    __threadfence_block();  // the neighbour slots are rewritten by the next item
+  }
+}
+
+// the lanes of the candidate hubs: partial sums in chunk order, then as above
+template <typename DT>
+__global__ __launch_bounds__(256) void k_bc_bottom_up_combine(const int32_t* __restrict__ cand, int nc,
+                                                              const int32_t* __restrict__ rp,
+                                                              const int32_t* __restrict__ hstart, int nsp, int d,
+                                                              DT* __restrict__ depth, double* __restrict__ sigma,
+                                                              const double* __restrict__ partial,
+                                                              uint8_t* __restrict__ next_flag) {
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const DT dn = (DT)(d + 1);
+  for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nc; f += gridDim.x * wpb) {
+    const int v = cand[f];
+    const int deg = rp[v + 1] - rp[v];
+    if (deg <= BC_CH) continue;  // single-item vertex: finished in the item pass
+    const int nchunk = (deg + BC_CH - 1) / BC_CH;
+    const int slot0 = hstart[f];
+    bool reached = false;
+    for (int c = 0; c < nsp; c += 64) {
+      const int64_t vi = (int64_t)v * nsp + c + lane;
+      if (depth[vi] == bc_unvisited<DT>()) {
+        double acc = 0.0;
+        for (int j = 0; j < nchunk; ++j) acc += partial[(int64_t)(slot0 + j) * nsp + c + lane];
+        if (acc > 0.0) {
+          depth[vi] = dn;
+          sigma[vi] = acc;
+          reached = true;
+        }
+      }
+    }
+    if (__ballot(reached) != 0ull && lane == 0) next_flag[v] = 1;
+  }
+}
+
+// flag[v] = 1 when vertex v has an edge and an unvisited lane (the
+// candidates of a bottom-up step)
+template <typename DT>
+__global__ __launch_bounds__(256) void k_unvisited_flags(const int32_t* __restrict__ rp, const DT* __restrict__ depth,
+                                                         int64_t n, int nsp, uint8_t* __restrict__ flag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += (int64_t)gridDim.x * wpb) {
+    if (rp[v + 1] == rp[v]) continue;
+    bool hit = false;
+    for (int c = lane; c < nsp; c += 64) hit |= depth[v * nsp + c] == bc_unvisited<DT>();
+    if (__ballot(hit) != 0ull && lane == 0) flag[v] = 1;
+  }
+}
+
+// out[0] += the edges of list a (na entries), out[1] += those of list b
+// (*nb_ptr entries, a device count)
+__global__ void k_list_edges(const int32_t* __restrict__ a, int na, const int32_t* __restrict__ b,
+                             const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ rp,
+                             unsigned long long* __restrict__ out) {
+  const int nb = *nb_ptr;
+  unsigned long long ea = 0, eb = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < max(na, nb); i += gridDim.x * blockDim.x) {
+    if (i < na) ea += (unsigned long long)(rp[a[i] + 1] - rp[a[i]]);
+    if (i < nb) eb += (unsigned long long)(rp[b[i] + 1] - rp[b[i]]);
+  }
+  ea = wave_reduce_sum(ea);
+  eb = wave_reduce_sum(eb);
+  if ((threadIdx.x & 63) == 0) {
+    if (ea) atomicAdd(&out[0], ea);
+    if (eb) atomicAdd(&out[1], eb);
+  }
+}
+
+// sigma of the hub vertices of level d: their per-chunk partial sums in chunk order
+template <typename DT>
+__global__ __launch_bounds__(256) void k_bc_sigma_combine(const int32_t* __restrict__ front, int nf,
+                                                          const int32_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ hstart, int nsp, int d,
+                                                          const DT* __restrict__ depth, double* __restrict__ sigma,
+                                                          const double* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int wpb = blockDim.x >> 6;
+  const DT dd = (DT)d;
+  for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nf; f += gridDim.x * wpb) {
+    const int v = front[f];
+    const int deg = rp[v + 1] - rp[v];
+    if (deg <= BC_CH) continue;  // single-item vertex: finished in the item pass
+    const int nc = (deg + BC_CH - 1) / BC_CH;
+    const int slot0 = hstart[f];
+    for (int c = 0; c < nsp; c += 64) {
+      const int64_t vi = (int64_t)v * nsp + c + lane;
+      if (depth[vi] == dd) {
+        double acc = 0.0;
+        for (int j = 0; j < nc; ++j) acc += partial[(int64_t)(slot0 + j) * nsp + c + lane];
+        sigma[vi] = acc;
+      }
+    }
   }
 }
 
@@ -360,8 +578,10 @@ static hipEvent_t bc_event(Context& c) {
   return ev;
 }
 
+// the flagged vertices as an ascending list (flags cleared); its size goes to
+// count_h (pinned) and, on the device, to w.off[ceil(n / BC_FLAG_CHUNK)]
 static void frontier_from_flags(uint8_t* flag, int64_t n, BcWork& w, int32_t* out, int32_t* count_h,
-                                hipStream_t s) {
+                                hipStream_t s, bool sync = true) {
   const int chunk = BC_FLAG_CHUNK;
   const int64_t ng = ceil_div(n, chunk);
   hipLaunchKernelGGL(k_flag_count, dim3(grid_for(ng, 256)), dim3(256), 0, s, flag, n, w.cnt.as<int32_t>(), chunk);
@@ -369,7 +589,7 @@ static void frontier_from_flags(uint8_t* flag, int64_t n, BcWork& w, int32_t* ou
   hipLaunchKernelGGL(k_flag_emit, dim3(grid_for(ng, 256)), dim3(256), 0, s, flag, n,
                      (const int32_t*)w.off.as<int32_t>(), chunk, out);
   GA_HIP(hipMemcpyAsync(count_h, w.off.as<int32_t>() + ng, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  GA_HIP(hipStreamSynchronize(s));
+  if (sync) GA_HIP(hipStreamSynchronize(s));
 }
 
 // Sizes of every BcWork buffer for an n-vertex graph with nnz entries, ns
@@ -384,7 +604,7 @@ static BcSizes bc_sizes(int64_t n, int64_t nnz, int64_t ns, int64_t nsp, int64_t
   z.total = n * z.nsp;
   z.item_cap = n + ceil_div(nnz, BC_CH) + 1;
   z.partial = (2 * ceil_div(nnz, BC_CH) + 1) * z.nsp;
-  z.levels = 2 * std::max<int64_t>(n, 1);
+  z.levels = 3 * std::max<int64_t>(n, 1);  // two frontier lists + the bottom-up candidates
   z.ng = ceil_div(std::max<int64_t>(n, 1), BC_FLAG_CHUNK);
   z.outs = outs;
   return z;
@@ -403,6 +623,7 @@ static void bc_reserve(BcWork& w, int64_t n, const BcSizes& z, int64_t ns) {
   w.it_slot.ensure(sizeof(int32_t) * z.item_cap);
   w.partial.ensure(sizeof(double) * z.partial);
   w.levels.ensure(sizeof(int32_t) * z.levels);
+  w.dsum.ensure(sizeof(unsigned long long) * 2);
   w.cnt.ensure(sizeof(int32_t) * z.ng);
   w.off.ensure(sizeof(int32_t) * (z.ng + 1));
   w.icnt.ensure(sizeof(int32_t) * n1);
@@ -419,7 +640,7 @@ static size_t bc_bytes(int64_t n, const BcSizes& z, int64_t ns) {
   b += lease_bytes(sizeof(double) * n1 * z.outs) + lease_bytes(4 * ceil_div(n1, 4)) +
        lease_bytes(sizeof(int32_t) * 2 * ns);
   b += 3 * lease_bytes(sizeof(int32_t) * z.item_cap) + lease_bytes(sizeof(double) * z.partial);
-  b += lease_bytes(sizeof(int32_t) * z.levels);
+  b += lease_bytes(sizeof(int32_t) * z.levels) + lease_bytes(sizeof(unsigned long long) * 2);
   b += lease_bytes(sizeof(int32_t) * z.ng) + lease_bytes(sizeof(int32_t) * (z.ng + 1));
   b += 2 * lease_bytes(sizeof(int32_t) * n1) + 2 * lease_bytes(sizeof(int32_t) * (n1 + 1));
   b += lease_bytes(scan_tmp_bytes(std::max(n1, z.ng)));
@@ -523,6 +744,52 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
   int32_t nf = *count_h;
   int d = 0;
   Context& cx = ctx();
+  // event pairs around the device work of each step (the host waits for the
+  // size of every new level list between them)
+  hipEvent_t ev0 = nullptr;
+  auto ev_open = [&] {
+    if (!cx.prof_events_on) return;
+    ev0 = bc_event(cx);
+    GA_HIP(hipEventRecord(ev0, s));
+  };
+  auto ev_close = [&] {
+    if (!cx.prof_events_on) return;
+    hipEvent_t ev1 = bc_event(cx);
+    GA_HIP(hipEventRecord(ev1, s));
+    cx.prof_events.emplace_back(ev0, ev1);
+    cx.prof_launches += 1;
+  };
+  int64_t timed = 0;  // event pairs of this pass
+  // direction-optimising forward: top-down (discovery push + sigma pull)
+  // while the level list has at most 1/alpha of the edges of the vertices
+  // with unvisited lanes, bottom-up from then on (the candidates only shrink)
+  int32_t* cand = w.levels.as<int32_t>() + 2 * std::max<int64_t>(n, 1);
+  auto* edges_h = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(count_h) + 64);
+  const int64_t ng = ceil_div(std::max<int64_t>(n, 1), BC_FLAG_CHUNK);
+  bool bottom_up = false;
+  int32_t nc = 0;
+  // candidates of a bottom-up step into `cand` (nc entries); with `costs`,
+  // also the edges of the level list and of the candidates (edges_h[0..1])
+  auto candidates = [&](const int32_t* front, int32_t nfront, bool costs) {
+    ev_open();
+    hipLaunchKernelGGL(k_unvisited_flags<DT>, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
+                       (const DT*)w.depth.as<DT>(), n, nsp, w.flag.as<uint8_t>());
+    frontier_from_flags(w.flag.as<uint8_t>(), n, w, cand, count_h, s, false);
+    if (costs) {
+      fill_u64(w.dsum.p, 0ull, 2, s);
+      hipLaunchKernelGGL(k_list_edges, dim3(grid_for(std::max<int64_t>(n, 1), 256, 1024)), dim3(256), 0, s, front,
+                         nfront, (const int32_t*)cand, (const int32_t*)(w.off.as<int32_t>() + ng), A.rp,
+                         w.dsum.as<unsigned long long>());
+      GA_HIP(hipMemcpyAsync(edges_h, w.dsum.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    }
+    GA_HIP(hipGetLastError());
+    ev_close();
+    ++timed;
+    GA_HIP(hipStreamSynchronize(s));
+    nc = *count_h;
+  };
+  ev_open();
+  bc_items(A, lists[0], nf, w, s);
   while (true) {
     // level d + 1 must stay below the unvisited marker
     if (d + 1 >= (int)bc_unvisited<DT>()) {
@@ -530,27 +797,59 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
       GA_HIP(hipStreamSynchronize(s));
       return false;
     }
-    const int32_t* front = lists[d & 1];
-    hipEvent_t fe0 = nullptr, fe1 = nullptr;
-    if (cx.prof_events_on) {
-      fe0 = bc_event(cx);
-      fe1 = bc_event(cx);
-      GA_HIP(hipEventRecord(fe0, s));
+    int32_t* next = lists[(d + 1) & 1];
+    if (!bottom_up && d >= 1 && cx.bc_bottom_up) {
+      candidates(lists[d & 1], nf, true);
+      bottom_up = (double)edges_h[0] * cx.bc_alpha > (double)edges_h[1];
+    } else if (bottom_up) {
+      candidates(nullptr, 0, false);
     }
-    bc_items(A, front, nf, w, s);
-    hipLaunchKernelGGL(k_bc_forward_items<DT>, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s, A.rp,
-                       A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+    if (bottom_up) {
+      if (nc == 0) break;  // no vertex can be reached any more
+      ev_open();
+      bc_items(A, cand, nc, w, s);
+      hipLaunchKernelGGL(k_bc_bottom_up_items<DT>, dim3(grid_for(nc + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0,
+                         s, A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+                         (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nc), nsp,
+                         d, w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(), w.flag.as<uint8_t>());
+      hipLaunchKernelGGL(k_bc_bottom_up_combine<DT>, dim3(grid_for(nc, 4, 65536)), dim3(256), 0, s,
+                         (const int32_t*)cand, nc, A.rp, (const int32_t*)w.hstart.as<int32_t>(), nsp, d,
+                         w.depth.as<DT>(), w.sigma.as<double>(), (const double*)w.partial.as<double>(),
+                         w.flag.as<uint8_t>());
+      GA_HIP(hipGetLastError());
+      ev_close();
+      ++timed;
+      frontier_from_flags(w.flag.as<uint8_t>(), n, w, next, count_h, s);
+      const int32_t nn = *count_h;
+      if (nn == 0) break;
+      nf = nn;
+      ++d;
+      continue;
+    }
+    // top-down: the items of level d are in w.it_* (count at istart[nf])
+    if (d > 0) ev_open();
+    hipLaunchKernelGGL(k_bc_discover_items<DT>, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s,
+                       A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
                        (const int32_t*)(w.istart.as<int32_t>() + nf), nsp, d, w.depth.as<DT>(),
-                       w.sigma.as<double>(), w.flag.as<uint8_t>());
+                       w.flag.as<uint8_t>());
     GA_HIP(hipGetLastError());
-    if (cx.prof_events_on) {
-      GA_HIP(hipEventRecord(fe1, s));
-      cx.prof_events.emplace_back(fe0, fe1);
-      cx.prof_launches += 1;
-    }
-    frontier_from_flags(w.flag.as<uint8_t>(), n, w, lists[(d + 1) & 1], count_h, s);
+    ev_close();
+    ++timed;
+    frontier_from_flags(w.flag.as<uint8_t>(), n, w, next, count_h, s);
     const int32_t nn = *count_h;
     if (nn == 0) break;
+    ev_open();
+    bc_items(A, next, nn, w, s);
+    hipLaunchKernelGGL(k_bc_sigma_items<DT>, dim3(grid_for(nn + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s, A.rp,
+                       A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+                       (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nn), nsp,
+                       d + 1, (const DT*)w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>());
+    hipLaunchKernelGGL(k_bc_sigma_combine<DT>, dim3(grid_for(nn, 4, 65536)), dim3(256), 0, s, (const int32_t*)next, nn,
+                       A.rp, (const int32_t*)w.hstart.as<int32_t>(), nsp, d + 1, (const DT*)w.depth.as<DT>(),
+                       w.sigma.as<double>(), (const double*)w.partial.as<double>());
+    GA_HIP(hipGetLastError());
+    ev_close();
+    ++timed;
     nf = nn;
     ++d;
   }
@@ -560,14 +859,9 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
     hipLaunchKernelGGL(k_bc_account<DT>, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
                        (const DT*)w.depth.as<DT>(), n, nsp, cx.prof_dev.as<unsigned long long>());
     cx.prof_bytes_host += (int64_t)(dmax + 1) * (4 * (n + 1) + 4 * A.nnz);
-    cx.prof_bytes_launches += 2 * dmax;
+    cx.prof_bytes_launches += timed + 1;  // the event pairs of the pass, backward sweep included
   }
-  hipEvent_t be0 = nullptr, be1 = nullptr;
-  if (cx.prof_events_on) {
-    be0 = bc_event(cx);
-    be1 = bc_event(cx);
-    GA_HIP(hipEventRecord(be0, s));
-  }
+  ev_open();
   if (dmax >= 1) {
     hipLaunchKernelGGL(k_bc_last<DT>, dim3(grid_for(nf, 4, 65536)), dim3(256), 0, s, (const int32_t*)lists[dmax & 1], nf,
                        nsp, dmax, (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(),
@@ -594,11 +888,7 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
       GA_HIP(hipGetLastError());
     }
   }
-  if (cx.prof_events_on) {
-    GA_HIP(hipEventRecord(be1, s));
-    cx.prof_events.emplace_back(be0, be1);
-    cx.prof_launches += 1;
-  }
+  ev_close();
   if (n) GA_HIP(hipMemcpyAsync(bc_h, w.bc.p, sizeof(double) * n * outs, hipMemcpyDeviceToHost, s));
   GA_HIP(hipStreamSynchronize(s));
   return true;

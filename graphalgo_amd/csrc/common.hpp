@@ -146,9 +146,22 @@ struct Context {
   // set by per-item costs, not products, and the selection launches cost
   // more than the products save (510 vs 493 ms per decomposition).
   bool selective_support = false;
+  // BC forward: switch to bottom-up steps once the level list has more than
+  // 1/bc_alpha of the edges of the vertices with unvisited lanes
+  // (GA_BC_BOTTOM_UP=0: top-down only; GA_BC_ALPHA sets the ratio).  R-MAT 18
+  // x 256 sources: 14.9 ms per pass at alpha 1, 12.7 at 2, 12.3 at 4 and 14.
+  bool bc_bottom_up = true;
+  double bc_alpha = 14.0;
   // target wedges per work item when rows are split by their measured work
   // (GA_ITEM_WEDGES overrides it at ga_init)
   long long item_wedges = 32768;
+  // rows per claim of the short-row kernels (0 = automatic; GA_ROW_CHUNK
+  // overrides it at ga_init, MxmWork::row_chunk per workspace)
+  int row_chunk = 0;
+  // short rows of pattern triangle counts: 0 = 16-lane wedge runs
+  // (k_sum_subrows), 8/16/32/64 = lanes per row walking its entries
+  // (k_sum_entries); GA_SUM_SUB overrides it at ga_init
+  int sum_sub = 0;
   // kernel timing (events around the masked-product launches) and
   // algorithmic byte accounting (extra counting kernels; kept out of timed runs)
   bool prof_events_on = false;
@@ -231,6 +244,9 @@ struct MxmWork {
   long long wave_flops = -1;     // binning threshold (products per row) for the wavefront kernel;
                                  // -1 = the context default
   int threads = 256;             // workgroup width of the item kernel (64, 128 or 256)
+  int row_chunk = 0;             // rows per claim of the short-row kernels (0 = automatic)
+  int sum_sub = -1;              // short pattern sum rows: 0 = wedge runs, 8..64 = lanes per row
+                                 // walking entries (-1 = the context default)
   int64_t max_mask_row = INT32_MAX;  // longest mask row (host-known bound); < 65536 enables the rank lookup
   int64_t max_b_row = INT32_MAX;     // longest B row (host-known bound; B.nnz is used when tighter)
   // Work balance over repeated products of one matrix (the k-truss support
@@ -249,6 +265,9 @@ struct MxmWork {
   int64_t plan_m = 0, plan_nnz = 0;
   int plan_tcap = 0;
   long long plan_wf = 0;
+  // short rows and work items of the kept plan, read back once when it is
+  // built (-1 = unknown): an empty kernel of a kept plan is not launched
+  int64_t plan_rows = -1, plan_items = -1;
 };
 
 // C<M> = A*B (structural mask).  mode 0: per-entry values into Cval (dtype);
@@ -306,6 +325,7 @@ size_t ktruss_workspace_bytes(int64_t n, int64_t nnz);
 struct BcWork {
   DevBuf depth, sigma, q, bc, flag, levels, cnt, off, scan_tmp, src;
   DevBuf icnt, hcnt, istart, hstart, it_v, it_t0, it_slot, partial;  // neighbour-chunked items
+  DevBuf dsum;                                                        // edge counts of two lists
 };
 void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h, BcWork& w, hipStream_t s);
 size_t bc_workspace_bytes(int64_t n, int64_t nnz, int64_t ns);

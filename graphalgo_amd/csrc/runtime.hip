@@ -304,6 +304,13 @@ ga_status ga_init(int device) {
     if (const char* di = std::getenv("GA_DIRECT_ITEMS")) c->direct_items = std::atoi(di) != 0;
     if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT")) c->selective_support = std::atoi(ss) != 0;
     if (const char* iw = std::getenv("GA_ITEM_WEDGES")) c->item_wedges = std::max(1LL, std::atoll(iw));
+    if (const char* rc = std::getenv("GA_ROW_CHUNK")) c->row_chunk = std::max(0, std::atoi(rc));
+    if (const char* bu = std::getenv("GA_BC_BOTTOM_UP")) c->bc_bottom_up = std::atoi(bu) != 0;
+    if (const char* ba = std::getenv("GA_BC_ALPHA")) c->bc_alpha = std::max(0.01, std::atof(ba));
+    if (const char* ss = std::getenv("GA_SUM_SUB")) {
+      const int v = std::atoi(ss);
+      c->sum_sub = (v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
+    }
     c->prof_dev.ensure(64);
     GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
     GA_HIP(hipStreamSynchronize(c->stream));

@@ -56,8 +56,12 @@ constexpr int MXM_CTR_WORDS = 2 * MXM_NSEG * MXM_CTR_STRIDE + 32;  // item segme
 // entry scan); MCAP = mask entries staged in LDS.  BATCHED: the operands are
 // many graphs stored back to back with graph-local column indices; row i
 // belongs to graph row_group[i] whose rows start at gbase[row_group[i]].
+// (pattern products with 4-byte or no accumulators: at most 64 VGPRs, eight
+// wavefronts per SIMD to hide the dependent loads; valued ones would spill
+// under that cap, 8-byte accumulators are bound by LDS instead)
 template <typename T, bool PAT_A, bool PAT_B, int MODE, int THREADS, bool BATCHED, bool RANK>
-__global__ __launch_bounds__(THREADS) void k_mxm_items(
+__global__ __launch_bounds__(THREADS)
+__attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 4) ? 8 : 1))) void k_mxm_items(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const T* __restrict__ Aval,
     const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol, const T* __restrict__ Bval,
     const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol, T* __restrict__ Cval,
@@ -300,40 +304,43 @@ __global__ __launch_bounds__(THREADS) void k_mxm_items(
     }
     __syncthreads();
 
-    // Wedge mapping: lanes form groups of MXM_G; group g of a trip owns the
-    // MXM_G * MXM_K consecutive wedges starting at base + g * MXM_G * MXM_K,
-    // lane r of the group takes every MXM_G-th of them.  The MXM_G lanes of a
-    // group read adjacent B columns (32-byte runs), and a lane's wedges are
-    // MXM_G apart, so its entry advances by a short linear walk; the entry of
-    // the first wedge of a trip is found by one binary search (from the
-    // lane's current entry) per MXM_K wedges.
+    // Wedge mapping: lanes form groups of MXM_G, and group g owns the
+    // contiguous wedge range [g0, g1) = [g * total / NG, (g + 1) * total / NG).
+    // A trip of the group covers its next MXM_G * MXM_K wedges, lane r taking
+    // every MXM_G-th of them: the MXM_G lanes read adjacent B columns
+    // (64-byte runs) and a lane's wedges are MXM_G apart, so its entry only
+    // moves forward by a short linear walk.  One binary search per lane and
+    // item finds the entry of its first wedge; there is none per trip.
     const int grp = tid / MXM_G, rl = tid % MXM_G;
+    constexpr int MXM_NG = MXM_THREADS / MXM_G;
+    const int g0 = (int)(((long long)total * grp) / MXM_NG);
+    const int g1 = (int)(((long long)total * (grp + 1)) / MXM_NG);
     // the loop is instantiated twice: mask row staged in LDS (sorted list and
     // LDS accumulator) or not (search and accumulate in global memory), so
     // the hot loop has no branch on it
     auto wedge_loop = [&](auto local_c) {
       constexpr bool LOCAL = decltype(local_c)::value;
       int e = 0;
-      for (int base = 0; base < total; base += MXM_THREADS * MXM_K) {
-        const int p0 = base + grp * (MXM_G * MXM_K) + rl;
-        if (p0 >= total) break;  // later trips start even further out
-        {  // entry holding wedge p0: largest e' >= e with s_eoff[e'] <= p0
-          int lo = e, hi = ne - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_eoff[mid] <= p0) lo = mid;
-            else hi = mid - 1;
-          }
-          e = lo;
+      if (g0 + rl < g1) {  // entry holding the lane's first wedge: largest e with s_eoff[e] <= g0 + rl
+        int lo = 0, hi = ne - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_eoff[mid] <= g0 + rl) lo = mid;
+          else hi = mid - 1;
         }
+        e = lo;
+      }
+      for (int pos = g0; pos < g1; pos += MXM_G * MXM_K) {
+        const int p0 = pos + rl;
+        if (p0 >= g1) break;  // later trips start even further out
         int jb = s_jb[e];
         int nb = s_eoff[e + 1];
         int cc[MXM_K], jj[MXM_K], ee[MXM_K];
 #pragma unroll
         for (int u = 0; u < MXM_K; ++u) {
-          // wedges past the end re-read the last one (in bounds) and are dropped
-          const bool ok = p0 + u * MXM_G < total;
-          const int w = ok ? p0 + u * MXM_G : total - 1;
+          // wedges past the range re-read its last one (in bounds) and are dropped
+          const bool ok = p0 + u * MXM_G < g1;
+          const int w = ok ? p0 + u * MXM_G : g1 - 1;
           while (w >= nb) {  // next non-empty entry (zero-length entries are skipped)
             ++e;
             nb = s_eoff[e + 1];
@@ -656,29 +663,34 @@ constexpr int SR_SUB = 16;
 constexpr int SR_GROUPS = WR_WAVES * (64 / SR_SUB);  // groups per workgroup
 constexpr int SR_K = 8;                               // wedges per lane per trip
 
+// (eight wavefronts per SIMD: the kernel is bound by the latency of its
+// dependent load chains, so occupancy matters more than a few registers)
 template <bool BATCHED>
-__global__ __launch_bounds__(64 * WR_WAVES) void k_sum_subrows(
+__global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void k_sum_subrows(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const int32_t* __restrict__ Brp,
     const int32_t* __restrict__ Bcol, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol,
     const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
     unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
   __shared__ uint32_t s_bm[SR_GROUPS][WR_BMW];
-  __shared__ uint16_t s_wpre[SR_GROUPS][WR_BMW];
   __shared__ int32_t s_jb[SR_GROUPS][SR_SUB];
   __shared__ int32_t s_eoff[SR_GROUPS][SR_SUB + 1];
   const int sg = threadIdx.x / SR_SUB;  // group within the workgroup
   const int sl = threadIdx.x % SR_SUB;  // lane within the group
   uint32_t* bm = s_bm[sg];
-  uint16_t* wpre = s_wpre[sg];
   int32_t* jbase = s_jb[sg];
   int32_t* eoff = s_eoff[sg];
   for (int q = sl; q < WR_BMW; q += SR_SUB) bm[q] = 0u;
   const int n_rows = *n_rows_ptr;
-  // segmented claiming as in k_mxm_rows, by each group's lane 0
+  // segmented claiming as in k_mxm_rows: the group's lane 0 takes `chunk`
+  // rows at a time from its current segment (one atomic); when that segment
+  // is drained the group reads the 32 segment counters at once (two per
+  // lane) and moves to the next segment with rows left, so the end of the
+  // launch costs two round trips per claim instead of one per segment
   const int seg_len = (n_rows + MXM_NSEG - 1) / MXM_NSEG;
-  int seg = blockIdx.x % MXM_NSEG, tries = 0;
-  int next = 0, end = 0;
+  int seg = blockIdx.x % MXM_NSEG;
+  bool drained = false;
+  int next = 0, end = 0;  // the group's current claim (group-uniform)
   long long lsum = 0;  // this lane's hits for graph cur_g
   int cur_g = -1;
   unsigned long long wedges = 0;
@@ -689,31 +701,44 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_sum_subrows(
     if (sl == 0 && t) atomicAdd(&sums[cur_g], (unsigned long long)t);
     lsum = 0;
   };
+  auto try_claim = [&](int sj) {  // group-uniform; true when rows were taken
+    const int lo = sj * seg_len;
+    const int hi = min(lo + seg_len, n_rows);
+    if (lo >= hi) return false;
+    int k = 0;
+    if (sl == 0) k = (int)atomicAdd(&counter[sj * MXM_CTR_STRIDE], (unsigned)chunk);
+    k = __shfl(k, 0, SR_SUB);
+    if (lo + k >= hi) return false;
+    next = lo + k;
+    end = min(lo + k + chunk, hi);
+    return true;
+  };
   while (true) {
-    int r = n_rows;
-    if (sl == 0) {
-      if (next < end) {
-        r = next++;
-      } else {
-        while (tries < MXM_NSEG) {
-          const int lo = seg * seg_len;
+    if (next >= end && !drained && !try_claim(seg)) {
+      while (true) {
+        uint32_t live = 0;
+        for (int j = sl; j < MXM_NSEG; j += SR_SUB) {
+          const int sj = (seg + 1 + j) % MXM_NSEG;
+          const int lo = sj * seg_len;
           const int hi = min(lo + seg_len, n_rows);
           if (lo < hi) {
-            const int k = (int)atomicAdd(&counter[seg * MXM_CTR_STRIDE], (unsigned)chunk);
-            if (lo + k < hi) {
-              r = lo + k;
-              next = r + 1;
-              end = min(lo + k + chunk, hi);
-              break;
-            }
+            const unsigned taken =
+                __hip_atomic_load(&counter[sj * MXM_CTR_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lo + (long long)taken < hi) live |= 1u << j;
           }
-          seg = (seg + 1) % MXM_NSEG;
-          ++tries;
         }
+#pragma unroll
+        for (int d = SR_SUB / 2; d >= 1; d >>= 1) live |= __shfl_xor(live, d, SR_SUB);
+        if (live == 0u) {
+          drained = true;
+          break;
+        }
+        seg = (seg + 1 + (__ffs((int)live) - 1)) % MXM_NSEG;
+        if (try_claim(seg)) break;
       }
     }
-    r = __shfl(r, 0, SR_SUB);
-    if (r >= n_rows) break;
+    if (next >= end) break;  // every segment handed out
+    const int r = next++;
     const int i = rows[r];
     const int ms = Mrp[i];
     const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // <= WR_MCAP by binning
@@ -724,7 +749,6 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_sum_subrows(
     }
     for (int p = sl; p < mnz; p += SR_SUB) {
       const int c = Mcol[ms + p];
-      if (p == 0 || (Mcol[ms + p - 1] >> 5) != (c >> 5)) wpre[c >> 5] = (uint16_t)p;
       atomicOr(&bm[c >> 5], 1u << (c & 31));
     }
     const int maxc = (trunc && mnz > 0) ? Mcol[ms + mnz - 1] : 0x7FFFFFFF;
@@ -807,6 +831,157 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_sum_subrows(
   }
   if (cur_g >= 0) flush();
   if (wedge_counter && sl == 0 && wedges) atomicAdd(wedge_counter, wedges);
+}
+
+// ---- short rows of pattern sum-mode products, entry by entry -------------
+// For rows whose B rows are short (a few dozen columns: kNN and other
+// near-regular graphs), a SUB-lane group walks the A entries of its row one
+// at a time and reads each entry's B row with SUB adjacent lanes (one
+// coalesced run per SUB columns).  No wedge-to-entry search is needed: the
+// search of k_sum_subrows is a chain of dependent LDS reads per trip, which
+// is what bounds it on such rows.  Four entries are read at once so that
+// their gathers are in flight together.  The mask row is an exact per-group
+// bitmap (<= WR_MAXCOLS columns).
+template <bool BATCHED, int SUB>
+__global__ __launch_bounds__(64 * WR_WAVES) void k_sum_entries(
+    const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const int32_t* __restrict__ Brp,
+    const int32_t* __restrict__ Bcol, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol,
+    const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
+    const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
+    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
+  static_assert(SUB == 8 || SUB == 16 || SUB == 32 || SUB == 64, "group width");
+  constexpr int NG = 64 * WR_WAVES / SUB;  // groups per workgroup
+  constexpr int EU = 4;                    // entries whose B rows are read at once
+  __shared__ uint32_t s_bm[NG][WR_BMW];
+  const int sg = threadIdx.x / SUB;  // group within the workgroup
+  const int sl = threadIdx.x % SUB;  // lane within the group
+  uint32_t* bm = s_bm[sg];
+  for (int q = sl; q < WR_BMW; q += SUB) bm[q] = 0u;
+  const int n_rows = *n_rows_ptr;
+  // segmented claims, as in k_sum_subrows (the group's lane 0 takes `chunk`
+  // rows of its current segment; a drained segment sends the group to a
+  // probe of all segment counters)
+  const int seg_len = (n_rows + MXM_NSEG - 1) / MXM_NSEG;
+  int seg = blockIdx.x % MXM_NSEG;
+  bool drained = false;
+  int next = 0, end = 0;
+  unsigned long long lsum = 0;  // this lane's hits for graph cur_g
+  int cur_g = -1;
+  unsigned long long wedges = 0;
+  auto flush = [&]() {  // group-uniform
+    unsigned long long t = lsum;
+#pragma unroll
+    for (int d = SUB / 2; d >= 1; d >>= 1) t += __shfl_xor(t, d, SUB);
+    if (sl == 0 && t) atomicAdd(&sums[cur_g], t);
+    lsum = 0;
+  };
+  auto try_claim = [&](int sj) {
+    const int lo = sj * seg_len;
+    const int hi = min(lo + seg_len, n_rows);
+    if (lo >= hi) return false;
+    int k = 0;
+    if (sl == 0) k = (int)atomicAdd(&counter[sj * MXM_CTR_STRIDE], (unsigned)chunk);
+    k = __shfl(k, 0, SUB);
+    if (lo + k >= hi) return false;
+    next = lo + k;
+    end = min(lo + k + chunk, hi);
+    return true;
+  };
+  while (true) {
+    if (next >= end && !drained && !try_claim(seg)) {
+      while (true) {
+        uint32_t live = 0;
+        for (int j = sl; j < MXM_NSEG; j += SUB) {
+          const int sj = (seg + 1 + j) % MXM_NSEG;
+          const int lo = sj * seg_len;
+          const int hi = min(lo + seg_len, n_rows);
+          if (lo < hi) {
+            const unsigned taken =
+                __hip_atomic_load(&counter[sj * MXM_CTR_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lo + (long long)taken < hi) live |= 1u << j;
+          }
+        }
+#pragma unroll
+        for (int d = SUB / 2; d >= 1; d >>= 1) live |= __shfl_xor(live, d, SUB);
+        if (live == 0u) {
+          drained = true;
+          break;
+        }
+        seg = (seg + 1 + (__ffs((int)live) - 1)) % MXM_NSEG;
+        if (try_claim(seg)) break;
+      }
+    }
+    if (next >= end) break;  // every segment handed out
+    const int r = next++;
+    const int i = rows[r];
+    const int ms = Mrp[i];
+    const int mnz = (Mend ? Mend[i] : Mrp[i + 1]) - ms;  // <= WR_MCAP by binning
+    const int g_row = row_group ? row_group[i] : 0;
+    if (g_row != cur_g) {
+      if (cur_g >= 0) flush();
+      cur_g = g_row;
+    }
+    for (int p = sl; p < mnz; p += SUB) {
+      const int c = Mcol[ms + p];
+      atomicOr(&bm[c >> 5], 1u << (c & 31));
+    }
+    const int maxc = (trunc && mnz > 0) ? Mcol[ms + mnz - 1] : 0x7FFFFFFF;
+    const int bbase = BATCHED ? gbase[row_group[i]] : 0;
+    __threadfence_block();
+    const int a_end = Arp[i + 1];
+    unsigned int hits = 0;  // this row's hits on this lane (32-bit adds)
+    for (int t0 = Arp[i]; t0 < a_end; t0 += SUB) {
+      const int ne = min(SUB, a_end - t0);
+      int b0 = 0, b1 = 0;  // lane sl: B row of entry sl of this chunk
+      if (sl < ne) {
+        const int k = bbase + Acol[t0 + sl];
+        b0 = Brp[k];
+        b1 = Brp[k + 1];
+        if (trunc) {
+          int lo = b0, hi = b1;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (Bcol[mid] <= maxc) lo = mid + 1;
+            else hi = mid;
+          }
+          b1 = lo;
+        }
+        wedges += (unsigned long long)(b1 - b0);
+      }
+      for (int e0 = 0; e0 < ne; e0 += EU) {
+        int bs[EU], be[EU], cc[EU];
+#pragma unroll
+        for (int u = 0; u < EU; ++u) {
+          const int e = e0 + u;
+          const int x = __shfl(b0, e < ne ? e : 0, SUB);
+          const int y = __shfl(b1, e < ne ? e : 0, SUB);
+          bs[u] = e < ne ? x : 0;
+          be[u] = e < ne ? y : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < EU; ++u) {
+          const int p = bs[u] + sl;
+          cc[u] = p < be[u] ? Bcol[p] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < EU; ++u)
+          if (cc[u] >= 0) hits += (bm[cc[u] >> 5] >> (cc[u] & 31)) & 1u;
+        // B rows longer than SUB columns: the rest, SUB columns at a time
+#pragma unroll
+        for (int u = 0; u < EU; ++u) {
+          for (int p = bs[u] + SUB + sl; p < be[u]; p += SUB) {
+            const int c = Bcol[p];
+            hits += (bm[c >> 5] >> (c & 31)) & 1u;
+          }
+        }
+      }
+    }
+    lsum += hits;
+    for (int p = sl; p < mnz; p += SUB) bm[Mcol[ms + p] >> 5] = 0u;
+    __threadfence_block();
+  }
+  if (cur_g >= 0) flush();
+  if (wedge_counter && wedges) atomicAdd(wedge_counter, wedges);
 }
 
 // Row binning: rows with an empty A row or empty mask get no work; rows
@@ -947,7 +1122,9 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // of its first launch; work-balanced lists change every launch
   const bool plan_hit = w.reuse_plan && !w.balance && !direct && w.plan_valid && w.plan_rp == A.rp &&
                         w.plan_m == m && w.plan_nnz == A.nnz && w.plan_tcap == TCAP && w.plan_wf == wave_flops;
-  if (w.reuse_plan && !w.balance && !direct) {
+  const bool keep_plan = w.reuse_plan && !w.balance && !direct;
+  if (keep_plan && !plan_hit) w.plan_rows = w.plan_items = -1;
+  if (keep_plan) {
     w.plan_valid = true;
     w.plan_rp = A.rp;
     w.plan_m = m;
@@ -1001,6 +1178,18 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     GA_HIP(hipGetLastError());
   }
   const bool wave_on = wave_flops >= 0;
+  if (keep_plan && !plan_hit) {
+    // the sizes of the kept plan, once (later launches skip an empty kernel)
+    auto* h = static_cast<int32_t*>(pinned_scratch());
+    h[0] = 0;
+    GA_HIP(hipMemcpyAsync(h + 1, w.start.as<int32_t>() + m, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    if (wave_on) GA_HIP(hipMemcpyAsync(h, w.rstart.as<int32_t>() + m, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    w.plan_rows = h[0];
+    w.plan_items = h[1];
+  }
+  const bool run_rows = wave_on && !(plan_hit && w.plan_rows == 0);
+  const bool run_items = !(plan_hit && w.plan_items == 0);
 
   auto krows = k_mxm_rows<T, PA, PB, MODE, BATCHED>;
   // rank lookup when the bitmap is at most 16 KB and every mask row fits
@@ -1019,7 +1208,9 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // claim sizes: short rows about 8 claims per resident wave (at most 32
   // rows per claim); items one at a time -- they are long and uneven, and
   // claiming several at once leaves a tail of heavy items on few workgroups
-  const int chunk_r = (int)std::max<int64_t>(1, std::min<int64_t>(32, m / (grid_r * WR_WAVES * 8)));
+  const int chunk_fix = w.row_chunk > 0 ? w.row_chunk : c.row_chunk;
+  const int chunk_r = chunk_fix > 0 ? chunk_fix
+                                    : (int)std::max<int64_t>(1, std::min<int64_t>(32, m / (grid_r * WR_WAVES * 8)));
   const int chunk_i = 1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c.prof_events_on) {
@@ -1028,26 +1219,53 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     GA_HIP(hipEventRecord(e0, s));
   }
   unsigned long long* wc = c.prof_bytes ? c.prof_dev.as<unsigned long long>() : nullptr;
-  if (wave_on && MODE == 1 && PA && PB) {
+  const int sum_sub = w.sum_sub >= 0 ? w.sum_sub : c.sum_sub;
+  if (run_rows && MODE == 1 && PA && PB && sum_sub > 0) {
+    // pattern sum mode, entry by entry: sum_sub lanes per short row
+    auto launch_entries = [&](auto kern, int groups) {
+      int occ = 0;
+      GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WR_WAVES, 0));
+      const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ, 1),
+                                                                 ceil_div(m, groups)));
+      const int chunk = chunk_fix > 0 ? chunk_fix
+                                      : (int)std::max<int64_t>(1, std::min<int64_t>(32, m / (grid * groups * 8)));
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
+                         M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
+                         (const int32_t*)(w.rstart.as<int32_t>() + m),
+                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, trunc, chunk);
+    };
+    switch (sum_sub) {
+      case 8: launch_entries(k_sum_entries<BATCHED, 8>, 64 * WR_WAVES / 8); break;
+      case 16: launch_entries(k_sum_entries<BATCHED, 16>, 64 * WR_WAVES / 16); break;
+      case 32: launch_entries(k_sum_entries<BATCHED, 32>, 64 * WR_WAVES / 32); break;
+      default: launch_entries(k_sum_entries<BATCHED, 64>, 64 * WR_WAVES / 64); break;
+    }
+  } else if (run_rows && MODE == 1 && PA && PB) {
     // pattern sum mode: 16 lanes per short row, four rows per wavefront
     auto ksub = k_sum_subrows<BATCHED>;
     int occ_s = 0;
     GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, ksub, 64 * WR_WAVES, 0));
     const int64_t grid_s = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ_s, 1),
                                                                  ceil_div(m, SR_GROUPS)));
-    const int chunk_s = (int)std::max<int64_t>(1, std::min<int64_t>(32, m / (grid_s * SR_GROUPS * 8)));
+    // rows per claim: about 8 claims per resident group, at most 4: the rows
+    // in flight on an XCD are its groups x the claim size, and their graphs'
+    // arrays share the XCD's L2 (4096 kNN graphs: 2.52 ms at 16 rows per
+    // claim, 2.28 ms at 4)
+    const int chunk_s = chunk_fix > 0 ? chunk_fix
+                                      : (int)std::max<int64_t>(1, std::min<int64_t>(4, m / (grid_s * SR_GROUPS * 8)));
     hipLaunchKernelGGL(ksub, dim3((unsigned)grid_s), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
                        M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
                        (const int32_t*)(w.rstart.as<int32_t>() + m),
                        w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, trunc, chunk_s);
-  } else if (wave_on)
+  } else if (run_rows)
     hipLaunchKernelGGL(krows, dim3((unsigned)grid_r), dim3(64 * WR_WAVES), 0, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col, Mend,
                      static_cast<T*>(Cval), sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
                      (const int32_t*)(w.rstart.as<int32_t>() + m), w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc,
                      trunc, chunk_r, mirror, opos);
   GA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(kitems, dim3((unsigned)grid_i), dim3(THREADS), dyn, s, A.rp, A.col,
+  if (run_items)
+    hipLaunchKernelGGL(kitems, dim3((unsigned)grid_i), dim3(THREADS), dyn, s, A.rp, A.col,
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col,
                      static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
                      direct ? (const int32_t*)n_extra : (const int32_t*)(w.start.as<int32_t>() + m),

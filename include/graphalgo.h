@@ -156,12 +156,17 @@ typedef struct ga_batch ga_batch;
 ga_status ga_batch_create(int64_t ngraphs, const ga_csr* graphs, ga_batch** out);
 ga_status ga_batch_destroy(ga_batch* b);
 /* counts[g] = triangles of graph g.  threads: workgroup width 64, 128 or 256
- * lanes (A entries per work item), 0 = the autotuned choice (256 before
- * tuning). */
+ * lanes (A entries per work item of a long row), 0 = the autotuned choice
+ * (256 before tuning); the short-row kernel takes the autotuned rows per
+ * claim (automatic before tuning). */
 ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts);
-/* Times the three widths on the resident batch, keeps the fastest;
- * ms_per_candidate (may be NULL) receives 3 times for 64, 128, 256. */
-ga_status ga_batch_autotune(ga_batch* b, int* best_threads, double* ms_per_candidate);
+/* Times the launch shapes on the resident batch -- item width 128 or 256 x
+ * rows per claim of the short-row kernel 2, 4, 8 or 16 (the rows in flight
+ * on an XCD share its L2) -- and keeps the fastest for later calls.
+ * ms_per_candidate (may be NULL) receives GA_BATCH_TUNE_CANDIDATES times,
+ * width-major: (128, 2), (128, 4), ..., (256, 16). */
+#define GA_BATCH_TUNE_CANDIDATES 8
+ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, double* ms_per_candidate);
 
 /* ---- serving ----------------------------------------------------------- */
 /* A request loop over graphs resident in HBM.  The reference has no serving

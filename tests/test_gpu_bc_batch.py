@@ -91,8 +91,8 @@ def test_batch_tc_knn(gpu, G, oracle):
     for threads in (64, 128, 256):
         print(f"case batch tc threads={threads}", flush=True)
         assert np.array_equal(B.triangle_counts(threads), want)
-    best, ms = B.autotune()
-    assert best in (64, 128, 256)
+    (width, claim), ms = B.autotune()
+    assert width in (128, 256) and claim in (2, 4, 8, 16) and len(ms) == 8
     assert np.array_equal(B.triangle_counts(), want)
     B.close()
 
