@@ -85,11 +85,13 @@ __global__ void k_filter_count(const int32_t* __restrict__ rp, const int32_t* __
   }
 }
 
+// nsup: the next pass's support array, cleared for the survivors (carry:
+// their supports carried over instead)
 __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                  const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t n,
                                  int thr, const int32_t* __restrict__ nrp, int32_t* __restrict__ ncol,
                                  int32_t* __restrict__ neid, int32_t* __restrict__ nold,
-                                 int32_t* __restrict__ newpos, int32_t* __restrict__ nsup) {
+                                 int32_t* __restrict__ newpos, int32_t* __restrict__ nsup, int carry) {
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t wpb = blockDim.x >> 6;
@@ -104,7 +106,7 @@ __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* 
         ncol[r] = col[j];
         neid[r] = eid[j];
         nold[r] = j;
-        if (nsup) nsup[r] = sup[j];  // carried over; the next pass recomputes the affected ones
+        if (nsup) nsup[r] = carry ? sup[j] : 0;
       }
       if (j < e) newpos[j] = keep ? r : -1;
       dst += __popcll(m);
@@ -115,7 +117,9 @@ __global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* 
 // The transposed entry of a survivor survives too (equal supports), so the
 // mirror index carries over through the old -> new position map.
 __global__ void k_mirror_update(const int32_t* __restrict__ mirror_old, const int32_t* __restrict__ nold,
-                                const int32_t* __restrict__ newpos, int64_t nnz, int32_t* __restrict__ mirror_new) {
+                                const int32_t* __restrict__ newpos, const int32_t* __restrict__ nnz_ptr,
+                                int32_t* __restrict__ mirror_new) {
+  const int64_t nnz = *nnz_ptr;  // survivors (the compacted row pointer's last entry)
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nnz; r += (int64_t)gridDim.x * blockDim.x) {
     const int mo = mirror_old[nold[r]];
     mirror_new[r] = mo >= 0 ? newpos[mo] : -1;
@@ -178,6 +182,18 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   int64_t passes = 0, steps = 0, wedges_host = 0;
   int32_t kmax = 2;
   int32_t* kept_h = static_cast<int32_t*>(pinned_scratch());
+  // the host waits for the survivor count only (an event after its copy);
+  // the compaction is queued behind it before the count is known, so the
+  // device never idles while the host decides (when nothing is removed the
+  // compaction writes the spare buffers, which the next real one rewrites)
+  hipEvent_t kept_ev;
+  GA_HIP(hipEventCreateWithFlags(&kept_ev, hipEventDisableTiming));
+  struct EventGuard {
+    hipEvent_t e;
+    ~EventGuard() { (void)hipEventDestroy(e); }
+  } kept_guard{kept_ev};
+  const bool selective = ctx().selective_support;
+  bool sup_zeroed = false;  // tw.sup[cur] was cleared by the compaction that made cur
   {
     while (true) {
       if (all_k && nnz == 0) break;
@@ -186,8 +202,8 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       if (!valid) {
         // first pass: every support; later passes: only the entries with an
         // endpoint that lost entries in the last compaction (the rest carry over)
-        if (passes == 0 || !ctx().selective_support)
-          support_symmetric_dev(C, tw.mirror[cur].as<int32_t>(), sup, tw.mw, s);
+        if (passes == 0 || !selective)
+          support_symmetric_dev(C, tw.mirror[cur].as<int32_t>(), sup, tw.mw, s, sup_zeroed);
         else
           support_selected_dev(C, tw.mirror[cur].as<int32_t>(), tw.dirty.as<uint8_t>(), sup, tw.sw, tw.mw, s);
         ++passes;
@@ -201,7 +217,18 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       int32_t* nrp = tw.cur[cur ^ 1].rp.as<int32_t>();
       scan_exclusive_i32(tw.cnt.as<int32_t>(), nrp, n, tw.scan_tmp, s);
       GA_HIP(hipMemcpyAsync(kept_h, nrp + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      GA_HIP(hipStreamSynchronize(s));
+      GA_HIP(hipEventRecord(kept_ev, s));
+      hipLaunchKernelGGL(k_filter_scatter, dim3(wgrid), dim3(256), 0, s, C.rp, C.col,
+                         (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, n, thr,
+                         (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>(),
+                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(), tw.sup[cur ^ 1].as<int32_t>(),
+                         selective ? 1 : 0);
+      hipLaunchKernelGGL(k_mirror_update, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s,
+                         (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
+                         (const int32_t*)tw.newpos.as<int32_t>(), (const int32_t*)(nrp + n),
+                         tw.mirror[cur ^ 1].as<int32_t>());
+      GA_HIP(hipGetLastError());
+      GA_HIP(hipEventSynchronize(kept_ev));
       const int64_t kept = *kept_h;
       ++steps;
       if (kept == nnz) {
@@ -219,18 +246,10 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
         ++k;
         continue;
       }
-      hipLaunchKernelGGL(k_filter_scatter, dim3(wgrid), dim3(256), 0, s, C.rp, C.col,
-                         (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, n, thr,
-                         (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>(),
-                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(),
-                         ctx().selective_support ? tw.sup[cur ^ 1].as<int32_t>() : nullptr);
-      hipLaunchKernelGGL(k_mirror_update, dim3(grid_for(kept, 256, 8192)), dim3(256), 0, s,
-                         (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
-                         (const int32_t*)tw.newpos.as<int32_t>(), kept, tw.mirror[cur ^ 1].as<int32_t>());
-      GA_HIP(hipGetLastError());
       cur ^= 1;
       nnz = kept;
       valid = false;
+      sup_zeroed = !selective;
     }
   }
   GA_HIP(hipStreamSynchronize(s));

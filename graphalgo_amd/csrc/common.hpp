@@ -280,7 +280,8 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
 // on the strict lower half and mirrored (the k-truss support pass).
 // mirror[t] = position of entry t's transposed entry; kept valid through
 // compaction by the k-truss loop.
-void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s);
+void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s,
+                           bool s_zeroed = false);
 void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s);
 // Support pass over the affected entries only (entries with an endpoint in a
 // row flagged dirty); the other entries of S keep their values.

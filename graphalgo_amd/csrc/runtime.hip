@@ -206,6 +206,44 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tile_apply(const int32_t* __res
   }
 }
 
+// Tile scan with the tile offsets summed in place: each workgroup adds the
+// sums of the tiles before it (at most SCAN_FUSED_TILES of them), so short
+// scans take two launches instead of three.  The last tile writes the total.
+constexpr int SCAN_FUSED_TILES = 1024;
+__global__ __launch_bounds__(SCAN_T) void k_scan_tile_apply_fused(const int32_t* __restrict__ in,
+                                                                  int32_t* __restrict__ out, int64_t n,
+                                                                  const long long* __restrict__ bsum) {
+  __shared__ long long wtot[SCAN_T / 64];
+  __shared__ long long prior[SCAN_T / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  long long before = 0;
+  for (int b = tid; b < (int)blockIdx.x; b += SCAN_T) before += bsum[b];
+  before = wave_reduce_sum(before);
+  if (lane == 0) prior[wv] = before;
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)tid * SCAN_PER;
+  int32_t v[SCAN_PER];
+  long long tsum = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_PER; ++q) {
+    v[q] = (base + q < n) ? in[base + q] : 0;
+    tsum += v[q];
+  }
+  const long long incl = wave_incl_scan(tsum);
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  long long off = incl - tsum;
+  for (int k = 0; k < SCAN_T / 64; ++k) {
+    off += prior[k];
+    if (k < wv) off += wtot[k];
+  }
+#pragma unroll
+  for (int q = 0; q < SCAN_PER; ++q) {
+    if (base + q < n) out[base + q] = (int32_t)off;
+    off += v[q];
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == SCAN_T - 1) out[n] = (int32_t)off;
+}
+
 size_t scan_tmp_bytes(int64_t n) { return n <= 0 ? 0 : lease_bytes(sizeof(long long) * ceil_div(n, SCAN_TILE)); }
 
 void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf& tmp, hipStream_t s) {
@@ -217,8 +255,14 @@ void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf& tmp,
   tmp.ensure(sizeof(long long) * nb);
   long long* bsum = tmp.as<long long>();
   hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, n, bsum);
-  hipLaunchKernelGGL(k_scan_tile_offsets, dim3(1), dim3(1024), 0, s, bsum, nb, out + n);
-  hipLaunchKernelGGL(k_scan_tile_apply, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, n, (const long long*)bsum);
+  if (nb <= SCAN_FUSED_TILES) {
+    hipLaunchKernelGGL(k_scan_tile_apply_fused, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, n,
+                       (const long long*)bsum);
+  } else {
+    hipLaunchKernelGGL(k_scan_tile_offsets, dim3(1), dim3(1024), 0, s, bsum, nb, out + n);
+    hipLaunchKernelGGL(k_scan_tile_apply, dim3((unsigned)nb), dim3(SCAN_T), 0, s, in, out, n,
+                       (const long long*)bsum);
+  }
   GA_HIP(hipGetLastError());
 }
 

@@ -305,16 +305,20 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
     __syncthreads();
 
     // Wedge mapping: lanes form groups of MXM_G, and group g owns the
-    // contiguous wedge range [g0, g1) = [g * total / NG, (g + 1) * total / NG).
-    // A trip of the group covers its next MXM_G * MXM_K wedges, lane r taking
-    // every MXM_G-th of them: the MXM_G lanes read adjacent B columns
-    // (64-byte runs) and a lane's wedges are MXM_G apart, so its entry only
-    // moves forward by a short linear walk.  One binary search per lane and
-    // item finds the entry of its first wedge; there is none per trip.
+    // contiguous wedge range [g0, g1) of span = ceil(total / NG) rounded up to
+    // whole trips (small items leave the last groups idle rather than run
+    // every group on a partly filled trip).  A trip of the group covers its
+    // next MXM_G * MXM_K wedges, lane r taking every MXM_G-th of them: the
+    // MXM_G lanes read adjacent B columns (64-byte runs) and a lane's wedges
+    // are MXM_G apart, so its entry only moves forward by a short linear
+    // walk.  One binary search per lane and item finds the entry of its
+    // first wedge; there is none per trip.
     const int grp = tid / MXM_G, rl = tid % MXM_G;
     constexpr int MXM_NG = MXM_THREADS / MXM_G;
-    const int g0 = (int)(((long long)total * grp) / MXM_NG);
-    const int g1 = (int)(((long long)total * (grp + 1)) / MXM_NG);
+    constexpr int MXM_RUN = MXM_G * MXM_K;
+    const long long span = (((long long)total + MXM_NG - 1) / MXM_NG + MXM_RUN - 1) / MXM_RUN * MXM_RUN;
+    const int g0 = (int)min<long long>(total, span * grp);
+    const int g1 = (int)min<long long>(total, span * (grp + 1));
     // the loop is instantiated twice: mask row staged in LDS (sorted list and
     // LDS accumulator) or not (search and accumulate in global memory), so
     // the hot loop has no branch on it
@@ -330,7 +334,7 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
         }
         e = lo;
       }
-      for (int pos = g0; pos < g1; pos += MXM_G * MXM_K) {
+      for (int pos = g0; pos < g1; pos += MXM_RUN) {
         const int p0 = pos + rl;
         if (p0 >= g1) break;  // later trips start even further out
         int jb = s_jb[e];
@@ -789,19 +793,12 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
         wedges += (unsigned long long)total;
       }
       __threadfence_block();
+      // the group's trips cover consecutive runs of the chunk's wedges, so a
+      // lane's entry only moves forward: a linear walk, no search
       int e = 0;
       for (int base = 0; base < total; base += SR_SUB * SR_K) {
         const int p0 = base + sl;
         if (p0 >= total) break;
-        {
-          int lo = e, hi = ne - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (eoff[mid] <= p0) lo = mid;
-            else hi = mid - 1;
-          }
-          e = lo;
-        }
         int jb = jbase[e];
         int nb = eoff[e + 1];
         int cc[SR_K];
@@ -1024,12 +1021,35 @@ __global__ void k_emit_rows(const int32_t* __restrict__ flag, const int32_t* __r
 // wedges per row are known -- enough items of about `target` wedges each
 // (entries split evenly), capped at MXM_SPLIT_MAX times the entry-based count
 // and at one entry per item.
+// Also clears what the launch that follows accumulates into (the work
+// counters, the per-row wedge counts of this launch) and, with split_col,
+// computes the mask prefix Mend[i] = first entry of row i with column >= i
+// (the strict lower half), so none of these needs a launch of its own.
 __global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp,
                              const int32_t* __restrict__ Mend, int64_t m, int tcap, int32_t* __restrict__ cnt,
-                             const unsigned long long* __restrict__ work_prev, long long target) {
+                             const unsigned long long* __restrict__ work_prev, long long target,
+                             uint32_t* __restrict__ ctr_clear, int ctr_words,
+                             unsigned long long* __restrict__ work_clear, const int32_t* __restrict__ split_col,
+                             int32_t* __restrict__ split_out) {
+  if (ctr_clear && blockIdx.x == 0)
+    for (int q = threadIdx.x; q < ctr_words; q += blockDim.x) ctr_clear[q] = 0u;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    if (work_clear) work_clear[i] = 0ull;
     const int a = Arp[i + 1] - Arp[i];
-    const int mz = (Mend ? Mend[i] : Mrp[i + 1]) - Mrp[i];
+    int mend;
+    if (split_col) {
+      int lo = Mrp[i], hi = Mrp[i + 1];
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (split_col[mid] < (int)i) lo = mid + 1;
+        else hi = mid;
+      }
+      split_out[i] = lo;
+      mend = lo;
+    } else {
+      mend = Mend ? Mend[i] : Mrp[i + 1];
+    }
+    const int mz = mend - Mrp[i];
     int c = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
     if (c > 0 && work_prev) {
       const long long byw = (long long)((work_prev[i] + (unsigned long long)target - 1) / (unsigned long long)target);
@@ -1074,6 +1094,20 @@ __global__ void k_item_fill(const int32_t* __restrict__ Arp, const int32_t* __re
   }
 }
 
+// first entry of row i with column >= i (the end of the strict lower half)
+__global__ void k_row_split(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                            int32_t* __restrict__ split) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = rp[i], hi = rp[i + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (col[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    split[i] = lo;
+  }
+}
+
 static hipEvent_t take_event(Context& c) {
   hipEvent_t ev;
   if (!c.event_pool.empty()) {
@@ -1092,7 +1126,9 @@ template <typename T, bool PA, bool PB, int MODE, int THREADS, bool BATCHED>
 static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval, unsigned long long* sums,
                           const int32_t* row_group, const int32_t* gbase, MxmWork& w, int bitmap_words,
                           hipStream_t s, const int32_t* Mend, int trunc, const int32_t* mirror = nullptr,
-                          const int32_t* opos = nullptr) {
+                          const int32_t* opos = nullptr, int32_t* lower_out = nullptr) {
+  // lower_out: the mask is the strict lower half of M; its row ends are
+  // computed here into lower_out (by the item count when it runs) and used as Mend
   constexpr int TCAP = THREADS;
   const int64_t m = A.nrows;
   const int64_t max_items = (w.balance ? MXM_SPLIT_MAX : 1) * (m + ceil_div(A.nnz, TCAP)) + 1;
@@ -1111,7 +1147,6 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // lines, then the extra-item cursor on a line of its own
   const int ctr_words = MXM_CTR_WORDS;
   w.counter.ensure(sizeof(uint32_t) * ctr_words);
-  fill_u32(w.counter.p, 0u, ctr_words, s);
   const long long wave_flops =
       (M.ncols <= WR_MAXCOLS) ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops) : -1;
   // direct mode (no wavefront bin): items 0..m-1 are the rows' first chunks and
@@ -1132,6 +1167,17 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     w.plan_tcap = TCAP;
     w.plan_wf = wave_flops;
   }
+  // the listed path's item count also clears the counters and the per-row
+  // work and computes the lower-half row ends; other paths launch them
+  const bool listed = !direct && !plan_hit && wave_flops < 0;
+  if (!listed) {
+    fill_u32(w.counter.p, 0u, ctr_words, s);
+    if (lower_out) {
+      hipLaunchKernelGGL(k_row_split, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, M.rp, M.col, m, lower_out);
+      GA_HIP(hipGetLastError());
+    }
+  }
+  if (lower_out) Mend = lower_out;
   // work-balanced items (listed path of repeated products): the previous
   // launch's wedges per row set this launch's split
   unsigned long long* rw_cur = nullptr;
@@ -1141,7 +1187,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     w.rwork[1].ensure(sizeof(unsigned long long) * m);
     rw_cur = w.rwork[w.rwork_cur].as<unsigned long long>();
     rw_prev = w.rwork_have ? w.rwork[w.rwork_cur ^ 1].as<unsigned long long>() : nullptr;
-    fill_u64(rw_cur, 0ull, m, s);
+    if (!listed) fill_u64(rw_cur, 0ull, m, s);
     w.rwork_cur ^= 1;
     w.rwork_have = true;
   }
@@ -1154,8 +1200,9 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     // same operands as the previous launch: its row bins and items stand
   } else if (wave_flops < 0) {
     // no wavefront bin: items of every non-empty row, listed in row order
-    hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp, Mend, m, TCAP,
-                       w.cnt.as<int32_t>(), rw_prev, ctx().item_wedges);
+    hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp,
+                       lower_out ? nullptr : Mend, m, TCAP, w.cnt.as<int32_t>(), rw_prev, ctx().item_wedges,
+                       w.counter.as<uint32_t>(), ctr_words, rw_cur, lower_out ? M.col : nullptr, lower_out);
     GA_HIP(hipGetLastError());
     scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
     hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
@@ -1339,18 +1386,6 @@ void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t*
 // (mask prefix Mend[i] = first entry of row i with column >= i, B rows cut
 // at the last lower mask column) and the values are mirrored to the upper
 // half.  Same values as the full product, about half the wedges.
-__global__ void k_row_split(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
-                            int32_t* __restrict__ split) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    int lo = rp[i], hi = rp[i + 1];
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (col[mid] < i) lo = mid + 1;
-      else hi = mid;
-    }
-    split[i] = lo;
-  }
-}
 
 // mirror[t] = position of the transposed entry of entry t (symmetric pattern)
 __global__ void k_mirror_index(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
@@ -1390,21 +1425,21 @@ size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend) {
   return b;
 }
 
-void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s) {
+void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s,
+                           bool s_zeroed) {
   const int64_t n = C.nrows;
   GA_REQUIRE(C.nrows == C.ncols, GA_ERR_DIM, "symmetric support needs a square matrix");
-  if (C.nnz > 0) fill_u32(S, 0u, C.nnz, s);
+  if (C.nnz > 0 && !s_zeroed) fill_u32(S, 0u, C.nnz, s);  // s_zeroed: the caller's compaction cleared S
   if (n == 0 || C.nnz == 0) return;
   w.mend.ensure(sizeof(int32_t) * n);
   int32_t* split = w.mend.as<int32_t>();
-  hipLaunchKernelGGL(k_row_split, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, C.rp, C.col, n, split);
   const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : 0;
   Context& c = ctx();
   if (c.prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
                        c.prof_dev.as<unsigned long long>() + 1);
-  launch_binned<int32_t, true, true, 0, MXM_WIDE, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, split,
-                                                          1, mirror);
+  launch_binned<int32_t, true, true, 0, MXM_WIDE, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, nullptr,
+                                                          1, mirror, nullptr, split);
 }
 
 // ---- support of the affected entries only ----------------------------------
