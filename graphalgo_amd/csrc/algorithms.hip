@@ -65,52 +65,111 @@ static void upper_triangle_dev(const DCsr& A, OwnedCsr& U, DevBuf& tmp, hipStrea
   GA_HIP(hipGetLastError());
 }
 
-// ---- filter / compaction (wavefront per row) -------------------------------
-// cnt[i] = entries of row i with support >= thr; dirty[i] = row i loses some
-__global__ void k_filter_count(const int32_t* __restrict__ rp, const int32_t* __restrict__ sup, int64_t n, int thr,
-                               int32_t* __restrict__ cnt, uint8_t* __restrict__ dirty) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wpb = blockDim.x >> 6;
-  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
-    const int b = rp[i], e = rp[i + 1];
-    int c = 0;
-    for (int j = b + lane; j - lane < e; j += 64) {
-      const bool keep = j < e && sup[j] >= thr;
-      c += __popcll(__ballot(keep));
-    }
-    if (lane == 0) {
-      cnt[i] = c;
-      dirty[i] = c < e - b ? 1 : 0;
-    }
+// ---- filter / compaction (entry-parallel) ---------------------------------
+// The entries with support >= thr survive.  The CSR keeps its order, so the
+// new position of a survivor is the number of survivors before it: one
+// exclusive scan over the entries (tiles of KEEP_TILE) places every entry,
+// and the new row pointer of row i is that count at rp[i].  Every thread
+// owns KEEP_PER consecutive entries; all loads and stores are coalesced and
+// the work does not depend on the row lengths.
+constexpr int KEEP_T = 256;
+constexpr int KEEP_PER = 8;
+constexpr int KEEP_TILE = KEEP_T * KEEP_PER;
+constexpr int KEEP_FUSED_TILES = 1024;  // up to here a tile sums the tiles before it itself
+
+__global__ __launch_bounds__(KEEP_T) void k_keep_tile_sums(const int32_t* __restrict__ sup, int64_t nnz, int thr,
+                                                           long long* __restrict__ bsum) {
+  __shared__ long long red[KEEP_T / 64];
+  const int64_t base = (int64_t)blockIdx.x * KEEP_TILE;
+  long long c = 0;
+  for (int q = threadIdx.x; q < KEEP_TILE; q += KEEP_T) {
+    const int64_t j = base + q;
+    if (j < nnz) c += sup[j] >= thr ? 1 : 0;
+  }
+  c = wave_reduce_sum(c);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long t = 0;
+    for (int w = 0; w < KEEP_T / 64; ++w) t += red[w];
+    bsum[blockIdx.x] = t;
   }
 }
 
-// nsup: the next pass's support array, cleared for the survivors (carry:
-// their supports carried over instead)
-__global__ void k_filter_scatter(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                 const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t n,
-                                 int thr, const int32_t* __restrict__ nrp, int32_t* __restrict__ ncol,
-                                 int32_t* __restrict__ neid, int32_t* __restrict__ nold,
-                                 int32_t* __restrict__ newpos, int32_t* __restrict__ nsup, int carry) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t wpb = blockDim.x >> 6;
-  for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * wpb) {
-    const int b = rp[i], e = rp[i + 1];
-    int dst = nrp[i];
-    for (int j = b + lane; j - lane < e; j += 64) {
-      const bool keep = j < e && sup[j] >= thr;
-      const unsigned long long m = __ballot(keep);
-      const int r = dst + __popcll(m & lt);
-      if (keep) {
-        ncol[r] = col[j];
-        neid[r] = eid[j];
-        nold[r] = j;
-        if (nsup) nsup[r] = carry ? sup[j] : 0;
-      }
-      if (j < e) newpos[j] = keep ? r : -1;
-      dst += __popcll(m);
+// Survivors to their new positions (col, entry id, old position, the next
+// pass's support: cleared, or carried with `carry`); newpos[j] = new position
+// of a survivor, -1 - (survivors before j) for a removed entry.  bsum: tile
+// sums, or exclusive tile offsets when offsets_ready.  The last tile writes
+// the survivor count to *kept.
+__global__ __launch_bounds__(KEEP_T) void k_keep_scatter(
+    const int32_t* __restrict__ col, const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t nnz,
+    int thr, const long long* __restrict__ bsum, int offsets_ready, int32_t* __restrict__ ncol,
+    int32_t* __restrict__ neid, int32_t* __restrict__ nold, int32_t* __restrict__ newpos, int32_t* __restrict__ nsup,
+    int carry, int32_t* __restrict__ kept) {
+  __shared__ long long wtot[KEEP_T / 64];
+  __shared__ long long prior[KEEP_T / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  long long before = 0;
+  if (offsets_ready) {
+    before = tid == 0 ? bsum[blockIdx.x] : 0;
+  } else {
+    for (int b = tid; b < (int)blockIdx.x; b += KEEP_T) before += bsum[b];
+  }
+  before = wave_reduce_sum(before);
+  if (lane == 0) prior[wv] = before;
+  const int64_t base = (int64_t)blockIdx.x * KEEP_TILE + (int64_t)tid * KEEP_PER;
+  int32_t sv[KEEP_PER];
+  int tsum = 0;
+#pragma unroll
+  for (int q = 0; q < KEEP_PER; ++q) {
+    sv[q] = base + q < nnz ? sup[base + q] : -1;
+    tsum += (base + q < nnz && sv[q] >= thr) ? 1 : 0;
+  }
+  const long long incl = wave_incl_scan((long long)tsum);
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  long long off = incl - tsum;
+  for (int k = 0; k < KEEP_T / 64; ++k) {
+    off += prior[k];
+    if (k < wv) off += wtot[k];
+  }
+#pragma unroll
+  for (int q = 0; q < KEEP_PER; ++q) {
+    const int64_t j = base + q;
+    if (j >= nnz) break;
+    const bool keep = sv[q] >= thr;
+    const int r = (int)off;
+    if (keep) {
+      ncol[r] = col[j];
+      neid[r] = eid[j];
+      nold[r] = (int32_t)j;
+      nsup[r] = carry ? sv[q] : 0;
     }
+    newpos[j] = keep ? r : -1 - r;
+    off += keep ? 1 : 0;
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == KEEP_T - 1) *kept = (int32_t)off;
+}
+
+// nrp[i] = survivors before row i (the count at rp[i]); dirty[i] = row i
+// loses entries (only when dirty is given)
+__global__ void k_keep_rows(const int32_t* __restrict__ rp, int64_t n, const int32_t* __restrict__ newpos,
+                            int64_t nnz, const int32_t* __restrict__ kept, int32_t* __restrict__ nrp,
+                            uint8_t* __restrict__ dirty) {
+  const int32_t total = *kept;
+  auto before = [&](int64_t j) -> int32_t {
+    if (j >= nnz) return total;
+    const int32_t p = newpos[j];
+    return p >= 0 ? p : -1 - p;
+  };
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i == n) {
+      nrp[n] = total;
+      continue;
+    }
+    const int32_t a = before(rp[i]);
+    nrp[i] = a;
+    if (dirty) dirty[i] = (before(rp[i + 1]) - a) < (rp[i + 1] - rp[i]) ? 1 : 0;
   }
 }
 
@@ -159,9 +218,9 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   }
   tw.nold.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
   tw.newpos.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
-  tw.cnt.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1));
   tw.dirty.ensure(std::max<int64_t>(n, 1));
   tw.kept.ensure(64);
+  tw.scan_tmp.ensure(sizeof(long long) * std::max<int64_t>(ceil_div(nnz0, KEEP_TILE), 1));
   tw.mw.balance = true;  // the support passes split rows by the previous pass's work
   tw.mw.rwork_have = false;
   GA_HIP(hipMemcpyAsync(tw.cur[0].rp.p, A.rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s));
@@ -183,9 +242,10 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   int32_t kmax = 2;
   int32_t* kept_h = static_cast<int32_t*>(pinned_scratch());
   // the host waits for the survivor count only (an event after its copy);
-  // the compaction is queued behind it before the count is known, so the
-  // device never idles while the host decides (when nothing is removed the
-  // compaction writes the spare buffers, which the next real one rewrites)
+  // the compaction runs before the count is known and its last kernels are
+  // queued behind the copy, so the device never idles while the host decides
+  // (when nothing is removed the compaction writes the spare buffers, which
+  // the next real one rewrites)
   hipEvent_t kept_ev;
   GA_HIP(hipEventCreateWithFlags(&kept_ev, hipEventDisableTiming));
   struct EventGuard {
@@ -210,22 +270,39 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
         valid = true;
       }
       const int thr = k - 2;
-      const unsigned wgrid = grid_for(n, 4, 16384);
-      hipLaunchKernelGGL(k_filter_count, dim3(wgrid), dim3(256), 0, s, C.rp, (const int32_t*)sup, n, thr,
-                         tw.cnt.as<int32_t>(), tw.dirty.as<uint8_t>());
-      GA_HIP(hipGetLastError());
+      // compaction at threshold thr, queued before the survivor count is known
       int32_t* nrp = tw.cur[cur ^ 1].rp.as<int32_t>();
-      scan_exclusive_i32(tw.cnt.as<int32_t>(), nrp, n, tw.scan_tmp, s);
-      GA_HIP(hipMemcpyAsync(kept_h, nrp + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      int32_t* kept_dev = tw.kept.as<int32_t>();
+      long long* bsum = tw.scan_tmp.as<long long>();
+      const int64_t nbk = ceil_div(nnz, KEEP_TILE);
+      if (nnz == 0) {  // nothing to filter (a k-truss that became empty)
+        ++steps;
+        if (!all_k) {
+          if (nnz_at) nnz_at[kidx] = 0;
+          if (passes_at) passes_at[kidx] = (int32_t)passes;
+          if (++kidx == nk) break;
+          k = ks[kidx];
+          continue;
+        }
+        break;
+      }
+      hipLaunchKernelGGL(k_keep_tile_sums, dim3((unsigned)nbk), dim3(KEEP_T), 0, s, (const int32_t*)sup, nnz, thr,
+                         bsum);
+      const bool offsets = nbk > KEEP_FUSED_TILES;
+      if (offsets) scan_tile_offsets(bsum, nbk, kept_dev, s);
+      hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nbk), dim3(KEEP_T), 0, s, (const int32_t*)C.col,
+                         (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, nnz, thr,
+                         (const long long*)bsum, offsets ? 1 : 0, tw.cur[cur ^ 1].col.as<int32_t>(),
+                         tw.eid[cur ^ 1].as<int32_t>(), tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(),
+                         tw.sup[cur ^ 1].as<int32_t>(), selective ? 1 : 0, kept_dev);
+      GA_HIP(hipMemcpyAsync(kept_h, kept_dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
       GA_HIP(hipEventRecord(kept_ev, s));
-      hipLaunchKernelGGL(k_filter_scatter, dim3(wgrid), dim3(256), 0, s, C.rp, C.col,
-                         (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, n, thr,
-                         (const int32_t*)nrp, tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>(),
-                         tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(), tw.sup[cur ^ 1].as<int32_t>(),
-                         selective ? 1 : 0);
+      hipLaunchKernelGGL(k_keep_rows, dim3(grid_for(n + 1, 256, 8192)), dim3(256), 0, s, C.rp, n,
+                         (const int32_t*)tw.newpos.as<int32_t>(), nnz, (const int32_t*)kept_dev, nrp,
+                         selective ? tw.dirty.as<uint8_t>() : nullptr);
       hipLaunchKernelGGL(k_mirror_update, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s,
                          (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
-                         (const int32_t*)tw.newpos.as<int32_t>(), (const int32_t*)(nrp + n),
+                         (const int32_t*)tw.newpos.as<int32_t>(), (const int32_t*)kept_dev,
                          tw.mirror[cur ^ 1].as<int32_t>());
       GA_HIP(hipGetLastError());
       GA_HIP(hipEventSynchronize(kept_ev));
@@ -292,9 +369,9 @@ size_t ktruss_workspace_bytes(int64_t n, int64_t nnz) {
   const size_t e = lease_bytes(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
   size_t b = 2 * (lease_bytes(sizeof(int32_t) * (n + 1)) + 4 * e);  // cur rp/col, eid, mirror, sup
   b += 2 * e;                                                       // nold, newpos
-  b += lease_bytes(sizeof(int32_t) * std::max<int64_t>(n, 1)) + lease_bytes(std::max<int64_t>(n, 1));
-  b += lease_bytes(64);
-  b += scan_tmp_bytes(n);
+  b += lease_bytes(std::max<int64_t>(n, 1));                                                  // dirty
+  b += lease_bytes(64);                                                                         // kept
+  b += lease_bytes(sizeof(long long) * std::max<int64_t>(ceil_div(std::max<int64_t>(nnz, 1), KEEP_TILE), 1));
   b += mxm_workspace_bytes(n, nnz, true);
   b += sel_workspace_bytes(n, nnz);
   return b;

@@ -233,6 +233,9 @@ __device__ inline void atomic_add_val(double* p, double v) { atomicAdd(p, v); }
 // Exclusive scan of n int32 counts into out[0..n]; out[n] = total.  Totals
 // must fit in int32 (they index CSR arrays).  tmp is grown as needed.
 void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf& tmp, hipStream_t s);
+// In place: bsum[0..nb) (per-tile sums) becomes exclusive tile offsets; the
+// total goes to *total (one workgroup).
+void scan_tile_offsets(long long* bsum, int64_t nb, int32_t* total, hipStream_t s);
 // Fill n 32-bit words with a value (graph-capture safe: a kernel, not a memset).
 void fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s);
 void fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s);

@@ -244,6 +244,11 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tile_apply_fused(const int32_t*
   if (blockIdx.x == gridDim.x - 1 && tid == SCAN_T - 1) out[n] = (int32_t)off;
 }
 
+void scan_tile_offsets(long long* bsum, int64_t nb, int32_t* total, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_tile_offsets, dim3(1), dim3(1024), 0, s, bsum, nb, total);
+  GA_HIP(hipGetLastError());
+}
+
 size_t scan_tmp_bytes(int64_t n) { return n <= 0 ? 0 : lease_bytes(sizeof(long long) * ceil_div(n, SCAN_TILE)); }
 
 void scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, DevBuf& tmp, hipStream_t s) {
