@@ -1021,19 +1021,32 @@ __global__ void k_emit_rows(const int32_t* __restrict__ flag, const int32_t* __r
 // wedges per row are known -- enough items of about `target` wedges each
 // (entries split evenly), capped at MXM_SPLIT_MAX times the entry-based count
 // and at one entry per item.
-// Also clears what the launch that follows accumulates into (the work
-// counters, the per-row wedge counts of this launch) and, with split_col,
-// computes the mask prefix Mend[i] = first entry of row i with column >= i
-// (the strict lower half), so none of these needs a launch of its own.
-__global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp,
-                             const int32_t* __restrict__ Mend, int64_t m, int tcap, int32_t* __restrict__ cnt,
-                             const unsigned long long* __restrict__ work_prev, long long target,
-                             uint32_t* __restrict__ ctr_clear, int ctr_words,
-                             unsigned long long* __restrict__ work_clear, const int32_t* __restrict__ split_col,
-                             int32_t* __restrict__ split_out) {
+// Item planning of the listed path in two launches.  k_item_plan_count (one
+// thread per row, PLAN_T rows per workgroup) counts each row's items -- one
+// per tcap entries, raised to the previous launch's wedges / target when
+// work_prev is given (at most MXM_SPLIT_MAX times the entry-based count) --
+// and leaves per-workgroup totals.  It also clears what the launch that
+// follows accumulates into (the work counters, this launch's per-row wedge
+// counts) and, with split_col, computes the mask prefix Mend[i] = first
+// entry of row i with column >= i (the strict lower half), so none of these
+// needs a launch of its own.  k_item_plan_fill scans the totals (each
+// workgroup adds the totals before it, or reads offsets prepared by
+// scan_tile_offsets beyond PLAN_FUSED_TILES workgroups), places each row's
+// items as k_item_fill does and writes the item count to start[m].
+constexpr int PLAN_T = 256;
+constexpr int PLAN_FUSED_TILES = 1024;
+
+__global__ __launch_bounds__(PLAN_T) void k_item_plan_count(
+    const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mend, int64_t m,
+    int tcap, int32_t* __restrict__ cnt, const unsigned long long* __restrict__ work_prev, long long target,
+    uint32_t* __restrict__ ctr_clear, int ctr_words, unsigned long long* __restrict__ work_clear,
+    const int32_t* __restrict__ split_col, int32_t* __restrict__ split_out, long long* __restrict__ bsum) {
+  __shared__ long long red[PLAN_T / 64];
   if (ctr_clear && blockIdx.x == 0)
-    for (int q = threadIdx.x; q < ctr_words; q += blockDim.x) ctr_clear[q] = 0u;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int q = threadIdx.x; q < ctr_words; q += PLAN_T) ctr_clear[q] = 0u;
+  const int64_t i = (int64_t)blockIdx.x * PLAN_T + threadIdx.x;
+  int c = 0;
+  if (i < m) {
     if (work_clear) work_clear[i] = 0ull;
     const int a = Arp[i + 1] - Arp[i];
     int mend;
@@ -1050,13 +1063,59 @@ __global__ void k_item_count(const int32_t* __restrict__ Arp, const int32_t* __r
       mend = Mend ? Mend[i] : Mrp[i + 1];
     }
     const int mz = mend - Mrp[i];
-    int c = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
+    c = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
     if (c > 0 && work_prev) {
       const long long byw = (long long)((work_prev[i] + (unsigned long long)target - 1) / (unsigned long long)target);
       c = (int)min<long long>(max<long long>(c, byw), min<long long>((long long)c * MXM_SPLIT_MAX, a));
     }
     cnt[i] = c;
   }
+  long long t = wave_reduce_sum((long long)c);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long b = 0;
+    for (int w = 0; w < PLAN_T / 64; ++w) b += red[w];
+    bsum[blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(PLAN_T) void k_item_plan_fill(const int32_t* __restrict__ Arp,
+                                                           const int32_t* __restrict__ cnt, int64_t m,
+                                                           const long long* __restrict__ bsum, int offsets_ready,
+                                                           int32_t* __restrict__ start, int32_t* __restrict__ item_row,
+                                                           int32_t* __restrict__ item_t0) {
+  __shared__ long long wtot[PLAN_T / 64];
+  __shared__ long long prior[PLAN_T / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  long long before = 0;
+  if (offsets_ready) {
+    before = tid == 0 ? bsum[blockIdx.x] : 0;
+  } else {
+    for (int b = tid; b < (int)blockIdx.x; b += PLAN_T) before += bsum[b];
+  }
+  before = wave_reduce_sum(before);
+  if (lane == 0) prior[wv] = before;
+  const int64_t i = (int64_t)blockIdx.x * PLAN_T + tid;
+  const int c = i < m ? cnt[i] : 0;
+  const long long incl = wave_incl_scan((long long)c);
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  long long s0 = incl - c;
+  for (int k = 0; k < PLAN_T / 64; ++k) {
+    s0 += prior[k];
+    if (k < wv) s0 += wtot[k];
+  }
+  if (i < m) {
+    start[i] = (int32_t)s0;
+    const int a0 = Arp[i];
+    const int64_t a = Arp[i + 1] - a0;
+    for (int q = 0; q < c; ++q) {
+      item_row[s0 + q] = (int32_t)i;
+      item_t0[s0 + q] = a0 + (int32_t)((int64_t)q * a / c);
+    }
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == PLAN_T - 1) start[m] = (int32_t)(s0 + c);
 }
 
 // Extra items of rows longer than one item (chunks 1..), appended in any
@@ -1147,6 +1206,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // lines, then the extra-item cursor on a line of its own
   const int ctr_words = MXM_CTR_WORDS;
   w.counter.ensure(sizeof(uint32_t) * ctr_words);
+  w.scan_tmp.ensure(sizeof(long long) * std::max<int64_t>(ceil_div(m, PLAN_T), 1));  // item plan and row scans
   const long long wave_flops =
       (M.ncols <= WR_MAXCOLS) ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops) : -1;
   // direct mode (no wavefront bin): items 0..m-1 are the rows' first chunks and
@@ -1200,14 +1260,16 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     // same operands as the previous launch: its row bins and items stand
   } else if (wave_flops < 0) {
     // no wavefront bin: items of every non-empty row, listed in row order
-    hipLaunchKernelGGL(k_item_count, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp, M.rp,
+    const int64_t nbp = ceil_div(m, PLAN_T);
+    long long* bsum = w.scan_tmp.as<long long>();
+    hipLaunchKernelGGL(k_item_plan_count, dim3((unsigned)nbp), dim3(PLAN_T), 0, s, A.rp, M.rp,
                        lower_out ? nullptr : Mend, m, TCAP, w.cnt.as<int32_t>(), rw_prev, ctx().item_wedges,
-                       w.counter.as<uint32_t>(), ctr_words, rw_cur, lower_out ? M.col : nullptr, lower_out);
-    GA_HIP(hipGetLastError());
-    scan_exclusive_i32(w.cnt.as<int32_t>(), w.start.as<int32_t>(), m, w.scan_tmp, s);
-    hipLaunchKernelGGL(k_item_fill, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, A.rp,
-                       (const int32_t*)w.cnt.as<int32_t>(), (const int32_t*)w.start.as<int32_t>(), m,
-                       w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
+                       w.counter.as<uint32_t>(), ctr_words, rw_cur, lower_out ? M.col : nullptr, lower_out, bsum);
+    const bool offsets = nbp > PLAN_FUSED_TILES;
+    if (offsets) scan_tile_offsets(bsum, nbp, w.start.as<int32_t>() + m, s);
+    hipLaunchKernelGGL(k_item_plan_fill, dim3((unsigned)nbp), dim3(PLAN_T), 0, s, A.rp,
+                       (const int32_t*)w.cnt.as<int32_t>(), m, (const long long*)bsum, offsets ? 1 : 0,
+                       w.start.as<int32_t>(), w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
     GA_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, A.rp, A.col, B.rp, M.rp, Mend,
@@ -1420,7 +1482,7 @@ size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend) {
   size_t b = 3 * lease_bytes(sizeof(int32_t) * m) + 2 * lease_bytes(sizeof(int32_t) * (m + 1));
   b += 2 * lease_bytes(sizeof(int32_t) * max_items);
   b += lease_bytes(sizeof(uint32_t) * MXM_CTR_WORDS);
-  b += scan_tmp_bytes(m);
+  b += lease_bytes(sizeof(long long) * std::max<int64_t>(ceil_div(m, PLAN_T), 1));  // >= scan_tmp_bytes(m)
   if (with_mend) b += lease_bytes(sizeof(int32_t) * m) + 2 * lease_bytes(sizeof(unsigned long long) * m);
   return b;
 }
