@@ -661,9 +661,11 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_mxm_rows(
 // (row bounds, mask, entries, B bounds, gathers) with little work.  Here a
 // 16-lane group owns a row and the four groups of a wavefront run four rows
 // at once, overlapping their chains.  Same per-row structure as k_mxm_rows:
-// an exact per-group bitmap + rank bases (<= WR_MAXCOLS columns), entries
-// in chunks of 16 with a 16-lane scan, 16-lane wedge runs with a short walk.
+// an exact per-group bitmap (<= WR_MAXCOLS columns), entries in chunks of
+// 32 (two per lane, so most rows of such graphs are one chunk) with a 16-lane
+// scan, 16-lane wedge runs with a short walk.
 constexpr int SR_SUB = 16;
+constexpr int SR_CH = 2 * SR_SUB;                     // entries per chunk
 constexpr int SR_GROUPS = WR_WAVES * (64 / SR_SUB);  // groups per workgroup
 constexpr int SR_K = 8;                               // wedges per lane per trip
 
@@ -677,8 +679,8 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
     unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
   __shared__ uint32_t s_bm[SR_GROUPS][WR_BMW];
-  __shared__ int32_t s_jb[SR_GROUPS][SR_SUB];
-  __shared__ int32_t s_eoff[SR_GROUPS][SR_SUB + 1];
+  __shared__ int32_t s_jb[SR_GROUPS][SR_CH];
+  __shared__ int32_t s_eoff[SR_GROUPS][SR_CH + 1];
   const int sg = threadIdx.x / SR_SUB;  // group within the workgroup
   const int sl = threadIdx.x % SR_SUB;  // lane within the group
   uint32_t* bm = s_bm[sg];
@@ -759,35 +761,49 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     const int bbase = BATCHED ? gbase[row_group[i]] : 0;
     __threadfence_block();
     const int a_end = Arp[i + 1];
-    for (int t0 = Arp[i]; t0 < a_end; t0 += SR_SUB) {
-      const int ne = min(SR_SUB, a_end - t0);
-      int len = 0, b0 = 0;
-      if (sl < ne) {
-        const int k = bbase + Acol[t0 + sl];
-        b0 = Brp[k];
-        int b1 = Brp[k + 1];
-        if (trunc) {
-          int lo = b0, hi = b1;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (Bcol[mid] <= maxc) lo = mid + 1;
-            else hi = mid;
+    for (int t0 = Arp[i]; t0 < a_end; t0 += SR_CH) {
+      const int ne = min(SR_CH, a_end - t0);
+      // lane sl stages entries sl and SR_SUB + sl of the chunk
+      int len[2] = {0, 0}, b0[2] = {0, 0};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int q = h * SR_SUB + sl;
+        if (q < ne) {
+          const int k = bbase + Acol[t0 + q];
+          b0[h] = Brp[k];
+          int b1 = Brp[k + 1];
+          if (trunc) {
+            int lo = b0[h], hi = b1;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (Bcol[mid] <= maxc) lo = mid + 1;
+              else hi = mid;
+            }
+            b1 = lo;
           }
-          b1 = lo;
+          len[h] = b1 - b0[h];
         }
-        len = b1 - b0;
       }
-      int incl = len;
+      int incl[2] = {len[0], len[1]};
 #pragma unroll
       for (int d = 1; d < SR_SUB; d <<= 1) {
-        const int t = __shfl_up(incl, d, SR_SUB);
-        if (sl >= d) incl += t;
+        const int t0s = __shfl_up(incl[0], d, SR_SUB);
+        const int t1s = __shfl_up(incl[1], d, SR_SUB);
+        if (sl >= d) {
+          incl[0] += t0s;
+          incl[1] += t1s;
+        }
       }
-      if (sl < ne) {
-        eoff[sl] = incl - len;
-        jbase[sl] = b0 - (incl - len);
+      incl[1] += __shfl(incl[0], SR_SUB - 1, SR_SUB);  // the second half follows the first
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int q = h * SR_SUB + sl;
+        if (q < ne) {
+          eoff[q] = incl[h] - len[h];
+          jbase[q] = b0[h] - (incl[h] - len[h]);
+        }
       }
-      const int total = __shfl(incl, SR_SUB - 1, SR_SUB);
+      const int total = __shfl(incl[1], SR_SUB - 1, SR_SUB);
       if (sl == 0) {
         eoff[ne] = total;
         wedges += (unsigned long long)total;
