@@ -74,10 +74,13 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
   constexpr int MXM_THREADS = THREADS;
   constexpr int MXM_TCAP = THREADS;
   constexpr int MXM_MCAP = 8 * THREADS;  // mask rows up to this long accumulate in LDS
-  // dynamic LDS: the n-bit mask bitmap, then (RANK) per bitmap word the
-  // position in the mask row of the word's first mask column
+  // dynamic LDS: the n-bit mask bitmap (RANK: plus one word that stays zero,
+  // tested by dropped wedges), then (RANK) per bitmap word the position in
+  // the mask row of the word's first mask column
   extern __shared__ uint32_t s_bitmap[];
-  uint16_t* s_wpre = reinterpret_cast<uint16_t*>(s_bitmap + bitmap_words);
+  constexpr int BM_EXTRA = RANK ? 1 : 0;
+  uint16_t* s_wpre = reinterpret_cast<uint16_t*>(s_bitmap + bitmap_words + BM_EXTRA);
+  const int drop_col = RANK ? 32 * bitmap_words : -1;  // column of a dropped wedge
   __shared__ int32_t s_mcol[RANK ? 1 : MXM_MCAP];
   __shared__ T s_acc[MODE == 0 ? MXM_MCAP : 1];
   // entry e of the item: wedges [s_eoff[e], s_eoff[e+1]) live at B.col[s_jb[e] + w]
@@ -92,7 +95,7 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
   const int lane = tid & 63;
   const int wv = tid >> 6;
   const bool use_bm = RANK || bitmap_words > 0;
-  for (int q = tid; q < bitmap_words; q += MXM_THREADS) s_bitmap[q] = 0u;
+  for (int q = tid; q < bitmap_words + BM_EXTRA; q += MXM_THREADS) s_bitmap[q] = 0u;
   // direct mode: items 0..n_list-1 are the listed extra chunks of long rows
   // (the heaviest work goes first), then item n_list + r is row r's first
   // chunk for r < direct_rows
@@ -353,7 +356,7 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
           ee[u] = e;
           jj[u] = jb + w;
           const int c = Bcol[(unsigned)jj[u]];
-          cc[u] = ok ? c : -1;
+          cc[u] = ok ? c : drop_col;
         }
         auto accumulate = [&](int p, T v) {
           if (MODE == 1) {
@@ -376,7 +379,7 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
           int wi[MXM_K], pre[MXM_K];
 #pragma unroll
           for (int u = 0; u < MXM_K; ++u) {
-            wi[u] = cc[u] >= 0 ? (cc[u] >> 5) : 0;
+            wi[u] = cc[u] >> 5;  // a dropped wedge reads the zero word
             wd[u] = s_bitmap[wi[u]];
           }
 #pragma unroll
@@ -384,7 +387,7 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
 #pragma unroll
           for (int u = 0; u < MXM_K; ++u) {
             const uint32_t bit = 1u << (cc[u] & 31);
-            if (cc[u] >= 0 && (wd[u] & bit))
+            if (wd[u] & bit)
               accumulate(pre[u] + __popc(wd[u] & (bit - 1u)),
                          (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
           }
@@ -678,7 +681,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
     unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
-  __shared__ uint32_t s_bm[SR_GROUPS][WR_BMW];
+  __shared__ uint32_t s_bm[SR_GROUPS][WR_BMW + 1];  // + one word that stays zero (dropped wedges)
   __shared__ int32_t s_jb[SR_GROUPS][SR_CH];
   __shared__ int32_t s_eoff[SR_GROUPS][SR_CH + 1];
   const int sg = threadIdx.x / SR_SUB;  // group within the workgroup
@@ -686,7 +689,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
   uint32_t* bm = s_bm[sg];
   int32_t* jbase = s_jb[sg];
   int32_t* eoff = s_eoff[sg];
-  for (int q = sl; q < WR_BMW; q += SR_SUB) bm[q] = 0u;
+  for (int q = sl; q <= WR_BMW; q += SR_SUB) bm[q] = 0u;
   const int n_rows = *n_rows_ptr;
   // segmented claiming as in k_mxm_rows: the group's lane 0 takes `chunk`
   // rows at a time from its current segment (one atomic); when that segment
@@ -761,6 +764,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     const int bbase = BATCHED ? gbase[row_group[i]] : 0;
     __threadfence_block();
     const int a_end = Arp[i + 1];
+    unsigned int hits = 0;  // this row's hits on this lane (32-bit adds)
     for (int t0 = Arp[i]; t0 < a_end; t0 += SR_CH) {
       const int ne = min(SR_CH, a_end - t0);
       // lane sl stages entries sl and SR_SUB + sl of the chunk
@@ -828,17 +832,17 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
             jb = jbase[e];
           }
           const int c = Bcol[(unsigned)(jb + w)];
-          cc[u] = ok ? c : -1;
+          cc[u] = ok ? c : 32 * WR_BMW;  // a dropped wedge tests the zero word
         }
         uint32_t wd[SR_K];
 #pragma unroll
-        for (int u = 0; u < SR_K; ++u) wd[u] = bm[cc[u] >= 0 ? (cc[u] >> 5) : 0];
+        for (int u = 0; u < SR_K; ++u) wd[u] = bm[cc[u] >> 5];
 #pragma unroll
-        for (int u = 0; u < SR_K; ++u)
-          lsum += (cc[u] >= 0 && ((wd[u] >> (cc[u] & 31)) & 1u)) ? 1 : 0;
+        for (int u = 0; u < SR_K; ++u) hits += (wd[u] >> (cc[u] & 31)) & 1u;
       }
       __threadfence_block();  // entry slots are rewritten by the next chunk
     }
+    lsum += hits;
     for (int p = sl; p < mnz; p += SR_SUB) bm[Mcol[ms + p] >> 5] = 0u;
     __threadfence_block();
   }
@@ -1322,7 +1326,8 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   const bool rank = bitmap_words > 0 && bitmap_words <= MXM_RANK_MAX_WORDS && w.max_mask_row < 65536;
   auto kitems = rank ? k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, true>
                      : k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, false>;
-  const size_t dyn = sizeof(uint32_t) * (size_t)bitmap_words + (rank ? sizeof(uint16_t) * ((bitmap_words + 1) & ~1) : 0);
+  const int bm_words = bitmap_words + (rank ? 1 : 0);  // rank: one zero word for dropped wedges
+  const size_t dyn = sizeof(uint32_t) * (size_t)bm_words + (rank ? sizeof(uint16_t) * ((bm_words + 1) & ~1) : 0);
   int occ_r = 0, occ_i = 0;
   GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, krows, 64 * WR_WAVES, 0));
   GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_i, kitems, THREADS, dyn));
