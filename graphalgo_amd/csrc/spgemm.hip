@@ -337,16 +337,18 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
         }
         e = lo;
       }
-      for (int pos = g0; pos < g1; pos += MXM_RUN) {
-        const int p0 = pos + rl;
-        if (p0 >= g1) break;  // later trips start even further out
+      // a trip; FULL: all MXM_RUN wedges lie in the range (every trip but a
+      // group's last one when its range is not whole trips), so no lane
+      // checks its wedges against the range end
+      auto trip = [&](auto full_c, const int p0) {
+        constexpr bool FULL = decltype(full_c)::value;
         int jb = s_jb[e];
         int nb = s_eoff[e + 1];
         int cc[MXM_K], jj[MXM_K], ee[MXM_K];
 #pragma unroll
         for (int u = 0; u < MXM_K; ++u) {
           // wedges past the range re-read its last one (in bounds) and are dropped
-          const bool ok = p0 + u * MXM_G < g1;
+          const bool ok = FULL || p0 + u * MXM_G < g1;
           const int w = ok ? p0 + u * MXM_G : g1 - 1;
           while (w >= nb) {  // next non-empty entry (zero-length entries are skipped)
             ++e;
@@ -418,6 +420,12 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
             if (found) accumulate(lo, (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
           }
         }
+      };
+      for (int pos = g0; pos < g1; pos += MXM_RUN) {
+        const int p0 = pos + rl;
+        if (p0 >= g1) break;  // later trips start even further out
+        if (pos + MXM_RUN <= g1) trip(std::true_type{}, p0);
+        else trip(std::false_type{}, p0);
       }
     };
     if (local) wedge_loop(std::true_type{});
