@@ -121,6 +121,8 @@ GA_SIGNATURES = {
     "ga_batch_destroy": (C.c_int, [C.c_void_p]),
     "ga_batch_triangle_count": (C.c_int, [C.c_void_p, C.c_int, I64P]),
     "ga_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), F64P]),
+    "ga_set_option": (C.c_int, [C.c_int, C.c_double]),
+    "ga_get_option": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "ga_prof_enable": (C.c_int, [C.c_int]),
     "ga_prof_read": (C.c_int, [C.POINTER(GaKernelProf), C.c_int]),
     "ga_stream_copy": (C.c_int, [C.c_int64, C.c_int, F64P]),

@@ -348,3 +348,41 @@ def device_info() -> dict:
     hbm = C.c_int64()
     N.check(N.galib().ga_device_info(name, 128, C.byref(cu), C.byref(hbm)))
     return {"name": name.value.decode(), "compute_units": cu.value, "hbm_bytes": hbm.value}
+
+
+# process-wide tuning options (include/graphalgo.h, GA_OPT_*): every value
+# gives the same results, only the speed differs
+OPTIONS = {"bc_bottom_up": 1, "bc_alpha": 2, "row_claim": 3, "sum_sub": 4, "item_wedges": 5,
+           "selective_support": 6}
+
+
+def set_option(name: str, value: float) -> None:
+    N.init()
+    N.check(N.galib().ga_set_option(OPTIONS[name], float(value)))
+
+
+def get_option(name: str) -> float:
+    N.init()
+    v = C.c_double()
+    N.check(N.galib().ga_get_option(OPTIONS[name], C.byref(v)))
+    return v.value
+
+
+class options:
+    """Context manager that sets options and restores the previous values:
+    ``with api.options(bc_bottom_up=0): ...``."""
+
+    def __init__(self, **values):
+        self.values = values
+        self.saved = {}
+
+    def __enter__(self):
+        for k, v in self.values.items():
+            self.saved[k] = get_option(k)
+            set_option(k, v)
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.saved.items():
+            set_option(k, v)
+        return False

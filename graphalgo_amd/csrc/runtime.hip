@@ -367,6 +367,58 @@ ga_status ga_init(int device) {
   });
 }
 
+ga_status ga_set_option(int option, double value) {
+  return guarded([&] {
+    Context& c = ctx();
+    const bool integral = value == (double)(long long)value;
+    switch (option) {
+      case GA_OPT_BC_BOTTOM_UP:
+        GA_REQUIRE(value == 0.0 || value == 1.0, GA_ERR_ARG, "bc_bottom_up is 0 or 1");
+        c.bc_bottom_up = value != 0.0;
+        break;
+      case GA_OPT_BC_ALPHA:
+        GA_REQUIRE(value > 0.0, GA_ERR_ARG, "bc_alpha must be positive");
+        c.bc_alpha = value;
+        break;
+      case GA_OPT_ROW_CLAIM:
+        GA_REQUIRE(integral && value >= 0.0 && value <= 1 << 20, GA_ERR_ARG, "row claim is 0..2^20");
+        c.row_chunk = (int)value;
+        break;
+      case GA_OPT_SUM_SUB:
+        GA_REQUIRE(value == 0.0 || value == 8.0 || value == 16.0 || value == 32.0 || value == 64.0, GA_ERR_ARG,
+                   "sum_sub is 0, 8, 16, 32 or 64");
+        c.sum_sub = (int)value;
+        break;
+      case GA_OPT_ITEM_WEDGES:
+        GA_REQUIRE(integral && value >= 1.0, GA_ERR_ARG, "item_wedges must be a positive integer");
+        c.item_wedges = (long long)value;
+        break;
+      case GA_OPT_SELECTIVE_SUPPORT:
+        GA_REQUIRE(value == 0.0 || value == 1.0, GA_ERR_ARG, "selective_support is 0 or 1");
+        c.selective_support = value != 0.0;
+        break;
+      default:
+        GA_REQUIRE(false, GA_ERR_ARG, "unknown option " + std::to_string(option));
+    }
+  });
+}
+
+ga_status ga_get_option(int option, double* value) {
+  return guarded([&] {
+    GA_REQUIRE(value != nullptr, GA_ERR_ARG, "null argument");
+    Context& c = ctx();
+    switch (option) {
+      case GA_OPT_BC_BOTTOM_UP: *value = c.bc_bottom_up ? 1.0 : 0.0; break;
+      case GA_OPT_BC_ALPHA: *value = c.bc_alpha; break;
+      case GA_OPT_ROW_CLAIM: *value = c.row_chunk; break;
+      case GA_OPT_SUM_SUB: *value = c.sum_sub; break;
+      case GA_OPT_ITEM_WEDGES: *value = (double)c.item_wedges; break;
+      case GA_OPT_SELECTIVE_SUPPORT: *value = c.selective_support ? 1.0 : 0.0; break;
+      default: GA_REQUIRE(false, GA_ERR_ARG, "unknown option " + std::to_string(option));
+    }
+  });
+}
+
 ga_status ga_device_info(char* name, int name_len, int* compute_units, int64_t* hbm_bytes) {
   return guarded([&] {
     Context& c = ctx();

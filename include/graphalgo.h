@@ -267,6 +267,26 @@ ga_status ga_prof_enable(int flags);
 ga_status ga_stream_copy(int64_t bytes, int reps, double* ms_per_copy);
 ga_status ga_prof_read(ga_kernel_prof* out, int reset);
 
+/* Process-wide tuning options; the GA_* environment variables of the same
+ * names set them at ga_init.  Every setting gives the same results (tests
+ * compare them); only the speed differs.  Set them while no other call of
+ * the library is running.  No reference counterpart. */
+enum {
+  GA_OPT_BC_BOTTOM_UP = 1,      /* 0/1: bottom-up BC forward steps (GA_BC_BOTTOM_UP) */
+  GA_OPT_BC_ALPHA = 2,          /* > 0: go bottom-up once the level list holds more than 1/alpha
+                                   of the unvisited vertices' edges (GA_BC_ALPHA) */
+  GA_OPT_ROW_CLAIM = 3,         /* >= 0: rows per claim of the short-row kernels, 0 = automatic
+                                   (GA_ROW_CHUNK) */
+  GA_OPT_SUM_SUB = 4,           /* 0 = 16-lane wedge runs, 8/16/32/64 = lanes per row walking its
+                                   entries, for short rows of pattern sums (GA_SUM_SUB) */
+  GA_OPT_ITEM_WEDGES = 5,       /* >= 1: target wedges per work item of repeated products
+                                   (GA_ITEM_WEDGES) */
+  GA_OPT_SELECTIVE_SUPPORT = 6  /* 0/1: k-truss passes after the first recompute only the
+                                   supports an endpoint change can affect (GA_SELECTIVE_SUPPORT) */
+};
+ga_status ga_set_option(int option, double value);
+ga_status ga_get_option(int option, double* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,109 @@
+"""GPU tests: every tuning option gives the same results.  BC with top-down
+forward steps only, with bottom-up steps from the first level on and with the
+default switch point (bit-identical scores, and the oracle); triangle counts
+with every short-row kernel and claim size (exact); truss numbers with tiny
+work items (rows split over many items, whose partial supports meet in global
+atomics) and with the selective support passes (exact)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+BC_RTOL = 1e-9
+
+
+def _bc_close(got, want):
+    scale = max(1.0, float(np.abs(want).max()))
+    return np.allclose(got, want, rtol=BC_RTOL, atol=BC_RTOL * scale * 1e-3)
+
+
+@pytest.mark.parametrize("scale,ns", [(12, 64), (14, 256)])
+def test_bc_directions_bit_identical(gpu, G, oracle, scale, ns):
+    g = G.rmat(scale, 16, seed=40 + scale)
+    src = G.pick_sources(g, ns, 3)
+    R = gpu.Graph(g)
+    print(f"case bc directions rmat{scale} ns={ns}", flush=True)
+    with gpu.options(bc_bottom_up=0):
+        top_down = R.bc(src)
+    with gpu.options(bc_bottom_up=1, bc_alpha=1e6):  # bottom-up from level 1 on
+        early = R.bc(src)
+    with gpu.options(bc_bottom_up=1, bc_alpha=1.0):
+        late = R.bc(src)
+    default = R.bc(src)
+    R.close()
+    assert np.array_equal(top_down, early)
+    assert np.array_equal(top_down, late)
+    assert np.array_equal(top_down, default)
+    assert _bc_close(default, oracle.bc(g, src))
+
+
+def test_bc_directions_deep_and_disconnected(gpu, G, oracle):
+    """A long path (levels of one vertex) plus a disjoint star: the bottom-up
+    candidates include vertices no source reaches."""
+    n_path, n_star = 300, 50
+    rows = list(range(n_path - 1)) + [n_path] * (n_star - 1)
+    cols = list(range(1, n_path)) + list(range(n_path + 1, n_path + n_star))
+    g = G.CSR.from_edges(n_path + n_star, rows, cols)
+    src = np.array([0, 7, 150, n_path + 3], dtype=np.int32)
+    want = oracle.bc(g, src)
+    for opts in ({"bc_bottom_up": 0}, {"bc_bottom_up": 1, "bc_alpha": 1e6}):
+        with gpu.options(**opts):
+            got = gpu.betweenness_centrality_batch(g, src)
+        assert _bc_close(got, want), opts
+
+
+@pytest.fixture(scope="module")
+def tc_cases(G, oracle):
+    graphs = G.knn_batch(base_seed=3000, count=12, k=30)
+    er = G.erdos_renyi(1000, 9990, seed=1)
+    return graphs, np.array([oracle.tc(g) for g in graphs], dtype=np.int64), er, oracle.tc(er)
+
+
+@pytest.mark.parametrize("sub", [0, 8, 16, 32, 64])
+@pytest.mark.parametrize("claim", [0, 1, 3, 16])
+def test_tc_short_row_kernels_and_claims(gpu, tc_cases, sub, claim):
+    graphs, want, er, want_er = tc_cases
+    print(f"case tc sum_sub={sub} row_claim={claim}", flush=True)
+    with gpu.options(sum_sub=sub, row_claim=claim):
+        B = gpu.Batch(graphs)
+        got = B.triangle_counts()
+        B.close()
+        R = gpu.Graph(er)
+        got_er = R.triangle_count()
+        R.close()
+    assert np.array_equal(got, want)
+    assert got_er == want_er
+
+
+@pytest.mark.parametrize("item_wedges", [64, 4096])
+def test_truss_small_items(gpu, G, oracle, item_wedges):
+    g = G.rmat(12, 16, seed=8)
+    want, kmax, _, _, _ = oracle.truss_decomposition(g)
+    print(f"case truss item_wedges={item_wedges}", flush=True)
+    with gpu.options(item_wedges=item_wedges):
+        got, got_kmax, _ = gpu.truss_decomposition(g)
+    assert got_kmax == kmax
+    assert np.array_equal(got, want)
+
+
+def test_truss_selective_support(gpu, G, oracle):
+    g = G.rmat(12, 16, seed=9)
+    want, kmax, _, _, _ = oracle.truss_decomposition(g)
+    with gpu.options(selective_support=1):
+        got, got_kmax, _ = gpu.truss_decomposition(g)
+        R = gpu.Graph(g)
+        nnz5 = R.ktruss(5)[0]
+        R.close()
+    assert got_kmax == kmax and np.array_equal(got, want)
+    assert nnz5 == int((want >= 5).sum())
+
+
+def test_option_validation(gpu):
+    for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
+                      ("item_wedges", 0), ("selective_support", 0.5)):
+        with pytest.raises(Exception):
+            gpu.set_option(name, bad)
+    before = gpu.get_option("row_claim")
+    with gpu.options(row_claim=5):
+        assert gpu.get_option("row_claim") == 5
+    assert gpu.get_option("row_claim") == before
