@@ -99,15 +99,19 @@ __global__ __launch_bounds__(KEEP_T) void k_keep_tile_sums(const int32_t* __rest
 // Survivors to their new positions (col, entry id, old position, the next
 // pass's support: cleared, or carried with `carry`); newpos[j] = new position
 // of a survivor, -1 - (survivors before j) for a removed entry.  bsum: tile
-// sums, or exclusive tile offsets when offsets_ready.  The last tile writes
-// the survivor count to *kept.
+// sums, or exclusive tile offsets when offsets_ready.  Wavefront w of a tile
+// owns its entries [w * 512, (w + 1) * 512) and reads them 64 at a time
+// (coalesced); a ballot per 64 entries ranks the survivors, so the stores of
+// consecutive survivors are consecutive too.  The last tile writes the
+// survivor count to *kept.
 __global__ __launch_bounds__(KEEP_T) void k_keep_scatter(
     const int32_t* __restrict__ col, const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t nnz,
     int thr, const long long* __restrict__ bsum, int offsets_ready, int32_t* __restrict__ ncol,
     int32_t* __restrict__ neid, int32_t* __restrict__ nold, int32_t* __restrict__ newpos, int32_t* __restrict__ nsup,
     int carry, int32_t* __restrict__ kept) {
-  __shared__ long long wtot[KEEP_T / 64];
-  __shared__ long long prior[KEEP_T / 64];
+  constexpr int NW = KEEP_T / 64;
+  __shared__ long long prior[NW];
+  __shared__ int wcount[NW];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   long long before = 0;
   if (offsets_ready) {
@@ -117,71 +121,78 @@ __global__ __launch_bounds__(KEEP_T) void k_keep_scatter(
   }
   before = wave_reduce_sum(before);
   if (lane == 0) prior[wv] = before;
-  const int64_t base = (int64_t)blockIdx.x * KEEP_TILE + (int64_t)tid * KEEP_PER;
+  const int64_t wbase = (int64_t)blockIdx.x * KEEP_TILE + (int64_t)wv * 64 * KEEP_PER;
+  unsigned long long mk[KEEP_PER];
   int32_t sv[KEEP_PER];
-  int tsum = 0;
+  int cnt = 0;
 #pragma unroll
   for (int q = 0; q < KEEP_PER; ++q) {
-    sv[q] = base + q < nnz ? sup[base + q] : -1;
-    tsum += (base + q < nnz && sv[q] >= thr) ? 1 : 0;
+    const int64_t j = wbase + q * 64 + lane;
+    sv[q] = j < nnz ? sup[j] : -1;
+    mk[q] = __ballot(j < nnz && sv[q] >= thr);
+    cnt += __popcll(mk[q]);
   }
-  const long long incl = wave_incl_scan((long long)tsum);
-  if (lane == 63) wtot[wv] = incl;
+  if (lane == 0) wcount[wv] = cnt;
   __syncthreads();
-  long long off = incl - tsum;
-  for (int k = 0; k < KEEP_T / 64; ++k) {
+  long long off = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
     off += prior[k];
-    if (k < wv) off += wtot[k];
+    if (k < wv) off += wcount[k];
   }
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
   for (int q = 0; q < KEEP_PER; ++q) {
-    const int64_t j = base + q;
-    if (j >= nnz) break;
-    const bool keep = sv[q] >= thr;
-    const int r = (int)off;
-    if (keep) {
-      ncol[r] = col[j];
-      neid[r] = eid[j];
-      nold[r] = (int32_t)j;
-      nsup[r] = carry ? sv[q] : 0;
+    const int64_t j = wbase + q * 64 + lane;
+    if (j < nnz) {
+      const bool keep = (mk[q] >> lane) & 1ull;
+      const int r = (int)(off + __popcll(mk[q] & lt));
+      if (keep) {
+        ncol[r] = col[j];
+        neid[r] = eid[j];
+        nold[r] = (int32_t)j;
+        nsup[r] = carry ? sv[q] : 0;
+      }
+      newpos[j] = keep ? r : -1 - r;
     }
-    newpos[j] = keep ? r : -1 - r;
-    off += keep ? 1 : 0;
+    off += __popcll(mk[q]);
   }
-  if (blockIdx.x == gridDim.x - 1 && tid == KEEP_T - 1) *kept = (int32_t)off;
+  if (blockIdx.x == gridDim.x - 1 && tid == 0) {
+    long long t = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) t += prior[k] + wcount[k];
+    *kept = (int32_t)t;
+  }
 }
 
-// nrp[i] = survivors before row i (the count at rp[i]); dirty[i] = row i
-// loses entries (only when dirty is given)
-__global__ void k_keep_rows(const int32_t* __restrict__ rp, int64_t n, const int32_t* __restrict__ newpos,
-                            int64_t nnz, const int32_t* __restrict__ kept, int32_t* __restrict__ nrp,
-                            uint8_t* __restrict__ dirty) {
+// The rest of a compaction, one launch: thread t is row t of the new row
+// pointers (nrp[i] = survivors before row i, the count at rp[i]; dirty[i] =
+// row i loses entries, only when dirty is given) and survivor t of the mirror
+// index update (the transposed entry of a survivor survives too -- equal
+// supports -- so the mirror index carries over through the old -> new map).
+__global__ void k_keep_finish(const int32_t* __restrict__ rp, int64_t n, const int32_t* __restrict__ newpos,
+                              int64_t nnz, const int32_t* __restrict__ kept, int32_t* __restrict__ nrp,
+                              uint8_t* __restrict__ dirty, const int32_t* __restrict__ mirror_old,
+                              const int32_t* __restrict__ nold, int32_t* __restrict__ mirror_new) {
   const int32_t total = *kept;
   auto before = [&](int64_t j) -> int32_t {
     if (j >= nnz) return total;
     const int32_t p = newpos[j];
     return p >= 0 ? p : -1 - p;
   };
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
-    if (i == n) {
+  const int64_t span = max<int64_t>(n + 1, total);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < span; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < n) {
+      const int32_t a = before(rp[t]);
+      nrp[t] = a;
+      if (dirty) dirty[t] = (before(rp[t + 1]) - a) < (rp[t + 1] - rp[t]) ? 1 : 0;
+    } else if (t == n) {
       nrp[n] = total;
-      continue;
     }
-    const int32_t a = before(rp[i]);
-    nrp[i] = a;
-    if (dirty) dirty[i] = (before(rp[i + 1]) - a) < (rp[i + 1] - rp[i]) ? 1 : 0;
-  }
-}
-
-// The transposed entry of a survivor survives too (equal supports), so the
-// mirror index carries over through the old -> new position map.
-__global__ void k_mirror_update(const int32_t* __restrict__ mirror_old, const int32_t* __restrict__ nold,
-                                const int32_t* __restrict__ newpos, const int32_t* __restrict__ nnz_ptr,
-                                int32_t* __restrict__ mirror_new) {
-  const int64_t nnz = *nnz_ptr;  // survivors (the compacted row pointer's last entry)
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nnz; r += (int64_t)gridDim.x * blockDim.x) {
-    const int mo = mirror_old[nold[r]];
-    mirror_new[r] = mo >= 0 ? newpos[mo] : -1;
+    if (t < total) {
+      const int mo = mirror_old[nold[t]];
+      mirror_new[t] = mo >= 0 ? newpos[mo] : -1;
+    }
   }
 }
 
@@ -297,13 +308,10 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                          tw.sup[cur ^ 1].as<int32_t>(), selective ? 1 : 0, kept_dev);
       GA_HIP(hipMemcpyAsync(kept_h, kept_dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
       GA_HIP(hipEventRecord(kept_ev, s));
-      hipLaunchKernelGGL(k_keep_rows, dim3(grid_for(n + 1, 256, 8192)), dim3(256), 0, s, C.rp, n,
-                         (const int32_t*)tw.newpos.as<int32_t>(), nnz, (const int32_t*)kept_dev, nrp,
-                         selective ? tw.dirty.as<uint8_t>() : nullptr);
-      hipLaunchKernelGGL(k_mirror_update, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s,
-                         (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
-                         (const int32_t*)tw.newpos.as<int32_t>(), (const int32_t*)kept_dev,
-                         tw.mirror[cur ^ 1].as<int32_t>());
+      hipLaunchKernelGGL(k_keep_finish, dim3(grid_for(std::max<int64_t>(n + 1, nnz), 256, 8192)), dim3(256), 0, s,
+                         C.rp, n, (const int32_t*)tw.newpos.as<int32_t>(), nnz, (const int32_t*)kept_dev, nrp,
+                         selective ? tw.dirty.as<uint8_t>() : nullptr, (const int32_t*)tw.mirror[cur].as<int32_t>(),
+                         (const int32_t*)tw.nold.as<int32_t>(), tw.mirror[cur ^ 1].as<int32_t>());
       GA_HIP(hipGetLastError());
       GA_HIP(hipEventSynchronize(kept_ev));
       const int64_t kept = *kept_h;
