@@ -124,6 +124,12 @@ __global__ __launch_bounds__(256) void k_bc_last(const int32_t* __restrict__ fro
 // least one item (a backward vertex without successors still needs q).
 constexpr int BC_CH = 256;
 constexpr int BC_UNROLL = 8;
+// pull sweeps (bottom-up, backward): BC_CG 64-lane chunks of sources are
+// walked together, BC_NU neighbours per step, so the 64-byte depth pieces
+// and 512-byte value pieces of one neighbour's row are read together (one
+// row per neighbour, not one piece per pass over the neighbour list)
+constexpr int BC_CG = 4;
+constexpr int BC_NU = 2;
 constexpr int BC_FLAG_CHUNK = 64;  // vertices per thread when flags become a list
 
 __global__ void k_bc_item_count(const int32_t* __restrict__ front, int nf, const int32_t* __restrict__ rp,
@@ -291,32 +297,52 @@ __global__ __launch_bounds__(256) void k_bc_bottom_up_items(
     for (int t = lane; t < cnt; t += 64) nb[t] = col[t0 + t];
     __threadfence_block();
     bool reached = false;
-    for (int c = 0; c < nsp; c += 64) {
-      const int64_t vi = (int64_t)v * nsp + c + lane;
-      const bool act = depth[vi] == bc_unvisited<DT>();
-      if (__ballot(act) == 0ull) continue;
-      double acc = 0.0;
-      for (int t = 0; t < cnt; t += BC_UNROLL) {
-        int uu[BC_UNROLL];
-        DT du[BC_UNROLL];
-        double su[BC_UNROLL];
+    for (int c0 = 0; c0 < nsp; c0 += 64 * BC_CG) {
+      bool act[BC_CG];
+      bool any = false;
 #pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) uu[j] = (t + j < cnt) ? nb[t + j] : -1;
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j)
-          du[j] = (act && uu[j] >= 0) ? depth[(int64_t)uu[j] * nsp + c + lane] : (DT)0;
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j)
-          su[j] = (act && uu[j] >= 0 && du[j] == dd) ? sigma[(int64_t)uu[j] * nsp + c + lane] : 0.0;
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) acc += su[j];
+      for (int g = 0; g < BC_CG; ++g) {
+        const int c = c0 + 64 * g;
+        act[g] = c < nsp && depth[(int64_t)v * nsp + c + lane] == bc_unvisited<DT>();
+        any |= __ballot(act[g]) != 0ull;
       }
-      if (slot >= 0) {
-        if (act) partial[(int64_t)slot * nsp + c + lane] = acc;
-      } else if (act && acc > 0.0) {
-        depth[vi] = dn;
-        sigma[vi] = acc;
-        reached = true;
+      if (!any) continue;
+      double acc[BC_CG];
+#pragma unroll
+      for (int g = 0; g < BC_CG; ++g) acc[g] = 0.0;
+      for (int t = 0; t < cnt; t += BC_NU) {
+        int uu[BC_NU];
+        DT du[BC_NU][BC_CG];
+        double su[BC_NU][BC_CG];
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j) uu[j] = (t + j < cnt) ? nb[t + j] : -1;
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j)
+#pragma unroll
+          for (int g = 0; g < BC_CG; ++g)
+            du[j][g] = (act[g] && uu[j] >= 0) ? depth[(int64_t)uu[j] * nsp + c0 + 64 * g + lane] : (DT)0;
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j)
+#pragma unroll
+          for (int g = 0; g < BC_CG; ++g)
+            su[j][g] = (act[g] && uu[j] >= 0 && du[j][g] == dd)
+                           ? sigma[(int64_t)uu[j] * nsp + c0 + 64 * g + lane]
+                           : 0.0;
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j)
+#pragma unroll
+          for (int g = 0; g < BC_CG; ++g) acc[g] += su[j][g];
+      }
+#pragma unroll
+      for (int g = 0; g < BC_CG; ++g) {
+        const int64_t vi = (int64_t)v * nsp + c0 + 64 * g + lane;
+        if (slot >= 0) {
+          if (act[g]) partial[(int64_t)slot * nsp + c0 + 64 * g + lane] = acc[g];
+        } else if (act[g] && acc[g] > 0.0) {
+          depth[vi] = dn;
+          sigma[vi] = acc[g];
+          reached = true;
+        }
       }
     }
     if (__ballot(reached) != 0ull && lane == 0) next_flag[v] = 1;
@@ -445,39 +471,60 @@ __global__ __launch_bounds__(256) void k_bc_backward_items(
     for (int t = lane; t < cnt; t += 64) nb[t] = col[t0 + t];
     __threadfence_block();
     double part = 0.0;
-    for (int c = 0; c < nsp; c += 64) {
-      const int64_t vi = (int64_t)v * nsp + c + lane;
-      const bool act = depth[vi] == dk;
-      if (__ballot(act) == 0ull) continue;
-      double acc = 0.0;
-      for (int t = 0; t < cnt; t += BC_UNROLL) {
-        int ww[BC_UNROLL];
-        DT dw[BC_UNROLL];
-        double qw[BC_UNROLL];
+    for (int c0 = 0; c0 < nsp; c0 += 64 * BC_CG) {
+      bool act[BC_CG];
+      unsigned live = 0u;  // chunks with an active lane (wavefront-uniform)
 #pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) ww[j] = (t + j < cnt) ? nb[t + j] : -1;
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j)
-          dw[j] = (act && ww[j] >= 0) ? depth[(int64_t)ww[j] * nsp + c + lane] : (DT)0;
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j)
-          qw[j] = (act && ww[j] >= 0 && dw[j] == dn) ? q[(int64_t)ww[j] * nsp + c + lane] : 0.0;
-#pragma unroll
-        for (int j = 0; j < BC_UNROLL; ++j) acc += qw[j];
+      for (int g = 0; g < BC_CG; ++g) {
+        const int c = c0 + 64 * g;
+        act[g] = c < nsp && depth[(int64_t)v * nsp + c + lane] == dk;
+        if (__ballot(act[g]) != 0ull) live |= 1u << g;
       }
-      if (slot >= 0) {
-        if (act) partial[(int64_t)slot * nsp + c + lane] = acc;
-      } else {
-        if (act) {
-          const double sg = sigma[vi];
-          const double delta = sg * acc;
-          q[vi] = (1.0 + delta) / sg;
-          part += delta;
-        }
-        if (bc_stride) {  // one score vector per 64-lane group
-          part = wave_reduce_sum(part);
-          if (lane == 0) bc[(c >> 6) * bc_stride + v] += part;
-          part = 0.0;
+      if (live == 0u) continue;
+      double acc[BC_CG];
+#pragma unroll
+      for (int g = 0; g < BC_CG; ++g) acc[g] = 0.0;
+      for (int t = 0; t < cnt; t += BC_NU) {
+        int ww[BC_NU];
+        DT dw[BC_NU][BC_CG];
+        double qw[BC_NU][BC_CG];
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j) ww[j] = (t + j < cnt) ? nb[t + j] : -1;
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j)
+#pragma unroll
+          for (int g = 0; g < BC_CG; ++g)
+            dw[j][g] = (act[g] && ww[j] >= 0) ? depth[(int64_t)ww[j] * nsp + c0 + 64 * g + lane] : (DT)0;
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j)
+#pragma unroll
+          for (int g = 0; g < BC_CG; ++g)
+            qw[j][g] = (act[g] && ww[j] >= 0 && dw[j][g] == dn) ? q[(int64_t)ww[j] * nsp + c0 + 64 * g + lane] : 0.0;
+#pragma unroll
+        for (int j = 0; j < BC_NU; ++j)
+#pragma unroll
+          for (int g = 0; g < BC_CG; ++g) acc[g] += qw[j][g];
+      }
+      // per chunk, in chunk order, as a chunk-at-a-time walk would finish them
+#pragma unroll
+      for (int g = 0; g < BC_CG; ++g) {
+        if (!((live >> g) & 1u)) continue;
+        const int c = c0 + 64 * g;
+        const int64_t vi = (int64_t)v * nsp + c + lane;
+        if (slot >= 0) {
+          if (act[g]) partial[(int64_t)slot * nsp + c + lane] = acc[g];
+        } else {
+          if (act[g]) {
+            const double sg = sigma[vi];
+            const double delta = sg * acc[g];
+            q[vi] = (1.0 + delta) / sg;
+            part += delta;
+          }
+          if (bc_stride) {  // one score vector per 64-lane group
+            part = wave_reduce_sum(part);
+            if (lane == 0) bc[(c >> 6) * bc_stride + v] += part;
+            part = 0.0;
+          }
         }
       }
     }
