@@ -129,7 +129,7 @@ constexpr int BC_UNROLL = 8;
 // and 512-byte value pieces of one neighbour's row are read together (one
 // row per neighbour, not one piece per pass over the neighbour list)
 constexpr int BC_CG = 4;
-constexpr int BC_NU = 2;
+constexpr int BC_NU = 4;
 constexpr int BC_FLAG_CHUNK = 64;  // vertices per thread when flags become a list
 
 __global__ void k_bc_item_count(const int32_t* __restrict__ front, int nf, const int32_t* __restrict__ rp,
