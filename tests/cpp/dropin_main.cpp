@@ -1,7 +1,7 @@
 // Reference-style client of the drop-in headers: the flow of the reference's
 // CLI (src/main.cpp:188-277) -- read an .mtx, build the adjacency matrix,
 // symmetrise its lower triangle, bind an mxmOp<int>, run triangle counting,
-// k-truss and batched BC -- compiled against include/graphalgo/*.h and
+// k-truss, batched BC and full-vertex BC -- compiled against include/graphalgo/*.h and
 // linked with libgraphalgo.so.  Prints one JSON object for the test to check.
 #include <cstdio>
 #include <string>
@@ -33,10 +33,14 @@ int main(int argc, char** argv) {
     for (float x : bc) bc_sum += x;
     int32_t kmax = 0;
     std::vector<int32_t> truss = truss_decomposition(TestMtx, &kmax);
+    // every vertex as a source, in blocks of 64 (include/betweenness_centrality.h:5-123)
+    std::vector<double> bcf = betweenness_centrality_f64(build_adjacency_matrix(gr), 64);
     std::printf("{\"n\": %zu, \"nnz\": %zu, \"tc\": %lld, \"tc_lu\": %lld, \"k3_nnz\": %zu, \"k3_it\": %d, "
-                "\"k4_nnz\": %zu, \"k4_it\": %d, \"sq_sum\": %lld, \"bc_sum\": %.6f, \"kmax\": %d}\n",
+                "\"k4_nnz\": %zu, \"k4_it\": %d, \"sq_sum\": %lld, \"bc_sum\": %.6f, \"kmax\": %d, \"bc_full\": [",
                 TestMtx.m, TestMtx.nz, (long long)tc, (long long)tc_lu, T3.nz, it3, T4.nz, it4, sq_sum, bc_sum,
                 kmax);
+    for (size_t v = 0; v < bcf.size(); ++v) std::printf(v ? ", %.17g" : "%.17g", bcf[v]);
+    std::printf("]}\n");
   } catch (const char* e) {
     std::fprintf(stderr, "error: %s\n", e);
     return 1;

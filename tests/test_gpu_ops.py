@@ -103,3 +103,9 @@ def test_cpp_dropin_client(gpu, golden, small_graphs, oracle):
     bc = oracle.bc(small_graphs["G43.mtx"][0], np.arange(16))
     assert out["bc_sum"] == pytest.approx(bc.astype(np.float32).astype(np.float64).sum(), rel=1e-6)
     assert out["kmax"] == rec["truss"]["kmax"]
+    # full-vertex BC (every vertex a source, 64 per block): FP64, 1e-9 relative to the oracle
+    g0 = small_graphs["G43.mtx"][0]
+    want = oracle.bc(g0, np.arange(g0.n))
+    got = np.array(out["bc_full"])
+    assert got.shape == want.shape
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-9 * max(1.0, float(want.max())) * 1e-3)
