@@ -13,8 +13,10 @@ uses algorithmic bytes per launch
 time of its launches, measured with HIP events on the library's stream.
 The CPU baseline is the OpenMP oracle's truss decomposition on the host.
 
-Other workloads: --workload tc (triangle count on R-MAT 16), bc (256-source
-batched BC on R-MAT 18), batch (4096 kNN graphs, batched TC), serve.
+Other workloads: --workload tc (triangle count on R-MAT 16), tc_er (one
+small triangle count), bc (256-source batched BC on R-MAT 18), batch (4096
+kNN graphs, batched TC), serve, ewise (eWiseAdd of two R-MAT 20 graphs),
+copy (calibration).
 """
 from __future__ import annotations
 
@@ -178,6 +180,39 @@ def bench_tc_er(args) -> dict:
         out["cpu_baseline"] = {"value": g.m_undirected / dtc, "unit": "edges/s", "cores": O.num_threads(),
                                "kind": "port", "sample": f"{reps} oracle triangle counts of the same graph "
                                                          f"({dtc * 1e6:.0f} us each)"}
+    return out
+
+
+def bench_ewise(args) -> dict:
+    """eWiseAdd C = A + B (include/matrix_la.h:375-436) of two R-MAT scale 20
+    graphs (edge factor 16, seeds 1 and 2) with int32 values.  The entry point
+    takes host arrays, so value = input entries per second of device time (the
+    count and fill passes, HIP events); the rate with the PCIe copies and the
+    host allocations is reported beside it."""
+    from graphalgo_amd import api, graphs as G
+    A, B = G.rmat(20, 16, seed=1), G.rmat(20, 16, seed=2)
+    rng = np.random.default_rng(0)
+    va = rng.integers(1, 100, size=A.nnz).astype(np.int32)
+    vb = rng.integers(1, 100, size=B.nnz).astype(np.int32)
+
+    def step():
+        return api.ewise_add(A, va, B, vb, dtype=np.int32)
+    dt, (Cm, cv), timing, acct = _measure(step, args.steps, args.warmup)
+    dev_s = timing["kernel_ms"] / timing["steps"] / 1e3
+    entries = A.nnz + B.nnz
+    out = {"value": entries / dev_s, "unit": "input entries/s", "ms_per_step": dev_s * 1e3, "dtype": "int32",
+           "config": {"workload": "eWiseAdd C = A + B of two R-MAT scale 20 graphs (edge factor 16), int32 values",
+                      "n": A.n, "nnz_A": A.nnz, "nnz_B": B.nnz, "nnz_C": Cm.nnz,
+                      "host_inclusive_entries_per_s": entries / dt, "host_inclusive_ms_per_call": dt * 1e3},
+           "roofline": _roofline(timing, acct, "k_ewise (count pass + scan, fill pass)", "k_ewise/ewise_add_rmat20")}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        O.ewise(O.EW_ADD, A.n, A.rowptr, A.col, va, B.rowptr, B.col, vb, dtype=np.int32)
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": entries / dtc, "unit": "input entries/s", "cores": 1, "kind": "port",
+                               "sample": f"one numpy oracle eWiseAdd of the same operands ({dtc:.1f} s)"}
     return out
 
 
@@ -393,7 +428,7 @@ def bench_serve(args) -> dict:
     return out
 
 
-WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "tc_er": bench_tc_er, "copy": bench_copy,
+WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "tc_er": bench_tc_er, "copy": bench_copy, "ewise": bench_ewise,
              "batch": bench_batch, "bc": bench_bc, "serve": bench_serve}
 
 

@@ -52,6 +52,7 @@ class GaServerConfig(C.Structure):
 
 
 GA_Q_TRIANGLES, GA_Q_KTRUSS, GA_Q_BC = 0, 1, 2
+GA_EW_ADD, GA_EW_MULT, GA_EW_ADD_DISJOINT = 0, 1, 2
 
 
 class GaQuery(C.Structure):
@@ -94,6 +95,9 @@ GA_SIGNATURES = {
     "ga_mxm_cmask": (C.c_int, [C.POINTER(GaCsr), C.POINTER(GaCsr), C.POINTER(GaCsr), C.c_int,
                                C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.c_void_p), I64P]),
     "ga_mxm_spd": (C.c_int, [C.POINTER(GaCsr), F64P, C.c_int64, C.POINTER(GaCsr), F64P]),
+    "ga_ewise": (C.c_int, [C.c_int, C.POINTER(GaCsr), C.POINTER(GaCsr), C.POINTER(GaCsr), C.c_int,
+                           C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.c_void_p), I64P]),
+    "ga_scale": (C.c_int, [C.POINTER(GaCsr), C.c_int, C.c_double, C.c_void_p]),
     "ga_free_host": (None, [C.c_void_p]),
     "ga_triangle_count": (C.c_int, [C.POINTER(GaCsr), I64P]),
     "ga_triangle_count_lu": (C.c_int, [C.POINTER(GaCsr), I64P]),

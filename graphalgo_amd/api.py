@@ -52,6 +52,59 @@ def mxm_masked(A: CSR, B: CSR, M: CSR, dtype=np.int64, Aval=None, Bval=None,
 mxmm_msa = mxmm_mca = mxmm_heap = mxm_masked
 
 
+def _ewise(op: int, A: CSR, Aval, B: CSR, Bval, M: CSR | None, dtype) -> tuple[CSR, np.ndarray]:
+    dt = np.dtype(dtype)
+    N.init()
+    av = None if Aval is None else np.ascontiguousarray(Aval, dtype=dt)
+    bv = None if Bval is None else np.ascontiguousarray(Bval, dtype=dt)
+    a, b = _ga(A, av), _ga(B, bv)
+    mm = None if M is None else _ga(M)
+    rp, col, val, nnz = N.I32P(), N.I32P(), C.c_void_p(), C.c_int64()
+    N.check(N.galib().ga_ewise(op, C.byref(a), C.byref(b), C.byref(mm) if mm is not None else None, _DT[dt],
+                               C.byref(rp), C.byref(col), C.byref(val), C.byref(nnz)))
+    nz = nnz.value
+    lib = N.galib()
+    try:
+        r = np.ctypeslib.as_array(rp, shape=(A.n + 1,)).copy()
+        c = np.ctypeslib.as_array(col, shape=(max(nz, 1),))[:nz].copy()
+        v = np.ctypeslib.as_array(C.cast(val, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(max(nz, 1),))[:nz].copy()
+    finally:
+        for p in (rp, col, val):
+            lib.ga_free_host(C.cast(p, C.c_void_p))
+    return CSR(A.n, r, c), v
+
+
+def ewise_add(A: CSR, Aval, B: CSR, Bval, M: CSR | None = None, dtype=np.int64) -> tuple[CSR, np.ndarray]:
+    """eWiseAdd (include/matrix_la.h:272-335 with a mask, :375-436 without):
+    the union of the patterns with values added; with M, C<M> = A + B on M's
+    pattern without the positions whose sum is zero.  Returns (pattern, values)."""
+    return _ewise(N.GA_EW_ADD, A, Aval, B, Bval, M, dtype)
+
+
+def ewise_mult(A: CSR, Aval, B: CSR, Bval, M: CSR | None = None, dtype=np.int64) -> tuple[CSR, np.ndarray]:
+    """eWiseMult (include/matrix_la.h:438-533): the intersection with values
+    multiplied, restricted to M's pattern when M is given."""
+    return _ewise(N.GA_EW_MULT, A, Aval, B, Bval, M, dtype)
+
+
+def add_nointersect(A: CSR, Aval, B: CSR, Bval, dtype=np.int64) -> tuple[CSR, np.ndarray]:
+    """add_nointersect (include/matrix_la.h:337-373): the union of two
+    patterns that share no position (raises when they do)."""
+    return _ewise(N.GA_EW_ADD_DISJOINT, A, Aval, B, Bval, None, dtype)
+
+
+def mult_scalar(A: CSR, Aval, alpha: float, dtype=np.int64) -> np.ndarray:
+    """multScalar (include/matrix_la.h:535-541): A's values times alpha in the
+    value type (integers converted back toward zero)."""
+    dt = np.dtype(dtype)
+    N.init()
+    av = None if Aval is None else np.ascontiguousarray(Aval, dtype=dt)
+    out = np.zeros(max(A.nnz, 1), dtype=dt)
+    a = _ga(A, av)
+    N.check(N.galib().ga_scale(C.byref(a), _DT[dt], float(alpha), out.ctypes.data_as(C.c_void_p)))
+    return out[: A.nnz]
+
+
 def triangle_count(A: CSR) -> int:
     """triangle_counting_masked_sandia (src/main.cpp:65-86): sum of C<L> = L*L."""
     N.init()

@@ -82,6 +82,25 @@ ga_status ga_mxm_masked(const ga_csr* A, const ga_csr* B, const ga_csr* M, ga_dt
 ga_status ga_mxm_cmask(const ga_csr* A, const ga_csr* B, const ga_csr* M, ga_dtype dtype, int32_t** Crowptr,
                        int32_t** Ccol, void** Cval, int64_t* Cnnz);
 
+/* Element-wise operators (include/matrix_la.h:272-541, defects fixed).
+ * op GA_EW_ADD: C = A + B over the union of the patterns (eWiseAdd, :375-436),
+ *   or with a mask M: C<M> = A + B on M's pattern, entries summing to zero
+ *   dropped (:272-335);
+ * op GA_EW_MULT: C = A .* B over the intersection (eWiseMult, :438-484), with a
+ *   mask restricted to M's pattern (:486-533);
+ * op GA_EW_ADD_DISJOINT: the union of two patterns that share no position
+ *   (add_nointersect, :337-373); GA_ERR_ARG if they do; no mask.
+ * Same shapes, sorted rows, values of dtype (NULL values = ones); M may be
+ * NULL.  Output arrays are allocated with malloc (free with ga_free_host):
+ * *Crowptr (nrows+1), *Ccol (*Cnnz), *Cval (*Cnnz values of dtype). */
+enum { GA_EW_ADD = 0, GA_EW_MULT = 1, GA_EW_ADD_DISJOINT = 2 };
+ga_status ga_ewise(int op, const ga_csr* A, const ga_csr* B, const ga_csr* M, ga_dtype dtype, int32_t** Crowptr,
+                   int32_t** Ccol, void** Cval, int64_t* Cnnz);
+/* multScalar (include/matrix_la.h:535-541) without its in-place write through
+ * a const operand: Cval[t] = A.val[t] * alpha in dtype (integers converted
+ * back toward zero, as `v *= alpha` does). */
+ga_status ga_scale(const ga_csr* A, ga_dtype dtype, double alpha, void* Cval);
+
 /* Masked sparse x dense: C<M> = A*B with B and C dense row-major
  * (A.ncols x ncols and A.nrows x ncols); row i of C is zero outside M's
  * row i.  Replaces mxmm_spd (include/matrix_la.h:239-270).  FP64 only. */

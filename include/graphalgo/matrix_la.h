@@ -5,6 +5,8 @@
 //                                     :998-1028 -> ga_mxm_masked
 //   mxmm_msa_cmask                    include/matrix_la.h:788-798 -> ga_mxm_cmask
 //   mxmm_spd                          include/matrix_la.h:239-270 -> ga_mxm_spd
+//   eWiseAdd, eWiseMult, add_nointersect, multScalar
+//                                     include/matrix_la.h:272-541 -> ga_ewise, ga_scale
 //
 // The three structural-mask variants are one GPU operator (the accumulator
 // choice of the reference -- dense MSA, mask-indexed MCA, heap -- becomes a
@@ -115,4 +117,79 @@ void mxmm_spd(const spMtx<T>& A, const denseMtx<T>& B, const spMtx<U>& M, denseM
   std::swap(C.capacity, Cbuf.capacity);
   C.m = M.m;
   C.n = B.n;
+}
+
+// ---- element-wise operators (include/matrix_la.h:272-541, defects fixed) ----
+namespace graphalgo_detail {
+template <typename T>
+spMtx<T> ewise(int op, const spMtx<T>& A, const spMtx<T>& B, const ga_csr* M) {
+  ensure_init();
+  const ga_csr a = A.view(true), b = B.view(true);
+  int32_t* rp = nullptr;
+  int32_t* col = nullptr;
+  void* val = nullptr;
+  int64_t nnz = 0;
+  check(ga_ewise(op, &a, &b, M, dtype_of<T>(), &rp, &col, &val, &nnz));
+  spMtx<T> C;
+  C.resizeRows(A.m);
+  C.n = A.n;
+  std::memcpy(C.Rst, rp, (A.m + 1) * sizeof(int));
+  C.resizeVals((size_t)nnz);
+  if (nnz) {
+    std::memcpy(C.Col, col, nnz * sizeof(int));
+    std::memcpy(C.Val, val, nnz * sizeof(T));
+  }
+  ga_free_host(rp);
+  ga_free_host(col);
+  ga_free_host(val);
+  return C;
+}
+}  // namespace graphalgo_detail
+
+// C<M> = A + B on M's pattern; positions whose sum is zero are left out
+template <typename T, typename U>
+spMtx<T> eWiseAdd(const spMtx<T>& A, const spMtx<T>& B, const spMtx<U>& M) {
+  if (A.m != B.m || A.n != B.n) throw -1;
+  const ga_csr mm = M.view(false);
+  return graphalgo_detail::ewise(GA_EW_ADD, A, B, &mm);
+}
+
+// the union of the patterns, values added where both hold an entry
+template <typename T>
+spMtx<T> eWiseAdd(const spMtx<T>& A, const spMtx<T>& B) {
+  if (A.m != B.m || A.n != B.n) throw -1;
+  return graphalgo_detail::ewise(GA_EW_ADD, A, B, nullptr);
+}
+
+// the union of two patterns that share no position
+template <typename T>
+spMtx<T> add_nointersect(const spMtx<T>& A, const spMtx<T>& B) {
+  if (A.m != B.m || A.n != B.n) throw -1;
+  return graphalgo_detail::ewise(GA_EW_ADD_DISJOINT, A, B, nullptr);
+}
+
+// the intersection of the patterns, values multiplied
+template <typename T>
+spMtx<T> eWiseMult(const spMtx<T>& A, const spMtx<T>& B) {
+  if (A.m != B.m || A.n != B.n) throw -1;
+  return graphalgo_detail::ewise(GA_EW_MULT, A, B, nullptr);
+}
+
+// the intersection restricted to M's pattern
+template <typename T, typename U>
+spMtx<T> eWiseMult(const spMtx<T>& A, const spMtx<T>& B, const spMtx<U>& M) {
+  if (A.m != B.m || A.n != B.n) throw -1;
+  const ga_csr mm = M.view(false);
+  return graphalgo_detail::ewise(GA_EW_MULT, A, B, &mm);
+}
+
+// A's values times alpha, returned as a new matrix (the operand is not
+// modified, unlike the reference's write through a const reference)
+template <typename MatrixValT, typename ScalarT>
+spMtx<MatrixValT> multScalar(const spMtx<MatrixValT>& A, const ScalarT& alpha) {
+  graphalgo_detail::ensure_init();
+  spMtx<MatrixValT> C = A;
+  const ga_csr a = A.view(true);
+  if (A.nz) graphalgo_detail::check(ga_scale(&a, graphalgo_detail::dtype_of<MatrixValT>(), (double)alpha, C.Val));
+  return C;
 }

@@ -195,3 +195,75 @@ def ref_bc(graph_path: str, batch: int, n: int) -> np.ndarray:
         out = os.path.join(d, "bc.bin")
         ref_run("bc", graph_path, str(batch), out)
         return np.fromfile(out, dtype=np.float32, count=n)
+
+
+# ---- element-wise operators (numpy restatement) -----------------------------
+# include/matrix_la.h:272-541 with the reference's defects fixed (the masked
+# eWiseAdd's second loop reads A's columns for B, :294/:321; the unmasked
+# eWiseMult's fill loop advances the row index instead of the output
+# position, :472; the masked eWiseMult's count and fill walks disagree,
+# :497-530).  Keys are row * n + col over sorted rows, so every result comes
+# out in CSR order.
+EW_ADD, EW_MULT, EW_ADD_DISJOINT = 0, 1, 2
+
+
+def _keys(n, rp, col):
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(np.asarray(rp, dtype=np.int64)))
+    return rows * n + np.asarray(col, dtype=np.int64)
+
+
+def _csr_from_keys(n, keys):
+    rows = keys // n
+    rp = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(np.bincount(rows, minlength=n), out=rp[1:])
+    return rp, (keys % n).astype(np.int32)
+
+
+def ewise(op, n, Arp, Acol, Aval, Brp, Bcol, Bval, Mrp=None, Mcol=None, dtype=np.int64):
+    """(rowptr, col, val) of eWiseAdd / eWiseMult / add_nointersect on n x n
+    operands; None values are ones; M (rowptr, col) optional."""
+    dt = np.dtype(dtype)
+    ka, kb = _keys(n, Arp, Acol), _keys(n, Brp, Bcol)
+    va = np.ones(len(ka), dt) if Aval is None else np.asarray(Aval, dtype=dt)
+    vb = np.ones(len(kb), dt) if Bval is None else np.asarray(Bval, dtype=dt)
+
+    def lookup(keys, k, v):
+        p = np.searchsorted(k, keys)
+        hit = (p < len(k)) & (k[np.minimum(p, max(len(k) - 1, 0))] == keys) if len(k) else np.zeros(len(keys), bool)
+        out = np.zeros(len(keys), dt)
+        out[hit] = v[p[hit]]
+        return hit, out
+
+    if Mrp is not None:
+        km = _keys(n, Mrp, Mcol)
+        ha, xa = lookup(km, ka, va)
+        hb, xb = lookup(km, kb, vb)
+        if op == EW_MULT:
+            keep = ha & hb
+            val = (xa * xb)[keep]
+        else:
+            s = xa + xb
+            keep = s != 0
+            val = s[keep]
+        rp, col = _csr_from_keys(n, km[keep])
+        return rp, col, val
+    if op == EW_MULT:
+        k = np.intersect1d(ka, kb)
+        _, xa = lookup(k, ka, va)
+        _, xb = lookup(k, kb, vb)
+        rp, col = _csr_from_keys(n, k)
+        return rp, col, xa * xb
+    if op == EW_ADD_DISJOINT and len(np.intersect1d(ka, kb)):
+        raise ValueError("add_nointersect: the operands share a position")
+    k = np.union1d(ka, kb)
+    ha, xa = lookup(k, ka, va)
+    hb, xb = lookup(k, kb, vb)
+    rp, col = _csr_from_keys(n, k)
+    return rp, col, xa + xb
+
+
+def scale(val, alpha, dtype=np.int64):
+    """multScalar (include/matrix_la.h:535-541): v * alpha in FP64, converted
+    back to the value type toward zero."""
+    dt = np.dtype(dtype)
+    return (np.asarray(val, dtype=dt).astype(np.float64) * float(alpha)).astype(dt)

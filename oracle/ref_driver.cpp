@@ -15,6 +15,10 @@
 //                             number of every entry (int32) to out.bin
 //   bc      <graph> batch out.bin  betweenness_centrality_batch (float32)
 //   mxm     <graph> out.bin   C<A> = A*A with mxmm_msa (int32 values)
+//   ewadd   <A> <B> out.bin   eWiseAdd(A, B) (unmasked, values 1), writes
+//                             rowptr, col, val (int32) of C
+//   ewdisj  <A> <B> out.bin   add_nointersect(A, B), same output
+// (ewadd / ewdisj load the operands as they are, without symmetrising)
 // <graph> is a Matrix Market file (read with the reference's reader, then
 // build_adjacency_matrix, and for tc/ktruss/mxm the symmetrised lower
 // triangle, exactly as launch_test does, src/main.cpp:221-222, :335) or a
@@ -85,6 +89,18 @@ int main(int argc, char** argv) {
   if (argc < 3) { fprintf(stderr, "usage: ref_driver tc|ktruss|ktall|bc|mxm <graph> ...\n"); return 2; }
   std::string mode = argv[1];
   try {
+    if (mode == "ewadd" || mode == "ewdisj") {
+      if (argc < 5) { fprintf(stderr, "usage: ref_driver ewadd|ewdisj <A> <B> out.bin\n"); return 2; }
+      spMtx<int> A = load(argv[2]), B = load(argv[3]);
+      spMtx<int> C = mode == "ewadd" ? eWiseAdd(A, B) : add_nointersect(A, B);
+      FILE* f = fopen(argv[4], "wb");
+      fwrite(C.Rst, sizeof(int), C.m + 1, f);
+      fwrite(C.Col, sizeof(int), C.Rst[C.m], f);
+      fwrite(C.Val, sizeof(int), C.Rst[C.m], f);
+      fclose(f);
+      printf("{\"mode\":\"%s\",\"n\":%zu,\"nnz\":%d}\n", mode.c_str(), C.m, C.Rst[C.m]);
+      return 0;
+    }
     spMtx<int> G = load(argv[2]);
     if (mode == "bc") {
       size_t batch = (size_t)atoll(argv[3]);

@@ -103,6 +103,16 @@ def test_cpp_dropin_client(gpu, golden, small_graphs, oracle):
     bc = oracle.bc(small_graphs["G43.mtx"][0], np.arange(16))
     assert out["bc_sum"] == pytest.approx(bc.astype(np.float32).astype(np.float64).sum(), rel=1e-6)
     assert out["kmax"] == rec["truss"]["kmax"]
+    # element-wise operators through the C++ headers, against the oracle
+    sq = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
+    half = oracle.scale(sq, 0.5, dtype=np.int32)
+    add = oracle.ewise(oracle.EW_ADD, g.n, g.rowptr, g.col, None, g.rowptr, g.col, half)
+    mult = oracle.ewise(oracle.EW_MULT, g.n, g.rowptr, g.col, sq, g.rowptr, g.col, sq, g.rowptr, g.col)
+    assert out["half_sum"] == half.sum()
+    assert (out["add_nnz"], out["add_sum"]) == (len(add[1]), add[2].sum())
+    assert (out["mult_nnz"], out["mult_sum"]) == (len(mult[1]), mult[2].sum())
+    assert out["cancel_nnz"] == 0
+    assert (out["disj_nnz"], out["disj_same"], out["disj_err"]) == (g.nnz, 1, 1)
     # full-vertex BC (every vertex a source, 64 per block): FP64, 1e-9 relative to the oracle
     g0 = small_graphs["G43.mtx"][0]
     want = oracle.bc(g0, np.arange(g0.n))

@@ -343,3 +343,113 @@ def test_planted_mistake_is_caught(G, oracle):
     Lt_col = rows[order].astype(np.int32)
     wrong = oracle.mxm_masked(n, n, L.rowptr, L.col, Lt_rp, Lt_col, L.rowptr, L.col)
     assert not np.array_equal(wrong, rows - L.col - 1)
+
+
+# ---------------------------------------------------------------- element-wise operators
+def _ew_operand(rng, m, p):
+    """Random pattern with random values, explicit zeros included (a stored zero is part
+    of the pattern)."""
+    P = rng.random((m, m)) < p
+    V = rng.integers(-3, 4, size=(m, m)) * P
+    rp = np.concatenate([[0], np.cumsum(P.sum(1))]).astype(np.int32)
+    r, c = np.nonzero(P)
+    return P, V, rp, c.astype(np.int32), V[r, c]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_ewise_dense_brute_force(oracle, seed):
+    """eWiseAdd / eWiseMult / add_nointersect, with and without a mask, against dense
+    arrays: the result pattern and values read off boolean and value matrices."""
+    rng = np.random.default_rng(100 + seed)
+    m = int(rng.integers(1, 12))
+    PA, VA, Arp, Acol, Aval = _ew_operand(rng, m, 0.4)
+    PB, VB, Brp, Bcol, Bval = _ew_operand(rng, m, 0.4)
+    PM, _, Mrp, Mcol, _ = _ew_operand(rng, m, 0.5)
+
+    def check(got, P, D):
+        rp, col, val = got
+        r, c = np.nonzero(P)
+        assert np.array_equal(rp, np.concatenate([[0], np.cumsum(P.sum(1))]))
+        assert np.array_equal(col, c) and np.array_equal(val, D[r, c])
+
+    S = VA + VB
+    for dt in (np.int64, np.float64):
+        check(oracle.ewise(oracle.EW_ADD, m, Arp, Acol, Aval, Brp, Bcol, Bval, dtype=dt), PA | PB, S)
+        check(oracle.ewise(oracle.EW_MULT, m, Arp, Acol, Aval, Brp, Bcol, Bval, dtype=dt), PA & PB, VA * VB)
+        check(oracle.ewise(oracle.EW_ADD, m, Arp, Acol, Aval, Brp, Bcol, Bval, Mrp, Mcol, dtype=dt),
+              PM & (S != 0), S)
+        check(oracle.ewise(oracle.EW_MULT, m, Arp, Acol, Aval, Brp, Bcol, Bval, Mrp, Mcol, dtype=dt),
+              PM & PA & PB, VA * VB)
+    # pattern operands: values are ones
+    check(oracle.ewise(oracle.EW_ADD, m, Arp, Acol, None, Brp, Bcol, None), PA | PB,
+          PA.astype(np.int64) + PB)
+    # add_nointersect: an error on a shared position, the union otherwise
+    if (PA & PB).any():
+        with pytest.raises(ValueError):
+            oracle.ewise(oracle.EW_ADD_DISJOINT, m, Arp, Acol, Aval, Brp, Bcol, Bval)
+    PC = PB & ~PA
+    Crp = np.concatenate([[0], np.cumsum(PC.sum(1))]).astype(np.int32)
+    r, c = np.nonzero(PC)
+    check(oracle.ewise(oracle.EW_ADD_DISJOINT, m, Arp, Acol, Aval, Crp, c.astype(np.int32), VB[r, c]),
+          PA | PC, VA + VB * PC)
+
+
+def test_ewise_invariants(G, oracle):
+    """On an R-MAT graph: A+B = B+A, A.*B = B.*A, the masked results are the unmasked
+    ones restricted to M (zeros of the masked sum dropped), A.*A has A's pattern."""
+    g = G.rmat(9, 8, seed=4)
+    h = G.rmat(9, 8, seed=5)
+    rng = np.random.default_rng(6)
+    va = rng.integers(-2, 3, size=g.nnz)
+    vb = rng.integers(-2, 3, size=h.nnz)
+    n = g.n
+    for op in (oracle.EW_ADD, oracle.EW_MULT):
+        x = oracle.ewise(op, n, g.rowptr, g.col, va, h.rowptr, h.col, vb)
+        y = oracle.ewise(op, n, h.rowptr, h.col, vb, g.rowptr, g.col, va)
+        assert all(np.array_equal(a, b) for a, b in zip(x, y))
+    m = G.rmat(9, 4, seed=7)
+    for op in (oracle.EW_ADD, oracle.EW_MULT):
+        rp, col, val = oracle.ewise(op, n, g.rowptr, g.col, va, h.rowptr, h.col, vb)
+        mrp, mcol, mval = oracle.ewise(op, n, g.rowptr, g.col, va, h.rowptr, h.col, vb, m.rowptr, m.col)
+        full = dict(zip(np.repeat(np.arange(n), np.diff(rp)) * n + col, val))
+        mk = np.repeat(np.arange(n), np.diff(m.rowptr)) * n + m.col
+        want = [(k, full[k]) for k in mk if k in full and (op == oracle.EW_MULT or full[k] != 0)]
+        got = list(zip(np.repeat(np.arange(n), np.diff(mrp)) * n + mcol, mval))
+        assert got == want
+    rp, col, val = oracle.ewise(oracle.EW_MULT, n, g.rowptr, g.col, va, g.rowptr, g.col, va)
+    assert np.array_equal(rp, g.rowptr) and np.array_equal(col, g.col) and np.array_equal(val, va * va)
+
+
+def test_ewise_reference(G, oracle, small_graphs, tmp_path):
+    """eWiseAdd(A, B) and add_nointersect(A, B) against the reference's own operators
+    (oracle/_ref, values 1), when built here: G43 with a random pattern, and G43's
+    lower triangle with its strictly upper part."""
+    if not oracle.ref_available():
+        pytest.skip("reference harness not built on this host")
+    g = small_graphs["G43.mtx"][1]
+    n = g.n
+    h = G.erdos_renyi(n, 3 * n, seed=11)
+    rows = np.repeat(np.arange(n), np.diff(g.rowptr))
+    up = g.col > rows
+    U = G.CSR(n, np.concatenate([[0], np.cumsum(np.bincount(rows[up], minlength=n))]).astype(np.int32),
+              g.col[up].astype(np.int32))
+    L = g.lower()
+    for mode, a, b in (("ewadd", g, h), ("ewdisj", L, U)):
+        pa, pb, out = tmp_path / "a.csr", tmp_path / "b.csr", tmp_path / "c.bin"
+        a.write_csr(str(pa))
+        b.write_csr(str(pb))
+        info = oracle.ref_run(mode, str(pa), str(pb), str(out))
+        raw = np.fromfile(out, dtype=np.int32)
+        nnz = info["nnz"]
+        rp, col, val = raw[:n + 1], raw[n + 1:n + 1 + nnz], raw[n + 1 + nnz:]
+        op = oracle.EW_ADD if mode == "ewadd" else oracle.EW_ADD_DISJOINT
+        want = oracle.ewise(op, n, a.rowptr, a.col, None, b.rowptr, b.col, None)
+        assert np.array_equal(rp, want[0]) and np.array_equal(col, want[1]) and np.array_equal(val, want[2])
+    assert (want[2] == 1).all() and np.array_equal(want[1], g.col)
+
+
+def test_scale(oracle):
+    """multScalar: the product in FP64, converted toward zero for integer values."""
+    assert np.array_equal(oracle.scale([7, -7, 3, 0], 0.5), [3, -3, 1, 0])
+    assert np.array_equal(oracle.scale([7, -7], -1.5, dtype=np.int32), [-10, 10])
+    assert np.allclose(oracle.scale([0.25, -2.0], 3.0, dtype=np.float64), [0.75, -6.0], rtol=0, atol=0)
