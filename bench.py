@@ -38,6 +38,7 @@ HBM_PEAK_GBS = 8000.0
 
 def _sync():
     from graphalgo_amd import _native as N
+    N.init()
     N.check(N.galib().ga_synchronize())
 
 

@@ -14,9 +14,9 @@
 //                         values: converted back toward zero, like `v *= alpha`).
 //
 // Entry-parallel: the driving operand's entries (A's without a mask, M's
-// with one) are cut into aligned groups of 64; a wavefront takes four groups
-// (four entries per lane, their binary searches in lockstep), so a hub row
-// is spread over as many wavefronts as it has groups of 256 entries.  Each
+// with one) are cut into aligned groups of 64; a wavefront takes EW_Q groups
+// (EW_Q entries per lane, their binary searches in lockstep), so a hub row
+// is spread over as many wavefronts as it has tiles of 64 EW_Q entries.  Each
 // lane finds its entry's row in the short range of rows its group spans, and
 // the entry's column in the other operand(s) by binary search.  The mark
 // pass stores one bit per driving entry (A: has a match in B; M: kept) and
@@ -28,7 +28,10 @@
 //                 an unmatched B entry u goes to u + pa - P(pa);
 //   intersection: C's row pointer i = P(Arp[i]); a matched A entry t goes to P(t);
 //   masked:       C's row pointer i = P(Mrp[i]); a kept M entry t goes to P(t).
-// The fill pass recomputes the searches and writes C in CSR order.
+// Without a mask the mark pass also stores each A entry's lower bound in B,
+// so the fill reads A's side back instead of searching again; B's entries
+// (and M's, with a mask) are searched again in the fill.  C is written in
+// CSR order.
 #include "common.hpp"
 
 #include <algorithm>
@@ -38,7 +41,7 @@
 namespace ga {
 
 constexpr int EW_ADD = GA_EW_ADD, EW_MULT = GA_EW_MULT, EW_DISJOINT = GA_EW_ADD_DISJOINT;
-constexpr int EW_Q = 4;             // entries per lane
+constexpr int EW_Q = 8;             // entries per lane
 constexpr int EW_TILE = 64 * EW_Q;  // entries per wavefront
 
 template <typename T>
@@ -46,11 +49,19 @@ __device__ inline T ew_val(const T* __restrict__ v, int64_t j) {
   return v ? v[j] : T(1);  // pattern operands hold ones
 }
 
-// tile_row[g] = the row that holds entry 64 g (one thread per row)
-__global__ void k_ew_group_rows(const int32_t* __restrict__ rp, int64_t m, int32_t* __restrict__ tile_row) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = rp[i], e = rp[i + 1];
-    for (int64_t g = (b + 63) >> 6; g * 64 < e; ++g) tile_row[g] = (int32_t)i;
+// tile_row[g] = the row that holds entry 64 g: the last row r with
+// rp[r] <= 64 g (one thread per group)
+__global__ void k_ew_group_rows(const int32_t* __restrict__ rp, int64_t m, int64_t ngroups,
+                                int32_t* __restrict__ tile_row) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ngroups; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = g * 64;
+    int lo = 0, hi = (int)(m - 1);
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rp[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    tile_row[g] = lo;
   }
 }
 
@@ -178,7 +189,7 @@ __global__ __launch_bounds__(256) void k_ew_mark(const int32_t* __restrict__ Xrp
                                                  const int32_t* __restrict__ Acol, const T* __restrict__ Aval,
                                                  const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol,
                                                  const T* __restrict__ Bval, uint64_t* __restrict__ bits,
-                                                 int32_t* __restrict__ gcnt) {
+                                                 int32_t* __restrict__ gcnt, int32_t* __restrict__ bound) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t w = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); w * EW_TILE < nX; w += nwaves) {
@@ -193,6 +204,7 @@ __global__ __launch_bounds__(256) void k_ew_mark(const int32_t* __restrict__ Xrp
         bits[g] = b;
         gcnt[g] = __popcll(b);
       }
+      if (!MASKED && e.valid[q]) bound[base + q * 64 + lane] = e.pb[q];  // the fill reads it back
     }
   }
 }
@@ -209,8 +221,10 @@ __global__ void k_ew_rowptr(const int32_t* __restrict__ Arp, const int32_t* __re
   }
 }
 
-// Fill: wavefronts [0, wX) take the driving entries; for a union, wavefronts
-// [wX, wX + wB) take B's entries without a match in A.
+// Fill: wavefronts take tiles of the driving entries; for a union, tiles of
+// A and of B alternate (tile j of A, then tile j of B, ...: both write the
+// same region of C at about the same time), and B's entries without a match
+// in A are written.
 template <typename T, int OP, bool MASKED>
 __global__ __launch_bounds__(256) void k_ew_fill(const int32_t* __restrict__ Xrp, const int32_t* __restrict__ Xcol,
                                                  const int32_t* __restrict__ Xtr, int64_t nX, int64_t ngroups,
@@ -219,17 +233,41 @@ __global__ __launch_bounds__(256) void k_ew_fill(const int32_t* __restrict__ Xrp
                                                  const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol,
                                                  const T* __restrict__ Bval, const int32_t* __restrict__ Btr,
                                                  int64_t ngroupsB, const uint64_t* __restrict__ bits,
-                                                 const int32_t* __restrict__ P, int32_t* __restrict__ Ccol,
-                                                 T* __restrict__ Cval) {
+                                                 const int32_t* __restrict__ P, const int32_t* __restrict__ bound,
+                                                 int32_t* __restrict__ Ccol, T* __restrict__ Cval) {
   constexpr bool UNION = !MASKED && OP != EW_MULT;
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t nB = UNION ? (int64_t)Brp[m] : 0;
   const int64_t wX = (nX + EW_TILE - 1) / EW_TILE, wB = UNION ? (nB + EW_TILE - 1) / EW_TILE : 0;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t both = wX < wB ? wX : wB;
   for (int64_t w = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); w < wX + wB; w += nwaves) {
-    if (w < wX) {
-      const int64_t base = w * EW_TILE;
+    const bool on_x = w < 2 * both ? (w & 1) == 0 : wX > wB;
+    const int64_t tile = w < 2 * both ? (w >> 1) : w - both;
+    if (on_x && !MASKED) {
+      // A's entries: lower bounds in B and match bits from the mark pass
+      const int64_t base = tile * EW_TILE;
+#pragma unroll
+      for (int q = 0; q < EW_Q; ++q) {
+        const int64_t t = base + q * 64 + lane;
+        if (t >= nX) break;
+        const int64_t g = t >> 6;
+        const unsigned long long b = bits[g];
+        const bool fb = (b >> lane) & 1ull;
+        const int before = P[g] + __popcll(b & lt);  // matches before t
+        const int pb = bound[t];
+        if (UNION) {
+          const int64_t r = t + pb - before;
+          Ccol[r] = Xcol[t];
+          Cval[r] = fb ? ew_val(Aval, t) + ew_val(Bval, pb) : ew_val(Aval, t);
+        } else if (fb) {
+          Ccol[before] = Xcol[t];
+          Cval[before] = ew_val(Aval, t) * ew_val(Bval, pb);
+        }
+      }
+    } else if (on_x) {
+      const int64_t base = tile * EW_TILE;
       EwTile<T, OP, MASKED> e;
       e.run(base, lane, Xrp, Xcol, Xtr, nX, ngroups, m, Arp, Acol, Aval, Brp, Bcol, Bval);
 #pragma unroll
@@ -250,7 +288,7 @@ __global__ __launch_bounds__(256) void k_ew_fill(const int32_t* __restrict__ Xrp
       }
     } else if (UNION) {
       // B's entries: their lower bounds in A; the unmatched ones are written
-      const int64_t base = (w - wX) * EW_TILE;
+      const int64_t base = tile * EW_TILE;
       bool valid[EW_Q];
       int c[EW_Q], lo[EW_Q], end[EW_Q];
 #pragma unroll
@@ -299,24 +337,24 @@ static hipEvent_t ew_event(Context& c) {
 
 template <typename T, int OP, bool MASKED>
 static void ew_launch_mark(const DCsr& X, const int32_t* Xtr, int64_t ngX, const DCsr& A, const DCsr& B,
-                           uint64_t* bits, int32_t* gcnt, hipStream_t s) {
+                           uint64_t* bits, int32_t* gcnt, int32_t* bound, hipStream_t s) {
   const unsigned grid = grid_for(ceil_div(X.nnz, EW_TILE), 4, 65536);
   hipLaunchKernelGGL((k_ew_mark<T, OP, MASKED>), dim3(grid), dim3(256), 0, s, X.rp, X.col, Xtr, X.nnz, ngX,
                      A.nrows, A.rp, A.col, static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val),
-                     bits, gcnt);
+                     bits, gcnt, bound);
   GA_HIP(hipGetLastError());
 }
 
 template <typename T, int OP, bool MASKED>
 static void ew_launch_fill(const DCsr& X, const int32_t* Xtr, int64_t ngX, const DCsr& A, const DCsr& B,
-                           const int32_t* Btr, int64_t ngB, const uint64_t* bits, const int32_t* P, int32_t* Ccol,
-                           T* Cval, hipStream_t s) {
+                           const int32_t* Btr, int64_t ngB, const uint64_t* bits, const int32_t* P,
+                           const int32_t* bound, int32_t* Ccol, T* Cval, hipStream_t s) {
   constexpr bool UNION = !MASKED && OP != EW_MULT;
   const int64_t waves = ceil_div(X.nnz, EW_TILE) + (UNION ? ceil_div(B.nnz, EW_TILE) : 0);
   const unsigned grid = grid_for(waves, 4, 65536);
   hipLaunchKernelGGL((k_ew_fill<T, OP, MASKED>), dim3(grid), dim3(256), 0, s, X.rp, X.col, Xtr, X.nnz, ngX,
                      A.nrows, A.rp, A.col, static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val),
-                     Btr, ngB, bits, P, Ccol, Cval);
+                     Btr, ngB, bits, P, bound, Ccol, Cval);
   GA_HIP(hipGetLastError());
 }
 
@@ -336,13 +374,14 @@ static void ewise_run(const DCsr& A, const DCsr& B, const DCsr* M, int32_t* Crp_
   const DCsr& X = MASKED ? *M : A;
   const int64_t ngX = ceil_div(X.nnz, 64), ngB = UNION ? ceil_div(B.nnz, 64) : 0;
   Context& c = ctx();
-  DevBuf trX, trB, bits, gcnt, P, rp, tmp, ccol, cval;
+  DevBuf trX, trB, bits, gcnt, P, rp, tmp, ccol, cval, bound;
   trX.ensure(sizeof(int32_t) * std::max<int64_t>(ngX, 1));
   trB.ensure(sizeof(int32_t) * std::max<int64_t>(ngB, 1));
   bits.ensure(sizeof(uint64_t) * std::max<int64_t>(ngX, 1));
   gcnt.ensure(sizeof(int32_t) * std::max<int64_t>(ngX, 1));
   P.ensure(sizeof(int32_t) * (ngX + 1));
   rp.ensure(sizeof(int32_t) * (m + 1));
+  bound.ensure(sizeof(int32_t) * (MASKED ? 1 : std::max<int64_t>(A.nnz, 1)));  // A's lower bounds in B
   hipEvent_t e_open = nullptr;
   auto mark = [&](bool open) {
     if (!c.prof_events_on) return;
@@ -356,12 +395,14 @@ static void ewise_run(const DCsr& A, const DCsr& B, const DCsr* M, int32_t* Crp_
     }
   };
   mark(true);
-  hipLaunchKernelGGL(k_ew_group_rows, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, X.rp, m, trX.as<int32_t>());
+  hipLaunchKernelGGL(k_ew_group_rows, dim3(grid_for(ngX, 256, 16384)), dim3(256), 0, s, X.rp, m, ngX,
+                     trX.as<int32_t>());
   if (UNION)
-    hipLaunchKernelGGL(k_ew_group_rows, dim3(grid_for(m, 256, 16384)), dim3(256), 0, s, B.rp, m,
+    hipLaunchKernelGGL(k_ew_group_rows, dim3(grid_for(ngB, 256, 16384)), dim3(256), 0, s, B.rp, m, ngB,
                        trB.as<int32_t>());
   GA_HIP(hipGetLastError());
-  ew_launch_mark<T, OP, MASKED>(X, trX.as<int32_t>(), ngX, A, B, bits.as<uint64_t>(), gcnt.as<int32_t>(), s);
+  ew_launch_mark<T, OP, MASKED>(X, trX.as<int32_t>(), ngX, A, B, bits.as<uint64_t>(), gcnt.as<int32_t>(),
+                                bound.as<int32_t>(), s);
   scan_exclusive_i32(gcnt.as<int32_t>(), P.as<int32_t>(), ngX, tmp, s);
   hipLaunchKernelGGL((k_ew_rowptr<OP, MASKED>), dim3(grid_for(m + 1, 256, 16384)), dim3(256), 0, s, A.rp, B.rp,
                      MASKED ? M->rp : nullptr, m, (const uint64_t*)bits.as<uint64_t>(), (const int32_t*)P.as<int32_t>(),
@@ -378,7 +419,7 @@ static void ewise_run(const DCsr& A, const DCsr& B, const DCsr* M, int32_t* Crp_
   cval.ensure(sizeof(T) * std::max<int64_t>(nz, 1));
   mark(true);
   ew_launch_fill<T, OP, MASKED>(X, trX.as<int32_t>(), ngX, A, B, trB.as<int32_t>(), ngB, bits.as<uint64_t>(),
-                                P.as<int32_t>(), ccol.as<int32_t>(), cval.as<T>(), s);
+                                P.as<int32_t>(), bound.as<int32_t>(), ccol.as<int32_t>(), cval.as<T>(), s);
   mark(false);
   if (c.prof_bytes) {
     const int64_t vs = (int64_t)sizeof(T), va = A.val ? vs : 0, vb = B.val ? vs : 0;
