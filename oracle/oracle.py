@@ -1,7 +1,10 @@
 """ORACLE -- test infrastructure only.
 
 ctypes wrapper of oracle/_build/liboracle.so (the C restatement in oracle.c)
-and of the reference harness oracle/_ref/ref_driver.  Only tests/, the smoke
+and of the reference harness oracle/_ref/ref_driver, plus a numpy restatement
+of the element-wise operators (ewise, scale; pinned in tests/test_oracle.py by
+dense brute force, invariants and the reference's own eWiseAdd and
+add_nointersect).  Only tests/, the smoke
 check and bench.py's cpu_baseline leg import this module, and only as the
 checker or the timed CPU baseline; the product (graphalgo_amd) never does.
 """
