@@ -72,27 +72,79 @@ static void upper_triangle_dev(const DCsr& A, OwnedCsr& U, DevBuf& tmp, hipStrea
 // and the new row pointer of row i is that count at rp[i].  Every thread
 // owns KEEP_PER consecutive entries; all loads and stores are coalesced and
 // the work does not depend on the row lengths.
+// Peeling state of a device-driven run (one block of device memory).  The
+// step kernels read it when they start; k_truss_state, the last kernel of a
+// step, updates it.  A run needs no host decision between steps: steps
+// enqueued after the run has finished see done = 1 and do nothing.
+constexpr int TRUSS_MAX_K = 64;  // k values of one shared k-truss run
+struct TrussDevState {
+  long long nnz;     // entries of the current graph
+  long long passes;  // support passes run
+  long long steps;   // filter steps run
+  long long ran;     // step kernels run, the no-ops after the end included
+  int k;             // the current k (threshold k - 2)
+  int valid;         // the supports of the current graph are computed
+  int done;
+  int kidx;          // k list: index of the current k
+  int kmax;          // truss decomposition: largest k whose truss is nonempty
+  int skip_support;  // the next step runs no support pass
+  int skip_compact;  // the next step runs no compaction
+  int assign_k;      // truss decomposition: k for the current entries (0 = none)
+  int all_k;         // truss decomposition (else a k list)
+  int nk;
+  int error;         // a survivor count outside [0, nnz] ended the run
+  int ks[TRUSS_MAX_K];
+  int passes_at[TRUSS_MAX_K];
+  long long nnz_at[TRUSS_MAX_K];
+};
+
 constexpr int KEEP_T = 256;
 constexpr int KEEP_PER = 8;
 constexpr int KEEP_TILE = KEEP_T * KEEP_PER;
 constexpr int KEEP_FUSED_TILES = 1024;  // up to here a tile sums the tiles before it itself
 
+// ds (device-driven steps): nnz and thr come from the state, the nnz
+// argument is the first graph's (its tiles are walked with a grid stride,
+// the ones past the current graph count zero), and the entries of a graph
+// that was stable at the previous step get their truss number assign_k
+// (eid, truss; truss decomposition).
 __global__ __launch_bounds__(KEEP_T) void k_keep_tile_sums(const int32_t* __restrict__ sup, int64_t nnz, int thr,
-                                                           long long* __restrict__ bsum) {
+                                                           long long* __restrict__ bsum,
+                                                           const TrussDevState* __restrict__ ds,
+                                                           const int32_t* __restrict__ eid,
+                                                           int32_t* __restrict__ truss) {
   __shared__ long long red[KEEP_T / 64];
-  const int64_t base = (int64_t)blockIdx.x * KEEP_TILE;
-  long long c = 0;
-  for (int q = threadIdx.x; q < KEEP_TILE; q += KEEP_T) {
-    const int64_t j = base + q;
-    if (j < nnz) c += sup[j] >= thr ? 1 : 0;
+  int64_t ntiles = gridDim.x;
+  int assign = 0;
+  if (ds) {
+    ntiles = (nnz + KEEP_TILE - 1) / KEEP_TILE;
+    nnz = ds->skip_compact ? 0 : ds->nnz;
+    thr = ds->k - 2;
+    assign = truss ? ds->assign_k : 0;
   }
-  c = wave_reduce_sum(c);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long t = 0;
-    for (int w = 0; w < KEEP_T / 64; ++w) t += red[w];
-    bsum[blockIdx.x] = t;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t base = tile * KEEP_TILE;
+    if (base >= nnz) {  // device-driven: past the current graph
+      if (threadIdx.x == 0) bsum[tile] = 0;
+      continue;
+    }
+    long long c = 0;
+    for (int q = threadIdx.x; q < KEEP_TILE; q += KEEP_T) {
+      const int64_t j = base + q;
+      if (j < nnz) {
+        c += sup[j] >= thr ? 1 : 0;
+        if (assign) truss[eid[j]] = assign;
+      }
+    }
+    c = wave_reduce_sum(c);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long t = 0;
+      for (int w = 0; w < KEEP_T / 64; ++w) t += red[w];
+      bsum[tile] = t;
+    }
+    __syncthreads();  // red is reused by the next tile
   }
 }
 
@@ -103,65 +155,75 @@ __global__ __launch_bounds__(KEEP_T) void k_keep_tile_sums(const int32_t* __rest
 // owns its entries [w * 512, (w + 1) * 512) and reads them 64 at a time
 // (coalesced); a ballot per 64 entries ranks the survivors, so the stores of
 // consecutive survivors are consecutive too.  The last tile writes the
-// survivor count to *kept.
+// survivor count to *kept.  ds: as in k_keep_tile_sums (tiles walked with a
+// grid stride up to the current graph's last).
 __global__ __launch_bounds__(KEEP_T) void k_keep_scatter(
     const int32_t* __restrict__ col, const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t nnz,
     int thr, const long long* __restrict__ bsum, int offsets_ready, int32_t* __restrict__ ncol,
     int32_t* __restrict__ neid, int32_t* __restrict__ nold, int32_t* __restrict__ newpos, int32_t* __restrict__ nsup,
-    int carry, int32_t* __restrict__ kept) {
+    int carry, int32_t* __restrict__ kept, const TrussDevState* __restrict__ ds) {
   constexpr int NW = KEEP_T / 64;
   __shared__ long long prior[NW];
   __shared__ int wcount[NW];
+  int64_t ntiles = gridDim.x;
+  if (ds) {
+    nnz = ds->skip_compact ? 0 : ds->nnz;
+    thr = ds->k - 2;
+    ntiles = (nnz + KEEP_TILE - 1) / KEEP_TILE;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  long long before = 0;
-  if (offsets_ready) {
-    before = tid == 0 ? bsum[blockIdx.x] : 0;
-  } else {
-    for (int b = tid; b < (int)blockIdx.x; b += KEEP_T) before += bsum[b];
-  }
-  before = wave_reduce_sum(before);
-  if (lane == 0) prior[wv] = before;
-  const int64_t wbase = (int64_t)blockIdx.x * KEEP_TILE + (int64_t)wv * 64 * KEEP_PER;
-  unsigned long long mk[KEEP_PER];
-  int32_t sv[KEEP_PER];
-  int cnt = 0;
-#pragma unroll
-  for (int q = 0; q < KEEP_PER; ++q) {
-    const int64_t j = wbase + q * 64 + lane;
-    sv[q] = j < nnz ? sup[j] : -1;
-    mk[q] = __ballot(j < nnz && sv[q] >= thr);
-    cnt += __popcll(mk[q]);
-  }
-  if (lane == 0) wcount[wv] = cnt;
-  __syncthreads();
-  long long off = 0;
-#pragma unroll
-  for (int k = 0; k < NW; ++k) {
-    off += prior[k];
-    if (k < wv) off += wcount[k];
-  }
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-  for (int q = 0; q < KEEP_PER; ++q) {
-    const int64_t j = wbase + q * 64 + lane;
-    if (j < nnz) {
-      const bool keep = (mk[q] >> lane) & 1ull;
-      const int r = (int)(off + __popcll(mk[q] & lt));
-      if (keep) {
-        ncol[r] = col[j];
-        neid[r] = eid[j];
-        nold[r] = (int32_t)j;
-        nsup[r] = carry ? sv[q] : 0;
-      }
-      newpos[j] = keep ? r : -1 - r;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    long long before = 0;
+    if (offsets_ready) {
+      before = tid == 0 ? bsum[tile] : 0;
+    } else {
+      for (int64_t b = tid; b < tile; b += KEEP_T) before += bsum[b];
     }
-    off += __popcll(mk[q]);
-  }
-  if (blockIdx.x == gridDim.x - 1 && tid == 0) {
-    long long t = 0;
+    before = wave_reduce_sum(before);
+    if (lane == 0) prior[wv] = before;
+    const int64_t wbase = tile * KEEP_TILE + (int64_t)wv * 64 * KEEP_PER;
+    unsigned long long mk[KEEP_PER];
+    int32_t sv[KEEP_PER];
+    int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) t += prior[k] + wcount[k];
-    *kept = (int32_t)t;
+    for (int q = 0; q < KEEP_PER; ++q) {
+      const int64_t j = wbase + q * 64 + lane;
+      sv[q] = j < nnz ? sup[j] : -1;
+      mk[q] = __ballot(j < nnz && sv[q] >= thr);
+      cnt += __popcll(mk[q]);
+    }
+    if (lane == 0) wcount[wv] = cnt;
+    __syncthreads();
+    long long off = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      off += prior[k];
+      if (k < wv) off += wcount[k];
+    }
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int q = 0; q < KEEP_PER; ++q) {
+      const int64_t j = wbase + q * 64 + lane;
+      if (j < nnz) {
+        const bool keep = (mk[q] >> lane) & 1ull;
+        const int r = (int)(off + __popcll(mk[q] & lt));
+        if (keep) {
+          ncol[r] = col[j];
+          neid[r] = eid[j];
+          nold[r] = (int32_t)j;
+          nsup[r] = carry ? sv[q] : 0;
+        }
+        newpos[j] = keep ? r : -1 - r;
+      }
+      off += __popcll(mk[q]);
+    }
+    if ((tile + 1) * KEEP_TILE >= nnz && tid == 0) {  // the last tile
+      long long t = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) t += prior[k] + wcount[k];
+      *kept = (int32_t)t;
+    }
+    __syncthreads();  // prior and wcount are reused by the next tile
   }
 }
 
@@ -173,7 +235,12 @@ __global__ __launch_bounds__(KEEP_T) void k_keep_scatter(
 __global__ void k_keep_finish(const int32_t* __restrict__ rp, int64_t n, const int32_t* __restrict__ newpos,
                               int64_t nnz, const int32_t* __restrict__ kept, int32_t* __restrict__ nrp,
                               uint8_t* __restrict__ dirty, const int32_t* __restrict__ mirror_old,
-                              const int32_t* __restrict__ nold, int32_t* __restrict__ mirror_new) {
+                              const int32_t* __restrict__ nold, int32_t* __restrict__ mirror_new,
+                              const TrussDevState* __restrict__ ds) {
+  if (ds) {
+    if (ds->skip_compact) return;
+    nnz = ds->nnz;
+  }
   const int32_t total = *kept;
   auto before = [&](int64_t j) -> int32_t {
     if (j >= nnz) return total;
@@ -204,6 +271,287 @@ __global__ void k_truss_assign(const int32_t* __restrict__ eid, int64_t nnz, int
 __global__ void k_iota(int32_t* __restrict__ p, int64_t n) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
     p[e] = (int32_t)e;
+}
+
+// ---- device-driven peeling steps -------------------------------------------
+// A step is the body of the host loop in truss_loop: a support pass when the
+// current graph has none, a compaction at threshold k - 2, and the state
+// update.  Its kernels take the same arguments every time (the graph always
+// lives in buffer set 0: a compaction that removes entries copies the
+// survivors back from set 1), so one captured step replays for the whole run.
+
+struct TrussInit {
+  long long nnz;
+  int all_k;
+  int nk;
+  int ks[TRUSS_MAX_K];
+};
+
+__global__ void k_truss_state_init(TrussDevState* __restrict__ ds, TrussInit in) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  TrussDevState& d = *ds;
+  d.nnz = in.nnz;
+  d.passes = 0;
+  d.steps = 0;
+  d.ran = 0;
+  d.all_k = in.all_k;
+  d.nk = in.nk;
+  for (int i = 0; i < TRUSS_MAX_K; ++i) {
+    d.ks[i] = i < in.nk ? in.ks[i] : 0;
+    d.passes_at[i] = 0;
+    d.nnz_at[i] = 0;
+  }
+  d.k = in.all_k ? 3 : in.ks[0];
+  d.valid = 0;
+  d.kidx = 0;
+  d.kmax = 2;
+  d.assign_k = 0;
+  d.error = 0;
+  d.done = (in.all_k && in.nnz == 0) ? 1 : 0;
+  d.skip_support = (d.done || in.nnz == 0) ? 1 : 0;
+  d.skip_compact = d.skip_support;
+}
+
+// Survivors back into buffer set 0 (row pointers, columns, edge ids, mirror
+// positions); their supports cleared for the next pass.  Only when the
+// step's compaction removed entries.
+__global__ void k_keep_copyback(const TrussDevState* __restrict__ ds, const int32_t* __restrict__ kept, int64_t n,
+                                const int32_t* __restrict__ rp1, const int32_t* __restrict__ col1,
+                                const int32_t* __restrict__ eid1, const int32_t* __restrict__ mir1,
+                                int32_t* __restrict__ rp0, int32_t* __restrict__ col0, int32_t* __restrict__ eid0,
+                                int32_t* __restrict__ mir0, int32_t* __restrict__ sup0) {
+  if (ds->skip_compact) return;
+  const int64_t total = *kept;
+  if (total == ds->nnz) return;
+  const int64_t span = max<int64_t>(n + 1, total);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < span; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t <= n) rp0[t] = rp1[t];
+    if (t < total) {
+      col0[t] = col1[t];
+      eid0[t] = eid1[t];
+      mir0[t] = mir1[t];
+      sup0[t] = 0;
+    }
+  }
+}
+
+static __device__ void truss_state_update(TrussDevState& d, const int32_t* __restrict__ kept) {
+  auto next_k = [&]() {
+    d.nnz_at[d.kidx] = d.nnz;
+    d.passes_at[d.kidx] = (int)d.passes;
+    if (++d.kidx == d.nk) d.done = 1;
+    else d.k = d.ks[d.kidx];
+  };
+  if (!d.valid) {  // this step ran the support pass (none for an empty graph)
+    d.passes += 1;
+    d.valid = 1;
+  }
+  d.steps += 1;
+  if (d.nnz != 0 && (*kept < 0 || (long long)*kept > d.nnz)) {  // cannot happen; never index with it
+    d.error = 1;
+    d.done = 1;
+  } else if (d.nnz == 0) {  // nothing to filter: a k-truss that became empty
+    if (d.all_k) d.done = 1;
+    else next_k();
+  } else if ((long long)*kept == d.nnz) {  // stable at k
+    if (d.all_k) {
+      d.assign_k = d.k;
+      d.kmax = d.k;
+      d.k += 1;
+    } else {
+      next_k();
+    }
+  } else {
+    d.nnz = *kept;
+    d.valid = 0;
+  }
+  if (d.all_k && d.nnz == 0) d.done = 1;
+  d.skip_support = (d.done || d.valid || d.nnz == 0) ? 1 : 0;
+  d.skip_compact = (d.done || d.nnz == 0) ? 1 : 0;
+}
+
+// k_truss_state runs the rest of one iteration of truss_loop's host loop
+// (truss_state_update, in the same order: passes, steps and the per-k
+// records count exactly as there), then the test at the top of the next
+// iteration.  progress (host memory): the step
+// kernels run so far and the done flag, for the host's flow control.
+__global__ void k_truss_state(TrussDevState* __restrict__ ds, const int32_t* __restrict__ kept,
+                              long long* __restrict__ progress) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  TrussDevState& d = *ds;
+  d.assign_k = 0;
+  d.ran += 1;
+  if (!d.done) truss_state_update(d, kept);
+  if (progress) {
+    __hip_atomic_store(&progress[1], (long long)d.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&progress[0], d.ran, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// The peeling loop of truss_loop as device-driven steps (buffers set up by
+// truss_loop; selective support passes and byte accounting stay on the host
+// loop).  graph: the step is captured once into a HIP graph and replayed;
+// otherwise its kernels are launched step by step.  The host keeps a few
+// steps queued ahead of the device (flow control below); the steps queued
+// after the last one are no-ops.
+static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s,
+                        bool graph, int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st, int64_t* nnz_at,
+                        int32_t* passes_at) {
+  const int64_t n = A.nrows, nnz0 = A.nnz;
+  GA_REQUIRE(all_k || nk <= TRUSS_MAX_K, GA_ERR_LIMIT, "at most 64 k values in one k-truss run");
+  Context& c = ctx();
+  tw.dstate.ensure(sizeof(TrussDevState));
+  auto* ds = tw.dstate.as<TrussDevState>();
+  if (nnz0) fill_u32(tw.sup[0].p, 0u, nnz0, s);
+  tw.mw.rwork[0].ensure(sizeof(unsigned long long) * std::max<int64_t>(n, 1));
+  if (n) fill_u64(tw.mw.rwork[0].p, 0ull, n, s);
+  TrussInit in{};
+  in.nnz = nnz0;
+  in.all_k = all_k ? 1 : 0;
+  in.nk = all_k ? 0 : nk;
+  for (int i = 0; i < in.nk; ++i) in.ks[i] = ks[i];
+  hipLaunchKernelGGL(k_truss_state_init, dim3(1), dim3(64), 0, s, ds, in);
+  GA_HIP(hipGetLastError());
+  struct FixedArgs {  // the support passes take fixed arguments for the run only
+    MxmWork& w;
+    ~FixedArgs() {
+      w.fixed_args = false;
+      w.skip = nullptr;
+    }
+  } fixed{tw.mw};
+  tw.mw.fixed_args = true;
+  tw.mw.skip = &ds->skip_support;
+  int32_t* kept_dev = tw.kept.as<int32_t>();
+  long long* bsum = tw.scan_tmp.as<long long>();
+  long long* prog_dev = nullptr;
+  long long* prog = progress_words(&prog_dev);
+  __atomic_store_n(&prog[0], 0ll, __ATOMIC_RELAXED);
+  __atomic_store_n(&prog[1], 0ll, __ATOMIC_RELEASE);
+  const int64_t nbk = ceil_div(nnz0, KEEP_TILE);
+  const bool offsets = nbk > KEEP_FUSED_TILES;
+  // grids of at most two workgroups per CU: the graph shrinks during the
+  // run, and a step that has little left to do should not dispatch
+  // thousands of workgroups that find no work
+  const int64_t gcap = 2 * (int64_t)c.num_cu;
+  const unsigned gtiles = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nbk, gcap));
+  const unsigned gspan = (unsigned)std::min<int64_t>(grid_for(std::max<int64_t>(n + 1, nnz0), 256, 8192), gcap);
+  int64_t support_launches = 0;  // enqueued item launches (profiling: the skipped ones are not passes)
+  auto step = [&]() {
+    if (nnz0 > 0) {
+      const DCsr C{n, n, nnz0, tw.cur[0].rp.as<int32_t>(), tw.cur[0].col.as<int32_t>(), nullptr};
+      support_symmetric_dev(C, tw.mirror[0].as<int32_t>(), tw.sup[0].as<int32_t>(), tw.mw, s, true);
+      ++support_launches;
+      hipLaunchKernelGGL(k_keep_tile_sums, dim3(gtiles), dim3(KEEP_T), 0, s,
+                         (const int32_t*)tw.sup[0].as<int32_t>(), nnz0, 0, bsum, (const TrussDevState*)ds,
+                         (const int32_t*)tw.eid[0].as<int32_t>(), all_k ? tw.truss.as<int32_t>() : nullptr);
+      if (offsets) scan_tile_offsets(bsum, nbk, kept_dev, s);
+      hipLaunchKernelGGL(k_keep_scatter, dim3(gtiles), dim3(KEEP_T), 0, s,
+                         (const int32_t*)tw.cur[0].col.as<int32_t>(), (const int32_t*)tw.eid[0].as<int32_t>(),
+                         (const int32_t*)tw.sup[0].as<int32_t>(), nnz0, 0, (const long long*)bsum, offsets ? 1 : 0,
+                         tw.cur[1].col.as<int32_t>(), tw.eid[1].as<int32_t>(), tw.nold.as<int32_t>(),
+                         tw.newpos.as<int32_t>(), tw.sup[1].as<int32_t>(), 0, kept_dev, (const TrussDevState*)ds);
+      hipLaunchKernelGGL(k_keep_finish, dim3(gspan), dim3(256), 0, s, (const int32_t*)tw.cur[0].rp.as<int32_t>(), n,
+                         (const int32_t*)tw.newpos.as<int32_t>(), nnz0, (const int32_t*)kept_dev,
+                         tw.cur[1].rp.as<int32_t>(), (uint8_t*)nullptr, (const int32_t*)tw.mirror[0].as<int32_t>(),
+                         (const int32_t*)tw.nold.as<int32_t>(), tw.mirror[1].as<int32_t>(), (const TrussDevState*)ds);
+      hipLaunchKernelGGL(k_keep_copyback, dim3(gspan), dim3(256), 0, s, (const TrussDevState*)ds,
+                         (const int32_t*)kept_dev, n, (const int32_t*)tw.cur[1].rp.as<int32_t>(),
+                         (const int32_t*)tw.cur[1].col.as<int32_t>(), (const int32_t*)tw.eid[1].as<int32_t>(),
+                         (const int32_t*)tw.mirror[1].as<int32_t>(), tw.cur[0].rp.as<int32_t>(),
+                         tw.cur[0].col.as<int32_t>(), tw.eid[0].as<int32_t>(), tw.mirror[0].as<int32_t>(),
+                         tw.sup[0].as<int32_t>());
+    }
+    // (a truss decomposition's entries get the k they were stable at from
+    // the next step's tile sums: that step always runs, on the same graph)
+    hipLaunchKernelGGL(k_truss_state, dim3(1), dim3(64), 0, s, ds, (const int32_t*)kept_dev, prog_dev);
+    GA_HIP(hipGetLastError());
+  };
+  // the first step runs eagerly: it sizes every workspace buffer, so the
+  // capture allocates nothing
+  step();
+  graph = graph && !c.prof_events_on;  // kernel timing needs events around every launch
+  if (graph) {
+    const MxmWork& w = tw.mw;
+    std::vector<const void*> key = {
+        (const void*)(intptr_t)n, (const void*)(intptr_t)nnz0, (const void*)(intptr_t)all_k,
+        (const void*)(intptr_t)c.item_wedges, (const void*)(intptr_t)w.max_mask_row,
+        (const void*)(intptr_t)w.max_b_row, tw.cur[0].rp.p, tw.cur[0].col.p, tw.cur[1].rp.p, tw.cur[1].col.p,
+        tw.eid[0].p, tw.eid[1].p, tw.mirror[0].p, tw.mirror[1].p, tw.sup[0].p, tw.sup[1].p, tw.nold.p,
+        tw.newpos.p, tw.kept.p, tw.scan_tmp.p, tw.dstate.p, tw.truss.p, prog_dev, w.cnt.p, w.start.p, w.item_row.p,
+        w.item_t0.p, w.scan_tmp.p, w.counter.p, w.mend.p, w.rwork[0].p};
+    if (tw.step_graph.exec == nullptr || tw.step_graph.key != key) {
+      tw.step_graph.reset();
+      hipGraph_t g = nullptr;
+      GA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      try {
+        step();
+      } catch (...) {
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(s, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+      }
+      GA_HIP(hipStreamEndCapture(s, &g));
+      const hipError_t e = hipGraphInstantiate(&tw.step_graph.exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      GA_HIP(e);
+      tw.step_graph.key = std::move(key);
+    }
+  }
+  auto launch = [&]() {
+    if (graph) GA_HIP(hipGraphLaunch(tw.step_graph.exec, s));
+    else step();
+  };
+  // flow control: at most kInFlight steps queued beyond the last one the
+  // device has finished (it reports through host memory, read here without
+  // a HIP call), so the device never waits for the host and at most
+  // kInFlight no-op steps run after the end
+  constexpr int64_t kInFlight = 3;
+  int64_t queued = 1;
+  const int64_t max_steps = nnz0 / 2 + n + nk + 64;  // a step removes an edge or moves k on
+  auto t_seen = std::chrono::steady_clock::now();
+  long long seen = -1;
+  while (true) {
+    const long long ran = __atomic_load_n(&prog[0], __ATOMIC_ACQUIRE);
+    if (__atomic_load_n(&prog[1], __ATOMIC_RELAXED)) break;
+    if (queued - ran < kInFlight) {
+      GA_REQUIRE(queued < max_steps, GA_ERR_STATE, "k-truss steps did not finish");
+      launch();
+      ++queued;
+      continue;
+    }
+    const auto now = std::chrono::steady_clock::now();
+    if (ran != seen) {
+      seen = ran;
+      t_seen = now;
+    } else if (now - t_seen > std::chrono::seconds(60)) {
+      GA_HIP(hipStreamSynchronize(s));  // a step that ran for a minute: report, do not spin on
+      GA_REQUIRE(__atomic_load_n(&prog[0], __ATOMIC_ACQUIRE) != seen, GA_ERR_STATE, "k-truss step made no progress");
+    }
+  }
+  TrussDevState h{};
+  GA_HIP(hipMemcpyAsync(&h, ds, sizeof(h), hipMemcpyDeviceToHost, s));
+  GA_HIP(hipStreamSynchronize(s));
+  GA_REQUIRE(h.done && !h.error, GA_ERR_STATE, "k-truss steps did not finish");
+  if (c.prof_events_on) {
+    // every enqueued step timed its item launch; the ones a skip flag emptied
+    // are not support passes (their few microseconds stay in the total)
+    // launches with work: every pass except the one a k list counts on a
+    // k-truss that became empty (no launch there, as in the host loop)
+    const int64_t real = (!all_k && h.nnz == 0 && h.passes > 0) ? h.passes - 1 : h.passes;
+    c.prof_launches -= std::max<int64_t>(0, support_launches - real);
+  }
+  if (nnz_out) *nnz_out = h.nnz;
+  if (kmax_out) *kmax_out = h.kmax;
+  if (!all_k)
+    for (int i = 0; i < nk; ++i) {
+      if (nnz_at) nnz_at[i] = h.nnz_at[i];
+      if (passes_at) passes_at[i] = h.passes_at[i];
+    }
+  if (st) {
+    st->passes = h.passes;
+    st->steps = h.steps;
+  }
 }
 
 // Runs the peeling loop on the device-resident graph A.  all_k: truss
@@ -243,9 +591,20 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     tw.truss.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
     fill_u32(tw.truss.p, 2u, nnz0, s);
   }
+  GA_REQUIRE(all_k || (ks && nk >= 1), GA_ERR_ARG, "no k given");
+  {
+    Context& c = ctx();
+    const int mode = c.truss_loop >= 0 ? c.truss_loop : (nnz0 <= c.truss_device_max_nnz ? 1 : 0);
+    if (mode > 0 && !c.selective_support && !c.prof_bytes) {
+      truss_steps(A, all_k, ks, nk, tw, s, mode == 2 && !tw.no_graph, nnz_out, kmax_out, st, nnz_at, passes_at);
+      if (cur_out) *cur_out = 0;
+      if (st)
+        st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+      return;
+    }
+  }
   int cur = 0;
   int64_t nnz = nnz0;
-  GA_REQUIRE(all_k || (ks && nk >= 1), GA_ERR_ARG, "no k given");
   int kidx = 0;
   int k = all_k ? 3 : ks[0];
   bool valid = false;
@@ -298,20 +657,21 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
         break;
       }
       hipLaunchKernelGGL(k_keep_tile_sums, dim3((unsigned)nbk), dim3(KEEP_T), 0, s, (const int32_t*)sup, nnz, thr,
-                         bsum);
+                         bsum, (const TrussDevState*)nullptr, (const int32_t*)nullptr, (int32_t*)nullptr);
       const bool offsets = nbk > KEEP_FUSED_TILES;
       if (offsets) scan_tile_offsets(bsum, nbk, kept_dev, s);
       hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nbk), dim3(KEEP_T), 0, s, (const int32_t*)C.col,
                          (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, nnz, thr,
                          (const long long*)bsum, offsets ? 1 : 0, tw.cur[cur ^ 1].col.as<int32_t>(),
                          tw.eid[cur ^ 1].as<int32_t>(), tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(),
-                         tw.sup[cur ^ 1].as<int32_t>(), selective ? 1 : 0, kept_dev);
+                         tw.sup[cur ^ 1].as<int32_t>(), selective ? 1 : 0, kept_dev, (const TrussDevState*)nullptr);
       GA_HIP(hipMemcpyAsync(kept_h, kept_dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
       GA_HIP(hipEventRecord(kept_ev, s));
       hipLaunchKernelGGL(k_keep_finish, dim3(grid_for(std::max<int64_t>(n + 1, nnz), 256, 8192)), dim3(256), 0, s,
                          C.rp, n, (const int32_t*)tw.newpos.as<int32_t>(), nnz, (const int32_t*)kept_dev, nrp,
                          selective ? tw.dirty.as<uint8_t>() : nullptr, (const int32_t*)tw.mirror[cur].as<int32_t>(),
-                         (const int32_t*)tw.nold.as<int32_t>(), tw.mirror[cur ^ 1].as<int32_t>());
+                         (const int32_t*)tw.nold.as<int32_t>(), tw.mirror[cur ^ 1].as<int32_t>(),
+                         (const TrussDevState*)nullptr);
       GA_HIP(hipGetLastError());
       GA_HIP(hipEventSynchronize(kept_ev));
       const int64_t kept = *kept_h;
@@ -379,6 +739,7 @@ size_t ktruss_workspace_bytes(int64_t n, int64_t nnz) {
   b += 2 * e;                                                       // nold, newpos
   b += lease_bytes(std::max<int64_t>(n, 1));                                                  // dirty
   b += lease_bytes(64);                                                                         // kept
+  b += lease_bytes(sizeof(TrussDevState));                                                      // dstate
   b += lease_bytes(sizeof(long long) * std::max<int64_t>(ceil_div(std::max<int64_t>(nnz, 1), KEEP_TILE), 1));
   b += mxm_workspace_bytes(n, nnz, true);
   b += sel_workspace_bytes(n, nnz);
@@ -495,6 +856,7 @@ ga_status ga_ktruss(const ga_csr* A, int k, int32_t* out_rowptr, int32_t* out_co
     vA.val = nullptr;
     TrussWork tw;
     tw.mw.max_mask_row = tw.mw.max_b_row = host_max_row(A);
+    tw.no_graph = true;  // one run per workspace: no capture to amortise
     int cur = 0;
     int64_t nnz = 0;
     ga_truss_stats st{};

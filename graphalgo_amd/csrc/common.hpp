@@ -124,6 +124,10 @@ struct DevBuf {
 // reads that steer the host loops (allocated on first use, kept for the
 // thread's lifetime: no per-call hipHostMalloc / hipHostFree).
 void* pinned_scratch();
+// Per host thread: 32 words of coherent pinned host memory that kernels write
+// with system-scope stores and the host reads without a HIP call (progress
+// of device-driven loops); *dev = the device's address of the same words.
+long long* progress_words(long long** dev);
 
 struct Context {
   int device = -1;
@@ -146,6 +150,17 @@ struct Context {
   // set by per-item costs, not products, and the selection launches cost
   // more than the products save (510 vs 493 ms per decomposition).
   bool selective_support = false;
+  // k-truss peeling loop: 0 = host-driven (a host wait for the survivor count
+  // after every filter step), 1 = device-driven steps launched one by one,
+  // 2 = device-driven steps replayed from a captured HIP graph, -1 = 1 for
+  // graphs of at most truss_device_max_nnz stored entries, else 0
+  // (GA_TRUSS_LOOP overrides it at ga_init).  Measured on MI355X: the device
+  // steps halve a small graph's k-truss time (R-MAT 12, k = 8: 1.49 -> 0.60
+  // ms) and lose 2% on R-MAT 16 (the survivors are copied back into the
+  // buffers the fixed launches name); graph replay is slower than launching
+  // the same kernels (R-MAT 16 decomposition 170.8 vs 167.6 ms).
+  int truss_loop = -1;
+  int64_t truss_device_max_nnz = int64_t(1) << 19;
   // BC forward: switch to bottom-up steps once the level list has more than
   // 1/bc_alpha of the edges of the vertices with unvisited lanes
   // (GA_BC_BOTTOM_UP=0: top-down only; GA_BC_ALPHA sets the ratio).  R-MAT 18
@@ -260,6 +275,13 @@ struct MxmWork {
   DevBuf rwork[2];
   int rwork_cur = 0;
   bool rwork_have = false;
+  // Device-driven launches (the k-truss steps replayed from a HIP graph):
+  // the kernel arguments may not change between launches, so the per-row
+  // work lives in one buffer (each row's previous count is read before it is
+  // cleared; the caller zeroes rwork[0] first), every row goes to the listed
+  // work items (no wavefront bin), and a nonzero *skip makes the launch a no-op.
+  const int32_t* skip = nullptr;
+  bool fixed_args = false;
   // reuse_plan: the operands do not change between launches (a resident
   // batch), so the row bins and item lists of the first launch are kept
   bool reuse_plan = false;
@@ -308,12 +330,34 @@ size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend);
 size_t scan_tmp_bytes(int64_t n);
 
 // k-truss workspace (algorithms.hip): ping-pong CSR, edge ids, mirror index.
+// An instantiated HIP graph of one device-driven k-truss step; it stays
+// valid while the buffers and sizes it was captured with (key) are unchanged.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> key;
+  StepGraph() = default;
+  StepGraph(const StepGraph&) = delete;
+  StepGraph& operator=(const StepGraph&) = delete;
+  ~StepGraph() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+};
+
 struct TrussWork {
   OwnedCsr cur[2];
   DevBuf eid[2], mirror[2], sup[2];
   DevBuf cnt, scan_tmp, truss, kept, nold, newpos, dirty;
+  DevBuf dstate;  // device-driven loop: the peeling state (k, entries, passes, ...)
   MxmWork mw;
   SelWork sw;
+  StepGraph step_graph;
+  // launch device-driven steps one by one even when the context replays a
+  // captured graph (a workspace used for one run only: the capture would
+  // cost more than it saves)
+  bool no_graph = false;
 };
 // k-truss for one k of a symmetric loop-free pattern on the device; returns
 // the surviving entry count (both triangles) and the support passes run.

@@ -36,11 +36,18 @@ struct PinnedScratch {
   }
 };
 thread_local PinnedScratch t_pinned;
+thread_local PinnedScratch t_progress;
 }  // namespace
 
 void* pinned_scratch() {
   if (!t_pinned.p) GA_HIP(hipHostMalloc(&t_pinned.p, 256, hipHostMallocDefault));
   return t_pinned.p;
+}
+
+long long* progress_words(long long** dev) {
+  if (!t_progress.p) GA_HIP(hipHostMalloc(&t_progress.p, 256, hipHostMallocMapped | hipHostMallocCoherent));
+  GA_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(dev), t_progress.p, 0));
+  return static_cast<long long*>(t_progress.p);
 }
 
 Context& ctx() {
@@ -352,6 +359,7 @@ ga_status ga_init(int device) {
     if (const char* wf = std::getenv("GA_WAVE_FLOPS")) c->wave_flops = std::atoll(wf);
     if (const char* di = std::getenv("GA_DIRECT_ITEMS")) c->direct_items = std::atoi(di) != 0;
     if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT")) c->selective_support = std::atoi(ss) != 0;
+    if (const char* tl = std::getenv("GA_TRUSS_LOOP")) c->truss_loop = std::min(2, std::max(-1, std::atoi(tl)));
     if (const char* iw = std::getenv("GA_ITEM_WEDGES")) c->item_wedges = std::max(1LL, std::atoll(iw));
     if (const char* rc = std::getenv("GA_ROW_CHUNK")) c->row_chunk = std::max(0, std::atoi(rc));
     if (const char* bu = std::getenv("GA_BC_BOTTOM_UP")) c->bc_bottom_up = std::atoi(bu) != 0;
@@ -397,6 +405,11 @@ ga_status ga_set_option(int option, double value) {
         GA_REQUIRE(value == 0.0 || value == 1.0, GA_ERR_ARG, "selective_support is 0 or 1");
         c.selective_support = value != 0.0;
         break;
+      case GA_OPT_TRUSS_LOOP:
+        GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0 || value == 2.0, GA_ERR_ARG,
+                   "truss_loop is -1, 0, 1 or 2");
+        c.truss_loop = (int)value;
+        break;
       default:
         GA_REQUIRE(false, GA_ERR_ARG, "unknown option " + std::to_string(option));
     }
@@ -414,6 +427,7 @@ ga_status ga_get_option(int option, double* value) {
       case GA_OPT_SUM_SUB: *value = c.sum_sub; break;
       case GA_OPT_ITEM_WEDGES: *value = (double)c.item_wedges; break;
       case GA_OPT_SELECTIVE_SUPPORT: *value = c.selective_support ? 1.0 : 0.0; break;
+      case GA_OPT_TRUSS_LOOP: *value = c.truss_loop; break;
       default: GA_REQUIRE(false, GA_ERR_ARG, "unknown option " + std::to_string(option));
     }
   });

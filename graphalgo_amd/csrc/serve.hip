@@ -216,6 +216,7 @@ void ga_server::run_batch(const std::vector<Job*>& set, size_t off, size_t fp, h
       std::vector<int32_t> passes(ks.size(), 0);
       TrussWork tw;
       tw.mw.max_mask_row = tw.mw.max_b_row = g.max_row;
+      tw.no_graph = true;  // a workspace of one query: no capture to amortise
       ktruss_multi_dev(g.A.view(), ks.data(), (int)ks.size(), tw, s, nnz.data(), passes.data());
       for (Job* x : set) {
         const size_t i = std::lower_bound(ks.begin(), ks.end(), x->q.k) - ks.begin();

@@ -70,7 +70,10 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
     const int32_t* __restrict__ n_items_ptr, unsigned int* __restrict__ work_counter,
     unsigned long long* __restrict__ wedge_counter, int bitmap_words, const int32_t* __restrict__ Mend,
     int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror, const int32_t* __restrict__ opos,
-    unsigned long long* __restrict__ row_work) {
+    unsigned long long* __restrict__ row_work, const int32_t* __restrict__ skip) {
+  // skip: a device flag that turns the launch into a no-op (device-driven
+  // peeling steps that need no support pass)
+  if (skip && *skip) return;
   constexpr int MXM_THREADS = THREADS;
   constexpr int MXM_TCAP = THREADS;
   constexpr int MXM_MCAP = 8 * THREADS;  // mask rows up to this long accumulate in LDS
@@ -1066,15 +1069,19 @@ constexpr int PLAN_FUSED_TILES = 1024;
 
 __global__ __launch_bounds__(PLAN_T) void k_item_plan_count(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mend, int64_t m,
-    int tcap, int32_t* __restrict__ cnt, const unsigned long long* __restrict__ work_prev, long long target,
-    uint32_t* __restrict__ ctr_clear, int ctr_words, unsigned long long* __restrict__ work_clear,
-    const int32_t* __restrict__ split_col, int32_t* __restrict__ split_out, long long* __restrict__ bsum) {
+    int tcap, int32_t* __restrict__ cnt, const unsigned long long* work_prev, long long target,
+    uint32_t* __restrict__ ctr_clear, int ctr_words, unsigned long long* work_clear,
+    const int32_t* __restrict__ split_col, int32_t* __restrict__ split_out, long long* __restrict__ bsum,
+    const int32_t* __restrict__ skip) {
   __shared__ long long red[PLAN_T / 64];
+  if (skip && *skip) return;
   if (ctr_clear && blockIdx.x == 0)
     for (int q = threadIdx.x; q < ctr_words; q += PLAN_T) ctr_clear[q] = 0u;
   const int64_t i = (int64_t)blockIdx.x * PLAN_T + threadIdx.x;
   int c = 0;
   if (i < m) {
+    // the previous launch's wedges of row i (work_prev may be work_clear itself)
+    const unsigned long long wprev = work_prev ? work_prev[i] : 0ull;
     if (work_clear) work_clear[i] = 0ull;
     const int a = Arp[i + 1] - Arp[i];
     int mend;
@@ -1093,7 +1100,7 @@ __global__ __launch_bounds__(PLAN_T) void k_item_plan_count(
     const int mz = mend - Mrp[i];
     c = (a > 0 && mz > 0) ? (a + tcap - 1) / tcap : 0;
     if (c > 0 && work_prev) {
-      const long long byw = (long long)((work_prev[i] + (unsigned long long)target - 1) / (unsigned long long)target);
+      const long long byw = (long long)((wprev + (unsigned long long)target - 1) / (unsigned long long)target);
       c = (int)min<long long>(max<long long>(c, byw), min<long long>((long long)c * MXM_SPLIT_MAX, a));
     }
     cnt[i] = c;
@@ -1112,8 +1119,10 @@ __global__ __launch_bounds__(PLAN_T) void k_item_plan_fill(const int32_t* __rest
                                                            const int32_t* __restrict__ cnt, int64_t m,
                                                            const long long* __restrict__ bsum, int offsets_ready,
                                                            int32_t* __restrict__ start, int32_t* __restrict__ item_row,
-                                                           int32_t* __restrict__ item_t0) {
+                                                           int32_t* __restrict__ item_t0,
+                                                           const int32_t* __restrict__ skip) {
   __shared__ long long wtot[PLAN_T / 64];
+  if (skip && *skip) return;
   __shared__ long long prior[PLAN_T / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   long long before = 0;
@@ -1236,11 +1245,14 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   w.counter.ensure(sizeof(uint32_t) * ctr_words);
   w.scan_tmp.ensure(sizeof(long long) * std::max<int64_t>(ceil_div(m, PLAN_T), 1));  // item plan and row scans
   const long long wave_flops =
-      (M.ncols <= WR_MAXCOLS) ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops) : -1;
+      (M.ncols <= WR_MAXCOLS && !w.fixed_args) ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops) : -1;
   // direct mode (no wavefront bin): items 0..m-1 are the rows' first chunks and
   // only the extra chunks of long rows are listed -- two small launches
   // instead of binning, two scans and two compactions
-  const bool direct = wave_flops < 0 && ctx().direct_items;
+  const bool direct = wave_flops < 0 && ctx().direct_items && !w.fixed_args;
+  GA_REQUIRE(!w.fixed_args || (w.balance && !w.reuse_plan), GA_ERR_STATE,
+             "fixed-argument launches are for work-balanced repeated products");
+  GA_REQUIRE(w.skip == nullptr || w.fixed_args, GA_ERR_STATE, "a skip flag needs fixed-argument launches");
   // an immutable operand set (reuse_plan) keeps the row bins and item lists
   // of its first launch; work-balanced lists change every launch
   const bool plan_hit = w.reuse_plan && !w.balance && !direct && w.plan_valid && w.plan_rp == A.rp &&
@@ -1270,7 +1282,11 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // launch's wedges per row set this launch's split
   unsigned long long* rw_cur = nullptr;
   const unsigned long long* rw_prev = nullptr;
-  if (w.balance && wave_flops < 0 && !direct) {
+  if (w.fixed_args) {
+    w.rwork[0].ensure(sizeof(unsigned long long) * m);
+    rw_cur = w.rwork[0].as<unsigned long long>();
+    rw_prev = rw_cur;
+  } else if (w.balance && wave_flops < 0 && !direct) {
     w.rwork[0].ensure(sizeof(unsigned long long) * m);
     w.rwork[1].ensure(sizeof(unsigned long long) * m);
     rw_cur = w.rwork[w.rwork_cur].as<unsigned long long>();
@@ -1292,12 +1308,13 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     long long* bsum = w.scan_tmp.as<long long>();
     hipLaunchKernelGGL(k_item_plan_count, dim3((unsigned)nbp), dim3(PLAN_T), 0, s, A.rp, M.rp,
                        lower_out ? nullptr : Mend, m, TCAP, w.cnt.as<int32_t>(), rw_prev, ctx().item_wedges,
-                       w.counter.as<uint32_t>(), ctr_words, rw_cur, lower_out ? M.col : nullptr, lower_out, bsum);
+                       w.counter.as<uint32_t>(), ctr_words, rw_cur, lower_out ? M.col : nullptr, lower_out, bsum,
+                       w.skip);
     const bool offsets = nbp > PLAN_FUSED_TILES;
     if (offsets) scan_tile_offsets(bsum, nbp, w.start.as<int32_t>() + m, s);
     hipLaunchKernelGGL(k_item_plan_fill, dim3((unsigned)nbp), dim3(PLAN_T), 0, s, A.rp,
                        (const int32_t*)w.cnt.as<int32_t>(), m, (const long long*)bsum, offsets ? 1 : 0,
-                       w.start.as<int32_t>(), w.item_row.as<int32_t>(), w.item_t0.as<int32_t>());
+                       w.start.as<int32_t>(), w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(), w.skip);
     GA_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, A.rp, A.col, B.rp, M.rp, Mend,
@@ -1408,7 +1425,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
                      static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
                      direct ? (const int32_t*)n_extra : (const int32_t*)(w.start.as<int32_t>() + m),
                      w.counter.as<unsigned int>(), wc, bitmap_words, Mend, trunc, chunk_i, direct ? (int)m : 0,
-                     mirror, opos, rw_cur);
+                     mirror, opos, rw_cur, w.skip);
   GA_HIP(hipGetLastError());
   if (c.prof_events_on) {
     GA_HIP(hipEventRecord(e1, s));

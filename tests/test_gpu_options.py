@@ -3,7 +3,8 @@ forward steps only, with bottom-up steps from the first level on and with the
 default switch point (bit-identical scores, and the oracle); triangle counts
 with every short-row kernel and claim size (exact); truss numbers with tiny
 work items (rows split over many items, whose partial supports meet in global
-atomics) and with the selective support passes (exact)."""
+atomics) and with the selective support passes (exact); the host-driven,
+device-driven and graph-replayed k-truss loops (identical counts)."""
 import numpy as np
 import pytest
 
@@ -98,9 +99,47 @@ def test_truss_selective_support(gpu, G, oracle):
     assert nnz5 == int((want >= 5).sum())
 
 
+def _truss_cases(G):
+    path = G.CSR.from_edges(40, list(range(39)), list(range(1, 40)))  # a tree: every k-truss is empty
+    empty = G.CSR.from_edges(16, [], [])
+    return {"rmat12": G.rmat(12, 16, seed=21), "rmat14": G.rmat(14, 8, seed=22),
+            "er1000": G.erdos_renyi(1000, 9990, seed=1), "path40": path, "empty16": empty}
+
+
+@pytest.mark.parametrize("name", ["rmat12", "rmat14", "er1000", "path40", "empty16"])
+def test_truss_loops_identical(gpu, G, oracle, name):
+    """Host-driven peeling (0), device-driven steps (1), steps replayed
+    from a captured HIP graph (2) and the size-based choice (-1): the oracle's truss numbers and support
+    pass count, and between the loops the same filter steps, k-truss sizes,
+    pass counts and k-truss patterns (resident graph queried twice, so the
+    captured step is reused)."""
+    g = _truss_cases(G)[name]
+    want, kmax, opasses, _, _ = oracle.truss_decomposition(g)
+    ks = sorted({3, 4, 5, max(kmax, 3), kmax + 1})
+    res = {}
+    for loop in (0, 1, 2, -1):
+        print(f"case truss loops {name} truss_loop={loop}", flush=True)
+        with gpu.options(truss_loop=loop):
+            t1, k1, s1 = gpu.truss_decomposition(g)
+            R = gpu.Graph(g)
+            t2, k2, s2 = R.truss()
+            t3, k3, s3 = R.truss()
+            kt = [R.ktruss(k) for k in ks] + [R.ktruss(k) for k in ks]
+            R.close()
+            pat, it = gpu.k_truss(g, 4)
+        for t, k, st in ((t1, k1, s1), (t2, k2, s2), (t3, k3, s3)):
+            assert k == kmax and np.array_equal(t, want)
+            assert st["passes"] == opasses
+        assert s1["steps"] == s2["steps"] == s3["steps"]
+        for k, (nnz, _) in zip(ks + ks, kt):
+            assert nnz == int((want >= k).sum()), (k, nnz)
+        res[loop] = (s1["steps"], kt, it, pat.rowptr.tolist(), pat.col.tolist())
+    assert res[0] == res[1] == res[2] == res[-1]
+
+
 def test_option_validation(gpu):
     for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
-                      ("item_wedges", 0), ("selective_support", 0.5)):
+                      ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 3), ("truss_loop", -2)):
         with pytest.raises(Exception):
             gpu.set_option(name, bad)
     before = gpu.get_option("row_claim")
