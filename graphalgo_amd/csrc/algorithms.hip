@@ -388,6 +388,10 @@ __global__ void k_truss_state(TrussDevState* __restrict__ ds, const int32_t* __r
   }
 }
 
+static bool selective_on(const Context& c, int64_t nnz0) {
+  return c.selective_support > 0 || (c.selective_support < 0 && nnz0 >= c.selective_min_nnz);
+}
+
 // The peeling loop of truss_loop as device-driven steps (buffers set up by
 // truss_loop; selective support passes and byte accounting stay on the host
 // loop).  graph: the step is captured once into a HIP graph and replayed;
@@ -595,7 +599,7 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   {
     Context& c = ctx();
     const int mode = c.truss_loop >= 0 ? c.truss_loop : (nnz0 <= c.truss_device_max_nnz ? 1 : 0);
-    if (mode > 0 && !c.selective_support && !c.prof_bytes) {
+    if (mode > 0 && !selective_on(c, nnz0) && !c.prof_bytes) {
       truss_steps(A, all_k, ks, nk, tw, s, mode == 2 && !tw.no_graph, nnz_out, kmax_out, st, nnz_at, passes_at);
       if (cur_out) *cur_out = 0;
       if (st)
@@ -622,7 +626,7 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     hipEvent_t e;
     ~EventGuard() { (void)hipEventDestroy(e); }
   } kept_guard{kept_ev};
-  const bool selective = ctx().selective_support;
+  const bool selective = selective_on(ctx(), nnz0);
   bool sup_zeroed = false;  // tw.sup[cur] was cleared by the compaction that made cur
   {
     while (true) {

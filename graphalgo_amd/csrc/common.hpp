@@ -144,12 +144,20 @@ struct Context {
   // and only extra chunks are listed (GA_DIRECT_ITEMS=1), which pays a claim
   // for every empty row and loses on peeled graphs
   bool direct_items = false;
-  // GA_SELECTIVE_SUPPORT=1: k-truss passes after the first recompute only
-  // the supports of entries with an endpoint that lost entries.  Off by
-  // default: on R-MAT 16 it forms 19% fewer products but the pass time is
-  // set by per-item costs, not products, and the selection launches cost
-  // more than the products save (510 vs 493 ms per decomposition).
-  bool selective_support = false;
+  // Selective support passes: k-truss passes after the first recompute only
+  // the supports of entries with an endpoint that lost entries (1 = on,
+  // 0 = off, -1 = on for graphs of at least selective_min_nnz stored
+  // entries; GA_SELECTIVE_SUPPORT overrides it at ga_init).  On R-MAT 16 it
+  // forms 19% fewer products but the pass time is set by per-item costs,
+  // not products, and the selection launches cost more than the products
+  // save (510 vs 493 ms per decomposition); on the R-MAT 18-20 graphs of
+  // the served pool one k-truss with k = 8 or 16 runs 6-27% faster (R-MAT 18,
+  // k = 8: 337 -> 247 ms), with k = 3 (two passes) 2-4% slower.
+  int selective_support = -1;
+  // support passes as triangle listing: C<L> = L*L over the strict lower
+  // half, each hit adding 1 to its three edges (GA_SUPPORT_TRI)
+  int support_tri = 0;
+  int64_t selective_min_nnz = int64_t(1) << 22;
   // k-truss peeling loop: 0 = host-driven (a host wait for the survivor count
   // after every filter step), 1 = device-driven steps launched one by one,
   // 2 = device-driven steps replayed from a captured HIP graph, -1 = 1 for

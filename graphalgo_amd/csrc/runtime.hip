@@ -358,7 +358,9 @@ ga_status ga_init(int device) {
     GA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* wf = std::getenv("GA_WAVE_FLOPS")) c->wave_flops = std::atoll(wf);
     if (const char* di = std::getenv("GA_DIRECT_ITEMS")) c->direct_items = std::atoi(di) != 0;
-    if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT")) c->selective_support = std::atoi(ss) != 0;
+    if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT"))
+      c->selective_support = std::min(1, std::max(-1, std::atoi(ss)));
+    if (const char* st = std::getenv("GA_SUPPORT_TRI")) c->support_tri = std::atoi(st) != 0;
     if (const char* tl = std::getenv("GA_TRUSS_LOOP")) c->truss_loop = std::min(2, std::max(-1, std::atoi(tl)));
     if (const char* iw = std::getenv("GA_ITEM_WEDGES")) c->item_wedges = std::max(1LL, std::atoll(iw));
     if (const char* rc = std::getenv("GA_ROW_CHUNK")) c->row_chunk = std::max(0, std::atoi(rc));
@@ -402,8 +404,12 @@ ga_status ga_set_option(int option, double value) {
         c.item_wedges = (long long)value;
         break;
       case GA_OPT_SELECTIVE_SUPPORT:
-        GA_REQUIRE(value == 0.0 || value == 1.0, GA_ERR_ARG, "selective_support is 0 or 1");
-        c.selective_support = value != 0.0;
+        GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0, GA_ERR_ARG, "selective_support is -1, 0 or 1");
+        c.selective_support = (int)value;
+        break;
+      case GA_OPT_SUPPORT_TRI:
+        GA_REQUIRE(value == 0.0 || value == 1.0, GA_ERR_ARG, "support_tri is 0 or 1");
+        c.support_tri = (int)value;
         break;
       case GA_OPT_TRUSS_LOOP:
         GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0 || value == 2.0, GA_ERR_ARG,
@@ -426,8 +432,9 @@ ga_status ga_get_option(int option, double* value) {
       case GA_OPT_ROW_CLAIM: *value = c.row_chunk; break;
       case GA_OPT_SUM_SUB: *value = c.sum_sub; break;
       case GA_OPT_ITEM_WEDGES: *value = (double)c.item_wedges; break;
-      case GA_OPT_SELECTIVE_SUPPORT: *value = c.selective_support ? 1.0 : 0.0; break;
+      case GA_OPT_SELECTIVE_SUPPORT: *value = c.selective_support; break;
       case GA_OPT_TRUSS_LOOP: *value = c.truss_loop; break;
+      case GA_OPT_SUPPORT_TRI: *value = c.support_tri; break;
       default: GA_REQUIRE(false, GA_ERR_ARG, "unknown option " + std::to_string(option));
     }
   });

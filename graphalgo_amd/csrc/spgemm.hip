@@ -85,7 +85,9 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
   uint16_t* s_wpre = reinterpret_cast<uint16_t*>(s_bitmap + bitmap_words + BM_EXTRA);
   const int drop_col = RANK ? 32 * bitmap_words : -1;  // column of a dropped wedge
   __shared__ int32_t s_mcol[RANK ? 1 : MXM_MCAP];
-  __shared__ T s_acc[MODE == 0 ? MXM_MCAP : 1];
+  __shared__ T s_acc[MODE != 1 ? MXM_MCAP : 1];
+  // MODE 2 (triangle support): hits per A entry of the item
+  __shared__ int32_t s_ecnt[MODE == 2 ? MXM_TCAP : 1];
   // entry e of the item: wedges [s_eoff[e], s_eoff[e+1]) live at B.col[s_jb[e] + w]
   // (the host guarantees an item's wedge count fits int32)
   __shared__ int32_t s_jb[MXM_TCAP];
@@ -238,7 +240,7 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
     const bool listed = q < n_list;
     const int i = listed ? item_row[q] : q - n_list;
     const int a_beg = Arp[i];
-    const int a_end = Arp[i + 1];
+    const int a_end = MODE == 2 ? Mend[i] : Arp[i + 1];  // MODE 2: A is the lower half too
     const int t0 = listed ? item_t0[q] : a_beg;
     // a listed item ends where the next item of its row starts; a direct
     // item (or the last of a row) takes up to MXM_TCAP entries
@@ -265,7 +267,7 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
       } else if (local) {
         s_mcol[p] = c;
       }
-      if (MODE == 0 && local) s_acc[p] = T(0);
+      if (MODE != 1 && local) s_acc[p] = T(0);
       if (use_bm) atomicOr(&s_bitmap[c >> 5], 1u << (c & 31));
     }
     // stage the item's entries: B row start and length, A value
@@ -274,8 +276,9 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
     if (tid < ne) {
       const int k = bbase + Acol[t0 + tid];
       b0 = Brp[k];
-      int b1 = Brp[k + 1];
-      if (trunc && mnz > 0) {
+      int b1 = MODE == 2 ? Mend[k] : Brp[k + 1];  // MODE 2: B is the lower half (columns < k < i)
+      if (MODE == 2) s_ecnt[tid] = 0;
+      if (MODE != 2 && trunc && mnz > 0) {
         // products beyond the last mask column are discarded by the mask:
         // stop B's row at the upper bound of that column
         const int maxc = Mcol[ms + mnz - 1];
@@ -392,9 +395,14 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
 #pragma unroll
           for (int u = 0; u < MXM_K; ++u) {
             const uint32_t bit = 1u << (cc[u] & 31);
-            if (wd[u] & bit)
+            if (wd[u] & bit) {
               accumulate(pre[u] + __popc(wd[u] & (bit - 1u)),
                          (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
+              if (MODE == 2) {  // the triangle's other two edges: the A entry and the B entry
+                atomicAdd(&s_ecnt[ee[u]], 1);
+                atomicAdd(reinterpret_cast<int*>(&Cval[jj[u]]), 1);
+              }
+            }
           }
         } else {
 #pragma unroll
@@ -420,7 +428,13 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
               }
               found = lo < mnz && mrow[lo] == c;
             }
-            if (found) accumulate(lo, (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
+            if (found) {
+              accumulate(lo, (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
+              if (MODE == 2) {
+                atomicAdd(&s_ecnt[ee[u]], 1);
+                atomicAdd(reinterpret_cast<int*>(&Cval[jj[u]]), 1);
+              }
+            }
           }
         }
       };
@@ -435,6 +449,17 @@ __attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 
     else wedge_loop(std::false_type{});
     __syncthreads();
 
+    if (MODE == 2) {
+      // every lower entry takes hits as a mask entry, an A entry and a B
+      // entry of other items and rows: all added atomically (the transposed
+      // entries are copied after the launch)
+      if (local)
+        for (int p = tid; p < mnz; p += MXM_THREADS) {
+          const T v = s_acc[p];
+          if (v != T(0)) atomic_add_val(&Cval[ms + p], v);
+        }
+      if (tid < ne && s_ecnt[tid] != 0) atomicAdd(reinterpret_cast<int*>(&Cval[t0 + tid]), s_ecnt[tid]);
+    }
     if (MODE == 0) {
       if (local) {
         // mirror: C is symmetric and only its lower half is computed; each
@@ -1072,7 +1097,7 @@ __global__ __launch_bounds__(PLAN_T) void k_item_plan_count(
     int tcap, int32_t* __restrict__ cnt, const unsigned long long* work_prev, long long target,
     uint32_t* __restrict__ ctr_clear, int ctr_words, unsigned long long* work_clear,
     const int32_t* __restrict__ split_col, int32_t* __restrict__ split_out, long long* __restrict__ bsum,
-    const int32_t* __restrict__ skip) {
+    const int32_t* __restrict__ skip, int a_lower) {
   __shared__ long long red[PLAN_T / 64];
   if (skip && *skip) return;
   if (ctr_clear && blockIdx.x == 0)
@@ -1083,7 +1108,7 @@ __global__ __launch_bounds__(PLAN_T) void k_item_plan_count(
     // the previous launch's wedges of row i (work_prev may be work_clear itself)
     const unsigned long long wprev = work_prev ? work_prev[i] : 0ull;
     if (work_clear) work_clear[i] = 0ull;
-    const int a = Arp[i + 1] - Arp[i];
+    int a = Arp[i + 1] - Arp[i];
     int mend;
     if (split_col) {
       int lo = Mrp[i], hi = Mrp[i + 1];
@@ -1094,6 +1119,7 @@ __global__ __launch_bounds__(PLAN_T) void k_item_plan_count(
       }
       split_out[i] = lo;
       mend = lo;
+      if (a_lower) a = lo - Arp[i];  // A is the lower half as well (Arp == Mrp)
     } else {
       mend = Mend ? Mend[i] : Mrp[i + 1];
     }
@@ -1120,7 +1146,8 @@ __global__ __launch_bounds__(PLAN_T) void k_item_plan_fill(const int32_t* __rest
                                                            const long long* __restrict__ bsum, int offsets_ready,
                                                            int32_t* __restrict__ start, int32_t* __restrict__ item_row,
                                                            int32_t* __restrict__ item_t0,
-                                                           const int32_t* __restrict__ skip) {
+                                                           const int32_t* __restrict__ skip,
+                                                           const int32_t* __restrict__ Aend) {
   __shared__ long long wtot[PLAN_T / 64];
   if (skip && *skip) return;
   __shared__ long long prior[PLAN_T / 64];
@@ -1146,7 +1173,7 @@ __global__ __launch_bounds__(PLAN_T) void k_item_plan_fill(const int32_t* __rest
   if (i < m) {
     start[i] = (int32_t)s0;
     const int a0 = Arp[i];
-    const int64_t a = Arp[i + 1] - a0;
+    const int64_t a = (Aend ? Aend[i] : Arp[i + 1]) - a0;
     for (int q = 0; q < c; ++q) {
       item_row[s0 + q] = (int32_t)i;
       item_t0[s0 + q] = a0 + (int32_t)((int64_t)q * a / c);
@@ -1244,12 +1271,15 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   const int ctr_words = MXM_CTR_WORDS;
   w.counter.ensure(sizeof(uint32_t) * ctr_words);
   w.scan_tmp.ensure(sizeof(long long) * std::max<int64_t>(ceil_div(m, PLAN_T), 1));  // item plan and row scans
-  const long long wave_flops =
-      (M.ncols <= WR_MAXCOLS && !w.fixed_args) ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops) : -1;
+  // MODE 2 (triangle support) runs on the listed items only
+  const long long wave_flops = (M.ncols <= WR_MAXCOLS && !w.fixed_args && MODE != 2)
+                                   ? (w.wave_flops >= 0 ? w.wave_flops : ctx().wave_flops)
+                                   : -1;
   // direct mode (no wavefront bin): items 0..m-1 are the rows' first chunks and
   // only the extra chunks of long rows are listed -- two small launches
   // instead of binning, two scans and two compactions
-  const bool direct = wave_flops < 0 && ctx().direct_items && !w.fixed_args;
+  const bool direct = wave_flops < 0 && ctx().direct_items && !w.fixed_args && MODE != 2;
+  GA_REQUIRE(MODE != 2 || lower_out != nullptr, GA_ERR_STATE, "triangle support needs the lower-half split");
   GA_REQUIRE(!w.fixed_args || (w.balance && !w.reuse_plan), GA_ERR_STATE,
              "fixed-argument launches are for work-balanced repeated products");
   GA_REQUIRE(w.skip == nullptr || w.fixed_args, GA_ERR_STATE, "a skip flag needs fixed-argument launches");
@@ -1309,12 +1339,13 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
     hipLaunchKernelGGL(k_item_plan_count, dim3((unsigned)nbp), dim3(PLAN_T), 0, s, A.rp, M.rp,
                        lower_out ? nullptr : Mend, m, TCAP, w.cnt.as<int32_t>(), rw_prev, ctx().item_wedges,
                        w.counter.as<uint32_t>(), ctr_words, rw_cur, lower_out ? M.col : nullptr, lower_out, bsum,
-                       w.skip);
+                       w.skip, MODE == 2 ? 1 : 0);
     const bool offsets = nbp > PLAN_FUSED_TILES;
     if (offsets) scan_tile_offsets(bsum, nbp, w.start.as<int32_t>() + m, s);
     hipLaunchKernelGGL(k_item_plan_fill, dim3((unsigned)nbp), dim3(PLAN_T), 0, s, A.rp,
                        (const int32_t*)w.cnt.as<int32_t>(), m, (const long long*)bsum, offsets ? 1 : 0,
-                       w.start.as<int32_t>(), w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(), w.skip);
+                       w.start.as<int32_t>(), w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(), w.skip,
+                       MODE == 2 ? (const int32_t*)lower_out : (const int32_t*)nullptr);
     GA_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_bin_rows, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, A.rp, A.col, B.rp, M.rp, Mend,
@@ -1434,7 +1465,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   }
   if (c.prof_bytes) {
     // host-known algorithmic bytes: A and M streamed, B row bounds, output
-    const int64_t out = MODE == 0 ? (int64_t)sizeof(T) * M.nnz : 8;
+    const int64_t out = MODE != 1 ? (int64_t)sizeof(T) * M.nnz : 8;
     c.prof_bytes_host += 4 * (A.nrows + 1) + 4 * A.nnz + 8 * A.nnz + 4 * (M.nrows + 1) + 4 * M.nnz + out;
     c.prof_bytes_launches += 1;
   }
@@ -1533,6 +1564,20 @@ size_t mxm_workspace_bytes(int64_t m, int64_t nnz_a, bool with_mend) {
   return b;
 }
 
+// Upper entries take the support of their transposed (lower) entry: the
+// transposed entry of p lies in a later row exactly when p is upper.  The
+// current entry count is rp[n] (device-driven steps size the grid for the
+// first graph); skip: as in k_mxm_items.
+__global__ void k_sup_mirror(const int32_t* __restrict__ rp, int64_t n, const int32_t* __restrict__ mirror,
+                             int32_t* __restrict__ S, const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t nnz = rp[n];
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+    const int m = mirror[p];
+    if (m > p) S[p] = S[m];
+  }
+}
+
 void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, MxmWork& w, hipStream_t s,
                            bool s_zeroed) {
   const int64_t n = C.nrows;
@@ -1546,6 +1591,18 @@ void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, Mxm
   if (c.prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
                        c.prof_dev.as<unsigned long long>() + 1);
+  Context& cx = ctx();
+  if (cx.support_tri) {
+    // each triangle i > k > j once, from the wedge (i, k, j) of C<L> = L*L
+    // (L = the strict lower half): +1 to (i, j) (mask), (i, k) (A) and
+    // (k, j) (B); then the upper half from the lower
+    launch_binned<int32_t, true, true, 2, MXM_WIDE, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, nullptr,
+                                                            0, nullptr, nullptr, split);
+    const unsigned g = (unsigned)std::min<int64_t>(grid_for(C.nnz, 256, 8192), 4 * (int64_t)cx.num_cu);
+    hipLaunchKernelGGL(k_sup_mirror, dim3(g), dim3(256), 0, s, C.rp, n, mirror, S, w.skip);
+    GA_HIP(hipGetLastError());
+    return;
+  }
   launch_binned<int32_t, true, true, 0, MXM_WIDE, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, nullptr,
                                                           1, mirror, nullptr, split);
 }

@@ -300,14 +300,19 @@ enum {
                                    entries, for short rows of pattern sums (GA_SUM_SUB) */
   GA_OPT_ITEM_WEDGES = 5,       /* >= 1: target wedges per work item of repeated products
                                    (GA_ITEM_WEDGES) */
-  GA_OPT_SELECTIVE_SUPPORT = 6, /* 0/1: k-truss passes after the first recompute only the
-                                   supports an endpoint change can affect (GA_SELECTIVE_SUPPORT) */
-  GA_OPT_TRUSS_LOOP = 7         /* k-truss peeling loop (GA_TRUSS_LOOP): 0 = the host reads the
+  GA_OPT_SELECTIVE_SUPPORT = 6, /* k-truss passes after the first recompute only the supports an
+                                   endpoint change can affect (GA_SELECTIVE_SUPPORT): 1 = on,
+                                   0 = off, -1 (default) = on for graphs of at least 2^22 stored
+                                   entries */
+  GA_OPT_TRUSS_LOOP = 7,        /* k-truss peeling loop (GA_TRUSS_LOOP): 0 = the host reads the
                                    survivor count after every filter step; 1 = device-driven
                                    steps (state in device memory, a few steps queued ahead);
                                    2 = the same steps replayed from a captured HIP graph;
                                    -1 (default) = 1 for graphs of at most 2^19 stored entries,
                                    else 0.  Selective support passes always use 0. */
+  GA_OPT_SUPPORT_TRI = 8        /* 0/1: k-truss support passes as triangle listing, C<L> = L*L
+                                   over the strict lower half with each hit added to its three
+                                   edges (GA_SUPPORT_TRI) */
 };
 ga_status ga_set_option(int option, double value);
 ga_status ga_get_option(int option, double* value);
