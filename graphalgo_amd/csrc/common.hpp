@@ -155,8 +155,14 @@ struct Context {
   // k = 8: 337 -> 247 ms), with k = 3 (two passes) 2-4% slower.
   int selective_support = -1;
   // support passes as triangle listing: C<L> = L*L over the strict lower
-  // half, each hit adding 1 to its three edges (GA_SUPPORT_TRI)
-  int support_tri = 0;
+  // half, each hit adding 1 to its three edges (1 = on, 0 = off, -1 = on for
+  // graphs of at least selective_min_nnz stored entries; GA_SUPPORT_TRI).
+  // Measured on MI355X: the full passes of R-MAT 18-20 run 1.45x faster
+  // (k-truss k = 3: 72.8 -> 48.7 ms on R-MAT 18), but the R-MAT 16
+  // decomposition runs 40% slower (233 vs 167 ms): its later passes are on
+  // dense cores where most wedges close a triangle, and every closed one
+  // costs a global atomic for the B entry.
+  int support_tri = -1;
   int64_t selective_min_nnz = int64_t(1) << 22;
   // k-truss peeling loop: 0 = host-driven (a host wait for the survivor count
   // after every filter step), 1 = device-driven steps launched one by one,

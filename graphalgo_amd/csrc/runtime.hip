@@ -360,7 +360,7 @@ ga_status ga_init(int device) {
     if (const char* di = std::getenv("GA_DIRECT_ITEMS")) c->direct_items = std::atoi(di) != 0;
     if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT"))
       c->selective_support = std::min(1, std::max(-1, std::atoi(ss)));
-    if (const char* st = std::getenv("GA_SUPPORT_TRI")) c->support_tri = std::atoi(st) != 0;
+    if (const char* st = std::getenv("GA_SUPPORT_TRI")) c->support_tri = std::min(1, std::max(-1, std::atoi(st)));
     if (const char* tl = std::getenv("GA_TRUSS_LOOP")) c->truss_loop = std::min(2, std::max(-1, std::atoi(tl)));
     if (const char* iw = std::getenv("GA_ITEM_WEDGES")) c->item_wedges = std::max(1LL, std::atoll(iw));
     if (const char* rc = std::getenv("GA_ROW_CHUNK")) c->row_chunk = std::max(0, std::atoi(rc));
@@ -408,7 +408,7 @@ ga_status ga_set_option(int option, double value) {
         c.selective_support = (int)value;
         break;
       case GA_OPT_SUPPORT_TRI:
-        GA_REQUIRE(value == 0.0 || value == 1.0, GA_ERR_ARG, "support_tri is 0 or 1");
+        GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0, GA_ERR_ARG, "support_tri is -1, 0 or 1");
         c.support_tri = (int)value;
         break;
       case GA_OPT_TRUSS_LOOP:

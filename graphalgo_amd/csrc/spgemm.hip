@@ -61,7 +61,7 @@ constexpr int MXM_CTR_WORDS = 2 * MXM_NSEG * MXM_CTR_STRIDE + 32;  // item segme
 // under that cap, 8-byte accumulators are bound by LDS instead)
 template <typename T, bool PAT_A, bool PAT_B, int MODE, int THREADS, bool BATCHED, bool RANK>
 __global__ __launch_bounds__(THREADS)
-__attribute__((amdgpu_waves_per_eu(PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 4) ? 8 : 1))) void k_mxm_items(
+__attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 4) ? 8 : 1)))) void k_mxm_items(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const T* __restrict__ Aval,
     const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol, const T* __restrict__ Bval,
     const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol, T* __restrict__ Cval,
@@ -1592,7 +1592,7 @@ void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, Mxm
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
                        c.prof_dev.as<unsigned long long>() + 1);
   Context& cx = ctx();
-  if (cx.support_tri) {
+  if (cx.support_tri > 0 || (cx.support_tri < 0 && C.nnz >= cx.selective_min_nnz)) {
     // each triangle i > k > j once, from the wedge (i, k, j) of C<L> = L*L
     // (L = the strict lower half): +1 to (i, j) (mask), (i, k) (A) and
     // (k, j) (B); then the upper half from the lower

@@ -310,9 +310,10 @@ enum {
                                    2 = the same steps replayed from a captured HIP graph;
                                    -1 (default) = 1 for graphs of at most 2^19 stored entries,
                                    else 0.  Selective support passes always use 0. */
-  GA_OPT_SUPPORT_TRI = 8        /* 0/1: k-truss support passes as triangle listing, C<L> = L*L
-                                   over the strict lower half with each hit added to its three
-                                   edges (GA_SUPPORT_TRI) */
+  GA_OPT_SUPPORT_TRI = 8        /* k-truss support passes as triangle listing, C<L> = L*L over
+                                   the strict lower half with each hit added to its three edges
+                                   (GA_SUPPORT_TRI): 1 = on, 0 = off, -1 (default) = on for
+                                   graphs of at least 2^22 stored entries */
 };
 ga_status ga_set_option(int option, double value);
 ga_status ga_get_option(int option, double* value);

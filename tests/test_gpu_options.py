@@ -156,9 +156,44 @@ def test_truss_loops_identical(gpu, G, oracle, name):
     assert res[0] == res[1] == res[2] == res[-1]
 
 
+@pytest.mark.parametrize("name", ["rmat12", "rmat14", "er1000", "path40", "empty16"])
+def test_truss_support_tri(gpu, G, oracle, name):
+    """Support passes as triangle listing (each triangle found once from
+    C<L> = L*L, its three edges incremented): the oracle's truss numbers and
+    pass count with the host-driven and the device-driven loop, and the
+    oracle's k-truss pattern and iterations."""
+    g = _truss_cases(G)[name]
+    want, kmax, opasses, _, _ = oracle.truss_decomposition(g)
+    for loop in (0, 1):
+        print(f"case truss support_tri {name} truss_loop={loop}", flush=True)
+        with gpu.options(support_tri=1, truss_loop=loop, selective_support=0):
+            got, got_kmax, st = gpu.truss_decomposition(g)
+            T, it = gpu.k_truss(g, 4)
+        assert got_kmax == kmax and np.array_equal(got, want) and st["passes"] == opasses
+        rp, col, it_o = oracle.ktruss(g, 4)
+        assert np.array_equal(T.rowptr, rp) and np.array_equal(T.col, col) and it == it_o
+
+
+def test_truss_support_tri_rmat16(gpu, G, golden):
+    """Triangle-listing support passes on the bench workload: the reference's
+    truss numbers (SHA-256 and histogram)."""
+    import hashlib
+    rec = golden["generated"]["rmat16_ef16_s1"]
+    g = G.rmat(**rec["cfg"])
+    print("case truss support_tri rmat16", flush=True)
+    R = gpu.Graph(g)
+    with gpu.options(support_tri=1):
+        truss, kmax, _ = R.truss()
+    R.close()
+    assert kmax == rec["truss"]["kmax"]
+    assert np.bincount(truss).tolist() == rec["truss"]["truss_hist"]
+    assert hashlib.sha256(truss.astype(np.int32).tobytes()).hexdigest() == rec["truss"]["truss_sha256"]
+
+
 def test_option_validation(gpu):
     for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
-                      ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 3), ("truss_loop", -2)):
+                      ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 3), ("truss_loop", -2),
+                      ("support_tri", 2)):
         with pytest.raises(Exception):
             gpu.set_option(name, bad)
     before = gpu.get_option("row_claim")
