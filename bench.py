@@ -16,7 +16,8 @@ The CPU baseline is the OpenMP oracle's truss decomposition on the host.
 Other workloads: --workload tc (triangle count on R-MAT 16), tc_er (one
 small triangle count), bc (256-source batched BC on R-MAT 18), batch (4096
 kNN graphs, batched TC), serve, ewise (eWiseAdd of two R-MAT 20 graphs),
-copy (calibration).
+ktruss_one (one k-truss query on graphs of scale 12..20 under each peeling
+setting), copy (calibration).
 """
 from __future__ import annotations
 
@@ -429,8 +430,61 @@ def bench_serve(args) -> dict:
     return out
 
 
+def bench_ktruss_one(args) -> dict:
+    """One k-truss query (k_truss for one k, src/main.cpp:89-146) on resident
+    R-MAT graphs from scale 12 to the served pool's scale 18 and 20, k = 3 and
+    8, with the default peeling settings and with each one forced: the loop
+    (0 = host-driven, 1 = device-driven steps, 2 = steps replayed from a HIP
+    graph), full support passes (no selective passes, no triangle listing)
+    and no triangle listing.  Median wall time of `steps` queries after one
+    warm-up query.  value = undirected edges / s of the scale-20 graph at
+    k = 8 with the defaults."""
+    from graphalgo_amd import api, graphs as G, serving as S
+    pool = {c["scale"]: c for c in S.pool_configs()}
+    graphs = [("rmat12_ef16_s5", dict(scale=12, edge_factor=16, seed=5)),
+              ("rmat14_ef16_s5", dict(scale=14, edge_factor=16, seed=5)),
+              ("rmat16_ef16_s1", dict(scale=16, edge_factor=16, seed=1))]
+    for sc in (18, 20):
+        c = pool[sc]
+        graphs.append((f"rmat{sc}_ef{c['edge_factor']}_s{c['seed']}",
+                       dict(scale=sc, edge_factor=c["edge_factor"], seed=c["seed"])))
+    variants = {"default": {}, "host_loop": {"truss_loop": 0}, "device_steps": {"truss_loop": 1},
+                "graph_replay": {"truss_loop": 2}, "full_passes": {"selective_support": 0, "support_tri": 0},
+                "no_tri": {"support_tri": 0}}
+    rows, headline = [], None
+    for name, cfg in graphs:
+        g = G.rmat(**cfg)
+        R = api.Graph(g)
+        for k in (3, 8):
+            for vname, opts in variants.items():
+                with api.options(**opts):
+                    R.ktruss(k)
+                    ts = []
+                    for _ in range(max(1, args.steps)):
+                        t0 = time.perf_counter()
+                        nnz, passes = R.ktruss(k)
+                        ts.append(time.perf_counter() - t0)
+                ms = float(np.median(ts)) * 1e3
+                rows.append({"graph": name, "nnz": g.nnz, "k": k, "variant": vname, "ms": round(ms, 3),
+                             "truss_nnz": nnz, "passes": passes})
+                if cfg["scale"] == 20 and k == 8 and vname == "default":
+                    headline = (g.m_undirected, ms)
+            _log(f"ktruss_one {name} k={k}: " + ", ".join(
+                f"{r['variant']} {r['ms']:.2f}" for r in rows if r["graph"] == name and r["k"] == k))
+        R.close()
+    m, ms = headline
+    return {"value": m / (ms / 1e3), "unit": "edges/s", "ms_per_step": ms, "dtype": "int32",
+            "config": {"workload": "k-truss for one k on resident R-MAT graphs (scale 12..20), each peeling "
+                                   "setting forced in turn; headline: the served pool's scale-20 graph, k = 8, "
+                                   "default settings", "queries_per_case": max(1, args.steps)},
+            "roofline": None,
+            "roofline_note": "wall time of whole queries (many launches and host steps); the support kernel's "
+                             "roofline is the ktruss workload's",
+            "cases": rows}
+
+
 WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "tc_er": bench_tc_er, "copy": bench_copy, "ewise": bench_ewise,
-             "batch": bench_batch, "bc": bench_bc, "serve": bench_serve}
+             "batch": bench_batch, "bc": bench_bc, "serve": bench_serve, "ktruss_one": bench_ktruss_one}
 
 
 def main() -> None:

@@ -174,7 +174,7 @@ struct Context {
   // buffers the fixed launches name); graph replay is slower than launching
   // the same kernels (R-MAT 16 decomposition 170.8 vs 167.6 ms).
   int truss_loop = -1;
-  int64_t truss_device_max_nnz = int64_t(1) << 19;
+  int64_t truss_device_max_nnz = int64_t(1) << 18;
   // BC forward: switch to bottom-up steps once the level list has more than
   // 1/bc_alpha of the edges of the vertices with unvisited lanes
   // (GA_BC_BOTTOM_UP=0: top-down only; GA_BC_ALPHA sets the ratio).  R-MAT 18

@@ -308,7 +308,7 @@ enum {
                                    survivor count after every filter step; 1 = device-driven
                                    steps (state in device memory, a few steps queued ahead);
                                    2 = the same steps replayed from a captured HIP graph;
-                                   -1 (default) = 1 for graphs of at most 2^19 stored entries,
+                                   -1 (default) = 1 for graphs of at most 2^18 stored entries,
                                    else 0.  Selective support passes always use 0. */
   GA_OPT_SUPPORT_TRI = 8        /* k-truss support passes as triangle listing, C<L> = L*L over
                                    the strict lower half with each hit added to its three edges
