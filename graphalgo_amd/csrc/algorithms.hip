@@ -760,6 +760,7 @@ struct ga_graph {
   OwnedCsr A;
   OwnedCsr L;
   bool have_L = false;
+  bool sums_clean = false;  // sums[0] is zero (the last count's publish kernel cleared it)
   TrussWork tw;
   MxmWork mw;
   BcWork bw;
@@ -937,14 +938,14 @@ ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles) {
       g->have_L = true;
     }
     g->sums.ensure(64);
-    fill_u64(g->sums.p, 0ull, 1, s);
+    if (!g->sums_clean) fill_u64(g->sums.p, 0ull, 1, s);
+    g->sums_clean = false;
     const DCsr vL = g->L.view();
     mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, g->sums.as<unsigned long long>(), nullptr, g->mw, s);
-    // the count comes back through pinned memory (a pageable copy is staged)
-    auto* t = static_cast<unsigned long long*>(pinned_scratch());
-    GA_HIP(hipMemcpyAsync(t, g->sums.p, sizeof(*t), hipMemcpyDeviceToHost, s));
-    GA_HIP(hipStreamSynchronize(s));
-    *triangles = (int64_t)*t;
+    // the count reaches the host through pinned memory, written by a kernel
+    // that also clears the sum and the claim counters for the next call
+    *triangles = (int64_t)publish_sum_and_wait(g->mw, g->sums.as<unsigned long long>(), s);
+    g->sums_clean = true;
   });
 }
 

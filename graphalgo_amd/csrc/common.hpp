@@ -296,6 +296,10 @@ struct MxmWork {
   // work items (no wavefront bin), and a nonzero *skip makes the launch a no-op.
   const int32_t* skip = nullptr;
   bool fixed_args = false;
+  // the claim counters are known to be zero (a publish kernel cleared them
+  // after the previous launch on this workspace), so the next launch skips
+  // its clearing fill
+  bool counters_clean = false;
   // reuse_plan: the operands do not change between launches (a resident
   // batch), so the row bins and item lists of the first launch are kept
   bool reuse_plan = false;
@@ -314,6 +318,12 @@ struct MxmWork {
 // sums[row_group ? row_group[i] : 0] (caller zeroes sums).
 void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int mode, void* Cval,
                     unsigned long long* sums, const int32_t* row_group, MxmWork& w, hipStream_t s);
+
+// After a pattern sum launch (mode 1) on workspace w: one kernel hands
+// sums[0] to the host through pinned host memory and clears sums[0] and the
+// claim counters for the next launch; the calling thread waits for the value
+// without a stream synchronisation.  Returns sums[0].
+unsigned long long publish_sum_and_wait(MxmWork& w, unsigned long long* sums, hipStream_t s);
 
 // S<C> = C*C (int32 per entry) for a symmetric loop-free pattern C, computed
 // on the strict lower half and mirrored (the k-truss support pass).

@@ -201,10 +201,7 @@ void ga_server::run_batch(const std::vector<Job*>& set, size_t off, size_t fp, h
       w.max_mask_row = w.max_b_row = g.max_row;
       const DCsr vL = g.L.view();
       mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
-      auto* h = static_cast<unsigned long long*>(pinned_scratch());
-      GA_HIP(hipMemcpyAsync(h, sums.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-      GA_HIP(hipStreamSynchronize(s));
-      j.res.value = (int64_t)*h;
+      j.res.value = (int64_t)publish_sum_and_wait(w, sums.as<unsigned long long>(), s);
       break;
     }
     case GA_Q_KTRUSS: {

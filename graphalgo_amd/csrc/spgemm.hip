@@ -1301,12 +1301,13 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // work and computes the lower-half row ends; other paths launch them
   const bool listed = !direct && !plan_hit && wave_flops < 0;
   if (!listed) {
-    fill_u32(w.counter.p, 0u, ctr_words, s);
+    if (!w.counters_clean) fill_u32(w.counter.p, 0u, ctr_words, s);
     if (lower_out) {
       hipLaunchKernelGGL(k_row_split, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, M.rp, M.col, m, lower_out);
       GA_HIP(hipGetLastError());
     }
   }
+  w.counters_clean = false;  // the launches below claim through them
   if (lower_out) Mend = lower_out;
   // work-balanced items (listed path of repeated products): the previous
   // launch's wedges per row set this launch's split
@@ -1482,6 +1483,48 @@ static void dispatch_pat(const DCsr& A, const DCsr& B, const DCsr& M, void* Cval
   else if (pa) launch_binned<T, true, false, MODE, TH, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
   else if (pb) launch_binned<T, false, true, MODE, TH, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
   else launch_binned<T, false, false, MODE, TH, false>(A, B, M, Cval, sums, row_group, nullptr, w, bm, s, nullptr, 0);
+}
+
+// One workgroup: the total to host memory (value, then the sequence number
+// with release order), then sums[0] and the claim counters cleared.
+__global__ void k_sum_publish(unsigned long long* __restrict__ sums, uint32_t* __restrict__ ctr, int ctr_words,
+                              long long* __restrict__ host, long long seq) {
+  for (int q = threadIdx.x; q < ctr_words; q += blockDim.x) ctr[q] = 0u;
+  if (threadIdx.x == 0) {
+    const unsigned long long v = sums[0];
+    sums[0] = 0ull;
+    __hip_atomic_store(&host[0], (long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&host[1], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+namespace {
+thread_local long long t_publish_seq = 0;
+}
+
+unsigned long long publish_sum_and_wait(MxmWork& w, unsigned long long* sums, hipStream_t s) {
+  long long* hdev = nullptr;
+  long long* h = progress_words(&hdev) + 4;  // words 4 (value) and 5 (sequence)
+  const long long seq = ++t_publish_seq;
+  w.counter.ensure(sizeof(uint32_t) * MXM_CTR_WORDS);
+  hipLaunchKernelGGL(k_sum_publish, dim3(1), dim3(256), 0, s, sums, w.counter.as<uint32_t>(), (int)MXM_CTR_WORDS,
+                     hdev + 4, seq);
+  GA_HIP(hipGetLastError());
+  w.counters_clean = true;
+  // wait for the sequence number; every 1024 polls ask the stream whether it
+  // failed (a fault must end the wait) or drained (then the word is there)
+  for (int64_t it = 1;; ++it) {
+    if (__atomic_load_n(&h[1], __ATOMIC_ACQUIRE) == seq) break;
+    if ((it & 1023) == 0) {
+      const hipError_t e = hipStreamQuery(s);
+      if (e == hipSuccess) {
+        GA_REQUIRE(__atomic_load_n(&h[1], __ATOMIC_ACQUIRE) == seq, GA_ERR_HIP, "the device result did not arrive");
+        break;
+      }
+      if (e != hipErrorNotReady) GA_HIP(e);
+    }
+  }
+  return (unsigned long long)__atomic_load_n(&h[0], __ATOMIC_RELAXED);
 }
 
 __global__ void k_count_wedges(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol,
