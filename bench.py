@@ -501,8 +501,8 @@ def main() -> None:
     ap.add_argument("--max-batch", type=int, default=16, help="serve: queued queries one run may take")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
-              "plan is described in DESIGN.md and is not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding over several "
+              "GPUs is designed (row blocks by wedges, BC by source columns) but not built", file=sys.stderr)
         sys.exit(2)
     res = WORKLOADS[args.workload](args)
     line = {

@@ -121,8 +121,8 @@ def test_tc_reference_fixtures(small_graphs, oracle, golden):
         assert oracle.tc(gs) == rec["tc_msa"], name
 
 
-def test_tc_survey_pins(small_graphs, oracle):
-    # values the reference printed in the survey run
+def test_tc_reference_pins(small_graphs, oracle):
+    # triangle counts the compiled reference printed for these four graphs
     assert oracle.tc(small_graphs["G43.mtx"][1]) == 1392
     assert oracle.tc(small_graphs["yeast_30NN.mtx.gz"][1]) == 180175
     assert oracle.tc(small_graphs["iris_dataset_30NN.mtx"][1]) == 25972
@@ -166,7 +166,7 @@ def test_ktruss_g43_sizes(small_graphs, oracle, golden):
     for k, nnz, iters in golden["graphs"]["G43.mtx"]["truss"]["per_k"]:
         rp, col, it = oracle.ktruss(gs, k)
         assert rp[-1] == nnz and it == iters, k
-    assert oracle.ktruss(gs, 3)[0][-1] == 6818  # survey pin
+    assert oracle.ktruss(gs, 3)[0][-1] == 6818  # the reference's k_truss output on G43
     assert oracle.ktruss(gs, 4)[0][-1] == 36
 
 
