@@ -566,7 +566,7 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
 // from that truss to ks[1] (the k2-truss lies inside the k1-truss), ...
 static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s,
                        int* cur_out, int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st,
-                       int64_t* nnz_at = nullptr, int32_t* passes_at = nullptr) {
+                       int64_t* nnz_at = nullptr, int32_t* passes_at = nullptr, const KDone* on_k = nullptr) {
   auto t_start = std::chrono::steady_clock::now();
   const int64_t n = A.nrows, nnz0 = A.nnz;
   GA_REQUIRE(A.nrows == A.ncols, GA_ERR_DIM, "k-truss needs a square matrix");
@@ -600,7 +600,15 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     Context& c = ctx();
     const int mode = c.truss_loop >= 0 ? c.truss_loop : (nnz0 <= c.truss_device_max_nnz ? 1 : 0);
     if (mode > 0 && !selective_on(c, nnz0) && !c.prof_bytes) {
-      truss_steps(A, all_k, ks, nk, tw, s, mode == 2 && !tw.no_graph, nnz_out, kmax_out, st, nnz_at, passes_at);
+      std::vector<int64_t> nz_at(all_k ? 0 : nk);
+      std::vector<int32_t> ps_at(all_k ? 0 : nk);
+      truss_steps(A, all_k, ks, nk, tw, s, mode == 2 && !tw.no_graph, nnz_out, kmax_out, st, nz_at.data(),
+                  ps_at.data());
+      for (int i = 0; i < (int)nz_at.size(); ++i) {
+        if (nnz_at) nnz_at[i] = nz_at[i];
+        if (passes_at) passes_at[i] = ps_at[i];
+        if (on_k) (*on_k)(i, nz_at[i], ps_at[i]);
+      }
       if (cur_out) *cur_out = 0;
       if (st)
         st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -654,6 +662,7 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
         if (!all_k) {
           if (nnz_at) nnz_at[kidx] = 0;
           if (passes_at) passes_at[kidx] = (int32_t)passes;
+          if (on_k) (*on_k)(kidx, 0, (int32_t)passes);
           if (++kidx == nk) break;
           k = ks[kidx];
           continue;
@@ -684,6 +693,7 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
         if (!all_k) {
           if (nnz_at) nnz_at[kidx] = nnz;
           if (passes_at) passes_at[kidx] = (int32_t)passes;
+          if (on_k) (*on_k)(kidx, nnz, (int32_t)passes);
           if (++kidx == nk) break;
           k = ks[kidx];
           continue;
@@ -725,14 +735,14 @@ void ktruss_dev(const DCsr& A, int k, TrussWork& tw, hipStream_t s, int64_t* nnz
 }
 
 void ktruss_multi_dev(const DCsr& A, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s, int64_t* nnz_at,
-                      int32_t* passes_at) {
+                      int32_t* passes_at, const KDone* on_k) {
   GA_REQUIRE(nk >= 1, GA_ERR_ARG, "no k given");
   for (int i = 0; i < nk; ++i) {
     GA_REQUIRE(ks[i] >= 3, GA_ERR_ARG, "k must be >= 3");
     GA_REQUIRE(i == 0 || ks[i] > ks[i - 1], GA_ERR_ARG, "k list must be ascending and distinct");
   }
   ga_truss_stats st{};
-  truss_loop(A, false, ks, nk, tw, s, nullptr, nullptr, nullptr, &st, nnz_at, passes_at);
+  truss_loop(A, false, ks, nk, tw, s, nullptr, nullptr, nullptr, &st, nnz_at, passes_at, on_k);
 }
 
 // Every buffer truss_loop sizes for one k (no per-entry truss array), at the

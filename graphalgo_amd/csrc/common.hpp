@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -389,8 +390,12 @@ void ktruss_dev(const DCsr& A, int k, TrussWork& tw, hipStream_t s, int64_t* nnz
 // Several k at once (ascending, distinct): one peeling run that stops at each
 // k in turn; nnz_at[i] / passes_at[i] = entries of the ks[i]-truss and the
 // support passes run up to it.
+// on_k (optional): called with (i, entries, passes) as soon as the ks[i]-truss
+// is known (the host-driven loop: while the run goes on to the next k; the
+// device-driven steps: at the end of the run).
+using KDone = std::function<void(int, int64_t, int32_t)>;
 void ktruss_multi_dev(const DCsr& A, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s, int64_t* nnz_at,
-                      int32_t* passes_at);
+                      int32_t* passes_at, const KDone* on_k = nullptr);
 size_t ktruss_workspace_bytes(int64_t n, int64_t nnz);
 
 // Batched betweenness centrality workspace and driver (bc.hip).

@@ -136,7 +136,9 @@ struct ga_server {
   }
   int max_batch = 1;
   void worker(int slot);
-  void run_batch(const std::vector<Job*>& set, size_t off, size_t fp, hipStream_t s);
+  // early[i] = 1: set[i] was completed (and may already be freed by its
+  // waiter) before the run ended
+  void run_batch(const std::vector<Job*>& set, std::vector<char>& early, size_t off, size_t fp, hipStream_t s);
   size_t footprint(const ga_query& q) const;
   size_t batch_footprint(const std::vector<Job*>& set) const;
   std::vector<Job*> merge_set(Job* h) const;
@@ -181,7 +183,8 @@ std::vector<Job*> ga_server::merge_set(Job* h) const {
   return set;
 }
 
-void ga_server::run_batch(const std::vector<Job*>& set, size_t off, size_t fp, hipStream_t s) {
+void ga_server::run_batch(const std::vector<Job*>& set, std::vector<char>& early, size_t off, size_t fp,
+                          hipStream_t s) {
   Context& c = ctx();
   GA_REQUIRE(!c.prof_events_on && !c.prof_bytes, GA_ERR_STATE,
              "kernel timing hooks are process-wide: disable them while a server runs queries");
@@ -214,11 +217,30 @@ void ga_server::run_batch(const std::vector<Job*>& set, size_t off, size_t fp, h
       TrussWork tw;
       tw.mw.max_mask_row = tw.mw.max_b_row = g.max_row;
       tw.no_graph = true;  // a workspace of one query: no capture to amortise
-      ktruss_multi_dev(g.A.view(), ks.data(), (int)ks.size(), tw, s, nnz.data(), passes.data());
-      for (Job* x : set) {
-        const size_t i = std::lower_bound(ks.begin(), ks.end(), x->q.k) - ks.begin();
-        x->res.value = nnz[i];
-        x->res.passes = passes[i];
+      // a query is answered as soon as its k-truss is known: the run goes on
+      // to the larger k of the other queries without it
+      const KDone on_k = [&](int i, int64_t nz, int32_t ps) {
+        std::lock_guard<std::mutex> lk(mu);
+        const double t = now_ms();
+        for (size_t q = 0; q < set.size(); ++q) {
+          Job* x = set[q];
+          if (early[q] || x->q.k != ks[i]) continue;
+          x->res.value = nz;
+          x->res.passes = ps;
+          x->res.status = GA_OK;
+          x->res.t_end_ms = t;
+          x->done = true;
+          early[q] = 1;
+          st.completed += 1;
+        }
+        cv_done.notify_all();
+      };
+      ktruss_multi_dev(g.A.view(), ks.data(), (int)ks.size(), tw, s, nnz.data(), passes.data(), &on_k);
+      for (size_t q = 0; q < set.size(); ++q) {
+        if (early[q]) continue;
+        const size_t i = std::lower_bound(ks.begin(), ks.end(), set[q]->q.k) - ks.begin();
+        set[q]->res.value = nnz[i];
+        set[q]->res.passes = passes[i];
       }
       break;
     }
@@ -293,6 +315,7 @@ void ga_server::worker(int slot) {
       x->res.footprint = (int64_t)fp;
       x->res.batched = (int32_t)set.size();
     }
+    std::vector<char> early(set.size(), 0);
     lk.unlock();
     int code = GA_OK;
     std::string msg;
@@ -300,7 +323,7 @@ void ga_server::worker(int slot) {
       code = GA_ERR_HIP;
       msg = "worker stream could not be created";
     } else {
-      code = guarded([&] { run_batch(set, off, fp, s); });
+      code = guarded([&] { run_batch(set, early, off, fp, s); });
       if (code != GA_OK) {
         msg = ga_last_error();
         (void)hipStreamSynchronize(s);  // the block is reused only after its kernels ended
@@ -310,7 +333,9 @@ void ga_server::worker(int slot) {
     lk.lock();
     arena.release(off, fp);
     inflight -= 1;
-    for (Job* x : set) {
+    for (size_t q = 0; q < set.size(); ++q) {
+      if (early[q]) continue;  // answered during the run (possibly freed by now)
+      Job* x = set[q];
       x->res.status = code;
       x->err = msg;
       x->res.t_end_ms = t_end;
