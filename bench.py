@@ -394,7 +394,7 @@ def bench_serve(args) -> dict:
                       "p50_ms_all_steps": [round(x["p50_ms"], 3) for x in served],
                       "p99_ms_all_steps": [round(x["p99_ms"], 3) for x in served],
                       "capacity_qps": round(cap_served, 2), "capacity_qps_without_batching": round(cap_nobatch, 2),
-                      "runs": st["batches"], "queries_completed": st["completed"],
+                      "runs": st["batches"], "joined_runs": st["joined"], "queries_completed": st["completed"],
                       "peak_inflight": st["peak_inflight"],
                       "peak_arena_bytes": st["peak_bytes"], "resident_bytes": st["resident_bytes"]},
            "plain_path": {**{k: round(v, 3) for k, v in plain.items()}, "capacity_qps": round(cap_plain, 2),

@@ -75,6 +75,7 @@ class GaServerStats(C.Structure):
         ("arena_bytes", C.c_int64), ("resident_bytes", C.c_int64), ("peak_bytes", C.c_int64),
         ("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
         ("batches", C.c_int64), ("peak_inflight", C.c_int32), ("slots", C.c_int32),
+        ("joined", C.c_int64),
     ]
 
 

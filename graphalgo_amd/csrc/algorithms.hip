@@ -607,7 +607,7 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       for (int i = 0; i < (int)nz_at.size(); ++i) {
         if (nnz_at) nnz_at[i] = nz_at[i];
         if (passes_at) passes_at[i] = ps_at[i];
-        if (on_k) (*on_k)(i, nz_at[i], ps_at[i]);
+        if (on_k) (*on_k)(ks[i], nz_at[i], ps_at[i], nullptr);
       }
       if (cur_out) *cur_out = 0;
       if (st)
@@ -619,8 +619,28 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   int64_t nnz = nnz0;
   int kidx = 0;
   int k = all_k ? 3 : ks[0];
+  // k list mode: the list (ascending) and its results; on_k may extend it
+  std::vector<int32_t> kv(ks, ks + (all_k ? 0 : nk));
+  std::vector<int64_t> nz_v(kv.size(), 0);
+  std::vector<int32_t> ps_v(kv.size(), 0);
   bool valid = false;
   int64_t passes = 0, steps = 0, wedges_host = 0;
+  auto record = [&](int64_t value) {
+    nz_v[kidx] = value;
+    ps_v[kidx] = (int32_t)passes;
+    if (!on_k) return;
+    std::vector<int32_t> more;
+    (*on_k)(kv[kidx], value, (int32_t)passes, &more);
+    for (int32_t x : more) {
+      if (x <= kv[kidx]) continue;  // only larger k can continue from this truss
+      auto it = std::lower_bound(kv.begin() + kidx + 1, kv.end(), x);
+      if (it != kv.end() && *it == x) continue;
+      const auto at = it - kv.begin();
+      kv.insert(it, x);
+      nz_v.insert(nz_v.begin() + at, 0);
+      ps_v.insert(ps_v.begin() + at, 0);
+    }
+  };
   int32_t kmax = 2;
   int32_t* kept_h = static_cast<int32_t*>(pinned_scratch());
   // the host waits for the survivor count only (an event after its copy);
@@ -660,11 +680,9 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       if (nnz == 0) {  // nothing to filter (a k-truss that became empty)
         ++steps;
         if (!all_k) {
-          if (nnz_at) nnz_at[kidx] = 0;
-          if (passes_at) passes_at[kidx] = (int32_t)passes;
-          if (on_k) (*on_k)(kidx, 0, (int32_t)passes);
-          if (++kidx == nk) break;
-          k = ks[kidx];
+          record(0);
+          if (++kidx == (int)kv.size()) break;
+          k = kv[kidx];
           continue;
         }
         break;
@@ -691,11 +709,9 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       ++steps;
       if (kept == nnz) {
         if (!all_k) {
-          if (nnz_at) nnz_at[kidx] = nnz;
-          if (passes_at) passes_at[kidx] = (int32_t)passes;
-          if (on_k) (*on_k)(kidx, nnz, (int32_t)passes);
-          if (++kidx == nk) break;
-          k = ks[kidx];
+          record(nnz);
+          if (++kidx == (int)kv.size()) break;
+          k = kv[kidx];
           continue;
         }
         hipLaunchKernelGGL(k_truss_assign, dim3(grid_for(nnz, 256, 8192)), dim3(256), 0, s,
@@ -712,6 +728,11 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     }
   }
   GA_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < (all_k ? 0 : nk); ++i) {  // the caller's k values (joined ones went to on_k)
+    const auto at = std::lower_bound(kv.begin(), kv.end(), ks[i]) - kv.begin();
+    if (nnz_at) nnz_at[i] = nz_v[at];
+    if (passes_at) passes_at[i] = ps_v[at];
+  }
   if (cur_out) *cur_out = cur;
   if (nnz_out) *nnz_out = nnz;
   if (kmax_out) *kmax_out = kmax;

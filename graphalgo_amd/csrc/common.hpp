@@ -164,6 +164,11 @@ struct Context {
   // dense cores where most wedges close a triangle, and every closed one
   // costs a global atomic for the B entry.
   int support_tri = -1;
+  // served k-truss queries joining a peeling run on their graph that is
+  // already going (GA_SERVE_JOIN): 0 = never, 1 = when the query's k lies
+  // within the run's k range (it adds no work), 2 = also when it is above it
+  // (the run goes on to it)
+  int serve_join = 2;
   int64_t selective_min_nnz = int64_t(1) << 22;
   // k-truss peeling loop: 0 = host-driven (a host wait for the survivor count
   // after every filter step), 1 = device-driven steps launched one by one,
@@ -390,10 +395,12 @@ void ktruss_dev(const DCsr& A, int k, TrussWork& tw, hipStream_t s, int64_t* nnz
 // Several k at once (ascending, distinct): one peeling run that stops at each
 // k in turn; nnz_at[i] / passes_at[i] = entries of the ks[i]-truss and the
 // support passes run up to it.
-// on_k (optional): called with (i, entries, passes) as soon as the ks[i]-truss
-// is known (the host-driven loop: while the run goes on to the next k; the
-// device-driven steps: at the end of the run).
-using KDone = std::function<void(int, int64_t, int32_t)>;
+// on_k (optional): called with (k, entries, passes, more) as soon as the
+// k-truss is known.  The host-driven loop calls it while the run can still go
+// on: `more` is then non-null, and the callee may append larger k values,
+// which the run takes on in ascending order (reported through on_k only).
+// The device-driven steps call it at the end of the run with more = null.
+using KDone = std::function<void(int32_t, int64_t, int32_t, std::vector<int32_t>*)>;
 void ktruss_multi_dev(const DCsr& A, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s, int64_t* nnz_at,
                       int32_t* passes_at, const KDone* on_k = nullptr);
 size_t ktruss_workspace_bytes(int64_t n, int64_t nnz);

@@ -23,7 +23,11 @@
 // the other slots' queries (continuous batching at the granularity of the
 // paths' launches: a short triangle count proceeds while a k-truss waits on
 // its filter step, and slots refill the moment a query ends).  Admission is
-// in arrival order (no overtaking), so a large query is never starved.
+// in arrival order (no overtaking), so a large query is never starved.  The
+// one exception takes nothing from the others: a queued k-truss query joins
+// a peeling run on its graph that is already going when that run reaches a
+// k at or below the query's (it uses the run's memory and slot), and every
+// k-truss query is answered as soon as its own k is reached.
 #include "common.hpp"
 
 #include <algorithm>
@@ -103,6 +107,8 @@ struct Job {
   ga_query_result res{};
 };
 
+constexpr int kMaxJoined = 64;  // queries one k-truss run answers at most (its own and joined ones)
+
 size_t graph_footprint(int64_t n, int64_t nnz, int64_t lnnz) {
   return 2 * lease_bytes(sizeof(int32_t) * (n + 1)) + lease_bytes(sizeof(int32_t) * std::max<int64_t>(nnz, 1)) +
          lease_bytes(sizeof(int32_t) * std::max<int64_t>(n, 1)) + scan_tmp_bytes(n) +
@@ -137,8 +143,9 @@ struct ga_server {
   int max_batch = 1;
   void worker(int slot);
   // early[i] = 1: set[i] was completed (and may already be freed by its
-  // waiter) before the run ended
-  void run_batch(const std::vector<Job*>& set, std::vector<char>& early, size_t off, size_t fp, hipStream_t s);
+  // waiter) before the run ended; a k-truss run appends the queries that
+  // join it to set (and early)
+  void run_batch(std::vector<Job*>& set, std::vector<char>& early, int slot, size_t off, size_t fp, hipStream_t s);
   size_t footprint(const ga_query& q) const;
   size_t batch_footprint(const std::vector<Job*>& set) const;
   std::vector<Job*> merge_set(Job* h) const;
@@ -183,7 +190,7 @@ std::vector<Job*> ga_server::merge_set(Job* h) const {
   return set;
 }
 
-void ga_server::run_batch(const std::vector<Job*>& set, std::vector<char>& early, size_t off, size_t fp,
+void ga_server::run_batch(std::vector<Job*>& set, std::vector<char>& early, int slot, size_t off, size_t fp,
                           hipStream_t s) {
   Context& c = ctx();
   GA_REQUIRE(!c.prof_events_on && !c.prof_bytes, GA_ERR_STATE,
@@ -217,14 +224,40 @@ void ga_server::run_batch(const std::vector<Job*>& set, std::vector<char>& early
       TrussWork tw;
       tw.mw.max_mask_row = tw.mw.max_b_row = g.max_row;
       tw.no_graph = true;  // a workspace of one query: no capture to amortise
-      // a query is answered as soon as its k-truss is known: the run goes on
-      // to the larger k of the other queries without it
-      const KDone on_k = [&](int i, int64_t nz, int32_t ps) {
+      // A query is answered as soon as its k-truss is known: the run goes on
+      // to the larger k of the other queries without it.  While the run can
+      // still go on, queued k-truss queries on the same graph join it: one
+      // with the k just reached is answered at once, one with a larger k is
+      // added to the run's list (no memory or slot of its own, so joining
+      // delays no other query).
+      const int gid = j.q.graph;
+      const int join = c.serve_join;
+      int32_t k_top = ks.back();  // the run's largest k
+      const KDone on_k = [&](int32_t k, int64_t nz, int32_t ps, std::vector<int32_t>* more) {
         std::lock_guard<std::mutex> lk(mu);
         const double t = now_ms();
+        if (more && max_batch > 1 && join > 0) {
+          for (auto it = queue.begin(); it != queue.end() && (int)set.size() < kMaxJoined;) {
+            Job* c = *it;
+            if (c->q.kind != GA_Q_KTRUSS || c->q.graph != gid || c->q.k < k || (join == 1 && c->q.k > k_top)) {
+              ++it;
+              continue;
+            }
+            k_top = std::max(k_top, c->q.k);
+            it = queue.erase(it);
+            c->res.t_start_ms = t;
+            c->res.slot = slot;
+            c->res.footprint = (int64_t)fp;
+            c->res.batched = (int32_t)set.size() + 1;
+            set.push_back(c);
+            early.push_back(0);
+            st.joined += 1;
+            if (c->q.k > k) more->push_back(c->q.k);
+          }
+        }
         for (size_t q = 0; q < set.size(); ++q) {
           Job* x = set[q];
-          if (early[q] || x->q.k != ks[i]) continue;
+          if (early[q] || x->q.k != k) continue;
           x->res.value = nz;
           x->res.passes = ps;
           x->res.status = GA_OK;
@@ -323,7 +356,7 @@ void ga_server::worker(int slot) {
       code = GA_ERR_HIP;
       msg = "worker stream could not be created";
     } else {
-      code = guarded([&] { run_batch(set, early, off, fp, s); });
+      code = guarded([&] { run_batch(set, early, slot, off, fp, s); });
       if (code != GA_OK) {
         msg = ga_last_error();
         (void)hipStreamSynchronize(s);  // the block is reused only after its kernels ended

@@ -239,6 +239,7 @@ typedef struct {
   int64_t submitted, completed, rejected;
   int64_t batches;                     /* runs started (queries / batches = mean batch) */
   int32_t peak_inflight, slots;
+  int64_t joined;                      /* k-truss queries that joined a run already going */
 } ga_server_stats;
 
 ga_status ga_server_create(const ga_server_config* cfg, ga_server** out);
@@ -310,10 +311,14 @@ enum {
                                    2 = the same steps replayed from a captured HIP graph;
                                    -1 (default) = 1 for graphs of at most 2^18 stored entries,
                                    else 0.  Selective support passes always use 0. */
-  GA_OPT_SUPPORT_TRI = 8        /* k-truss support passes as triangle listing, C<L> = L*L over
+  GA_OPT_SUPPORT_TRI = 8,       /* k-truss support passes as triangle listing, C<L> = L*L over
                                    the strict lower half with each hit added to its three edges
                                    (GA_SUPPORT_TRI): 1 = on, 0 = off, -1 (default) = on for
                                    graphs of at least 2^22 stored entries */
+  GA_OPT_SERVE_JOIN = 9         /* served k-truss queries joining a peeling run on their graph
+                                   that is already going (GA_SERVE_JOIN): 0 = never, 1 = when
+                                   the query's k lies within the run's k range, 2 = also above
+                                   it (the run goes on to it) */
 };
 ga_status ga_set_option(int option, double value);
 ga_status ga_get_option(int option, double* value);

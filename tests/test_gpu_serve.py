@@ -119,6 +119,32 @@ def test_batched_runs_equal_alone(gpu, G):
     alone.close()
 
 
+def test_ktruss_queries_join_a_running_peel(gpu, G):
+    """One slot: a k-truss query on a graph large enough for the host-driven
+    loop starts a peeling run; queries submitted while it runs (k at or above
+    the run's) join it and are answered as their k is reached.  Every answer
+    equals the query run alone, and answers come in ascending k order."""
+    import time
+    g = G.rmat(18, 16, seed=33)
+    srv = gpu.Server(hbm_budget=8 << 30, slots=1)
+    gid = srv.add_graph(g)
+    print(f"case serve join: rmat18 nnz={g.nnz}", flush=True)
+    first = srv.submit({"kind": "ktruss", "graph": gid, "k": 8})
+    time.sleep(0.01)
+    later = {k: srv.submit({"kind": "ktruss", "graph": gid, "k": k}) for k in (8, 12, 10, 9)}
+    got = {8: srv.wait(first)}
+    got.update({k: srv.wait(t) for k, t in later.items()})
+    st = srv.stats()
+    srv.close()
+    assert st["joined"] >= 1 and st["completed"] == 5
+    R = gpu.Graph(g)
+    for k in (8, 9, 10, 12):
+        assert got[k]["value"] == R.ktruss(k)[0], k
+    R.close()
+    ends = [got[k]["t_end_ms"] for k in (8, 9, 10, 12)]
+    assert ends == sorted(ends)
+
+
 def test_admission_by_footprint(gpu, G):
     g = G.rmat(12, 16, seed=9)
     probe = gpu.Server(hbm_budget=1 << 30, slots=1, max_batch=1)
