@@ -193,7 +193,7 @@ def test_truss_support_tri_rmat16(gpu, G, golden):
 def test_option_validation(gpu):
     for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
                       ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 3), ("truss_loop", -2),
-                      ("support_tri", 2)):
+                      ("support_tri", 2), ("serve_join", 3)):
         with pytest.raises(Exception):
             gpu.set_option(name, bad)
     before = gpu.get_option("row_claim")
