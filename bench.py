@@ -444,13 +444,18 @@ def bench_ktruss_one(args) -> dict:
     graphs = [("rmat12_ef16_s5", dict(scale=12, edge_factor=16, seed=5)),
               ("rmat14_ef16_s5", dict(scale=14, edge_factor=16, seed=5)),
               ("rmat16_ef16_s1", dict(scale=16, edge_factor=16, seed=1))]
+    scales = {int(x) for x in args.ktruss_scales.split(",")} if args.ktruss_scales else None
     for sc in (18, 20):
         c = pool[sc]
         graphs.append((f"rmat{sc}_ef{c['edge_factor']}_s{c['seed']}",
                        dict(scale=sc, edge_factor=c["edge_factor"], seed=c["seed"])))
     variants = {"default": {}, "host_loop": {"truss_loop": 0}, "device_steps": {"truss_loop": 1},
                 "graph_replay": {"truss_loop": 2}, "full_passes": {"selective_support": 0, "support_tri": 0},
-                "no_tri": {"support_tri": 0}}
+                "no_tri": {"support_tri": 0}, "tri_full_passes": {"selective_support": 0, "support_tri": 1}}
+    if args.ktruss_variants:
+        variants = {k: v for k, v in variants.items() if k in args.ktruss_variants.split(",")}
+    if scales:
+        graphs = [(nm, cfg) for nm, cfg in graphs if cfg["scale"] in scales]
     rows, headline = [], None
     for name, cfg in graphs:
         g = G.rmat(**cfg)
@@ -472,6 +477,8 @@ def bench_ktruss_one(args) -> dict:
             _log(f"ktruss_one {name} k={k}: " + ", ".join(
                 f"{r['variant']} {r['ms']:.2f}" for r in rows if r["graph"] == name and r["k"] == k))
         R.close()
+    if headline is None:  # a subset without the headline case: the first case
+        headline = (rows[0]["nnz"] // 2, rows[0]["ms"]) if rows else (0, float("nan"))
     m, ms = headline
     return {"value": m / (ms / 1e3), "unit": "edges/s", "ms_per_step": ms, "dtype": "int32",
             "config": {"workload": "k-truss for one k on resident R-MAT graphs (scale 12..20), each peeling "
@@ -499,6 +506,8 @@ def main() -> None:
     ap.add_argument("--slots", type=int, default=4, help="serve: queries in flight")
     ap.add_argument("--budget-gb", type=int, default=96, help="serve: device arena size")
     ap.add_argument("--max-batch", type=int, default=16, help="serve: queued queries one run may take")
+    ap.add_argument("--ktruss-scales", default="", help="ktruss_one: R-MAT scales to run (comma list; all)")
+    ap.add_argument("--ktruss-variants", default="", help="ktruss_one: settings to run (comma list; all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding over several "

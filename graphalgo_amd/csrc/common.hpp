@@ -147,8 +147,9 @@ struct Context {
   bool direct_items = false;
   // Selective support passes: k-truss passes after the first recompute only
   // the supports of entries with an endpoint that lost entries (1 = on,
-  // 0 = off, -1 = on for graphs of at least selective_min_nnz stored
-  // entries; GA_SELECTIVE_SUPPORT overrides it at ga_init).  On R-MAT 16 it
+  // 0 = off, -1 = on for graphs of at least selective_min_nnz stored entries
+  // when triangle-listing passes are switched off, since those beat it;
+  // GA_SELECTIVE_SUPPORT overrides it at ga_init).  On R-MAT 16 it
   // forms 19% fewer products but the pass time is set by per-item costs,
   // not products, and the selection launches cost more than the products
   // save (510 vs 493 ms per decomposition); on the R-MAT 18-20 graphs of
@@ -157,13 +158,16 @@ struct Context {
   int selective_support = -1;
   // support passes as triangle listing: C<L> = L*L over the strict lower
   // half, each hit adding 1 to its three edges (1 = on, 0 = off, -1 = on for
-  // graphs of at least selective_min_nnz stored entries; GA_SUPPORT_TRI).
-  // Measured on MI355X: the full passes of R-MAT 18-20 run 1.45x faster
-  // (k-truss k = 3: 72.8 -> 48.7 ms on R-MAT 18), but the R-MAT 16
-  // decomposition runs 40% slower (233 vs 167 ms): its later passes are on
-  // dense cores where most wedges close a triangle, and every closed one
-  // costs a global atomic for the B entry.
+  // k-truss runs on graphs of at least tri_list_min_nnz stored entries and
+  // decompositions of graphs of at least selective_min_nnz; GA_SUPPORT_TRI).
+  // Measured on MI355X: one k-truss on the served pool's R-MAT 18 / 20 runs
+  // 2.1-2.2x faster than with the symmetric half pass plus selective passes
+  // (k = 8: 224 -> 108 ms, 318 -> 147 ms; one full pass 31 -> 10 ms on
+  // R-MAT 18), but the R-MAT 16 decomposition runs 40% slower (227 vs
+  // 164 ms): its passes at large k are on dense cores where most wedges close
+  // a triangle, and every closed one costs a global atomic for the B entry.
   int support_tri = -1;
+  int64_t tri_list_min_nnz = int64_t(1) << 20;
   // served k-truss queries joining a peeling run on their graph that is
   // already going (GA_SERVE_JOIN): 0 = never, 1 = when the query's k lies
   // within the run's k range (it adds no work), 2 = also when it is above it
@@ -306,6 +310,9 @@ struct MxmWork {
   // after the previous launch on this workspace), so the next launch skips
   // its clearing fill
   bool counters_clean = false;
+  // support passes: the smallest graph (stored entries) that the default
+  // triangle-listing choice takes (-1: the context's threshold)
+  int64_t tri_min_nnz = -1;
   // reuse_plan: the operands do not change between launches (a resident
   // batch), so the row bins and item lists of the first launch are kept
   bool reuse_plan = false;

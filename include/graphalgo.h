@@ -304,7 +304,7 @@ enum {
   GA_OPT_SELECTIVE_SUPPORT = 6, /* k-truss passes after the first recompute only the supports an
                                    endpoint change can affect (GA_SELECTIVE_SUPPORT): 1 = on,
                                    0 = off, -1 (default) = on for graphs of at least 2^22 stored
-                                   entries */
+                                   entries when GA_OPT_SUPPORT_TRI is 0 */
   GA_OPT_TRUSS_LOOP = 7,        /* k-truss peeling loop (GA_TRUSS_LOOP): 0 = the host reads the
                                    survivor count after every filter step; 1 = device-driven
                                    steps (state in device memory, a few steps queued ahead);
@@ -314,7 +314,8 @@ enum {
   GA_OPT_SUPPORT_TRI = 8,       /* k-truss support passes as triangle listing, C<L> = L*L over
                                    the strict lower half with each hit added to its three edges
                                    (GA_SUPPORT_TRI): 1 = on, 0 = off, -1 (default) = on for
-                                   graphs of at least 2^22 stored entries */
+                                   k-truss runs on graphs of at least 2^20 stored entries and
+                                   decompositions of graphs of at least 2^22 */
   GA_OPT_SERVE_JOIN = 9         /* served k-truss queries joining a peeling run on their graph
                                    that is already going (GA_SERVE_JOIN): 0 = never, 1 = when
                                    the query's k lies within the run's k range, 2 = also above

@@ -99,22 +99,25 @@ def test_truss_selective_support(gpu, G, oracle):
     assert nnz5 == int((want >= 5).sum())
 
 
-def test_selective_default_large(gpu, G):
-    """A graph past the size where selective passes are on by default
-    (2^22 stored entries): the same k-truss sizes and pass counts as with
-    full passes, and the sizes the decomposition's truss numbers give."""
+def test_large_graph_pass_kinds(gpu, G):
+    """A graph past the sizes where the defaults change pass kind (2^20 / 2^22
+    stored entries: triangle-listing passes): the same k-truss sizes and
+    pass counts as with symmetric half passes and as with selective passes,
+    and the sizes the decomposition's truss numbers give."""
     g = G.rmat(18, 16, seed=31)
     assert g.nnz >= 1 << 22
     ks = (3, 6, 12)
-    print(f"case selective default rmat18 nnz={g.nnz}", flush=True)
+    print(f"case large graph pass kinds rmat18 nnz={g.nnz}", flush=True)
     R = gpu.Graph(g)
-    with gpu.options(selective_support=0):
+    with gpu.options(selective_support=0, support_tri=0):
         truss, kmax, _ = R.truss()
-        full = [R.ktruss(k) for k in ks]
-    auto = [R.ktruss(k) for k in ks]
+        half = [R.ktruss(k) for k in ks]
+    with gpu.options(selective_support=1, support_tri=0):
+        selective = [R.ktruss(k) for k in ks]
+    default = [R.ktruss(k) for k in ks]
     R.close()
-    assert auto == full
-    for k, (nnz, _) in zip(ks, auto):
+    assert default == half == selective
+    for k, (nnz, _) in zip(ks, default):
         assert nnz == int((truss >= k).sum())
 
 
