@@ -116,7 +116,9 @@ def bench_ktruss(args) -> dict:
                    "graph": "rmat(scale=16, edge_factor=16, a=.57, b=.19, c=.19, seed=1)",
                    "n": g.n, "m_undirected": m, "nnz": g.nnz, "kmax": kmax,
                    "support_passes": stats.get("passes"), "filter_steps": stats.get("steps")},
-        "roofline": _roofline(timing, acct, "k_mxm_items (symmetric support pass)", "k_mxm_items/truss_rmat16"),
+        "roofline": _roofline(timing, acct, "k_mxm_items (support passes: symmetric half pass; triangle "
+                                            "listing while the graph holds >= 2^20 entries)",
+                              "k_mxm_items/truss_rmat16"),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_ktruss(g)

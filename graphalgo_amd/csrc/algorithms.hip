@@ -586,10 +586,6 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   tw.scan_tmp.ensure(sizeof(long long) * std::max<int64_t>(ceil_div(nnz0, KEEP_TILE), 1));
   tw.mw.balance = true;  // the support passes split rows by the previous pass's work
   tw.mw.rwork_have = false;
-  {
-    const Context& c = ctx();
-    tw.mw.tri_min_nnz = all_k ? c.selective_min_nnz : c.tri_list_min_nnz;
-  }
   GA_HIP(hipMemcpyAsync(tw.cur[0].rp.p, A.rp, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToDevice, s));
   if (nnz0) GA_HIP(hipMemcpyAsync(tw.cur[0].col.p, A.col, sizeof(int32_t) * nnz0, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz0, 256, 8192)), dim3(256), 0, s, tw.eid[0].as<int32_t>(), nnz0);

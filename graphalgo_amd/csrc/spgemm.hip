@@ -1635,7 +1635,7 @@ void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, Mxm
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
                        c.prof_dev.as<unsigned long long>() + 1);
   Context& cx = ctx();
-  const int64_t tri_min = w.tri_min_nnz >= 0 ? w.tri_min_nnz : cx.selective_min_nnz;
+  const int64_t tri_min = w.tri_min_nnz >= 0 ? w.tri_min_nnz : cx.tri_list_min_nnz;
   if (cx.support_tri > 0 || (cx.support_tri < 0 && C.nnz >= tri_min)) {
     // each triangle i > k > j once, from the wedge (i, k, j) of C<L> = L*L
     // (L = the strict lower half): +1 to (i, j) (mask), (i, k) (A) and

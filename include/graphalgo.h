@@ -314,8 +314,7 @@ enum {
   GA_OPT_SUPPORT_TRI = 8,       /* k-truss support passes as triangle listing, C<L> = L*L over
                                    the strict lower half with each hit added to its three edges
                                    (GA_SUPPORT_TRI): 1 = on, 0 = off, -1 (default) = on for
-                                   k-truss runs on graphs of at least 2^20 stored entries and
-                                   decompositions of graphs of at least 2^22 */
+                                   every pass whose graph holds at least 2^20 stored entries */
   GA_OPT_SERVE_JOIN = 9         /* served k-truss queries joining a peeling run on their graph
                                    that is already going (GA_SERVE_JOIN): 0 = never, 1 = when
                                    the query's k lies within the run's k range, 2 = also above
