@@ -515,6 +515,13 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
   const int64_t max_steps = nnz0 / 2 + n + nk + 64;  // a step removes an edge or moves k on
   auto t_seen = std::chrono::steady_clock::now();
   long long seen = -1;
+  struct DrainOnThrow {  // an error leaves no step running on buffers the caller may free
+    hipStream_t s;
+    bool armed = true;
+    ~DrainOnThrow() {
+      if (armed) (void)hipStreamSynchronize(s);
+    }
+  } drain{s};
   while (true) {
     const long long ran = __atomic_load_n(&prog[0], __ATOMIC_ACQUIRE);
     if (__atomic_load_n(&prog[1], __ATOMIC_RELAXED)) break;
@@ -536,6 +543,7 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
   TrussDevState h{};
   GA_HIP(hipMemcpyAsync(&h, ds, sizeof(h), hipMemcpyDeviceToHost, s));
   GA_HIP(hipStreamSynchronize(s));
+  drain.armed = false;
   GA_REQUIRE(h.done && !h.error, GA_ERR_STATE, "k-truss steps did not finish");
   if (c.prof_events_on) {
     // every enqueued step timed its item launch; the ones a skip flag emptied
