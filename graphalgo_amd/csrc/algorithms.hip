@@ -161,7 +161,8 @@ __global__ __launch_bounds__(KEEP_T) void k_keep_scatter(
     const int32_t* __restrict__ col, const int32_t* __restrict__ eid, const int32_t* __restrict__ sup, int64_t nnz,
     int thr, const long long* __restrict__ bsum, int offsets_ready, int32_t* __restrict__ ncol,
     int32_t* __restrict__ neid, int32_t* __restrict__ nold, int32_t* __restrict__ newpos, int32_t* __restrict__ nsup,
-    int carry, int32_t* __restrict__ kept, const TrussDevState* __restrict__ ds) {
+    int carry, int32_t* __restrict__ kept, const TrussDevState* __restrict__ ds, long long* __restrict__ host_kept,
+    long long seq) {
   constexpr int NW = KEEP_T / 64;
   __shared__ long long prior[NW];
   __shared__ int wcount[NW];
@@ -222,6 +223,10 @@ __global__ __launch_bounds__(KEEP_T) void k_keep_scatter(
 #pragma unroll
       for (int k = 0; k < NW; ++k) t += prior[k] + wcount[k];
       *kept = (int32_t)t;
+      if (host_kept) {  // the host's copy: the count, then the sequence number
+        __hip_atomic_store(&host_kept[0], t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&host_kept[1], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
     __syncthreads();  // prior and wcount are reused by the next tile
   }
@@ -453,7 +458,8 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
                          (const int32_t*)tw.cur[0].col.as<int32_t>(), (const int32_t*)tw.eid[0].as<int32_t>(),
                          (const int32_t*)tw.sup[0].as<int32_t>(), nnz0, 0, (const long long*)bsum, offsets ? 1 : 0,
                          tw.cur[1].col.as<int32_t>(), tw.eid[1].as<int32_t>(), tw.nold.as<int32_t>(),
-                         tw.newpos.as<int32_t>(), tw.sup[1].as<int32_t>(), 0, kept_dev, (const TrussDevState*)ds);
+                         tw.newpos.as<int32_t>(), tw.sup[1].as<int32_t>(), 0, kept_dev, (const TrussDevState*)ds,
+                         (long long*)nullptr, 0ll);
       hipLaunchKernelGGL(k_keep_finish, dim3(gspan), dim3(256), 0, s, (const int32_t*)tw.cur[0].rp.as<int32_t>(), n,
                          (const int32_t*)tw.newpos.as<int32_t>(), nnz0, (const int32_t*)kept_dev,
                          tw.cur[1].rp.as<int32_t>(), (uint8_t*)nullptr, (const int32_t*)tw.mirror[0].as<int32_t>(),
@@ -650,18 +656,16 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     }
   };
   int32_t kmax = 2;
-  int32_t* kept_h = static_cast<int32_t*>(pinned_scratch());
-  // the host waits for the survivor count only (an event after its copy);
-  // the compaction runs before the count is known and its last kernels are
-  // queued behind the copy, so the device never idles while the host decides
-  // (when nothing is removed the compaction writes the spare buffers, which
-  // the next real one rewrites)
-  hipEvent_t kept_ev;
-  GA_HIP(hipEventCreateWithFlags(&kept_ev, hipEventDisableTiming));
-  struct EventGuard {
-    hipEvent_t e;
-    ~EventGuard() { (void)hipEventDestroy(e); }
-  } kept_guard{kept_ev};
+  // the host waits for the survivor count only: the scatter's last tile
+  // stores it with a sequence number to pinned host memory (words 8 and 9),
+  // read without a copy or an event; the compaction's last kernel is queued
+  // before the host reads it, so the device never idles while the host
+  // decides (when nothing is removed the compaction writes the spare
+  // buffers, which the next real one rewrites)
+  long long* hk_dev = nullptr;
+  long long* hk = progress_words(&hk_dev) + 8;
+  hk_dev += 8;
+  long long kept_seq = 0;
   const bool selective = selective_on(ctx(), nnz0);
   bool sup_zeroed = false;  // tw.sup[cur] was cleared by the compaction that made cur
   {
@@ -699,21 +703,21 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                          bsum, (const TrussDevState*)nullptr, (const int32_t*)nullptr, (int32_t*)nullptr);
       const bool offsets = nbk > KEEP_FUSED_TILES;
       if (offsets) scan_tile_offsets(bsum, nbk, kept_dev, s);
+      kept_seq = next_host_seq();
       hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nbk), dim3(KEEP_T), 0, s, (const int32_t*)C.col,
                          (const int32_t*)tw.eid[cur].as<int32_t>(), (const int32_t*)sup, nnz, thr,
                          (const long long*)bsum, offsets ? 1 : 0, tw.cur[cur ^ 1].col.as<int32_t>(),
                          tw.eid[cur ^ 1].as<int32_t>(), tw.nold.as<int32_t>(), tw.newpos.as<int32_t>(),
-                         tw.sup[cur ^ 1].as<int32_t>(), selective ? 1 : 0, kept_dev, (const TrussDevState*)nullptr);
-      GA_HIP(hipMemcpyAsync(kept_h, kept_dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      GA_HIP(hipEventRecord(kept_ev, s));
+                         tw.sup[cur ^ 1].as<int32_t>(), selective ? 1 : 0, kept_dev, (const TrussDevState*)nullptr,
+                         hk_dev, kept_seq);
       hipLaunchKernelGGL(k_keep_finish, dim3(grid_for(std::max<int64_t>(n + 1, nnz), 256, 8192)), dim3(256), 0, s,
                          C.rp, n, (const int32_t*)tw.newpos.as<int32_t>(), nnz, (const int32_t*)kept_dev, nrp,
                          selective ? tw.dirty.as<uint8_t>() : nullptr, (const int32_t*)tw.mirror[cur].as<int32_t>(),
                          (const int32_t*)tw.nold.as<int32_t>(), tw.mirror[cur ^ 1].as<int32_t>(),
                          (const TrussDevState*)nullptr);
       GA_HIP(hipGetLastError());
-      GA_HIP(hipEventSynchronize(kept_ev));
-      const int64_t kept = *kept_h;
+      wait_host_word(&hk[1], kept_seq, s);
+      const int64_t kept = __atomic_load_n(&hk[0], __ATOMIC_RELAXED);
       ++steps;
       if (kept == nnz) {
         if (!all_k) {

@@ -338,6 +338,11 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
 // claim counters for the next launch; the calling thread waits for the value
 // without a stream synchronisation.  Returns sums[0].
 unsigned long long publish_sum_and_wait(MxmWork& w, unsigned long long* sums, hipStream_t s);
+// Host-memory handoff: a sequence number unique on the calling thread, and a
+// wait until a kernel has stored it to *w (system scope) -- a stream error or
+// a drained stream without the store ends the wait with an error.
+long long next_host_seq();
+void wait_host_word(const long long* w, long long seq, hipStream_t s);
 
 // S<C> = C*C (int32 per entry) for a symmetric loop-free pattern C, computed
 // on the strict lower half and mirrored (the k-truss support pass).

@@ -1502,28 +1502,34 @@ namespace {
 thread_local long long t_publish_seq = 0;
 }
 
+long long next_host_seq() { return ++t_publish_seq; }
+
+void wait_host_word(const long long* w, long long seq, hipStream_t s) {
+  // every 1024 polls ask the stream whether it failed (a fault must end the
+  // wait) or drained (then the word is there)
+  for (int64_t it = 1;; ++it) {
+    if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == seq) return;
+    if ((it & 1023) == 0) {
+      const hipError_t e = hipStreamQuery(s);
+      if (e == hipSuccess) {
+        GA_REQUIRE(__atomic_load_n(w, __ATOMIC_ACQUIRE) == seq, GA_ERR_HIP, "the device result did not arrive");
+        return;
+      }
+      if (e != hipErrorNotReady) GA_HIP(e);
+    }
+  }
+}
+
 unsigned long long publish_sum_and_wait(MxmWork& w, unsigned long long* sums, hipStream_t s) {
   long long* hdev = nullptr;
   long long* h = progress_words(&hdev) + 4;  // words 4 (value) and 5 (sequence)
-  const long long seq = ++t_publish_seq;
+  const long long seq = next_host_seq();
   w.counter.ensure(sizeof(uint32_t) * MXM_CTR_WORDS);
   hipLaunchKernelGGL(k_sum_publish, dim3(1), dim3(256), 0, s, sums, w.counter.as<uint32_t>(), (int)MXM_CTR_WORDS,
                      hdev + 4, seq);
   GA_HIP(hipGetLastError());
   w.counters_clean = true;
-  // wait for the sequence number; every 1024 polls ask the stream whether it
-  // failed (a fault must end the wait) or drained (then the word is there)
-  for (int64_t it = 1;; ++it) {
-    if (__atomic_load_n(&h[1], __ATOMIC_ACQUIRE) == seq) break;
-    if ((it & 1023) == 0) {
-      const hipError_t e = hipStreamQuery(s);
-      if (e == hipSuccess) {
-        GA_REQUIRE(__atomic_load_n(&h[1], __ATOMIC_ACQUIRE) == seq, GA_ERR_HIP, "the device result did not arrive");
-        break;
-      }
-      if (e != hipErrorNotReady) GA_HIP(e);
-    }
-  }
+  wait_host_word(&h[1], seq, s);
   return (unsigned long long)__atomic_load_n(&h[0], __ATOMIC_RELAXED);
 }
 
