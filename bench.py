@@ -1,29 +1,44 @@
-"""Benchmark: truss decomposition (k-truss for every k until empty) on R-MAT
-scale 16, edge factor 16 (a=.57 b=.19 c=.19 d=.05, symmetrised, deduplicated,
-loops removed), one GPU.
+"""Benchmark of the masked-SpGEMM path on one GPU.
 
-One step = one full truss decomposition of the HBM-resident graph through the
-masked-SpGEMM support loop.  value = undirected edges / seconds per step.
-The dominant kernel is the masked SpGEMM (k_mxm_items); its roofline entry
-uses algorithmic bytes per launch
+The default run measures every workload in one process and prints one JSON
+line.  Its headline (value, roofline, cpu_baseline at the top level) is the
+truss decomposition (k-truss for every k until empty) of R-MAT scale 16, edge
+factor 16 (a=.57 b=.19 c=.19 d=.05, symmetrised, deduplicated, loops
+removed): one step = one full decomposition of the HBM-resident graph with
+the truss numbers read back to the host; value = undirected edges / seconds
+per step.  The line's "workloads" object holds the others, each with its own
+value, roofline and cpu_baseline:
 
-  4(m+1) + 4 nnz(A) + sum_{(i,k) in A} (8 + 4 nnz(B_k)) + 4(m+1) + 4 nnz(M) + 4 nnz(M)
+  tc_er_n1000     triangle count C<L>=L*L of the Erdos-Renyi graph n=1000,
+                  m=9990 (latency-bound single launch)
+  bc_rmat18_256   batched BC, 256 seeded random sources, R-MAT scale 18
+  batch_tc_knn4096  triangle counts of 4096 kNN graphs in one launch
+  serve_rmat_pool served stream over eleven resident R-MAT graphs of scale
+                  10..20 (p50 / p99 latency and throughput, vs the plain path)
+  mxm_rmat16      C<A>=A*A on R-MAT scale 16 (the reference's mxm action)
 
-(A = B = M = the current graph; values are implicit ones) divided by the GPU
-time of its launches, measured with HIP events on the library's stream.
-The CPU baseline is the OpenMP oracle's truss decomposition on the host.
+The dominant kernel's roofline entry uses algorithmic bytes per launch
 
-Other workloads: --workload tc (triangle count on R-MAT 16), tc_er (one
-small triangle count), bc (256-source batched BC on R-MAT 18), batch (4096
-kNN graphs, batched TC), serve, ewise (eWiseAdd of two R-MAT 20 graphs),
-ktruss_one (one k-truss query on graphs of scale 12..20 under each peeling
-setting), copy (calibration).
+  4(m+1) + 4 nnz(A) + sum_{(i,k) in A} (8 + 4 nnz(B_k)) + 4(m+1) + 4 nnz(M) + out
+
+(values implicit ones) divided by the GPU time of its launches, measured
+with HIP events on the stream the kernels run on; "traffic" is the HBM bytes
+per launch rocprofv3's FETCH_SIZE + WRITE_SIZE counted (scaled by a copy
+kernel of known size) and "traffic_frac" that traffic's rate against the
+peak.  CPU baselines are the OpenMP oracle on the host cores, timed in a
+child process of its own (OMP_PROC_BIND=spread, OMP_PLACES=cores, one
+warm-up, median of several runs) that regenerates the same seeded inputs
+and never touches the GPU; the GPU results are checked against the oracle
+in this process.
+
+--workload NAME runs one workload alone (also: tc, ktruss_one, ewise, copy).
 """
 from __future__ import annotations
 
+import os
 import argparse
 import json
-import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -48,6 +63,80 @@ def _cpu_cores() -> int:
         return len(os.sched_getaffinity(0))
     except Exception:
         return os.cpu_count() or 1
+
+
+_HOST = None
+_HOST_CPUS = None
+
+
+def host_info() -> dict:
+    """CPU model, sockets, physical cores and the cores this process may use."""
+    global _HOST, _HOST_CPUS
+    if _HOST is None:
+        try:
+            _HOST_CPUS = os.sched_getaffinity(0)
+        except (AttributeError, OSError):
+            _HOST_CPUS = None
+        info = {}
+        try:
+            out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+            for line in out.splitlines():
+                k, _, v = line.partition(":")
+                info[k.strip()] = v.strip()
+        except (OSError, subprocess.SubprocessError):
+            pass
+
+        def num(key):
+            try:
+                return int(info.get(key, ""))
+            except ValueError:
+                return None
+        sockets, cps, tpc = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+        _HOST = {"cpu_model": info.get("Model name"), "sockets": sockets,
+                 "physical_cores": sockets * cps if sockets and cps else None, "threads_per_core": tpc,
+                 "logical_cpus": num("CPU(s)"), "cpus_available": _cpu_cores(),
+                 "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+                 "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    return _HOST
+
+
+def _oracle():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    O.lib()
+    return O
+
+
+def _cpu_median(fn, reps: int, warm=None) -> tuple[float, list[float]]:
+    """One warm-up call (warm, or fn itself), then the median of reps timed calls."""
+    (warm or fn)()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def _cpu_entry(value: float, unit: str, sample: str, ts: list[float]) -> dict:
+    O = _oracle()
+    return {"value": value, "unit": unit, "cores": O.num_threads(), "kind": "port", "sample": sample,
+            "runs_s": [round(t, 4) for t in ts], "statistic": f"median of {len(ts)} after one warm-up",
+            "host": host_info()}
+
+
+def cpu_baseline(name: str, args) -> dict:
+    """Runs the CPU leg `name` (a key of CPU_LEGS) in a child process with the
+    OpenMP threads spread over the cores, and returns its cpu_baseline entry.
+    A child of its own because binding the OpenMP threads also pins the
+    thread that starts them, which would confine this process's later
+    threads (graph generation, the HIP runtime's) to one core."""
+    env = dict(os.environ, OMP_PROC_BIND="spread", OMP_PLACES="cores")
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-leg", name, "--queries", str(args.queries)]
+    proc = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if proc.returncode != 0:
+        raise RuntimeError(f"CPU leg {name} failed:\n{proc.stderr[-2000:]}")
+    return json.loads(proc.stdout.strip().splitlines()[-1])
 
 
 def _measure(step, steps: int, warmup: int):
@@ -87,14 +176,23 @@ def _traffic(key: str):
 
 
 def _roofline(timing: dict, acct: dict, kernel: str, traffic_key: str | None = None) -> dict:
+    """achieved / frac: algorithmic bytes per launch / average launch time (the
+    rate the kernel would need from HBM if every algorithmic byte came from
+    there); traffic_frac: the measured HBM bytes per launch at the same launch
+    time, against the same peak (below frac when the graph stays in L2/MALL)."""
     steps = max(timing.get("steps", 1), 1)
     launches = max(timing["launches"] // steps, 1)
     kernel_s = timing["kernel_ms"] / steps / 1e3
     achieved = acct["bytes_alg"] / kernel_s / 1e9 if kernel_s > 0 else 0.0
     traffic, source = _traffic(traffic_key) if traffic_key else (None, None)
+    avg_s = kernel_s / launches
+    traffic_gbs = traffic / avg_s / 1e9 if traffic and avg_s > 0 else None
     return {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
+        "frac_basis": "algorithmic bytes per launch / average launch time",
+        "traffic_gbs": None if traffic_gbs is None else round(traffic_gbs, 1),
+        "traffic_frac": None if traffic_gbs is None else round(traffic_gbs / HBM_PEAK_GBS, 4),
         "kernel": kernel, "launches_per_step": launches,
         "avg_launch_us": round(kernel_s * 1e6 / launches, 2),
         "bytes_alg_per_launch": int(acct["bytes_alg"] / max(acct["accounted_launches"], 1)),
@@ -107,34 +205,54 @@ def bench_ktruss(args) -> dict:
     g = G.rmat(16, 16, seed=1)
     m = g.m_undirected
     R = api.Graph(g)
-    dt, (_, kmax, stats), timing, acct = _measure(lambda: R.truss(want_values=False), args.steps, args.warmup)
+    # a step ends with the truss number of every stored entry on the host
+    dt, (truss, kmax, stats), timing, acct = _measure(lambda: R.truss(want_values=True), args.steps, args.warmup)
     R.close()
-    # per-step averages for the roofline (every step repeats the same launches)
+    digest = _sha256_i32(truss)
     out = {
         "value": m / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
-        "config": {"workload": "truss decomposition (k-truss for all k) on R-MAT scale 16",
+        "config": {"workload": "truss decomposition (k-truss for all k) on R-MAT scale 16, truss numbers read back",
                    "graph": "rmat(scale=16, edge_factor=16, a=.57, b=.19, c=.19, seed=1)",
                    "n": g.n, "m_undirected": m, "nnz": g.nnz, "kmax": kmax,
-                   "support_passes": stats.get("passes"), "filter_steps": stats.get("steps")},
+                   "support_passes": stats.get("passes"), "filter_steps": stats.get("steps"),
+                   "truss_sha256": digest},
         "roofline": _roofline(timing, acct, "k_mxm_items (support passes: symmetric half pass; triangle "
                                             "listing while the graph holds >= 2^20 entries)",
                               "k_mxm_items/truss_rmat16"),
     }
+    golden = _golden_truss_rmat16()
+    if golden:
+        out["config"]["truss_sha256_matches_reference"] = digest == golden
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_ktruss(g)
+        out["cpu_baseline"] = cpu_baseline("ktruss", args)
     return out
 
 
-def cpu_ktruss(g) -> dict:
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle as O
-    O.lib()
-    t0 = time.perf_counter()
-    O.truss_decomposition(g, max_passes=1)
-    dt = time.perf_counter() - t0
-    return {"value": g.m_undirected / dt, "unit": "edges/s", "cores": O.num_threads(), "kind": "port",
-            "sample": "one full truss decomposition of the same R-MAT 16 graph by the OpenMP oracle "
-                      f"({dt:.1f} s)"}
+def _sha256_i32(a) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()
+
+
+def _golden_truss_rmat16():
+    """SHA-256 of the reference's per-entry truss numbers of the same graph
+    (tests/golden/golden.json), when the fixture is present."""
+    try:
+        gold = json.loads((ROOT / "tests" / "golden" / "golden.json").read_text())
+        return gold["generated"]["rmat16_ef16_s1"]["truss"]["truss_sha256"]
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
+def cpu_ktruss(args) -> dict:
+    from graphalgo_amd import graphs as G
+    O = _oracle()
+    g = G.rmat(16, 16, seed=1)
+    warm = G.rmat(14, 16, seed=1)
+    med, ts = _cpu_median(lambda: O.truss_decomposition(g, max_passes=1), 3,
+                          warm=lambda: O.truss_decomposition(warm, max_passes=1))
+    return _cpu_entry(g.m_undirected / med, "edges/s",
+                      "full truss decompositions of the same R-MAT 16 graph by the OpenMP oracle (warm-up: "
+                      f"R-MAT 14), median {med:.2f} s", ts)
 
 
 def bench_tc(args) -> dict:
@@ -148,16 +266,21 @@ def bench_tc(args) -> dict:
                       "m_undirected": g.m_undirected, "triangles": t},
            "roofline": _roofline(timing, acct, "k_mxm_items (C<L> = L*L, sum mode)", "k_mxm_items/tc_rmat16")}
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle as O
-        reps, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < 5.0:
-            O.tc(g)
-            reps += 1
-        dtc = (time.perf_counter() - t0) / reps
-        out["cpu_baseline"] = {"value": g.m_undirected / dtc, "unit": "edges/s", "cores": O.num_threads(),
-                               "kind": "port", "sample": f"{reps} oracle triangle counts of the same graph"}
+        O = _oracle()
+        want = O.tc(g)
+        assert t == want, f"GPU triangle count {t} differs from the oracle's {want}"
+        out["config"]["oracle_triangles_equal"] = True
+        out["cpu_baseline"] = cpu_baseline("tc", args)
     return out
+
+
+def cpu_tc(args) -> dict:
+    from graphalgo_amd import graphs as G
+    O = _oracle()
+    g = G.rmat(16, 16, seed=1)
+    med, ts = _cpu_median(lambda: O.tc(g), 5)
+    return _cpu_entry(g.m_undirected / med, "edges/s",
+                      f"oracle triangle counts of the same graph, median {med * 1e3:.1f} ms", ts)
 
 
 def bench_tc_er(args) -> dict:
@@ -171,20 +294,66 @@ def bench_tc_er(args) -> dict:
     R.close()
     out = {"value": g.m_undirected / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": "triangle count C<L>=L*L on Erdos-Renyi n=1000 m=9990 (seed 1)", "n": g.n,
-                      "m_undirected": g.m_undirected, "triangles": t, "latency_us": round(dt * 1e6, 2)},
+                      "m_undirected": g.m_undirected, "triangles": t, "latency_us": round(dt * 1e6, 2),
+                      "note": "latency-bound: one call moves ~0.43 MB of algorithmic bytes (54 ns at 8 TB/s)"},
            "roofline": _roofline(timing, acct, "k_sum_subrows (C<L> = L*L, sum mode)")}
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle as O
-        reps, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < 2.0:
-            O.tc(g)
-            reps += 1
-        dtc = (time.perf_counter() - t0) / reps
-        out["cpu_baseline"] = {"value": g.m_undirected / dtc, "unit": "edges/s", "cores": O.num_threads(),
-                               "kind": "port", "sample": f"{reps} oracle triangle counts of the same graph "
-                                                         f"({dtc * 1e6:.0f} us each)"}
+        O = _oracle()
+        want = O.tc(g)
+        assert t == want, f"GPU triangle count {t} differs from the oracle's {want}"
+        out["config"]["oracle_triangles_equal"] = True
+        out["cpu_baseline"] = cpu_baseline("tc_er", args)
     return out
+
+
+def cpu_tc_er(args) -> dict:
+    from graphalgo_amd import graphs as G
+    O = _oracle()
+    g = G.erdos_renyi(1000, 9990, seed=1)
+
+    def many():  # one timed sample = 1000 counts (a single count is ~50 us)
+        for _ in range(1000):
+            O.tc(g)
+    med, ts = _cpu_median(many, 5)
+    return _cpu_entry(1000 * g.m_undirected / med, "edges/s",
+                      f"1000 oracle triangle counts of the same graph per run ({med * 1e3:.0f} us each)", ts)
+
+
+def bench_mxm(args) -> dict:
+    """C<A> = A*A on R-MAT scale 16 (the reference's mxm action,
+    src/main.cpp:235-240), int32 values of ones: host-pointer entry point, so
+    value = undirected edges per second of device time (HIP events around the
+    product launches); the host-inclusive rate (PCIe copies, allocations) is
+    beside it."""
+    from graphalgo_amd import api, graphs as G
+    g = G.rmat(16, 16, seed=1)
+    ones = np.ones(g.nnz, dtype=np.int32)
+
+    def step():
+        return api.mxm_masked(g, g, g, dtype=np.int32, Aval=ones, Bval=ones)
+    dt, C, timing, acct = _measure(step, args.steps, args.warmup)
+    dev_s = timing["kernel_ms"] / timing["steps"] / 1e3
+    out = {"value": g.m_undirected / dev_s, "unit": "edges/s", "ms_per_step": dev_s * 1e3, "dtype": "int32",
+           "config": {"workload": "C<A>=A*A on R-MAT scale 16, int32 values", "n": g.n, "nnz": g.nnz,
+                      "sum_C": int(C.astype(np.int64).sum()), "host_inclusive_ms_per_call": dt * 1e3,
+                      "host_inclusive_edges_per_s": g.m_undirected / dt},
+           "roofline": _roofline(timing, acct, "k_mxm_items (C<A> = A*A, int32 values per entry)")}
+    if not args.no_cpu:
+        O = _oracle()
+        want = O.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
+        assert np.array_equal(C.astype(np.int64), want), "C<A>=A*A differs from the oracle"
+        out["config"]["oracle_equal"] = True
+        out["cpu_baseline"] = cpu_baseline("mxm", args)
+    return out
+
+
+def cpu_mxm(args) -> dict:
+    from graphalgo_amd import graphs as G
+    O = _oracle()
+    g = G.rmat(16, 16, seed=1)
+    med, ts = _cpu_median(lambda: O.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col), 5)
+    return _cpu_entry(g.m_undirected / med, "edges/s",
+                      f"oracle C<A>=A*A of the same graph, median {med * 1e3:.0f} ms", ts)
 
 
 def bench_ewise(args) -> dict:
@@ -271,17 +440,32 @@ def bench_batch(args) -> dict:
            "plain_path": {"value": m_total / dt_plain, "unit": "edges/s", "ms": dt_plain * 1e3,
                           "speedup_batched_vs_plain": dt_plain / dt}}
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle as O
-        sample = graphs[:512]
-        t0 = time.perf_counter()
+        O = _oracle()
+        want = [O.tc(g) for g in graphs]
+        assert counts.tolist() == want, "batched counts differ from the oracle"
+        out["config"]["oracle_counts_equal"] = True
+        out["cpu_baseline"] = cpu_baseline("batch", args)
+    return out
+
+
+def cpu_batch(args) -> dict:
+    from graphalgo_amd import graphs as G
+    O = _oracle()
+    # the generator's worker threads inherit the affinity of the thread that
+    # starts them, which the OpenMP runtime bound to one core: lift it while
+    # they run
+    bound = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, _HOST_CPUS or bound)
+    sample = G.knn_batch(base_seed=20240, count=512, k=30, workers=min(16, len(_HOST_CPUS or bound)))
+    os.sched_setaffinity(0, bound)
+
+    def run():
         for g in sample:
             O.tc(g)
-        dtc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": sum(g.m_undirected for g in sample) / dtc, "unit": "edges/s",
-                               "cores": O.num_threads(), "kind": "port",
-                               "sample": f"oracle triangle counts of the first 512 of the {count} graphs"}
-    return out
+    med, ts = _cpu_median(run, 5)
+    return _cpu_entry(sum(g.m_undirected for g in sample) / med, "edges/s",
+                      f"oracle triangle counts of the first 512 of the batch's kNN graphs, median {med * 1e3:.0f} ms",
+                      ts)
 
 
 def bench_bc(args) -> dict:
@@ -303,16 +487,24 @@ def bench_bc(args) -> dict:
                       "sources": len(src), "source_seed": 7, "bc_sum": float(bc.sum())},
            "roofline": roof}
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle as O
-        sample = src[:32]
-        t0 = time.perf_counter()
-        O.bc(g, sample)
-        dtc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": m * len(sample) / dtc, "unit": "edges*sources/s",
-                               "cores": O.num_threads(), "kind": "port",
-                               "sample": f"oracle FP64 batched BC with the first 32 of the 256 sources ({dtc:.1f} s)"}
+        O = _oracle()
+        want = O.bc(g, src)
+        rel = float((np.abs(bc - want) / np.maximum(np.abs(want), 1.0)).max())
+        assert rel <= 1e-9, f"BC differs from the FP64 oracle (max relative {rel:.3e})"
+        out["config"]["oracle_max_rel_diff"] = rel
+        out["cpu_baseline"] = cpu_baseline("bc", args)
     return out
+
+
+def cpu_bc(args) -> dict:
+    from graphalgo_amd import graphs as G
+    O = _oracle()
+    g = G.rmat(18, 16, seed=1)
+    sample = G.pick_sources(g, 256, seed=7)[:32]
+    med, ts = _cpu_median(lambda: O.bc(g, sample), 5)
+    return _cpu_entry(g.m_undirected * len(sample) / med, "edges*sources/s",
+                      f"oracle FP64 batched BC on the same graph with the first 32 of the 256 sources, "
+                      f"median {med:.2f} s", ts)
 
 
 def _log(msg: str) -> None:
@@ -365,7 +557,7 @@ def bench_serve(args) -> dict:
     specs, arr = S.make_stream(pool, nq, qps, seed=3)
     served, lat = [], []
     t_served = 0.0
-    for step in range(args.steps):
+    for step in range(args.serve_replays):
         t0 = time.perf_counter()
         res = srv.replay(specs, arr)
         t_served += time.perf_counter() - t0
@@ -385,7 +577,7 @@ def bench_serve(args) -> dict:
         h.close()
     last = served[-1]
     kinds = {k: sum(q["kind"] == k for q in specs) for k, _ in S.MIX}
-    out = {"value": last["throughput_qps"], "unit": "queries/s", "ms_per_step": 1e3 * t_served / args.steps,
+    out = {"value": last["throughput_qps"], "unit": "queries/s", "ms_per_step": 1e3 * t_served / args.serve_replays,
            "dtype": "int32+f64",
            "config": {"workload": "served query stream over 11 resident R-MAT graphs (scale 10..20, edge factor "
                                   "8..32): Poisson arrivals, 60% TC / 25% k-truss / 15% 64-source BC",
@@ -408,28 +600,44 @@ def bench_serve(args) -> dict:
            "roofline_note": "queries of every kind run concurrently on 'slots' streams; per-kernel roofline "
                             "is reported by the ktruss, tc, bc and batch workloads"}
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle as O
-        t0 = time.perf_counter()
-        done = 0
-        for q in specs:
-            g = pool[q["graph"]]
-            if g.n > (1 << 15):
-                continue
-            if q["kind"] == "tc":
-                O.tc(g)
-            elif q["kind"] == "ktruss":
-                O.ktruss(g, q["k"])
-            else:
-                O.bc(g, q["sources"])
-            done += 1
-            if time.perf_counter() - t0 > 15.0:
-                break
-        dtc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": done / dtc, "unit": "queries/s", "cores": O.num_threads(), "kind": "port",
-                               "sample": f"oracle on the stream's first {done} queries over pool graphs of scale "
-                                         f"<= 15, back to back ({dtc:.1f} s)"}
+        out["cpu_baseline"] = cpu_baseline("serve", args)
     return out
+
+
+def cpu_serve(args) -> dict:
+    """The served stream's queries (the same seeded stream: its kinds, graphs,
+    k and sources do not depend on the arrival rate) over the pool graphs of
+    scale <= 15, answered back to back by the oracle."""
+    from graphalgo_amd import serving as S
+    O = _oracle()
+    pool = S.make_pool(S.pool_configs(1))
+    specs, _ = S.make_stream(pool, args.queries, 1.0, seed=3)
+    small = [q for q in specs if pool[q["graph"]].n <= (1 << 15)]
+
+    def run_one(q):
+        g = pool[q["graph"]]
+        if q["kind"] == "tc":
+            O.tc(g)
+        elif q["kind"] == "ktruss":
+            O.ktruss(g, q["k"])
+        else:
+            O.bc(g, q["sources"])
+    # the sample: the first of those queries that the oracle answers within
+    # about 3 s (this first pass is the warm-up), then timed as a whole
+    t0, sample = time.perf_counter(), []
+    for q in small:
+        run_one(q)
+        sample.append(q)
+        if time.perf_counter() - t0 > 3.0:
+            break
+
+    def run():
+        for q in sample:
+            run_one(q)
+    med, ts = _cpu_median(run, 5, warm=lambda: None)
+    return _cpu_entry(len(sample) / med, "queries/s",
+                      f"oracle on the stream's first {len(sample)} queries over pool graphs of scale <= 15, "
+                      f"back to back, median {med:.2f} s", ts)
 
 
 def bench_ktruss_one(args) -> dict:
@@ -493,7 +701,38 @@ def bench_ktruss_one(args) -> dict:
 
 
 WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "tc_er": bench_tc_er, "copy": bench_copy, "ewise": bench_ewise,
-             "batch": bench_batch, "bc": bench_bc, "serve": bench_serve, "ktruss_one": bench_ktruss_one}
+             "batch": bench_batch, "bc": bench_bc, "serve": bench_serve, "ktruss_one": bench_ktruss_one,
+             "mxm": bench_mxm}
+
+# the workloads of the default run, after the headline truss decomposition
+ALL = (("tc_er_n1000", "tc_er"), ("bc_rmat18_256", "bc"), ("batch_tc_knn4096", "batch"),
+       ("serve_rmat_pool", "serve"), ("mxm_rmat16", "mxm"))
+
+
+def bench_all(args) -> dict:
+    """The headline (truss decomposition of R-MAT 16) plus every other
+    workload, each measured with the same steps / warm-up (the served stream
+    with its own replay count) and reported under "workloads"."""
+    t0 = time.perf_counter()
+    _log("truss decomposition, R-MAT 16")
+    res = bench_ktruss(args)
+    _log(f"  {res['ms_per_step']:.1f} ms/step ({time.perf_counter() - t0:.0f} s)")
+    res["workloads"] = {}
+    for key, name in ALL:
+        _log(f"{key}")
+        r = WORKLOADS[name](args)
+        entry = {"value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"], "dtype": r["dtype"]}
+        entry.update({k: v for k, v in r.items() if k not in entry})
+        res["workloads"][key] = entry
+        _log(f"  {key}: {r['value']:.4g} {r['unit']} ({time.perf_counter() - t0:.0f} s)")
+    res["bench_wall_s"] = round(time.perf_counter() - t0, 1)
+    return res
+
+
+WORKLOADS["all"] = bench_all
+
+CPU_LEGS = {"ktruss": cpu_ktruss, "tc": cpu_tc, "tc_er": cpu_tc_er, "mxm": cpu_mxm, "batch": cpu_batch,
+            "bc": cpu_bc, "serve": cpu_serve}
 
 
 def main() -> None:
@@ -501,16 +740,23 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="ktruss", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="all", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--queries", type=int, default=400, help="serve: queries per stream")
     ap.add_argument("--qps", type=float, default=0.0, help="serve: offered load (0 = 0.8 x served capacity)")
     ap.add_argument("--slots", type=int, default=4, help="serve: queries in flight")
     ap.add_argument("--budget-gb", type=int, default=96, help="serve: device arena size")
     ap.add_argument("--max-batch", type=int, default=16, help="serve: queued queries one run may take")
+    ap.add_argument("--serve-replays", type=int, default=2, help="serve: timed replays of the stream")
     ap.add_argument("--ktruss-scales", default="", help="ktruss_one: R-MAT scales to run (comma list; all)")
     ap.add_argument("--ktruss-variants", default="", help="ktruss_one: settings to run (comma list; all)")
+    ap.add_argument("--cpu-leg", default="", choices=[""] + sorted(CPU_LEGS),
+                    help="internal: time one CPU baseline (oracle only, no GPU) and print its entry")
     args = ap.parse_args()
+    if args.cpu_leg:
+        host_info()  # before the OpenMP runtime binds this thread to one core
+        print(json.dumps(CPU_LEGS[args.cpu_leg](args)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding over several "
               "GPUs is designed (row blocks by wedges, BC by source columns) but not built", file=sys.stderr)

@@ -27,7 +27,16 @@
 // one exception takes nothing from the others: a queued k-truss query joins
 // a peeling run on its graph that is already going when that run reaches a
 // k at or below the query's (it uses the run's memory and slot), and every
-// k-truss query is answered as soon as its own k is reached.
+// k-truss query is answered as soon as its own k is reached.  Both hold for
+// runs of the host-driven peeling loop (graphs above 2^18 stored entries by
+// default); on smaller graphs the device-driven steps run the whole k list
+// before the host sees any k, so their queries are answered when the run
+// ends and queries arriving meanwhile start a run of their own (such runs
+// take about a millisecond).
+//
+// Graphs may be added while queries run: a query holds a pointer to its
+// graph (taken under the server lock at submit), never an index into the
+// growing graph table.
 #include "common.hpp"
 
 #include <algorithm>
@@ -99,6 +108,9 @@ struct ServerGraph {
 struct Job {
   int64_t ticket = 0;
   ga_query q{};
+  // the query's graph, taken under the server lock at submit: workers read it
+  // without the lock while ga_server_add_graph may grow the graph table
+  const ServerGraph* g = nullptr;
   std::vector<int32_t> src;
   size_t footprint = 0;
   size_t off = Arena::kNone;
@@ -166,7 +178,7 @@ size_t ga_server::footprint(const ga_query& q) const {
 
 size_t ga_server::batch_footprint(const std::vector<Job*>& set) const {
   if (set.size() == 1) return set[0]->footprint;
-  const ServerGraph& g = *graphs[set[0]->q.graph];
+  const ServerGraph& g = *set[0]->g;
   if (set[0]->q.kind == GA_Q_BC) {
     std::vector<int64_t> sizes;
     for (const Job* j : set) sizes.push_back((int64_t)j->src.size());
@@ -196,7 +208,7 @@ void ga_server::run_batch(std::vector<Job*>& set, std::vector<char>& early, int 
   GA_REQUIRE(!c.prof_events_on && !c.prof_bytes, GA_ERR_STATE,
              "kernel timing hooks are process-wide: disable them while a server runs queries");
   Job& j = *set[0];
-  const ServerGraph& g = *graphs[j.q.graph];
+  const ServerGraph& g = *j.g;
   Lease lease{base + off, fp, 0};
   t_lease = &lease;
   struct Unset {
@@ -502,7 +514,8 @@ ga_status ga_server_submit(ga_server* srv, const ga_query* q, int64_t* ticket) {
     std::lock_guard<std::mutex> lk(srv->mu);
     GA_REQUIRE(!srv->stop, GA_ERR_STATE, "server is shutting down");
     j->footprint = srv->footprint(*q);  // validates graph id, kind and source count
-    const ServerGraph& g = *srv->graphs[q->graph];
+    j->g = srv->graphs[q->graph].get();
+    const ServerGraph& g = *j->g;
     if (q->kind == GA_Q_KTRUSS) GA_REQUIRE(q->k >= 3, GA_ERR_ARG, "k must be >= 3");
     if (q->kind == GA_Q_BC) {
       GA_REQUIRE(q->sources && q->bc, GA_ERR_ARG, "BC query needs sources and an output buffer");

@@ -130,7 +130,12 @@ def test_ktruss_queries_join_a_running_peel(gpu, G):
     gid = srv.add_graph(g)
     print(f"case serve join: rmat18 nnz={g.nnz}", flush=True)
     first = srv.submit({"kind": "ktruss", "graph": gid, "k": 8})
-    time.sleep(0.01)
+    # the worker must have taken the first query (its own run) before the
+    # others arrive, or they would all start together in one shared run
+    deadline = time.monotonic() + 30.0
+    while srv.stats()["batches"] < 1:
+        assert time.monotonic() < deadline, "the first query was never admitted"
+        time.sleep(0.0005)
     later = {k: srv.submit({"kind": "ktruss", "graph": gid, "k": k}) for k in (8, 12, 10, 9)}
     got = {8: srv.wait(first)}
     got.update({k: srv.wait(t) for k, t in later.items()})
@@ -143,6 +148,31 @@ def test_ktruss_queries_join_a_running_peel(gpu, G):
     R.close()
     ends = [got[k]["t_end_ms"] for k in (8, 9, 10, 12)]
     assert ends == sorted(ends)
+
+
+def test_graphs_added_while_queries_run(gpu, G, oracle):
+    """ga_server_add_graph while queries on earlier graphs are in flight: the
+    graph table grows under the server lock and running queries keep the
+    graph they were submitted on.  Every answer equals the oracle's."""
+    base = [G.rmat(12 + (i % 3), 16, seed=40 + i) for i in range(3)]
+    extra = [G.rmat(9 + (i % 4), 8, seed=60 + i) for i in range(24)]
+    srv = gpu.Server(hbm_budget=4 << 30, slots=4, max_batch=1)
+    ids = [srv.add_graph(g) for g in base]
+    print("case serve add_graph during queries: 3 graphs + 24 added while 36 queries run", flush=True)
+    tickets = []
+    for i in range(36):
+        tickets.append((i % 3, srv.submit({"kind": "ktruss" if i % 2 else "tc", "graph": ids[i % 3], "k": 4})))
+        if i < len(extra):
+            ids_new = srv.add_graph(extra[i])
+            assert ids_new == len(base) + i
+    got = [(b, srv.wait(t)) for b, t in tickets]
+    late = [srv.submit({"kind": "tc", "graph": len(base) + i}) for i in range(len(extra))]
+    late_got = [srv.wait(t)["value"] for t in late]
+    srv.close()
+    for i, (b, r) in enumerate(got):
+        want = len(oracle.ktruss(base[b], 4)[1]) if i % 2 else oracle.tc(base[b])
+        assert r["value"] == want, (i, b)
+    assert late_got == [oracle.tc(g) for g in extra]
 
 
 def test_admission_by_footprint(gpu, G):
