@@ -18,11 +18,12 @@ GA_ERRORS = {
     1: "GA_ERR_ARG", 2: "GA_ERR_DIM", 3: "GA_ERR_UNSORTED", 4: "GA_ERR_OOM",
     5: "GA_ERR_HIP", 6: "GA_ERR_LIMIT", 7: "GA_ERR_STATE",
 }
-GA_INT32, GA_INT64, GA_FLOAT64 = 0, 1, 2
+GA_INT32, GA_INT64, GA_FLOAT64, GA_FLOAT32 = 0, 1, 2, 3
 
 I32P = C.POINTER(C.c_int32)
 I64P = C.POINTER(C.c_int64)
 F64P = C.POINTER(C.c_double)
+F32P = C.POINTER(C.c_float)
 
 
 class GaCsr(C.Structure):
@@ -96,6 +97,9 @@ GA_SIGNATURES = {
     "ga_mxm_cmask": (C.c_int, [C.POINTER(GaCsr), C.POINTER(GaCsr), C.POINTER(GaCsr), C.c_int,
                                C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.c_void_p), I64P]),
     "ga_mxm_spd": (C.c_int, [C.POINTER(GaCsr), F64P, C.c_int64, C.POINTER(GaCsr), F64P]),
+    "ga_mxm_spd_f32": (C.c_int, [C.POINTER(GaCsr), F32P, C.c_int64, C.POINTER(GaCsr), F32P]),
+    "ga_dense_fuse_mult_add": (C.c_int, [C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "ga_dense_mult_masked": (C.c_int, [C.POINTER(GaCsr), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ga_ewise": (C.c_int, [C.c_int, C.POINTER(GaCsr), C.POINTER(GaCsr), C.POINTER(GaCsr), C.c_int,
                            C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.c_void_p), I64P]),
     "ga_scale": (C.c_int, [C.POINTER(GaCsr), C.c_int, C.c_double, C.c_void_p]),

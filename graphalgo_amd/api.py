@@ -24,7 +24,8 @@ def _ga(A: CSR, val: np.ndarray | None = None, ncols: int | None = None) -> N.Ga
     return g
 
 
-_DT = {np.dtype(np.int32): N.GA_INT32, np.dtype(np.int64): N.GA_INT64, np.dtype(np.float64): N.GA_FLOAT64}
+_DT = {np.dtype(np.int32): N.GA_INT32, np.dtype(np.int64): N.GA_INT64, np.dtype(np.float64): N.GA_FLOAT64,
+       np.dtype(np.float32): N.GA_FLOAT32}
 
 
 def mxm_masked(A: CSR, B: CSR, M: CSR, dtype=np.int64, Aval=None, Bval=None,
@@ -103,6 +104,59 @@ def mult_scalar(A: CSR, Aval, alpha: float, dtype=np.int64) -> np.ndarray:
     a = _ga(A, av)
     N.check(N.galib().ga_scale(C.byref(a), _DT[dt], float(alpha), out.ctypes.data_as(C.c_void_p)))
     return out[: A.nnz]
+
+
+def mxm_spd(A: CSR, B: np.ndarray, M: CSR, Aval=None, dtype=np.float64) -> np.ndarray:
+    """mxmm_spd (include/matrix_la.h:239-270): C<M> = A*B with B (A.ncols x n)
+    and C (A.nrows x n) dense row-major; C is zero outside M's pattern.
+    dtype float64 or float32 (the reference's float instantiation,
+    include/betweenness_centrality.h:210)."""
+    dt = np.dtype(dtype)
+    N.init()
+    Bd = np.ascontiguousarray(B, dtype=dt)
+    ncols = Bd.shape[1]
+    av = None if Aval is None else np.ascontiguousarray(Aval, dtype=dt)
+    a = _ga(A, av, Bd.shape[0])
+    mm = _ga(M, None, ncols)
+    out = np.zeros((A.n, ncols), dtype=dt)
+    if dt == np.float64:
+        N.check(N.galib().ga_mxm_spd(C.byref(a), Bd.ctypes.data_as(N.F64P), ncols, C.byref(mm),
+                                     out.ctypes.data_as(N.F64P)))
+    elif dt == np.float32:
+        N.check(N.galib().ga_mxm_spd_f32(C.byref(a), Bd.ctypes.data_as(N.F32P), ncols, C.byref(mm),
+                                         out.ctypes.data_as(N.F32P)))
+    else:
+        raise TypeError("mxm_spd runs in float64 or float32")
+    return out
+
+
+def fuse_ewise_mult_add(A: np.ndarray, B: np.ndarray, Cm: np.ndarray) -> np.ndarray:
+    """fuseEWiseMultAdd (include/matrix_la.h:192-198): returns C + A .* B for
+    dense arrays of one shape and value type (C is not modified)."""
+    N.init()
+    dt = np.asarray(Cm).dtype
+    a, b = (np.ascontiguousarray(x, dtype=dt) for x in (A, B))
+    out = np.array(Cm, dtype=dt, order="C", copy=True)
+    if not (a.shape == b.shape == out.shape):
+        raise ValueError("shapes differ")
+    N.check(N.galib().ga_dense_fuse_mult_add(out.size, _DT[dt], a.ctypes.data_as(C.c_void_p),
+                                             b.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def ewise_mult_dense_masked(A: np.ndarray, B: np.ndarray, M: CSR) -> np.ndarray:
+    """The dense eWiseMult (include/matrix_la.h:200-219): C<M> = A .* B, zero
+    outside M's pattern; A, B dense (M.n x ncols) of one value type."""
+    N.init()
+    dt = np.asarray(A).dtype
+    a, b = (np.ascontiguousarray(x, dtype=dt) for x in (A, B))
+    if a.shape != b.shape or a.ndim != 2 or a.shape[0] != M.n:
+        raise ValueError("shapes differ")
+    mm = _ga(M, None, a.shape[1])
+    out = np.empty_like(a)
+    N.check(N.galib().ga_dense_mult_masked(C.byref(mm), _DT[dt], a.ctypes.data_as(C.c_void_p),
+                                           b.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p)))
+    return out
 
 
 def triangle_count(A: CSR) -> int:

@@ -268,6 +268,7 @@ __device__ inline void atomic_add_val(int64_t* p, int64_t v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }
 __device__ inline void atomic_add_val(double* p, double v) { atomicAdd(p, v); }
+__device__ inline void atomic_add_val(float* p, float v) { atomicAdd(p, v); }
 
 // ---- host launchers shared between translation units -------------------
 // Exclusive scan of n int32 counts into out[0..n]; out[n] = total.  Totals

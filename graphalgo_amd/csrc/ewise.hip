@@ -496,6 +496,7 @@ ga_status ga_ewise(int op, const ga_csr* A, const ga_csr* B, const ga_csr* M, ga
       switch (dtype) {
         case GA_INT32: ewise_dispatch<int32_t>(op, dA.view(), dB.view(), pm, rp, Ccol, Cval, Cnnz, s); break;
         case GA_INT64: ewise_dispatch<int64_t>(op, dA.view(), dB.view(), pm, rp, Ccol, Cval, Cnnz, s); break;
+        case GA_FLOAT32: ewise_dispatch<float>(op, dA.view(), dB.view(), pm, rp, Ccol, Cval, Cnnz, s); break;
         default: ewise_dispatch<double>(op, dA.view(), dB.view(), pm, rp, Ccol, Cval, Cnnz, s); break;
       }
     } catch (...) {
@@ -530,6 +531,10 @@ ga_status ga_scale(const ga_csr* A, ga_dtype dtype, double alpha, void* Cval) {
       case GA_INT64:
         hipLaunchKernelGGL(k_scale<int64_t>, dim3(grid), dim3(256), 0, s, v.as<int64_t>(), A->nnz, alpha,
                            out.as<int64_t>());
+        break;
+      case GA_FLOAT32:
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(256), 0, s, v.as<float>(), A->nnz, alpha,
+                           out.as<float>());
         break;
       default:
         hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(256), 0, s, v.as<double>(), A->nnz, alpha,

@@ -25,16 +25,20 @@
 namespace ga {
 
 // ---- mxmm_spd ---------------------------------------------------------------
+// Products and sums rounded separately (no contraction into fused
+// multiply-adds), as the reference's scalar loop rounds them.
+template <typename T>
 __global__ __launch_bounds__(256) void k_spd(const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol,
-                                             const double* __restrict__ Aval, const double* __restrict__ B,
-                                             int64_t ncols, const int32_t* __restrict__ Mrp,
-                                             const int32_t* __restrict__ Mcol, double* __restrict__ C, int64_t m) {
+                                             const T* __restrict__ Aval, const T* __restrict__ B, int64_t ncols,
+                                             const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol,
+                                             T* __restrict__ C, int64_t m) {
+#pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const int64_t wpb = blockDim.x >> 6;
   for (int64_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < m; i += (int64_t)gridDim.x * wpb) {
     const int mb = Mrp[i], me = Mrp[i + 1];
     const int ab = Arp[i], ae = Arp[i + 1];
-    double* crow = C + i * ncols;
+    T* crow = C + i * ncols;
     for (int64_t c = lane; c < ncols; c += 64) {
       int lo = mb, hi = me;
       while (lo < hi) {
@@ -42,9 +46,9 @@ __global__ __launch_bounds__(256) void k_spd(const int32_t* __restrict__ Arp, co
         if (Mcol[mid] < c) lo = mid + 1;
         else hi = mid;
       }
-      double acc = 0.0;
+      T acc = T(0);
       if (lo < me && Mcol[lo] == c) {
-        for (int t = ab; t < ae; ++t) acc += (Aval ? Aval[t] : 1.0) * B[(int64_t)Acol[t] * ncols + c];
+        for (int t = ab; t < ae; ++t) acc += (Aval ? Aval[t] : T(1)) * B[(int64_t)Acol[t] * ncols + c];
       }
       crow[c] = acc;
     }
@@ -271,6 +275,30 @@ static void cmask_run(const DCsr& A, const DCsr& B, const DCsr& M, int32_t* Crp_
   GA_HIP(hipStreamSynchronize(s));
 }
 
+template <typename T>
+void spd_run(const ga_csr* A, const T* B, int64_t ncols, const ga_csr* M, T* C, int dtype) {
+  validate_host_csr(A, "A");
+  validate_host_csr(M, "M");
+  GA_REQUIRE(ncols >= 0 && M->ncols == ncols && A->nrows == M->nrows, GA_ERR_DIM,
+             "shapes: A m x k, B k x ncols, M m x ncols");
+  GA_REQUIRE((A->ncols == 0 || ncols == 0 || B) && (A->nrows == 0 || ncols == 0 || C), GA_ERR_ARG, "null buffer");
+  hipStream_t s = ctx().stream;
+  OwnedCsr dA, dM;
+  upload_csr(A, dtype, dA, s);
+  upload_csr(M, dtype, dM, s);
+  const int64_t m = A->nrows, k = A->ncols;
+  DevBuf dB, dC;
+  dB.ensure(sizeof(T) * std::max<int64_t>(k * ncols, 1));
+  dC.ensure(sizeof(T) * std::max<int64_t>(m * ncols, 1));
+  if (k * ncols) GA_HIP(hipMemcpyAsync(dB.p, B, sizeof(T) * k * ncols, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_spd<T>, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, dA.rp.as<int32_t>(),
+                     dA.col.as<int32_t>(), (const T*)dA.val.p, (const T*)dB.as<T>(), ncols, dM.rp.as<int32_t>(),
+                     dM.col.as<int32_t>(), dC.as<T>(), m);
+  GA_HIP(hipGetLastError());
+  if (m * ncols) GA_HIP(hipMemcpyAsync(C, dC.p, sizeof(T) * m * ncols, hipMemcpyDeviceToHost, s));
+  GA_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace ga
 
 using namespace ga;
@@ -278,28 +306,11 @@ using namespace ga;
 extern "C" {
 
 ga_status ga_mxm_spd(const ga_csr* A, const double* B, int64_t ncols, const ga_csr* M, double* C) {
-  return guarded([&] {
-    validate_host_csr(A, "A");
-    validate_host_csr(M, "M");
-    GA_REQUIRE(ncols >= 0 && M->ncols == ncols && A->nrows == M->nrows, GA_ERR_DIM,
-               "shapes: A m x k, B k x ncols, M m x ncols");
-    GA_REQUIRE((A->ncols == 0 || ncols == 0 || B) && (A->nrows == 0 || ncols == 0 || C), GA_ERR_ARG, "null buffer");
-    hipStream_t s = ctx().stream;
-    OwnedCsr dA, dM;
-    upload_csr(A, GA_FLOAT64, dA, s);
-    upload_csr(M, GA_FLOAT64, dM, s);
-    const int64_t m = A->nrows, k = A->ncols;
-    DevBuf dB, dC;
-    dB.ensure(sizeof(double) * std::max<int64_t>(k * ncols, 1));
-    dC.ensure(sizeof(double) * std::max<int64_t>(m * ncols, 1));
-    if (k * ncols) GA_HIP(hipMemcpyAsync(dB.p, B, sizeof(double) * k * ncols, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_spd, dim3(grid_for(m, 4, 16384)), dim3(256), 0, s, dA.rp.as<int32_t>(), dA.col.as<int32_t>(),
-                       (const double*)dA.val.p, (const double*)dB.as<double>(), ncols, dM.rp.as<int32_t>(),
-                       dM.col.as<int32_t>(), dC.as<double>(), m);
-    GA_HIP(hipGetLastError());
-    if (m * ncols) GA_HIP(hipMemcpyAsync(C, dC.p, sizeof(double) * m * ncols, hipMemcpyDeviceToHost, s));
-    GA_HIP(hipStreamSynchronize(s));
-  });
+  return guarded([&] { spd_run<double>(A, B, ncols, M, C, GA_FLOAT64); });
+}
+
+ga_status ga_mxm_spd_f32(const ga_csr* A, const float* B, int64_t ncols, const ga_csr* M, float* C) {
+  return guarded([&] { spd_run<float>(A, B, ncols, M, C, GA_FLOAT32); });
 }
 
 ga_status ga_mxm_cmask(const ga_csr* A, const ga_csr* B, const ga_csr* M, ga_dtype dtype, int32_t** Crowptr,
@@ -327,6 +338,7 @@ ga_status ga_mxm_cmask(const ga_csr* A, const ga_csr* B, const ga_csr* M, ga_dty
       switch (dtype) {
         case GA_INT32: cmask_run<int32_t>(dA.view(), dB.view(), vM, rp, Ccol, Cval, Cnnz, s); break;
         case GA_INT64: cmask_run<int64_t>(dA.view(), dB.view(), vM, rp, Ccol, Cval, Cnnz, s); break;
+        case GA_FLOAT32: cmask_run<float>(dA.view(), dB.view(), vM, rp, Ccol, Cval, Cnnz, s); break;
         default: cmask_run<double>(dA.view(), dB.view(), vM, rp, Ccol, Cval, Cnnz, s); break;
       }
     } catch (...) {

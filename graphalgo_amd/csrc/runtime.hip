@@ -61,6 +61,7 @@ size_t dtype_size(int dtype) {
     case GA_INT32: return 4;
     case GA_INT64: return 8;
     case GA_FLOAT64: return 8;
+    case GA_FLOAT32: return 4;
   }
   throw Error{GA_ERR_ARG, "unknown dtype " + std::to_string(dtype)};
 }

@@ -1776,6 +1776,7 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
     case GA_INT32: dispatch_pat<int32_t, 0>(A, B, M, Cval, sums, row_group, w, bm, s); break;
     case GA_INT64: dispatch_pat<int64_t, 0>(A, B, M, Cval, sums, row_group, w, bm, s); break;
     case GA_FLOAT64: dispatch_pat<double, 0>(A, B, M, Cval, sums, row_group, w, bm, s); break;
+    case GA_FLOAT32: dispatch_pat<float, 0>(A, B, M, Cval, sums, row_group, w, bm, s); break;
     default: throw Error{GA_ERR_ARG, "unknown dtype"};
   }
 }

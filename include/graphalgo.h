@@ -40,7 +40,10 @@ typedef int ga_status;
 #define GA_ERR_LIMIT 6     /* size beyond a documented limit */
 #define GA_ERR_STATE 7     /* library not initialised / handle misuse */
 
-typedef enum { GA_INT32 = 0, GA_INT64 = 1, GA_FLOAT64 = 2 } ga_dtype;
+/* Value types of the operators.  GA_FLOAT32 is the reference's `float`
+ * (spMtx<float> / denseMtx<float> in include/betweenness_centrality.h:137,
+ * :141-143); floating-point sums are formed in the kernels' own order. */
+typedef enum { GA_INT32 = 0, GA_INT64 = 1, GA_FLOAT64 = 2, GA_FLOAT32 = 3 } ga_dtype;
 
 /* A CSR matrix view (host or device pointers depending on the call). */
 typedef struct {
@@ -103,8 +106,24 @@ ga_status ga_scale(const ga_csr* A, ga_dtype dtype, double alpha, void* Cval);
 
 /* Masked sparse x dense: C<M> = A*B with B and C dense row-major
  * (A.ncols x ncols and A.nrows x ncols); row i of C is zero outside M's
- * row i.  Replaces mxmm_spd (include/matrix_la.h:239-270).  FP64 only. */
+ * row i.  Replaces mxmm_spd (include/matrix_la.h:239-270).  FP64; the
+ * _f32 form is the reference's float instantiation
+ * (include/betweenness_centrality.h:210).  Each C entry sums A's row in CSR
+ * order, as the reference does. */
 ga_status ga_mxm_spd(const ga_csr* A, const double* B, int64_t ncols, const ga_csr* M, double* C);
+ga_status ga_mxm_spd_f32(const ga_csr* A, const float* B, int64_t ncols, const ga_csr* M, float* C);
+
+/* Dense element-wise operators of the reference's BC backward sweep, on
+ * count = m * n row-major values of dtype (host memory, synchronous).
+ * ga_dense_fuse_mult_add: C += A .* B (fuseEWiseMultAdd,
+ *   include/matrix_la.h:192-198);
+ * ga_dense_mult_masked: C<M> = A .* B -- C is zero outside M's pattern and
+ *   A(i,j) * B(i,j) at M's entries (the dense eWiseMult,
+ *   include/matrix_la.h:200-219); M is m x n.
+ * Floating-point products and sums are rounded separately (no fused
+ * multiply-add), as the reference's scalar loop computes them. */
+ga_status ga_dense_fuse_mult_add(int64_t count, ga_dtype dtype, const void* A, const void* B, void* C);
+ga_status ga_dense_mult_masked(const ga_csr* M, ga_dtype dtype, const void* A, const void* B, void* C);
 
 void ga_free_host(void* p);
 

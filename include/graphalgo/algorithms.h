@@ -79,10 +79,10 @@ inline int* triangle_counting_vertex(const spMtx<int>& A, mxmOp<int> matrixMult,
   return out;
 }
 
-// src/main.cpp:89-146: the k-truss of A (values 1).  iterations (optional)
-// receives the number of support passes.
-inline spMtx<int> k_truss(const spMtx<int>& A, int k, mxmOp<int> matrixMult, bool isParallel,
-                          int* iterations = nullptr) {
+// src/main.cpp:89-146: the k-truss of A (values 1); *iterations receives
+// the number of support passes (the reference's totalIterationNum, which it
+// prints).  The four-argument form below has the reference's exact type.
+inline spMtx<int> k_truss(const spMtx<int>& A, int k, mxmOp<int> matrixMult, bool isParallel, int* iterations) {
   if (graphalgo_detail::is_gpu_op(matrixMult)) {
     graphalgo_detail::ensure_init();
     spMtx<int> C(A.m, A.n, A.nz);
@@ -118,6 +118,10 @@ inline spMtx<int> k_truss(const spMtx<int>& A, int k, mxmOp<int> matrixMult, boo
   }
   if (iterations) *iterations = it;
   return C;
+}
+
+inline spMtx<int> k_truss(const spMtx<int>& A, int k, mxmOp<int> matrixMult, bool isParallel) {
+  return k_truss(A, k, matrixMult, isParallel, nullptr);
 }
 
 // Truss number of every stored entry of A (the loop over k around k_truss).

@@ -7,12 +7,15 @@
 //   mxmm_spd                          include/matrix_la.h:239-270 -> ga_mxm_spd
 //   eWiseAdd, eWiseMult, add_nointersect, multScalar
 //                                     include/matrix_la.h:272-541 -> ga_ewise, ga_scale
+//   fuseEWiseMultAdd, eWiseMult (dense, masked)
+//                                     include/matrix_la.h:192-219
+//                                     -> ga_dense_fuse_mult_add, ga_dense_mult_masked
 //
 // The three structural-mask variants are one GPU operator (the accumulator
 // choice of the reference -- dense MSA, mask-indexed MCA, heap -- becomes a
 // per-row decision inside the kernel), so binding any of them to
 // mxmOp<T> (src/main.cpp:11-12) selects the same path.  `isParallel` is
-// accepted and ignored.  Value types: int, int64_t, double.  On error the
+// accepted and ignored.  Value types: int, int64_t, float, double.  On error the
 // functions throw the C ABI's message (const char*), the reference's error
 // style.
 #pragma once
@@ -25,9 +28,13 @@ namespace graphalgo_detail {
 
 template <typename T>
 constexpr ga_dtype dtype_of() {
-  static_assert(std::is_same<T, int>::value || std::is_same<T, int64_t>::value || std::is_same<T, double>::value,
-                "the GPU operators support int, int64_t and double values");
-  return std::is_same<T, int>::value ? GA_INT32 : (std::is_same<T, int64_t>::value ? GA_INT64 : GA_FLOAT64);
+  static_assert(std::is_same<T, int>::value || std::is_same<T, int64_t>::value || std::is_same<T, float>::value ||
+                    std::is_same<T, double>::value,
+                "the GPU operators support int, int64_t, float and double values");
+  return std::is_same<T, int>::value       ? GA_INT32
+         : std::is_same<T, int64_t>::value ? GA_INT64
+         : std::is_same<T, float>::value   ? GA_FLOAT32
+                                           : GA_FLOAT64;
 }
 
 inline void check(ga_status s) {
@@ -108,15 +115,39 @@ void mxmm_msa_cmask(bool /*isParallel*/, const spMtx<T>& A, const spMtx<T>& B, c
 // result buffer into C, include/matrix_la.h:242-270; Cbuf is left unspecified).
 template <typename T, typename U>
 void mxmm_spd(const spMtx<T>& A, const denseMtx<T>& B, const spMtx<U>& M, denseMtx<T>& C, denseMtx<T>& Cbuf) {
-  static_assert(std::is_same<T, double>::value, "mxmm_spd runs in FP64 on the GPU");
+  static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+                "mxmm_spd runs in FP64 or FP32 on the GPU");
   graphalgo_detail::ensure_init();
   Cbuf.resize(M.m, B.n);
   const ga_csr a = A.view(true), mm = M.view(false);
-  graphalgo_detail::check(ga_mxm_spd(&a, B.Val, (int64_t)B.n, &mm, Cbuf.Val));
+  if constexpr (std::is_same<T, double>::value)
+    graphalgo_detail::check(ga_mxm_spd(&a, B.Val, (int64_t)B.n, &mm, Cbuf.Val));
+  else
+    graphalgo_detail::check(ga_mxm_spd_f32(&a, B.Val, (int64_t)B.n, &mm, Cbuf.Val));
   std::swap(C.Val, Cbuf.Val);
   std::swap(C.capacity, Cbuf.capacity);
   C.m = M.m;
   C.n = B.n;
+}
+
+// ---- dense element-wise operators (include/matrix_la.h:192-219) ----
+// C += A .* B over all m * n values.
+template <typename T>
+void fuseEWiseMultAdd(const denseMtx<T>& A, const denseMtx<T>& B, denseMtx<T>& C) {
+  graphalgo_detail::ensure_init();
+  if (A.m != B.m || A.n != B.n || A.m != C.m || A.n != C.n) throw -1;
+  graphalgo_detail::check(
+      ga_dense_fuse_mult_add((int64_t)(A.m * A.n), graphalgo_detail::dtype_of<T>(), A.Val, B.Val, C.Val));
+}
+
+// C<M> = A .* B: C is zero outside M's pattern.  C is taken by const
+// reference and written through its value array, as in the reference.
+template <typename T, typename U>
+void eWiseMult(const denseMtx<T>& A, const denseMtx<T>& B, const spMtx<U>& M, const denseMtx<T>& C) {
+  graphalgo_detail::ensure_init();
+  if (A.m != B.m || A.n != B.n || A.m != C.m || A.n != C.n || M.m != C.m || M.n != C.n) throw -1;
+  const ga_csr mm = M.view(false);
+  graphalgo_detail::check(ga_dense_mult_masked(&mm, graphalgo_detail::dtype_of<T>(), A.Val, B.Val, C.Val));
 }
 
 // ---- element-wise operators (include/matrix_la.h:272-541, defects fixed) ----

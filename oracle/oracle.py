@@ -265,6 +265,46 @@ def ewise(op, n, Arp, Acol, Aval, Brp, Bcol, Bval, Mrp=None, Mcol=None, dtype=np
     return rp, col, xa + xb
 
 
+def mxm_spd_seq(Arp, Acol, Aval, B, Mrp, Mcol, dtype=np.float32):
+    """mxmm_spd (include/matrix_la.h:239-270) in the value type `dtype`, in
+    the reference's order: row i of C starts at zero and, for each stored
+    A(i,k) in CSR order, C(i, M's columns) += A(i,k) * B(k, M's columns),
+    every product and sum rounded to dtype (numpy rounds each operation).
+    Columns of row i outside M stay zero.  Aval None = ones."""
+    dt = np.dtype(dtype)
+    Bd = np.asarray(B, dtype=dt)
+    m = len(Arp) - 1
+    out = np.zeros((m, Bd.shape[1]), dtype=dt)
+    for i in range(m):
+        cols = np.asarray(Mcol[Mrp[i]:Mrp[i + 1]], dtype=np.int64)
+        if not len(cols):
+            continue
+        acc = np.zeros(len(cols), dtype=dt)
+        for t in range(Arp[i], Arp[i + 1]):
+            a = dt.type(1) if Aval is None else dt.type(Aval[t])
+            acc = acc + a * Bd[Acol[t], cols]
+        out[i, cols] = acc
+    return out
+
+
+def dense_fuse_mult_add(A, B, Cm):
+    """fuseEWiseMultAdd (include/matrix_la.h:192-198): C + A .* B, the
+    product and the sum each rounded to C's type."""
+    dt = np.asarray(Cm).dtype
+    return np.asarray(Cm, dtype=dt) + np.asarray(A, dtype=dt) * np.asarray(B, dtype=dt)
+
+
+def dense_mult_masked(A, B, Mrp, Mcol):
+    """The dense eWiseMult (include/matrix_la.h:200-219): C zeroed, then
+    C(i,j) = A(i,j) * B(i,j) at M's entries."""
+    A = np.asarray(A)
+    out = np.zeros_like(A)
+    rows = np.repeat(np.arange(len(Mrp) - 1), np.diff(np.asarray(Mrp, dtype=np.int64)))
+    cols = np.asarray(Mcol, dtype=np.int64)
+    out[rows, cols] = A[rows, cols] * np.asarray(B, dtype=A.dtype)[rows, cols]
+    return out
+
+
 def scale(val, alpha, dtype=np.int64):
     """multScalar (include/matrix_la.h:535-541): v * alpha in FP64, converted
     back to the value type toward zero."""

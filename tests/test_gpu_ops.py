@@ -33,7 +33,8 @@ def _cmask(N, A, B, M, dtype):
     m = A.nrows
     r = np.ctypeslib.as_array(rp, shape=(m + 1,)).copy()
     c = np.ctypeslib.as_array(col, shape=(max(nz, 1),))[:nz].copy()
-    vp = C.cast(val, C.POINTER(C.c_double if dtype == N.GA_FLOAT64 else C.c_int64))
+    ct = {N.GA_FLOAT64: C.c_double, N.GA_FLOAT32: C.c_float, N.GA_INT64: C.c_int64, N.GA_INT32: C.c_int32}[dtype]
+    vp = C.cast(val, C.POINTER(ct))
     v = np.ctypeslib.as_array(vp, shape=(max(nz, 1),))[:nz].copy()
     for p in (rp, col, val):
         N.galib().ga_free_host(C.cast(p, C.c_void_p))
@@ -86,6 +87,88 @@ def test_spd_vs_oracle_bit_exact(gpu, G, oracle):
     assert np.array_equal(out, want)  # same accumulation order as the reference
 
 
+def test_float32_masked_product(gpu, G, oracle):
+    """C<M> = A*B in FP32 (the reference's spMtx<float>): exact on
+    integer-valued operands; within FP32 rounding of the FP64 oracle on
+    random values (sums formed in the kernel's order)."""
+    g = G.rmat(12, 16, seed=31)
+    rng = np.random.default_rng(8)
+    vi = rng.integers(-3, 4, size=g.nnz).astype(np.float32)
+    got = gpu.mxm_masked(g, g, g, dtype=np.float32, Aval=vi, Bval=vi)
+    want = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col,
+                             Aval=vi.astype(np.float64), Bval=vi.astype(np.float64), dtype=np.float64)
+    assert got.dtype == np.float32 and np.array_equal(got.astype(np.float64), want)
+    vr = rng.random(g.nnz).astype(np.float32)
+    got = gpu.mxm_masked(g, g, g, dtype=np.float32, Aval=vr, Bval=vr)
+    v64 = vr.astype(np.float64)
+    want = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col, Aval=v64, Bval=v64,
+                             dtype=np.float64)
+    # bound: FP32 rounding of a sum of nonnegative terms, relative 1e-5 of the sum
+    assert np.allclose(got.astype(np.float64), want, rtol=1e-5, atol=1e-6)
+
+
+def test_float32_cmask_and_ewise(gpu, G, oracle):
+    from graphalgo_amd import _native as N
+    rng = np.random.default_rng(9)
+    m, k, n = 120, 90, 150
+    Arp, Acol, Aval = _rand_csr(rng, m, k, 0.06)
+    Brp, Bcol, Bval = _rand_csr(rng, k, n, 0.06)
+    Mrp, Mcol, _ = _rand_csr(rng, m, n, 0.2, values=False)
+    want = oracle.mxm_cmask(m, n, Arp, Acol, Brp, Bcol, Mrp, Mcol, Aval=Aval, Bval=Bval)
+    a32, b32 = Aval.astype(np.float32), Bval.astype(np.float32)
+    got = _cmask(N, _view(N, Arp, Acol, m, k, a32), _view(N, Brp, Bcol, k, n, b32), _view(N, Mrp, Mcol, m, n),
+                 N.GA_FLOAT32)
+    # integer-valued operands: FP32 sums are exact
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    assert np.array_equal(got[2].astype(np.float64), want[2])
+    g = G.rmat(10, 8, seed=3)
+    h = G.rmat(10, 8, seed=4)
+    va = rng.random(g.nnz).astype(np.float32)
+    vb = rng.random(h.nnz).astype(np.float32)
+    P, pv = gpu.ewise_add(g, va, h, vb, dtype=np.float32)
+    wrp, wcol, wval = oracle.ewise(oracle.EW_ADD, g.n, g.rowptr, g.col, va, h.rowptr, h.col, vb, dtype=np.float32)
+    assert np.array_equal(P.rowptr, wrp) and np.array_equal(P.col, wcol) and np.array_equal(pv, wval)
+    P, pv = gpu.ewise_mult(g, va, h, vb, dtype=np.float32)
+    wrp, wcol, wval = oracle.ewise(oracle.EW_MULT, g.n, g.rowptr, g.col, va, h.rowptr, h.col, vb, dtype=np.float32)
+    assert np.array_equal(P.col, wcol) and np.array_equal(pv, wval)
+    assert np.array_equal(gpu.mult_scalar(g, va, 0.75, dtype=np.float32)[: g.nnz],
+                          oracle.scale(va, 0.75, dtype=np.float32))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_spd_sequential_bit_exact(gpu, G, oracle, dt):
+    """mxmm_spd in FP32 (include/betweenness_centrality.h:210) and FP64:
+    bit-identical to the sequential restatement (A's row in CSR order,
+    product and sum rounded separately)."""
+    g = G.rmat(11, 8, seed=6)
+    rng = np.random.default_rng(11)
+    ncols = 64
+    B = (1.0 / (1.0 + rng.integers(0, 11, size=(g.n, ncols)))).astype(dt)
+    Aval = rng.random(g.nnz).astype(dt)
+    Mrp, Mcol, _ = _rand_csr(rng, g.n, ncols, 0.3, values=False)
+    M = G.CSR(g.n, Mrp, Mcol)
+    got = gpu.mxm_spd(g, B, M, Aval=Aval, dtype=dt)
+    want = oracle.mxm_spd_seq(g.rowptr, g.col, Aval, B, Mrp, Mcol, dtype=dt)
+    assert got.dtype == np.dtype(dt) and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64, np.int32, np.int64])
+def test_dense_elementwise_ops(gpu, G, oracle, dt):
+    """fuseEWiseMultAdd and the masked dense eWiseMult (the BC backward
+    sweep's operators, include/matrix_la.h:192-219) on an R-MAT 12 x 256
+    layout: bit-identical to the oracle in every value type."""
+    rng = np.random.default_rng(12)
+    m, n = 4096, 256
+    if np.dtype(dt).kind == "f":
+        A, B, Cm = (rng.random((m, n)).astype(dt) for _ in range(3))
+    else:
+        A, B, Cm = (rng.integers(-1000, 1000, size=(m, n)).astype(dt) for _ in range(3))
+    Mrp, Mcol, _ = _rand_csr(rng, m, n, 0.1, values=False)
+    M = G.CSR(m, Mrp, Mcol)
+    assert np.array_equal(gpu.fuse_ewise_mult_add(A, B, Cm), oracle.dense_fuse_mult_add(A, B, Cm))
+    assert np.array_equal(gpu.ewise_mult_dense_masked(A, B, M), oracle.dense_mult_masked(A, B, Mrp, Mcol))
+
+
 def test_cpp_dropin_client(gpu, golden, small_graphs, oracle):
     exe = ROOT / "tests" / "cpp" / "_build" / "dropin"
     if not exe.exists():
@@ -119,3 +202,61 @@ def test_cpp_dropin_client(gpu, golden, small_graphs, oracle):
     got = np.array(out["bc_full"])
     assert got.shape == want.shape
     assert np.allclose(got, want, rtol=1e-9, atol=1e-9 * max(1.0, float(want.max())) * 1e-3)
+    # k_truss through a pointer of the reference's own type (src/main.cpp:90)
+    assert out["k5_nnz"] == len(oracle.ktruss(g, 5)[1])
+    # float instantiations and the dense operators (same inputs as the client builds)
+    assert out["sqf_sum"] == out["sq_sum"]
+    n, nc = g.n, 8
+    i = np.arange(n)[:, None]
+    j = np.arange(nc)[None, :]
+    Bd = (np.float32(1) / (1 + (7 * i + 3 * j) % 11).astype(np.float32)).astype(np.float32)
+    Xd = (np.float32(1) / (1 + (5 * i + j) % 13).astype(np.float32)).astype(np.float32)
+    c1, c2 = np.arange(n) % nc, (np.arange(n) // nc) % nc
+    cols = [sorted({a, b}) for a, b in zip(c1, c2)]
+    Mrp = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int32)
+    Mcol = np.concatenate(cols).astype(np.int32)
+    spd = oracle.mxm_spd_seq(g.rowptr, g.col, None, Bd, Mrp, Mcol, dtype=np.float32)
+    assert np.array_equal(np.array(out["spd_f32"], dtype=np.float32).reshape(n, nc), spd)
+    wm = oracle.dense_mult_masked(Bd, Xd, Mrp, Mcol)
+    assert np.array_equal(np.array(out["wmul_f32"], dtype=np.float32).reshape(n, nc), wm)
+    assert np.array_equal(np.array(out["fma_f32"], dtype=np.float32).reshape(n, nc),
+                          oracle.dense_fuse_mult_add(wm, Bd, Xd))
+    b64 = Bd.astype(np.float64)
+    assert np.array_equal(np.array(out["fma_f64"]).reshape(n, nc),
+                          oracle.dense_fuse_mult_add(b64, b64, Xd.astype(np.float64)))
+
+
+def _write_reference_bin(path, g):
+    """The reference's binary CSR cache layout (include/matrix.h:259-277):
+    4-byte matcode, size_t m, n, nz, int Rst[m+1], int Col[nz], int Val[nz]."""
+    with open(path, "wb") as f:
+        f.write(b"MCPS")
+        f.write(np.array([g.n, g.n, g.nnz], dtype=np.uint64).tobytes())
+        f.write(np.ascontiguousarray(g.rowptr, dtype=np.int32).tobytes())
+        f.write(np.ascontiguousarray(g.col, dtype=np.int32).tobytes())
+        f.write(np.ones(g.nnz, dtype=np.int32).tobytes())
+
+
+def test_cpp_bin_cache_roundtrip_rmat16(gpu, G, golden, tmp_path):
+    """R-MAT 16 through the bin cache of the C++ drop-in headers: read with
+    spMtx(path, "bin"), written back with write_crs_to_bin (byte-identical
+    file), read again; triangle count and per-entry truss numbers of the
+    copy equal the reference's (golden count and SHA-256)."""
+    import hashlib
+    exe = ROOT / "tests" / "cpp" / "_build" / "dropin"
+    if not exe.exists():
+        pytest.fail("tests/cpp/_build/dropin not built (run __graft_entry__.build())")
+    rec = golden["generated"]["rmat16_ef16_s1"]
+    g = G.rmat(**rec["cfg"])
+    src, tmp, tr = tmp_path / "g.bin", tmp_path / "copy.bin", tmp_path / "truss.bin"
+    _write_reference_bin(src, g)
+    print("case bin cache: rmat16 round trip", flush=True)
+    r = subprocess.run([str(exe), "--bin", str(src), str(tmp), str(tr)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["same"] == 1 and (out["n"], out["nnz"]) == (g.n, g.nnz)
+    assert src.read_bytes() == tmp.read_bytes()
+    assert out["tc"] == rec["tc"]
+    truss = np.fromfile(tr, dtype=np.int32)
+    assert out["kmax"] == rec["truss"]["kmax"]
+    assert hashlib.sha256(truss.tobytes()).hexdigest() == rec["truss"]["truss_sha256"]

@@ -453,3 +453,54 @@ def test_scale(oracle):
     assert np.array_equal(oracle.scale([7, -7, 3, 0], 0.5), [3, -3, 1, 0])
     assert np.array_equal(oracle.scale([7, -7], -1.5, dtype=np.int32), [-10, 10])
     assert np.allclose(oracle.scale([0.25, -2.0], 3.0, dtype=np.float64), [0.75, -6.0], rtol=0, atol=0)
+
+
+# ---------------------------------------------------------------- float32 and dense operators
+def test_spd_sequential_restatement(G, oracle):
+    """mxm_spd_seq (numpy, any value type) against the C oracle's FP64
+    mxmm_spd: bit-identical in FP64 (both sum A's row in CSR order with
+    separate roundings); in FP32 exact on integer-valued operands and within
+    FP32 rounding of the FP64 result otherwise."""
+    g = G.rmat(9, 8, seed=23)
+    rng = np.random.default_rng(2)
+    ncols = 6
+    M = rng.random((g.n, ncols)) < 0.5
+    Mrp = np.concatenate([[0], np.cumsum(M.sum(1))]).astype(np.int32)
+    Mcol = np.nonzero(M)[1].astype(np.int32)
+    Aval = rng.random(g.nnz)
+    B = rng.random((g.n, ncols))
+    want = oracle.mxm_spd(g.n, ncols, g.rowptr, g.col, B, Mrp, Mcol, Aval=Aval)
+    got = oracle.mxm_spd_seq(g.rowptr, g.col, Aval, B, Mrp, Mcol, dtype=np.float64)
+    assert np.array_equal(got, want)
+    Bi = rng.integers(-8, 9, size=(g.n, ncols)).astype(np.float64)
+    exact = oracle.mxm_spd(g.n, ncols, g.rowptr, g.col, Bi, Mrp, Mcol)
+    assert np.array_equal(oracle.mxm_spd_seq(g.rowptr, g.col, None, Bi, Mrp, Mcol, dtype=np.float32), exact)
+    f32 = oracle.mxm_spd_seq(g.rowptr, g.col, Aval, B, Mrp, Mcol, dtype=np.float32)
+    assert f32.dtype == np.float32
+    assert np.allclose(f32, want, rtol=2e-5, atol=1e-6)
+    assert not np.array_equal(f32.astype(np.float64), want)  # the FP32 rounding is really there
+
+
+def test_dense_elementwise_brute_force(oracle):
+    """fuseEWiseMultAdd and the masked dense eWiseMult against scalar loops
+    in the value type (each product and sum rounded separately)."""
+    rng = np.random.default_rng(3)
+    m, n = 7, 5
+    M = rng.random((m, n)) < 0.4
+    Mrp = np.concatenate([[0], np.cumsum(M.sum(1))]).astype(np.int32)
+    Mcol = np.nonzero(M)[1].astype(np.int32)
+    for dt in (np.float32, np.float64, np.int32, np.int64):
+        if np.dtype(dt).kind == "f":
+            A, B, Cm = (rng.random((m, n)).astype(dt) for _ in range(3))
+        else:
+            A, B, Cm = (rng.integers(-50, 50, size=(m, n)).astype(dt) for _ in range(3))
+        want = np.empty_like(Cm)
+        wmul = np.zeros_like(Cm)
+        for i in range(m):
+            for j in range(n):
+                p = A[i, j] * B[i, j]  # numpy scalars of type dt: rounded to dt
+                want[i, j] = Cm[i, j] + p
+                if M[i, j]:
+                    wmul[i, j] = p
+        assert np.array_equal(oracle.dense_fuse_mult_add(A, B, Cm), want), dt
+        assert np.array_equal(oracle.dense_mult_masked(A, B, Mrp, Mcol), wmul), dt
