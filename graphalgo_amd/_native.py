@@ -107,6 +107,7 @@ GA_SIGNATURES = {
     "ga_triangle_count": (C.c_int, [C.POINTER(GaCsr), I64P]),
     "ga_triangle_count_lu": (C.c_int, [C.POINTER(GaCsr), I64P]),
     "ga_triangle_count_vertex": (C.c_int, [C.POINTER(GaCsr), I64P]),
+    "ga_triangle_count_noalloc": (C.c_int, [C.POINTER(GaCsr), C.c_int64, C.c_void_p, C.c_void_p]),
     "ga_ktruss": (C.c_int, [C.POINTER(GaCsr), C.c_int, I32P, I32P, I64P, I32P]),
     "ga_truss_decomposition": (C.c_int, [C.POINTER(GaCsr), I32P, I32P, C.POINTER(GaTrussStats)]),
     "ga_bc_batch": (C.c_int, [C.POINTER(GaCsr), I32P, C.c_int64, F64P]),

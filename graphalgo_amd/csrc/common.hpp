@@ -233,6 +233,9 @@ struct OwnedCsr {
 };
 
 size_t dtype_size(int dtype);
+// resolves the device's copy of the allocation-free entry point's scratch
+// array (noalloc.hip); called by ga_init with the device current
+void noalloc_setup(int device);
 void validate_host_csr(const ga_csr* A, const char* name);
 // Longest row of a host CSR.
 inline int64_t host_max_row(const ga_csr* A) {

@@ -375,6 +375,7 @@ ga_status ga_init(int device) {
     c->prof_dev.ensure(64);
     GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
     GA_HIP(hipStreamSynchronize(c->stream));
+    noalloc_setup(device);
     g_ctx = c;
   });
 }

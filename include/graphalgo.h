@@ -138,6 +138,22 @@ ga_status ga_triangle_count_lu(const ga_csr* A, int64_t* triangles);
  * triangle_counting_vertex (src/main.cpp:14-37); int64 instead of int. */
 ga_status ga_triangle_count_vertex(const ga_csr* A, int64_t* per_vertex);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ga_triangle_count's result computed without any
+ * allocation.  L is the strict lower triangle of the symmetric pattern (the
+ * matrix the reference multiplies, src/main.cpp:67-73) in DEVICE memory;
+ * max_row bounds its longest row; *triangles is a DEVICE int64 (8-byte
+ * aligned) that receives T = sum of C<L> = L*L.  Runs on `stream` (a
+ * hipStream_t; NULL = the library stream) and returns without waiting.
+ * Every work buffer comes from one fixed device array of
+ * GA_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a graph
+ * whose buffers exceed it (about 2 million rows) is refused with
+ * GA_ERR_LIMIT before anything runs.  Calls that overlap on one GPU share
+ * that array and race; calls in stream order do not. */
+#define GA_NOALLOC_SCRATCH_BYTES (64ll << 20)
+ga_status ga_triangle_count_noalloc(const ga_csr* L, int64_t max_row, int64_t* triangles, void* stream);
+
 /* ---- k-truss ----------------------------------------------------------- */
 /* The k-truss of the symmetric pattern A: repeat { S<C> = C*C; keep S >= k-2 }
  * until nnz is unchanged.  Replaces k_truss (src/main.cpp:89-146).
