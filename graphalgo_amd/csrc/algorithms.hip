@@ -572,6 +572,382 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
   }
 }
 
+// ---- peeling by support updates -------------------------------------------
+// The reference recomputes every support, S<C> = C*C, after each filter
+// (src/main.cpp:98-129).  With C = C' + D (D the entries removed by a filter,
+// C' the survivors) the product splits by distributivity:
+//   (C*C)<C'> = (C'*C')<C'> + (D*C' + C'*D + D*D)<C'>,
+// so the supports of C' are the old supports minus, for each surviving entry,
+// the triangles of C that contain it and at least one entry of D.  A round
+// therefore lists the triangles through the entries it removes -- one
+// intersection of the two endpoint rows per removed edge -- and decrements the
+// supports of their surviving edges; every support stays exactly the one the
+// reference computes for the same graph, so the filters remove the same
+// entries in the same rounds and the pass counts are the reference's
+// (1 + the rounds that removed something).
+//
+// Per entry (both directions of an edge hold the same stamp):
+//   sup    the support, kept on the lower entry (row > col), the edge's
+//          canonical position; the upper entry's value is not maintained
+//   stamp  the round that removes the edge (PEEL_ALIVE while none does): a
+//          round r removes the edges stamped r; a stamp below r is an edge
+//          removed earlier, above r an edge still alive in round r.
+// A triangle of C is destroyed once, by its edge of least canonical position
+// among those stamped r; triangles with an edge removed earlier were
+// destroyed then.  The decrement that takes a support from thr to thr - 1
+// stamps the edge r + 1 and appends it to the next round's list, so rounds
+// never scan the graph: only a new k does (k_peel_scan, entry-parallel: an
+// entry is canonical when its transposed entry lies before it).  Dead
+// entries stay in the rows until a compaction at a k boundary drops them
+// (when they are at least half of the stored entries).
+constexpr int32_t PEEL_ALIVE = 0x7fffffff;
+
+struct PeelCtl {
+  long long bytes;   // algorithmic bytes of the destroy launches (accounting on)
+  int cnt[2];        // sizes of the two alternating removal lists
+  unsigned done;     // blocks of the running launch that finished (last one publishes)
+};
+
+// Wavefront-aggregated append; entries past cap are dropped (the count still
+// grows, and the host refuses a count above cap).
+__device__ inline void peel_push(bool sel, int32_t pos, int32_t* __restrict__ list, int* __restrict__ cnt, int cap,
+                                 int lane) {
+  const unsigned long long m = __ballot(sel);
+  if (!m) return;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, 0);
+  if (sel) {
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int at = base + __popcll(m & lt);
+    if (at < cap) list[at] = pos;
+  }
+}
+
+// The last workgroup of a launch to finish stores list `idx`'s size and a
+// sequence number (release order) to pinned host memory and re-arms `done`.
+__device__ inline void peel_finish_block(PeelCtl* __restrict__ ctl, int idx, long long* __restrict__ host,
+                                         long long seq) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(&ctl->done, 1u);
+    if (prev == gridDim.x - 1) {
+      __threadfence();
+      const int v = __hip_atomic_load(&ctl->cnt[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ctl->done = 0u;
+      __hip_atomic_store(&host[0], (long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&host[1], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
+// A new threshold: alive canonical entries with support below thr are
+// stamped r and listed (the list's count was cleared by the caller).
+__global__ __launch_bounds__(256) void k_peel_scan(const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
+                                                   int32_t* __restrict__ stamp, int64_t nnz, int thr, int r,
+                                                   int32_t* __restrict__ list, PeelCtl* __restrict__ ctl, int idx,
+                                                   int cap, long long* __restrict__ host, long long seq) {
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nnz; base += stride) {
+    const int64_t p = base + threadIdx.x;
+    bool sel = false;
+    if (p < nnz) {
+      const int m = mirror[p];
+      if (m < p && sup[p] < thr && stamp[p] == PEEL_ALIVE) {
+        sel = true;
+        stamp[p] = r;
+        stamp[m] = r;
+      }
+    }
+    peel_push(sel, (int32_t)p, list, &ctl->cnt[idx], cap, lane);
+  }
+  peel_finish_block(ctl, idx, host, seq);
+}
+
+// Round r: the triangles through each edge (i, j) of the list (i > j: row i
+// is the column of the transposed entry).  The shorter row is probed against
+// the longer one by binary search (wavefront per edge, a lane per probed
+// entry); each triangle this edge owns decrements the supports of its
+// surviving edges, and an edge pushed below thr is stamped r + 1 and joins
+// the next list.  The edge's own truss number is tval (decomposition).
+__global__ __launch_bounds__(256) void k_peel_destroy(const int32_t* __restrict__ cur, PeelCtl* __restrict__ ctl, int ic,
+                                                      const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                      const int32_t* __restrict__ mirror,
+                                                      const int32_t* __restrict__ eid, int32_t* __restrict__ sup,
+                                                      int32_t* __restrict__ stamp, int32_t* __restrict__ truss,
+                                                      int tval, int thr, int r, int32_t* __restrict__ next, int cap,
+                                                      int account, long long* __restrict__ host, long long seq) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wpb = blockDim.x >> 6;
+  const int nc = ctl->cnt[ic];
+  int* ncnt = &ctl->cnt[ic ^ 1];
+  long long bytes = 0;
+  for (int64_t t = blockIdx.x * wpb + (threadIdx.x >> 6); t < nc; t += (int64_t)gridDim.x * wpb) {
+    const int c = cur[t];
+    const int mc = mirror[c];
+    const int j = col[c], i = col[mc];
+    if (truss && lane == 0) {
+      truss[eid[c]] = tval;
+      truss[eid[mc]] = tval;
+    }
+    const int ib = rp[i], ie = rp[i + 1], jb = rp[j], je = rp[j + 1];
+    const bool pi = (ie - ib) <= (je - jb);  // probe row i
+    const int ab = pi ? ib : jb, ae = pi ? ie : je, arow = pi ? i : j;
+    const int bb = pi ? jb : ib, be = pi ? je : ie, brow = pi ? j : i;
+    if (account && lane == 0) bytes += 28 + 4ll * ((ie - ib) + (je - jb));
+    for (int q0 = ab; q0 < ae; q0 += 64) {
+      const int q = q0 + lane;
+      bool push_a = false, push_b = false;
+      int cid_a = 0, cid_b = 0;
+      if (q < ae) {
+        const int w = col[q];
+        const int sa = stamp[q];
+        if (sa >= r && w != brow) {
+          int lo = bb, hi = be;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (col[mid] < w) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo < be && col[lo] == w) {
+            const int sb = stamp[lo];
+            if (sb >= r) {
+              cid_a = w < arow ? q : mirror[q];
+              cid_b = w < brow ? lo : mirror[lo];
+              if ((sa != r || c < cid_a) && (sb != r || c < cid_b)) {
+                if (account) bytes += 16;
+                if (sa > r && atomicSub(&sup[cid_a], 1) == thr) {
+                  push_a = true;
+                  stamp[q] = r + 1;
+                  stamp[mirror[q]] = r + 1;
+                }
+                if (sb > r && atomicSub(&sup[cid_b], 1) == thr) {
+                  push_b = true;
+                  stamp[lo] = r + 1;
+                  stamp[mirror[lo]] = r + 1;
+                }
+              }
+            }
+          }
+        }
+      }
+      peel_push(push_a, cid_a, next, ncnt, cap, lane);
+      peel_push(push_b, cid_b, next, ncnt, cap, lane);
+    }
+  }
+  if (account) {
+    bytes = wave_reduce_sum(bytes);
+    if (lane == 0 && bytes) atomicAdd((unsigned long long*)&ctl->bytes, (unsigned long long)bytes);
+  }
+  peel_finish_block(ctl, ic ^ 1, host, seq);
+}
+
+// Entries removed at any round lose their support value (-1), so a
+// compaction at threshold 0 keeps exactly the alive ones.
+__global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __restrict__ sup, int64_t nnz) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
+    if (stamp[p] != PEEL_ALIVE) sup[p] = -1;
+}
+
+__global__ void k_peel_ctl_init(PeelCtl* __restrict__ ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->bytes = 0;
+    ctl->cnt[0] = ctl->cnt[1] = 0;
+    ctl->done = 0u;
+  }
+}
+
+__global__ void k_peel_cnt_zero(PeelCtl* __restrict__ ctl, int idx) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl->cnt[idx] = 0;
+}
+
+static hipEvent_t peel_event(Context& c) {
+  hipEvent_t ev;
+  if (!c.event_pool.empty()) {
+    ev = c.event_pool.back();
+    c.event_pool.pop_back();
+  } else {
+    GA_HIP(hipEventCreate(&ev));
+  }
+  return ev;
+}
+
+// Peeling with support updates (GA_OPT_TRUSS_LOOP 3): the buffers and the
+// first graph are set up by truss_loop; arguments and results as there.
+static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s,
+                       int* cur_out, int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st, int64_t* nnz_at,
+                       int32_t* passes_at, const KDone* on_k) {
+  const int64_t n = A.nrows, nnz0 = A.nnz;
+  Context& c = ctx();
+  const int64_t lcap = nnz0 / 2 + 64;  // canonical entries: every list fits
+  tw.plist.ensure(sizeof(int32_t) * 2 * lcap);
+  tw.pcnt.ensure(sizeof(PeelCtl));
+  tw.stamp.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
+  int32_t* lists[2] = {tw.plist.as<int32_t>(), tw.plist.as<int32_t>() + lcap};
+  auto* ctl = tw.pcnt.as<PeelCtl>();
+  int32_t* stamp = tw.stamp.as<int32_t>();
+  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(64), 0, s, ctl);
+  fill_u32(stamp, (uint32_t)PEEL_ALIVE, nnz0, s);
+  const bool account = c.prof_bytes;
+  // the first supports: one full product of the input graph
+  int cur = 0;
+  {
+    const DCsr C{n, n, nnz0, tw.cur[0].rp.as<int32_t>(), tw.cur[0].col.as<int32_t>(), nullptr};
+    tw.mw.balance = false;
+    support_symmetric_dev(C, tw.mirror[0].as<int32_t>(), tw.sup[0].as<int32_t>(), tw.mw, s, false);
+  }
+  int64_t stored = nnz0;  // entries in the current buffers (alive or not)
+  int64_t alive = nnz0;   // alive entries (both directions)
+  int64_t passes = (all_k && nnz0 == 0) ? 0 : 1, steps = 0;  // a decomposition of no entries runs no product
+  int32_t kmax = 2;
+  std::vector<int32_t> kv(ks, ks + (all_k ? 0 : nk));
+  std::vector<int64_t> nz_v(kv.size(), 0);
+  std::vector<int32_t> ps_v(kv.size(), 0);
+  int kidx = 0;
+  int k = all_k ? 3 : kv[0];
+  long long* hw_dev = nullptr;
+  long long* hw = progress_words(&hw_dev) + 12;  // words 12 (value) and 13 (sequence)
+  hw_dev += 12;
+  auto wait_count = [&](long long seq) -> int64_t {
+    wait_host_word(&hw[1], seq, s);
+    const int64_t v = __atomic_load_n(&hw[0], __ATOMIC_RELAXED);
+    GA_REQUIRE(v >= 0 && v <= lcap, GA_ERR_STATE, "peeling list overflow");
+    return v;
+  };
+  const unsigned cap = (unsigned)(8 * (int64_t)c.num_cu);
+  int ic = 0;   // the list of the next round to run
+  int r = 0;    // round number (stamps)
+  auto scan = [&](int thr) -> int64_t {
+    hipLaunchKernelGGL(k_peel_cnt_zero, dim3(1), dim3(64), 0, s, ctl, ic);
+    const long long seq = next_host_seq();
+    hipLaunchKernelGGL(k_peel_scan, dim3(std::min<unsigned>(grid_for(stored, 256, 1 << 20), cap)), dim3(256), 0, s,
+                       (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.sup[cur].as<int32_t>(), stamp,
+                       stored, thr, r + 1, lists[ic], ctl, ic, (int)lcap, hw_dev, seq);
+    GA_HIP(hipGetLastError());
+    return wait_count(seq);
+  };
+  auto compact = [&]() {
+    // drop every entry that is not alive, supports carried over; every entry
+    // left is alive and unstamped (a k boundary: no round is pending)
+    hipLaunchKernelGGL(k_peel_drop_dead, dim3(grid_for(stored, 256, 8192)), dim3(256), 0, s, (const int32_t*)stamp,
+                       tw.sup[cur].as<int32_t>(), stored);
+    int32_t* kept_dev = tw.kept.as<int32_t>();
+    long long* bsum = tw.scan_tmp.as<long long>();
+    const int64_t nbk = ceil_div(stored, KEEP_TILE);
+    hipLaunchKernelGGL(k_keep_tile_sums, dim3((unsigned)nbk), dim3(KEEP_T), 0, s,
+                       (const int32_t*)tw.sup[cur].as<int32_t>(), stored, 0, bsum, (const TrussDevState*)nullptr,
+                       (const int32_t*)nullptr, (int32_t*)nullptr);
+    const bool offsets = nbk > KEEP_FUSED_TILES;
+    if (offsets) scan_tile_offsets(bsum, nbk, kept_dev, s);
+    hipLaunchKernelGGL(k_keep_scatter, dim3((unsigned)nbk), dim3(KEEP_T), 0, s,
+                       (const int32_t*)tw.cur[cur].col.as<int32_t>(), (const int32_t*)tw.eid[cur].as<int32_t>(),
+                       (const int32_t*)tw.sup[cur].as<int32_t>(), stored, 0, (const long long*)bsum, offsets ? 1 : 0,
+                       tw.cur[cur ^ 1].col.as<int32_t>(), tw.eid[cur ^ 1].as<int32_t>(), tw.nold.as<int32_t>(),
+                       tw.newpos.as<int32_t>(), tw.sup[cur ^ 1].as<int32_t>(), 1, kept_dev,
+                       (const TrussDevState*)nullptr, (long long*)nullptr, 0ll);
+    hipLaunchKernelGGL(k_keep_finish, dim3(grid_for(std::max<int64_t>(n + 1, stored), 256, 8192)), dim3(256), 0, s,
+                       (const int32_t*)tw.cur[cur].rp.as<int32_t>(), n, (const int32_t*)tw.newpos.as<int32_t>(),
+                       stored, (const int32_t*)kept_dev, tw.cur[cur ^ 1].rp.as<int32_t>(), (uint8_t*)nullptr,
+                       (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
+                       tw.mirror[cur ^ 1].as<int32_t>(), (const TrussDevState*)nullptr);
+    GA_HIP(hipGetLastError());
+    cur ^= 1;
+    stored = alive;
+    fill_u32(stamp, (uint32_t)PEEL_ALIVE, stored, s);
+  };
+  auto record = [&](int64_t value) {
+    nz_v[kidx] = value;
+    ps_v[kidx] = (int32_t)passes;
+    if (!on_k) return;
+    std::vector<int32_t> more;
+    (*on_k)(kv[kidx], value, (int32_t)passes, &more);
+    for (int32_t x : more) {
+      if (x <= kv[kidx]) continue;
+      auto itx = std::lower_bound(kv.begin() + kidx + 1, kv.end(), x);
+      if (itx != kv.end() && *itx == x) continue;
+      const auto at = itx - kv.begin();
+      kv.insert(itx, x);
+      nz_v.insert(nz_v.begin() + at, 0);
+      ps_v.insert(ps_v.begin() + at, 0);
+    }
+  };
+  int64_t ncur = alive > 0 ? scan(k - 2) : 0;
+  const int64_t max_rounds = nnz0 / 2 + n + (int64_t)kv.size() + 64;  // a round removes an edge or moves k on
+  int64_t rounds = 0;
+  while (true) {
+    GA_REQUIRE(alive >= 0 && ++rounds <= max_rounds && k <= n + 3, GA_ERR_STATE, "peeling did not converge");
+    if (ncur == 0) {  // the k-truss is stable: one filter step that removed nothing
+      if (all_k && alive == 0) break;
+      ++steps;
+      if (all_k) {
+        kmax = k;
+        ++k;
+      } else {
+        record(alive);
+        if (++kidx == (int)kv.size()) break;
+        k = kv[kidx];
+      }
+      if (alive == 0) {
+        ncur = 0;
+        continue;
+      }
+      if (2 * alive <= stored) compact();
+      ncur = scan(k - 2);
+      continue;
+    }
+    // round r + 1: the listed edges go, their triangles with them
+    ++r;
+    hipLaunchKernelGGL(k_peel_cnt_zero, dim3(1), dim3(64), 0, s, ctl, ic ^ 1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c.prof_events_on) {
+      e0 = peel_event(c);
+      e1 = peel_event(c);
+      GA_HIP(hipEventRecord(e0, s));
+    }
+    if (account) c.prof_bytes_launches += 1;
+    const long long seq = next_host_seq();
+    hipLaunchKernelGGL(k_peel_destroy, dim3(std::min<unsigned>(grid_for(ncur, 4, 1 << 20), cap)), dim3(256), 0, s,
+                       (const int32_t*)lists[ic], ctl, ic, (const int32_t*)tw.cur[cur].rp.as<int32_t>(),
+                       (const int32_t*)tw.cur[cur].col.as<int32_t>(), (const int32_t*)tw.mirror[cur].as<int32_t>(),
+                       (const int32_t*)tw.eid[cur].as<int32_t>(), tw.sup[cur].as<int32_t>(), stamp,
+                       all_k ? tw.truss.as<int32_t>() : nullptr, k - 1, k - 2, r, lists[ic ^ 1], (int)lcap,
+                       account ? 1 : 0, hw_dev, seq);
+    GA_HIP(hipGetLastError());
+    if (c.prof_events_on) {
+      GA_HIP(hipEventRecord(e1, s));
+      c.prof_events.emplace_back(e0, e1);
+      c.prof_launches += 1;
+    }
+    alive -= 2 * ncur;
+    ++steps;
+    if (!all_k || alive > 0) ++passes;  // a decomposition stops without a product once the graph is empty
+    ic ^= 1;
+    ncur = wait_count(seq);
+  }
+  if (cur_out && stored != alive) compact();
+  if (account) {
+    PeelCtl h{};
+    GA_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    c.prof_bytes_host += h.bytes;
+  }
+  GA_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < (all_k ? 0 : nk); ++i) {  // the caller's k values (joined ones went to on_k)
+    const auto at = std::lower_bound(kv.begin(), kv.end(), ks[i]) - kv.begin();
+    if (nnz_at) nnz_at[i] = nz_v[at];
+    if (passes_at) passes_at[i] = ps_v[at];
+  }
+  if (cur_out) *cur_out = cur;
+  if (nnz_out) *nnz_out = alive;
+  if (kmax_out) *kmax_out = kmax;
+  if (st) {
+    st->passes = passes;
+    st->steps = steps;
+  }
+}
+
 // Runs the peeling loop on the device-resident graph A.  all_k: truss
 // decomposition (truss[] per entry of A); otherwise k-truss for one k, and
 // the surviving pattern is left in tw.cur[*cur_out].
@@ -612,7 +988,13 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   GA_REQUIRE(all_k || (ks && nk >= 1), GA_ERR_ARG, "no k given");
   {
     Context& c = ctx();
-    const int mode = c.truss_loop >= 0 ? c.truss_loop : (nnz0 <= c.truss_device_max_nnz ? 1 : 0);
+    const int mode = c.truss_loop >= 0 ? c.truss_loop : 3;
+    if (mode == 3) {
+      truss_peel(A, all_k, ks, nk, tw, s, cur_out, nnz_out, kmax_out, st, nnz_at, passes_at, on_k);
+      if (st)
+        st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+      return;
+    }
     if (mode > 0 && !selective_on(c, nnz0) && !c.prof_bytes) {
       std::vector<int64_t> nz_at(all_k ? 0 : nk);
       std::vector<int32_t> ps_at(all_k ? 0 : nk);
@@ -790,6 +1172,8 @@ size_t ktruss_workspace_bytes(int64_t n, int64_t nnz) {
   b += lease_bytes(sizeof(long long) * std::max<int64_t>(ceil_div(std::max<int64_t>(nnz, 1), KEEP_TILE), 1));
   b += mxm_workspace_bytes(n, nnz, true);
   b += sel_workspace_bytes(n, nnz);
+  b += lease_bytes(sizeof(int32_t) * 2 * (nnz / 2 + 64)) + lease_bytes(sizeof(PeelCtl));  // peeling lists
+  b += e;                                                                                   // stamps
   return b;
 }
 

@@ -397,6 +397,7 @@ struct TrussWork {
   OwnedCsr cur[2];
   DevBuf eid[2], mirror[2], sup[2];
   DevBuf cnt, scan_tmp, truss, kept, nold, newpos, dirty;
+  DevBuf plist, pcnt, stamp;  // peeling by support updates: removal lists, their counters, round stamps
   DevBuf dstate;  // device-driven loop: the peeling state (k, entries, passes, ...)
   MxmWork mw;
   SelWork sw;

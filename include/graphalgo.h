@@ -340,12 +340,15 @@ enum {
                                    endpoint change can affect (GA_SELECTIVE_SUPPORT): 1 = on,
                                    0 = off, -1 (default) = on for graphs of at least 2^22 stored
                                    entries when GA_OPT_SUPPORT_TRI is 0 */
-  GA_OPT_TRUSS_LOOP = 7,        /* k-truss peeling loop (GA_TRUSS_LOOP): 0 = the host reads the
-                                   survivor count after every filter step; 1 = device-driven
-                                   steps (state in device memory, a few steps queued ahead);
-                                   2 = the same steps replayed from a captured HIP graph;
-                                   -1 (default) = 1 for graphs of at most 2^18 stored entries,
-                                   else 0.  Selective support passes always use 0. */
+  GA_OPT_TRUSS_LOOP = 7,        /* k-truss peeling loop (GA_TRUSS_LOOP): 0 = every filter step
+                                   recomputes the supports, the host reads the survivor count
+                                   after each; 1 = the same as device-driven steps (state in
+                                   device memory, a few steps queued ahead); 2 = those steps
+                                   replayed from a captured HIP graph; 3 = supports computed
+                                   once and then updated: each round lists the triangles
+                                   through the entries it removes and decrements the supports
+                                   of their surviving edges; -1 (default) = 3.  Selective
+                                   support passes run with 0. */
   GA_OPT_SUPPORT_TRI = 8,       /* k-truss support passes as triangle listing, C<L> = L*L over
                                    the strict lower half with each hit added to its three edges
                                    (GA_SUPPORT_TRI): 1 = on, 0 = off, -1 (default) = on for

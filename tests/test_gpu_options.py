@@ -81,7 +81,7 @@ def test_truss_small_items(gpu, G, oracle, item_wedges):
     g = G.rmat(12, 16, seed=8)
     want, kmax, _, _, _ = oracle.truss_decomposition(g)
     print(f"case truss item_wedges={item_wedges}", flush=True)
-    with gpu.options(item_wedges=item_wedges):
+    with gpu.options(item_wedges=item_wedges, truss_loop=0):
         got, got_kmax, _ = gpu.truss_decomposition(g)
     assert got_kmax == kmax
     assert np.array_equal(got, want)
@@ -90,7 +90,7 @@ def test_truss_small_items(gpu, G, oracle, item_wedges):
 def test_truss_selective_support(gpu, G, oracle):
     g = G.rmat(12, 16, seed=9)
     want, kmax, _, _, _ = oracle.truss_decomposition(g)
-    with gpu.options(selective_support=1):
+    with gpu.options(selective_support=1, truss_loop=0):
         got, got_kmax, _ = gpu.truss_decomposition(g)
         R = gpu.Graph(g)
         nnz5 = R.ktruss(5)[0]
@@ -109,14 +109,18 @@ def test_large_graph_pass_kinds(gpu, G):
     ks = (3, 6, 12)
     print(f"case large graph pass kinds rmat18 nnz={g.nnz}", flush=True)
     R = gpu.Graph(g)
-    with gpu.options(selective_support=0, support_tri=0):
+    with gpu.options(selective_support=0, support_tri=0, truss_loop=0):
         truss, kmax, _ = R.truss()
         half = [R.ktruss(k) for k in ks]
-    with gpu.options(selective_support=1, support_tri=0):
+    with gpu.options(selective_support=1, support_tri=0, truss_loop=0):
         selective = [R.ktruss(k) for k in ks]
+    with gpu.options(truss_loop=0):
+        tri = [R.ktruss(k) for k in ks]
     default = [R.ktruss(k) for k in ks]
+    peel_truss, peel_kmax, _ = R.truss()
     R.close()
-    assert default == half == selective
+    assert default == half == selective == tri
+    assert peel_kmax == kmax and np.array_equal(peel_truss, truss)
     for k, (nnz, _) in zip(ks, default):
         assert nnz == int((truss >= k).sum())
 
@@ -131,7 +135,8 @@ def _truss_cases(G):
 @pytest.mark.parametrize("name", ["rmat12", "rmat14", "er1000", "path40", "empty16"])
 def test_truss_loops_identical(gpu, G, oracle, name):
     """Host-driven peeling (0), device-driven steps (1), steps replayed
-    from a captured HIP graph (2) and the size-based choice (-1): the oracle's truss numbers and support
+    from a captured HIP graph (2), peeling by support updates (3) and the
+    default (-1): the oracle's truss numbers and support
     pass count, and between the loops the same filter steps, k-truss sizes,
     pass counts and k-truss patterns (resident graph queried twice, so the
     captured step is reused)."""
@@ -139,7 +144,7 @@ def test_truss_loops_identical(gpu, G, oracle, name):
     want, kmax, opasses, _, _ = oracle.truss_decomposition(g)
     ks = sorted({3, 4, 5, max(kmax, 3), kmax + 1})
     res = {}
-    for loop in (0, 1, 2, -1):
+    for loop in (0, 1, 2, 3, -1):
         print(f"case truss loops {name} truss_loop={loop}", flush=True)
         with gpu.options(truss_loop=loop):
             t1, k1, s1 = gpu.truss_decomposition(g)
@@ -156,7 +161,7 @@ def test_truss_loops_identical(gpu, G, oracle, name):
         for k, (nnz, _) in zip(ks + ks, kt):
             assert nnz == int((want >= k).sum()), (k, nnz)
         res[loop] = (s1["steps"], kt, it, pat.rowptr.tolist(), pat.col.tolist())
-    assert res[0] == res[1] == res[2] == res[-1]
+    assert res[0] == res[1] == res[2] == res[3] == res[-1]
 
 
 @pytest.mark.parametrize("name", ["rmat12", "rmat14", "er1000", "path40", "empty16"])
@@ -195,7 +200,7 @@ def test_truss_support_tri_rmat16(gpu, G, golden):
 
 def test_option_validation(gpu):
     for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
-                      ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 3), ("truss_loop", -2),
+                      ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 4), ("truss_loop", -2),
                       ("support_tri", 2), ("serve_join", 3)):
         with pytest.raises(Exception):
             gpu.set_option(name, bad)

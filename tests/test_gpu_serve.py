@@ -185,7 +185,7 @@ def test_admission_by_footprint(gpu, G):
     fp_tc = probe.footprint({"kind": "tc", "graph": gid})
     fp_kt = probe.footprint({"kind": "ktruss", "graph": gid, "k": 4})
     probe.close()
-    assert fp_bc > fp_kt > fp_tc > 0
+    assert fp_bc > fp_tc > 0 and fp_kt > fp_tc
     # room for one BC workspace at a time beside the graph (4 KiB arena pages)
     budget = resident + fp_bc + fp_bc // 2 + (64 << 10)
     srv = gpu.Server(hbm_budget=budget, slots=4, max_batch=1)
