@@ -140,28 +140,36 @@ def cpu_baseline(name: str, args) -> dict:
 
 
 def _measure(step, steps: int, warmup: int):
-    """Runs `warmup` untimed steps, then `steps` timed steps with HIP events
-    around every masked-SpGEMM launch, then one extra untimed step with the
-    algorithmic byte accounting on (every step issues the same launches).
+    """Runs `warmup` untimed steps, then `steps` timed steps with no hooks
+    (seconds per step: the reported value), then `steps` more with HIP events
+    around every launch of the dominant kernel on the stream it runs on (the
+    roofline's kernel time), then one extra step with the algorithmic byte
+    accounting on (every step issues the same launches).
     Returns (seconds per step, last step result, timing, accounting)."""
     from graphalgo_amd import api
     for _ in range(warmup):
         step()
     _sync()
-    api.prof_enable(api.PROF_EVENTS)
-    api.prof_read(reset=True)
     t0 = time.perf_counter()
     out = None
     for _ in range(steps):
         out = step()
     _sync()
     dt = (time.perf_counter() - t0) / steps
+    api.prof_enable(api.PROF_EVENTS)
+    api.prof_read(reset=True)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    _sync()
+    dt_events = (time.perf_counter() - t1) / steps
     timing = api.prof_read(reset=True)
     api.prof_enable(api.PROF_BYTES)
     step()
     acct = api.prof_read(reset=True)
     api.prof_enable(0)
     timing["steps"] = steps
+    timing["ms_per_step_with_events"] = dt_events * 1e3
     return dt, out, timing, acct
 
 
@@ -196,6 +204,7 @@ def _roofline(timing: dict, acct: dict, kernel: str, traffic_key: str | None = N
         "kernel": kernel, "launches_per_step": launches,
         "avg_launch_us": round(kernel_s * 1e6 / launches, 2),
         "bytes_alg_per_launch": int(acct["bytes_alg"] / max(acct["accounted_launches"], 1)),
+        "ms_per_step_with_events": round(timing.get("ms_per_step_with_events", 0.0), 3),
         "wedges_per_step": acct["wedges"], "wedges_processed_per_step": acct["wedges_processed"],
     }
 

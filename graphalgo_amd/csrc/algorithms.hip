@@ -602,10 +602,26 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
 // (when they are at least half of the stored entries).
 constexpr int32_t PEEL_ALIVE = 0x7fffffff;
 
+// The peeling state in device memory.  Every kernel of a round reads it when
+// it starts; the last workgroup of the kernel to finish (every other one has
+// read it by then) applies the kernel's changes.
 struct PeelCtl {
   long long bytes;   // algorithmic bytes of the destroy launches (accounting on)
+  long long alive;   // alive entries (both directions)
+  long long passes;  // support products of the reference's schedule (1 + rounds that removed)
+  long long steps;   // filter steps (removal rounds + the step that finds a k-truss stable)
+  long long ticks;   // device-driven: advance kernels run (flow control)
   int cnt[2];        // sizes of the two alternating removal lists
-  unsigned done;     // blocks of the running launch that finished (last one publishes)
+  int adv_stable;    // device-driven: the last advance found the k-truss stable
+  int scnt;          // device-driven: entries the advance's scan listed (the post moves it into cnt)
+  int ic;            // list of the next round
+  int r;             // rounds run (a round r removes the edges stamped r)
+  int k;             // current k (threshold k - 2)
+  int kidx, nk, all_k, fin, kmax;
+  int err;           // a list outgrew its capacity: the run stopped
+  int ks[TRUSS_MAX_K];
+  int passes_at[TRUSS_MAX_K];
+  long long nnz_at[TRUSS_MAX_K];
 };
 
 // Wavefront-aggregated append; entries past cap are dropped (the count still
@@ -624,143 +640,363 @@ __device__ inline void peel_push(bool sel, int32_t pos, int32_t* __restrict__ li
   }
 }
 
-// The last workgroup of a launch to finish stores list `idx`'s size and a
-// sequence number (release order) to pinned host memory and re-arms `done`.
-__device__ inline void peel_finish_block(PeelCtl* __restrict__ ctl, int idx, long long* __restrict__ host,
-                                         long long seq) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(&ctl->done, 1u);
-    if (prev == gridDim.x - 1) {
-      __threadfence();
-      const int v = __hip_atomic_load(&ctl->cnt[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      ctl->done = 0u;
-      __hip_atomic_store(&host[0], (long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-      __hip_atomic_store(&host[1], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-  }
+// host words: [0] next list size, [1] alive, [2] fin, [3] ticks, [4] sequence (release)
+__device__ inline void peel_publish(const PeelCtl* __restrict__ ctl, long long* __restrict__ host, long long seq) {
+  if (!host) return;
+  __hip_atomic_store(&host[0], (long long)ctl->cnt[ctl->ic], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[1], ctl->alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[2], (long long)ctl->fin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[3], ctl->ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[4], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// A new threshold: alive canonical entries with support below thr are
-// stamped r and listed (the list's count was cleared by the caller).
-__global__ __launch_bounds__(256) void k_peel_scan(const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
-                                                   int32_t* __restrict__ stamp, int64_t nnz, int thr, int r,
-                                                   int32_t* __restrict__ list, PeelCtl* __restrict__ ctl, int idx,
-                                                   int cap, long long* __restrict__ host, long long seq) {
+// alive canonical entries with support below thr: stamped r, listed
+__device__ inline void peel_scan_body(const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
+                                      int32_t* __restrict__ stamp, int64_t nnz, int thr, int r,
+                                      int32_t* __restrict__ list, int* __restrict__ cnt, int cap) {
   const int lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nnz; base += stride) {
     const int64_t p = base + threadIdx.x;
     bool sel = false;
     if (p < nnz) {
-      const int m = mirror[p];
-      if (m < p && sup[p] < thr && stamp[p] == PEEL_ALIVE) {
+      const int m = mirror[p], sv = sup[p], sp = stamp[p];  // independent loads
+      if (m < p && sv < thr && sp == PEEL_ALIVE) {
         sel = true;
         stamp[p] = r;
         stamp[m] = r;
       }
     }
-    peel_push(sel, (int32_t)p, list, &ctl->cnt[idx], cap, lane);
+    peel_push(sel, (int32_t)p, list, cnt, cap, lane);
   }
-  peel_finish_block(ctl, idx, host, seq);
 }
 
-// Round r: the triangles through each edge (i, j) of the list (i > j: row i
-// is the column of the transposed entry).  The shorter row is probed against
-// the longer one by binary search (wavefront per edge, a lane per probed
-// entry); each triangle this edge owns decrements the supports of its
-// surviving edges, and an edge pushed below thr is stamped r + 1 and joins
-// the next list.  The edge's own truss number is tval (decomposition).
-__global__ __launch_bounds__(256) void k_peel_destroy(const int32_t* __restrict__ cur, PeelCtl* __restrict__ ctl, int ic,
-                                                      const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+// The scan of a new threshold (host-steered runs): k and the list from the
+// state; the list's count starts at zero (its round cleared it).
+__global__ __launch_bounds__(256) void k_peel_scan(const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
+                                                   int32_t* __restrict__ stamp, int64_t nnz,
+                                                   int32_t* __restrict__ list0, int32_t* __restrict__ list1,
+                                                   PeelCtl* __restrict__ ctl, int cap) {
+  const int ic = ctl->ic;
+  peel_scan_body(mirror, sup, stamp, nnz, ctl->k - 2, ctl->r + 1, ic ? list1 : list0, &ctl->cnt[ic], cap);
+}
+
+// The state after a round that removed nc edges (the list it consumed is
+// emptied by k_peel_post).
+struct PeelView {
+  int ic, r, fin;
+  long long alive, passes, steps;
+};
+__device__ inline PeelView peel_after_round(const PeelCtl* __restrict__ ctl, int nc) {
+  PeelView v{ctl->ic, ctl->r, ctl->fin, ctl->alive, ctl->passes, ctl->steps};
+  if (nc > 0) {
+    v.alive -= 2ll * nc;
+    v.steps += 1;
+    if (!ctl->all_k || v.alive > 0) v.passes += 1;  // a decomposition stops at an empty graph
+    v.r += 1;
+    v.ic ^= 1;
+  }
+  return v;
+}
+
+// Host-steered runs: the k to scan next (and the filter step that found the
+// previous one stable).
+__global__ void k_peel_set_k(PeelCtl* __restrict__ ctl, int k, int stable_step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->k = k;
+    ctl->steps += stable_step;
+  }
+}
+
+__global__ void k_peel_ctl_init(PeelCtl* __restrict__ ctl, long long alive, int all_k, int nk, int k0,
+                                long long passes) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->bytes = 0;
+    ctl->alive = alive;
+    ctl->passes = passes;
+    ctl->steps = 0;
+    ctl->ticks = 0;
+    ctl->cnt[0] = ctl->cnt[1] = 0;
+    ctl->adv_stable = 0;
+    ctl->scnt = 0;
+    ctl->ic = 0;
+    ctl->r = 0;
+    ctl->k = k0;
+    ctl->kidx = 0;
+    ctl->nk = nk;
+    ctl->all_k = all_k;
+    ctl->fin = alive == 0 && all_k;
+    ctl->kmax = 2;
+    ctl->err = 0;
+  }
+}
+
+// Round r = ctl->r + 1 removes the edges of list ic: the triangles through
+// each edge (i, j) of the list (i > j: row i is the column of the transposed
+// entry).  The shorter row is probed against the longer one: the longer row,
+// or every stride-th entry of it when it is longer than the LDS sample, is
+// staged in LDS; a probed column is found by a binary search of the sample
+// and, between two sampled entries, of at most stride - 1 entries in global
+// memory.  Each triangle the edge owns decrements the supports of its
+// surviving edges, and an edge pushed below thr is stamped r + 1 and joins the
+// next list.  The edge's own truss number is k - 1 (decomposition).
+struct PeelEdge {
+  int c, mc, ab, ae, arow, bb, be, brow, stride, m;
+};
+
+__device__ inline PeelEdge peel_edge(int c, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                     const int32_t* __restrict__ mirror, int cap) {
+  PeelEdge e;
+  e.c = c;
+  e.mc = mirror[c];
+  const int j = col[c], i = col[e.mc];
+  const int ib = rp[i], ie = rp[i + 1], jb = rp[j], je = rp[j + 1];
+  const bool pi = (ie - ib) <= (je - jb);  // probe row i
+  e.ab = pi ? ib : jb;
+  e.ae = pi ? ie : je;
+  e.arow = pi ? i : j;
+  e.bb = pi ? jb : ib;
+  e.be = pi ? je : ie;
+  e.brow = pi ? j : i;
+  const int tl = e.be - e.bb;
+  e.stride = tl > cap ? (tl + cap - 1) / cap : 1;
+  e.m = (tl + e.stride - 1) / e.stride;
+  return e;
+}
+
+// The same edge searched directly: the sample is the whole target row.
+__device__ inline PeelEdge peel_direct(PeelEdge e) {
+  e.stride = 1;
+  e.m = e.be - e.bb;
+  return e;
+}
+
+// One probed entry q of the edge's probe row: the decrements it owns and the
+// canonical positions to push (-1: none).
+__device__ inline void peel_probe(const PeelEdge& e, int q, const int32_t* __restrict__ st,
+                                  const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
+                                  int32_t* __restrict__ sup, int32_t* __restrict__ stamp, int thr, int r,
+                                  int& push_a, int& push_b, long long& found) {
+  push_a = push_b = -1;
+  const int w = col[q];
+  const int sa = stamp[q];
+  if (sa < r || w == e.brow) return;
+  int lo = 0, hi = e.m;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (st[mid] < w) lo = mid + 1;
+    else hi = mid;
+  }
+  int pos = -1;
+  if (lo < e.m && st[lo] == w) {
+    pos = e.bb + lo * e.stride;
+  } else if (e.stride > 1 && lo > 0) {
+    int glo = e.bb + (lo - 1) * e.stride + 1;
+    int ghi = lo < e.m ? e.bb + lo * e.stride : e.be;
+    while (glo < ghi) {
+      const int mid = (glo + ghi) >> 1;
+      if (col[mid] < w) glo = mid + 1;
+      else ghi = mid;
+    }
+    if (glo < e.be && col[glo] == w) pos = glo;
+  }
+  if (pos < 0) return;
+  const int sb = stamp[pos];
+  if (sb < r) return;
+  const int cid_a = w < e.arow ? q : mirror[q];
+  const int cid_b = w < e.brow ? pos : mirror[pos];
+  if ((sa == r && cid_a < e.c) || (sb == r && cid_b < e.c)) return;  // another removed edge owns it
+  ++found;
+  if (sa > r && atomicSub(&sup[cid_a], 1) == thr) {
+    push_a = cid_a;
+    stamp[q] = r + 1;
+    stamp[mirror[q]] = r + 1;
+  }
+  if (sb > r && atomicSub(&sup[cid_b], 1) == thr) {
+    push_b = cid_b;
+    stamp[pos] = r + 1;
+    stamp[mirror[pos]] = r + 1;
+  }
+}
+
+constexpr int PEEL_WSTAGE = 1024;         // sample entries per wavefront (large rounds)
+constexpr int PEEL_BSTAGE = 4096;         // sample entries per workgroup (small rounds)
+constexpr int PEEL_SPLIT = 8;             // workgroups per edge in small rounds
+constexpr int PEEL_SPLIT_MAX_EDGES = 64;  // rounds of at most this many edges split them
+constexpr int PEEL_GRID = PEEL_SPLIT * PEEL_SPLIT_MAX_EDGES;  // device-driven destroy grid
+
+// A round.  Large rounds: a wavefront per edge (a wavefront probing one chunk
+// searches the target row in global memory, a longer probe row stages the
+// sample first).  Small rounds (few edges, usually long rows): PEEL_SPLIT
+// workgroups per edge, each staging the sample once and probing every
+// PEEL_SPLIT-th run of 256 probe entries.  The last workgroup applies the
+// round to the state.
+__global__ __launch_bounds__(256) void k_peel_destroy(const int32_t* __restrict__ list0, int32_t* __restrict__ list1,
+                                                      PeelCtl* __restrict__ ctl, const int32_t* __restrict__ rp,
+                                                      const int32_t* __restrict__ col,
                                                       const int32_t* __restrict__ mirror,
                                                       const int32_t* __restrict__ eid, int32_t* __restrict__ sup,
-                                                      int32_t* __restrict__ stamp, int32_t* __restrict__ truss,
-                                                      int tval, int thr, int r, int32_t* __restrict__ next, int cap,
+                                                      int32_t* __restrict__ stamp, int32_t* __restrict__ truss, int cap,
                                                       int account, long long* __restrict__ host, long long seq) {
+  __shared__ int32_t s_stage[PEEL_BSTAGE];
   const int lane = threadIdx.x & 63;
-  const int64_t wpb = blockDim.x >> 6;
-  const int nc = ctl->cnt[ic];
+  const int ic = ctl->ic, nc = ctl->fin ? 0 : min(ctl->cnt[ic], cap);
+  (void)host;
+  (void)seq;
+  const int r = ctl->r + 1, thr = ctl->k - 2, tval = ctl->k - 1;
+  const int32_t* cur = ic ? list1 : list0;
+  int32_t* next = ic ? const_cast<int32_t*>(list0) : list1;
   int* ncnt = &ctl->cnt[ic ^ 1];
-  long long bytes = 0;
-  for (int64_t t = blockIdx.x * wpb + (threadIdx.x >> 6); t < nc; t += (int64_t)gridDim.x * wpb) {
-    const int c = cur[t];
-    const int mc = mirror[c];
-    const int j = col[c], i = col[mc];
-    if (truss && lane == 0) {
-      truss[eid[c]] = tval;
-      truss[eid[mc]] = tval;
-    }
-    const int ib = rp[i], ie = rp[i + 1], jb = rp[j], je = rp[j + 1];
-    const bool pi = (ie - ib) <= (je - jb);  // probe row i
-    const int ab = pi ? ib : jb, ae = pi ? ie : je, arow = pi ? i : j;
-    const int bb = pi ? jb : ib, be = pi ? je : ie, brow = pi ? j : i;
-    if (account && lane == 0) bytes += 28 + 4ll * ((ie - ib) + (je - jb));
-    for (int q0 = ab; q0 < ae; q0 += 64) {
-      const int q = q0 + lane;
-      bool push_a = false, push_b = false;
-      int cid_a = 0, cid_b = 0;
-      if (q < ae) {
-        const int w = col[q];
-        const int sa = stamp[q];
-        if (sa >= r && w != brow) {
-          int lo = bb, hi = be;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (col[mid] < w) lo = mid + 1;
-            else hi = mid;
-          }
-          if (lo < be && col[lo] == w) {
-            const int sb = stamp[lo];
-            if (sb >= r) {
-              cid_a = w < arow ? q : mirror[q];
-              cid_b = w < brow ? lo : mirror[lo];
-              if ((sa != r || c < cid_a) && (sb != r || c < cid_b)) {
-                if (account) bytes += 16;
-                if (sa > r && atomicSub(&sup[cid_a], 1) == thr) {
-                  push_a = true;
-                  stamp[q] = r + 1;
-                  stamp[mirror[q]] = r + 1;
-                }
-                if (sb > r && atomicSub(&sup[cid_b], 1) == thr) {
-                  push_b = true;
-                  stamp[lo] = r + 1;
-                  stamp[mirror[lo]] = r + 1;
-                }
-              }
-            }
-          }
-        }
+  long long bytes = 0, found = 0;
+  if (nc > PEEL_SPLIT_MAX_EDGES) {
+    int32_t* st = s_stage + (threadIdx.x >> 6) * PEEL_WSTAGE;
+    const int64_t wpb = blockDim.x >> 6;
+    for (int64_t t = blockIdx.x * wpb + (threadIdx.x >> 6); t < nc; t += (int64_t)gridDim.x * wpb) {
+      const PeelEdge e = peel_edge(cur[t], rp, col, mirror, PEEL_WSTAGE);
+      if (truss && lane == 0) {
+        truss[eid[e.c]] = tval;
+        truss[eid[e.mc]] = tval;
       }
-      peel_push(push_a, cid_a, next, ncnt, cap, lane);
-      peel_push(push_b, cid_b, next, ncnt, cap, lane);
+      if (account && lane == 0) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
+      const bool staged = e.ae - e.ab > 64;
+      const PeelEdge g = staged ? e : peel_direct(e);
+      __threadfence_block();  // the previous edge's sample reads are done
+      if (staged)
+        for (int u = lane; u < e.m; u += 64) st[u] = col[e.bb + u * e.stride];
+      __threadfence_block();
+      for (int q0 = e.ab; q0 < e.ae; q0 += 64) {
+        const int q = q0 + lane;
+        int pa = -1, pb = -1;
+        if (q < e.ae) peel_probe(g, q, staged ? st : col + e.bb, col, mirror, sup, stamp, thr, r, pa, pb, found);
+        peel_push(pa >= 0, pa, next, ncnt, cap, lane);
+        peel_push(pb >= 0, pb, next, ncnt, cap, lane);
+      }
+    }
+  } else {
+    for (int64_t b = blockIdx.x; b < (int64_t)nc * PEEL_SPLIT; b += gridDim.x) {
+      const int t = (int)(b / PEEL_SPLIT), part = (int)(b % PEEL_SPLIT);
+      const PeelEdge e = peel_edge(cur[t], rp, col, mirror, PEEL_BSTAGE);
+      if (part == 0 && threadIdx.x == 0) {
+        if (truss) {
+          truss[eid[e.c]] = tval;
+          truss[eid[e.mc]] = tval;
+        }
+        if (account) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
+      }
+      __syncthreads();  // the previous edge's sample reads are done
+      for (int u = threadIdx.x; u < e.m; u += blockDim.x) s_stage[u] = col[e.bb + u * e.stride];
+      __syncthreads();
+      for (int q0 = e.ab + part * 256; q0 < e.ae; q0 += PEEL_SPLIT * 256) {
+        const int q = q0 + threadIdx.x;
+        int pa = -1, pb = -1;
+        if (q < e.ae) peel_probe(e, q, s_stage, col, mirror, sup, stamp, thr, r, pa, pb, found);
+        peel_push(pa >= 0, pa, next, ncnt, cap, lane);
+        peel_push(pb >= 0, pb, next, ncnt, cap, lane);
+      }
     }
   }
   if (account) {
-    bytes = wave_reduce_sum(bytes);
+    bytes = wave_reduce_sum(bytes + 16 * found);
     if (lane == 0 && bytes) atomicAdd((unsigned long long*)&ctl->bytes, (unsigned long long)bytes);
   }
-  peel_finish_block(ctl, ic ^ 1, host, seq);
 }
 
-// Entries removed at any round lose their support value (-1), so a
-// compaction at threshold 0 keeps exactly the alive ones.
-__global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __restrict__ sup, int64_t nnz) {
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
-    if (stamp[p] != PEEL_ALIVE) sup[p] = -1;
-}
-
-__global__ void k_peel_ctl_init(PeelCtl* __restrict__ ctl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    ctl->bytes = 0;
-    ctl->cnt[0] = ctl->cnt[1] = 0;
-    ctl->done = 0u;
+// Device-driven runs: after each round, a list that came back empty means the
+// current k-truss is stable -- the new threshold is scanned here (every
+// workgroup derives the same next k from the state as the round left it);
+// k_peel_post then records the step and moves k on.  With a list still
+// pending this kernel does nothing.
+__device__ inline void peel_next_k(const PeelCtl* __restrict__ ctl, long long alive, int& k, int& kidx, int& fin) {
+  if (ctl->all_k) {
+    if (alive == 0) fin = 1;
+    else ++k;
+  } else {
+    if (++kidx == ctl->nk) fin = 1;
+    else k = ctl->ks[kidx];
   }
 }
 
-__global__ void k_peel_cnt_zero(PeelCtl* __restrict__ ctl, int idx) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) ctl->cnt[idx] = 0;
+__global__ __launch_bounds__(256) void k_peel_advance(const int32_t* __restrict__ mirror,
+                                                      const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
+                                                      int64_t nnz, int32_t* __restrict__ list0,
+                                                      int32_t* __restrict__ list1, PeelCtl* __restrict__ ctl, int cap) {
+  const int nc = ctl->fin ? 0 : min(ctl->cnt[ctl->ic], cap);
+  const PeelView v = peel_after_round(ctl, nc);
+  const bool stable = !v.fin && ctl->cnt[v.ic] == 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->adv_stable = stable ? 1 : 0;
+  if (!stable) return;
+  int k = ctl->k, kidx = ctl->kidx, fin = 0;
+  peel_next_k(ctl, v.alive, k, kidx, fin);
+  // the stability test above reads cnt[v.ic]: the scan counts into scnt
+  if (!fin && v.alive > 0)
+    peel_scan_body(mirror, sup, stamp, nnz, k - 2, v.r + 1, v.ic ? list1 : list0, &ctl->scnt, cap);
+}
+
+// One workgroup after the big kernels of a round: applies the round to the
+// state (mode 0), the round and the advance (mode 1: device-driven tick), or
+// only checks a host-steered scan (mode 2); then publishes to the host.
+__global__ void k_peel_post(PeelCtl* __restrict__ ctl, int mode, int cap, long long* __restrict__ host,
+                            long long seq) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (mode != 2) {
+    const int nc = ctl->fin ? 0 : min(ctl->cnt[ctl->ic], cap);
+    const PeelView v = peel_after_round(ctl, nc);
+    if (nc > 0) ctl->cnt[ctl->ic] = 0;  // consumed; the round after next fills it
+    ctl->alive = v.alive;
+    ctl->passes = v.passes;
+    ctl->steps = v.steps;
+    ctl->r = v.r;
+    ctl->ic = v.ic;
+    if (mode == 1 && ctl->adv_stable) {
+      if (ctl->all_k) {
+        if (v.alive > 0) {
+          ctl->kmax = ctl->k;
+          ctl->steps += 1;
+        }
+      } else {
+        ctl->nnz_at[ctl->kidx] = v.alive;
+        ctl->passes_at[ctl->kidx] = (int)v.passes;
+        ctl->steps += 1;
+      }
+      int k = ctl->k, kidx = ctl->kidx, fin = 0;
+      peel_next_k(ctl, v.alive, k, kidx, fin);
+      ctl->k = k;
+      ctl->kidx = kidx;
+      ctl->fin = fin;
+      ctl->cnt[v.ic] = ctl->scnt;  // the scanned list (cnt[v.ic] was empty: the truss was stable)
+    }
+    if (mode == 1) {
+      ctl->scnt = 0;
+      ctl->ticks += 1;
+    }
+  }
+  if (ctl->cnt[0] > cap || ctl->cnt[1] > cap) ctl->err = ctl->fin = 1;
+  peel_publish(ctl, host, seq);
+}
+
+// Entries of no round still pending or to come (stamp <= r) lose their
+// support value (-1), so a compaction at threshold 0 keeps exactly the alive
+// ones, the ones the pending round removes included.
+__global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __restrict__ sup, int64_t nnz,
+                                 const PeelCtl* __restrict__ ctl) {
+  const int r = ctl->r;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
+    if (stamp[p] <= r) sup[p] = -1;
+}
+
+// After a compaction: the pending list's entries move to their new positions
+// and keep their stamp r + 1 (both directions); every other stamp is reset.
+__global__ void k_peel_remap(int32_t* __restrict__ list0, int32_t* __restrict__ list1,
+                             const PeelCtl* __restrict__ ctl, const int32_t* __restrict__ newpos,
+                             const int32_t* __restrict__ mirror_new, int32_t* __restrict__ stamp) {
+  const int ic = ctl->ic, n = ctl->cnt[ic], r1 = ctl->r + 1;
+  int32_t* list = ic ? list1 : list0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int c = newpos[list[t]];
+    list[t] = c;
+    stamp[c] = r1;
+    stamp[mirror_new[c]] = r1;
+  }
 }
 
 static hipEvent_t peel_event(Context& c) {
@@ -774,21 +1010,32 @@ static hipEvent_t peel_event(Context& c) {
   return ev;
 }
 
-// Peeling with support updates (GA_OPT_TRUSS_LOOP 3): the buffers and the
-// first graph are set up by truss_loop; arguments and results as there.
+// Peeling with support updates (GA_OPT_TRUSS_LOOP 3 and 4): the buffers and
+// the first graph are set up by truss_loop; arguments and results as there.
+// host_steer (always with a k callback): the host reads each round's list
+// size, answers each k as it is reached and takes on the callback's extra k
+// values.  Otherwise the rounds are queued on the device a few ahead of it.
 static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s,
-                       int* cur_out, int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st, int64_t* nnz_at,
-                       int32_t* passes_at, const KDone* on_k) {
+                       bool host_steer, int* cur_out, int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st,
+                       int64_t* nnz_at, int32_t* passes_at, const KDone* on_k) {
   const int64_t n = A.nrows, nnz0 = A.nnz;
   Context& c = ctx();
+  host_steer = host_steer || on_k != nullptr;
+  GA_REQUIRE(all_k || host_steer || nk <= TRUSS_MAX_K, GA_ERR_LIMIT, "at most 64 k values in one k-truss run");
   const int64_t lcap = nnz0 / 2 + 64;  // canonical entries: every list fits
   tw.plist.ensure(sizeof(int32_t) * 2 * lcap);
   tw.pcnt.ensure(sizeof(PeelCtl));
   tw.stamp.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
-  int32_t* lists[2] = {tw.plist.as<int32_t>(), tw.plist.as<int32_t>() + lcap};
+  int32_t* list0 = tw.plist.as<int32_t>();
+  int32_t* list1 = list0 + lcap;
   auto* ctl = tw.pcnt.as<PeelCtl>();
   int32_t* stamp = tw.stamp.as<int32_t>();
-  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(64), 0, s, ctl);
+  const int64_t passes0 = (all_k && nnz0 == 0) ? 0 : 1;  // a decomposition of no entries runs no product
+  hipLaunchKernelGGL(k_peel_ctl_init, dim3(1), dim3(64), 0, s, ctl, (long long)nnz0, all_k ? 1 : 0, all_k ? 0 : nk,
+                     all_k ? 3 : ks[0], (long long)passes0);
+  if (!all_k && !host_steer)
+    GA_HIP(hipMemcpyAsync(reinterpret_cast<char*>(ctl) + offsetof(PeelCtl, ks), ks, sizeof(int32_t) * nk,
+                          hipMemcpyHostToDevice, s));
   fill_u32(stamp, (uint32_t)PEEL_ALIVE, nnz0, s);
   const bool account = c.prof_bytes;
   // the first supports: one full product of the input graph
@@ -799,40 +1046,50 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     support_symmetric_dev(C, tw.mirror[0].as<int32_t>(), tw.sup[0].as<int32_t>(), tw.mw, s, false);
   }
   int64_t stored = nnz0;  // entries in the current buffers (alive or not)
-  int64_t alive = nnz0;   // alive entries (both directions)
-  int64_t passes = (all_k && nnz0 == 0) ? 0 : 1, steps = 0;  // a decomposition of no entries runs no product
-  int32_t kmax = 2;
-  std::vector<int32_t> kv(ks, ks + (all_k ? 0 : nk));
-  std::vector<int64_t> nz_v(kv.size(), 0);
-  std::vector<int32_t> ps_v(kv.size(), 0);
-  int kidx = 0;
-  int k = all_k ? 3 : kv[0];
   long long* hw_dev = nullptr;
-  long long* hw = progress_words(&hw_dev) + 12;  // words 12 (value) and 13 (sequence)
-  hw_dev += 12;
-  auto wait_count = [&](long long seq) -> int64_t {
-    wait_host_word(&hw[1], seq, s);
+  long long* hw = progress_words(&hw_dev) + 16;  // words 16..20 (see peel_publish)
+  hw_dev += 16;
+  const unsigned cap = (unsigned)(8 * (int64_t)c.num_cu);
+  const unsigned gscan = std::min<unsigned>(grid_for(std::max<int64_t>(stored, 1), 256, 1 << 20), cap);
+  auto list_size = [&]() -> int64_t {
     const int64_t v = __atomic_load_n(&hw[0], __ATOMIC_RELAXED);
     GA_REQUIRE(v >= 0 && v <= lcap, GA_ERR_STATE, "peeling list overflow");
     return v;
   };
-  const unsigned cap = (unsigned)(8 * (int64_t)c.num_cu);
-  int ic = 0;   // the list of the next round to run
-  int r = 0;    // round number (stamps)
-  auto scan = [&](int thr) -> int64_t {
-    hipLaunchKernelGGL(k_peel_cnt_zero, dim3(1), dim3(64), 0, s, ctl, ic);
-    const long long seq = next_host_seq();
-    hipLaunchKernelGGL(k_peel_scan, dim3(std::min<unsigned>(grid_for(stored, 256, 1 << 20), cap)), dim3(256), 0, s,
-                       (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.sup[cur].as<int32_t>(), stamp,
-                       stored, thr, r + 1, lists[ic], ctl, ic, (int)lcap, hw_dev, seq);
+  auto destroy = [&](unsigned grid, long long* host, long long seq) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c.prof_events_on) {
+      e0 = peel_event(c);
+      e1 = peel_event(c);
+      GA_HIP(hipEventRecord(e0, s));
+    }
+    if (account) c.prof_bytes_launches += 1;
+    hipLaunchKernelGGL(k_peel_destroy, dim3(grid), dim3(256), 0, s, (const int32_t*)list0, list1, ctl,
+                       (const int32_t*)tw.cur[cur].rp.as<int32_t>(), (const int32_t*)tw.cur[cur].col.as<int32_t>(),
+                       (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.eid[cur].as<int32_t>(),
+                       tw.sup[cur].as<int32_t>(), stamp, all_k ? tw.truss.as<int32_t>() : nullptr, (int)lcap,
+                       account ? 1 : 0, host, seq);
     GA_HIP(hipGetLastError());
-    return wait_count(seq);
+    if (c.prof_events_on) {
+      GA_HIP(hipEventRecord(e1, s));
+      c.prof_events.emplace_back(e0, e1);
+      c.prof_launches += 1;
+    }
   };
-  auto compact = [&]() {
-    // drop every entry that is not alive, supports carried over; every entry
-    // left is alive and unstamped (a k boundary: no round is pending)
+  auto scan = [&]() {
+    hipLaunchKernelGGL(k_peel_scan, dim3(gscan), dim3(256), 0, s, (const int32_t*)tw.mirror[cur].as<int32_t>(),
+                       (const int32_t*)tw.sup[cur].as<int32_t>(), stamp, stored, list0, list1, ctl, (int)lcap);
+    GA_HIP(hipGetLastError());
+  };
+  auto post = [&](int mode, long long* host, long long seq) {
+    hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, mode, (int)lcap, host, seq);
+    GA_HIP(hipGetLastError());
+  };
+  // compaction (the device idle behind it in stream order): the dead entries
+  // go, supports carried; the pending list moves with its entries
+  auto compact = [&](int64_t alive) {
     hipLaunchKernelGGL(k_peel_drop_dead, dim3(grid_for(stored, 256, 8192)), dim3(256), 0, s, (const int32_t*)stamp,
-                       tw.sup[cur].as<int32_t>(), stored);
+                       tw.sup[cur].as<int32_t>(), stored, (const PeelCtl*)ctl);
     int32_t* kept_dev = tw.kept.as<int32_t>();
     long long* bsum = tw.scan_tmp.as<long long>();
     const int64_t nbk = ceil_div(stored, KEEP_TILE);
@@ -852,99 +1109,152 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                        stored, (const int32_t*)kept_dev, tw.cur[cur ^ 1].rp.as<int32_t>(), (uint8_t*)nullptr,
                        (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
                        tw.mirror[cur ^ 1].as<int32_t>(), (const TrussDevState*)nullptr);
+    fill_u32(stamp, (uint32_t)PEEL_ALIVE, alive, s);
+    hipLaunchKernelGGL(k_peel_remap, dim3(grid_for(lcap, 256, 4096)), dim3(256), 0, s, list0, list1,
+                       (const PeelCtl*)ctl, (const int32_t*)tw.newpos.as<int32_t>(),
+                       (const int32_t*)tw.mirror[cur ^ 1].as<int32_t>(), stamp);
     GA_HIP(hipGetLastError());
     cur ^= 1;
     stored = alive;
-    fill_u32(stamp, (uint32_t)PEEL_ALIVE, stored, s);
   };
-  auto record = [&](int64_t value) {
-    nz_v[kidx] = value;
-    ps_v[kidx] = (int32_t)passes;
-    if (!on_k) return;
-    std::vector<int32_t> more;
-    (*on_k)(kv[kidx], value, (int32_t)passes, &more);
-    for (int32_t x : more) {
-      if (x <= kv[kidx]) continue;
-      auto itx = std::lower_bound(kv.begin() + kidx + 1, kv.end(), x);
-      if (itx != kv.end() && *itx == x) continue;
-      const auto at = itx - kv.begin();
-      kv.insert(itx, x);
-      nz_v.insert(nz_v.begin() + at, 0);
-      ps_v.insert(ps_v.begin() + at, 0);
+  struct DrainOnThrow {  // an error leaves no kernel running on buffers the caller may free
+    hipStream_t s;
+    bool armed = true;
+    ~DrainOnThrow() {
+      if (armed) (void)hipStreamSynchronize(s);
     }
-  };
-  int64_t ncur = alive > 0 ? scan(k - 2) : 0;
-  const int64_t max_rounds = nnz0 / 2 + n + (int64_t)kv.size() + 64;  // a round removes an edge or moves k on
-  int64_t rounds = 0;
-  while (true) {
-    GA_REQUIRE(alive >= 0 && ++rounds <= max_rounds && k <= n + 3, GA_ERR_STATE, "peeling did not converge");
-    if (ncur == 0) {  // the k-truss is stable: one filter step that removed nothing
-      if (all_k && alive == 0) break;
-      ++steps;
-      if (all_k) {
-        kmax = k;
-        ++k;
-      } else {
-        record(alive);
-        if (++kidx == (int)kv.size()) break;
-        k = kv[kidx];
-      }
-      if (alive == 0) {
-        ncur = 0;
+  } drain{s};
+  PeelCtl h{};
+  if (host_steer) {
+    std::vector<int32_t> kv(ks, ks + (all_k ? 0 : nk));
+    std::vector<int64_t> nz_v(kv.size(), 0);
+    std::vector<int32_t> ps_v(kv.size(), 0);
+    int kidx = 0, k = all_k ? 3 : kv[0];
+    int32_t kmax = 2;
+    int64_t alive = nnz0, passes = passes0, ncur = 0;
+    if (alive > 0) {
+      const long long seq = next_host_seq();
+      scan();
+      post(2, hw_dev, seq);
+      wait_host_word(&hw[4], seq, s);
+      ncur = list_size();
+    }
+    const int64_t max_rounds = nnz0 / 2 + n + 4 * (int64_t)kv.size() + 64;  // a round removes an edge or moves k on
+    int64_t rounds = 0;
+    while (true) {
+      GA_REQUIRE(++rounds <= max_rounds && k <= n + 3, GA_ERR_STATE, "peeling did not converge");
+      if (ncur == 0) {  // the k-truss is stable: one filter step that removed nothing
+        if (all_k && alive == 0) break;
+        bool last = false;
+        if (all_k) {
+          kmax = k++;
+        } else {
+          nz_v[kidx] = alive;
+          ps_v[kidx] = (int32_t)passes;
+          if (on_k) {
+            std::vector<int32_t> more;
+            (*on_k)(kv[kidx], alive, (int32_t)passes, &more);
+            for (int32_t x : more) {
+              if (x <= kv[kidx]) continue;  // only larger k can continue from this truss
+              auto itx = std::lower_bound(kv.begin() + kidx + 1, kv.end(), x);
+              if (itx != kv.end() && *itx == x) continue;
+              const auto at = itx - kv.begin();
+              kv.insert(itx, x);
+              nz_v.insert(nz_v.begin() + at, 0);
+              ps_v.insert(ps_v.begin() + at, 0);
+            }
+          }
+          last = ++kidx == (int)kv.size();
+          if (!last) k = kv[kidx];
+        }
+        hipLaunchKernelGGL(k_peel_set_k, dim3(1), dim3(64), 0, s, ctl, k, 1);
+        if (last) break;
+        if (alive == 0) continue;  // (a k list past an empty truss: every later k is empty too)
+        if (2 * alive <= stored) compact(alive);
+        const long long seq = next_host_seq();
+        scan();
+        post(2, hw_dev, seq);
+        wait_host_word(&hw[4], seq, s);
+        ncur = list_size();
         continue;
       }
-      if (2 * alive <= stored) compact();
-      ncur = scan(k - 2);
-      continue;
+      const long long seq = next_host_seq();
+      destroy(ncur <= PEEL_SPLIT_MAX_EDGES ? (unsigned)(ncur * PEEL_SPLIT)
+                                           : std::min<unsigned>(grid_for(ncur, 4, 1 << 20), cap),
+              nullptr, 0);
+      post(0, hw_dev, seq);
+      wait_host_word(&hw[4], seq, s);
+      alive -= 2 * ncur;
+      if (!all_k || alive > 0) ++passes;
+      ncur = list_size();
     }
-    // round r + 1: the listed edges go, their triangles with them
-    ++r;
-    hipLaunchKernelGGL(k_peel_cnt_zero, dim3(1), dim3(64), 0, s, ctl, ic ^ 1);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c.prof_events_on) {
-      e0 = peel_event(c);
-      e1 = peel_event(c);
-      GA_HIP(hipEventRecord(e0, s));
-    }
-    if (account) c.prof_bytes_launches += 1;
-    const long long seq = next_host_seq();
-    hipLaunchKernelGGL(k_peel_destroy, dim3(std::min<unsigned>(grid_for(ncur, 4, 1 << 20), cap)), dim3(256), 0, s,
-                       (const int32_t*)lists[ic], ctl, ic, (const int32_t*)tw.cur[cur].rp.as<int32_t>(),
-                       (const int32_t*)tw.cur[cur].col.as<int32_t>(), (const int32_t*)tw.mirror[cur].as<int32_t>(),
-                       (const int32_t*)tw.eid[cur].as<int32_t>(), tw.sup[cur].as<int32_t>(), stamp,
-                       all_k ? tw.truss.as<int32_t>() : nullptr, k - 1, k - 2, r, lists[ic ^ 1], (int)lcap,
-                       account ? 1 : 0, hw_dev, seq);
-    GA_HIP(hipGetLastError());
-    if (c.prof_events_on) {
-      GA_HIP(hipEventRecord(e1, s));
-      c.prof_events.emplace_back(e0, e1);
-      c.prof_launches += 1;
-    }
-    alive -= 2 * ncur;
-    ++steps;
-    if (!all_k || alive > 0) ++passes;  // a decomposition stops without a product once the graph is empty
-    ic ^= 1;
-    ncur = wait_count(seq);
-  }
-  if (cur_out && stored != alive) compact();
-  if (account) {
-    PeelCtl h{};
     GA_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
-    c.prof_bytes_host += h.bytes;
+    GA_REQUIRE(!h.err && h.alive == alive && h.passes == passes, GA_ERR_STATE, "peeling state disagrees with the host");
+    for (int i = 0; i < (all_k ? 0 : nk); ++i) {  // the caller's k values (joined ones went to on_k)
+      const auto at = std::lower_bound(kv.begin(), kv.end(), ks[i]) - kv.begin();
+      if (nnz_at) nnz_at[i] = nz_v[at];
+      if (passes_at) passes_at[i] = ps_v[at];
+    }
+    h.kmax = kmax;
+  } else {
+    // device-driven: ticks of (round, advance) queued a few ahead of the
+    // device, which reports through the progress words; the host drains the
+    // queue only to compact, when the dead entries are at least half of the
+    // stored ones
+    constexpr int64_t kInFlight = 4;
+    const int64_t max_ticks = nnz0 / 2 + n + nk + 64;  // a tick removes an edge or moves k on
+    __atomic_store_n(&hw[1], (long long)nnz0, __ATOMIC_RELAXED);
+    __atomic_store_n(&hw[2], 0ll, __ATOMIC_RELAXED);
+    __atomic_store_n(&hw[3], 0ll, __ATOMIC_RELEASE);
+    if (nnz0 > 0) scan();  // the first threshold
+    const unsigned gdes = std::min<unsigned>(cap, PEEL_GRID);
+    int64_t queued = 0;
+    auto t_seen = std::chrono::steady_clock::now();
+    int64_t seen = -1;
+    while (true) {
+      const int64_t done_ticks = __atomic_load_n(&hw[3], __ATOMIC_ACQUIRE);
+      if (__atomic_load_n(&hw[2], __ATOMIC_RELAXED)) break;
+      const int64_t alive_seen = __atomic_load_n(&hw[1], __ATOMIC_RELAXED);
+      const bool want_compact = alive_seen > 0 && 2 * alive_seen <= stored;
+      if (want_compact && queued == done_ticks) {
+        compact(alive_seen);  // every queued tick ran: the state is final up to here
+      } else if (!want_compact && queued - done_ticks < kInFlight) {
+        GA_REQUIRE(queued < max_ticks, GA_ERR_STATE, "peeling did not converge");
+        destroy(gdes, nullptr, 0);
+        hipLaunchKernelGGL(k_peel_advance, dim3(gscan), dim3(256), 0, s, (const int32_t*)tw.mirror[cur].as<int32_t>(),
+                           (const int32_t*)tw.sup[cur].as<int32_t>(), stamp, stored, list0, list1, ctl, (int)lcap);
+        post(1, hw_dev, 0);
+        ++queued;
+        continue;
+      }
+      const auto now = std::chrono::steady_clock::now();
+      if (done_ticks != seen) {
+        seen = done_ticks;
+        t_seen = now;
+      } else if (now - t_seen > std::chrono::seconds(60)) {
+        GA_HIP(hipStreamSynchronize(s));  // a tick that ran for a minute: report, do not spin on
+        GA_REQUIRE(__atomic_load_n(&hw[3], __ATOMIC_ACQUIRE) != seen, GA_ERR_STATE, "peeling made no progress");
+      }
+    }
+    GA_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipStreamSynchronize(s));
+    GA_REQUIRE(h.fin && !h.err, GA_ERR_STATE, "peeling did not finish (a removal list outgrew its capacity)");
+    for (int i = 0; i < (all_k ? 0 : nk); ++i) {
+      if (nnz_at) nnz_at[i] = h.nnz_at[i];
+      if (passes_at) passes_at[i] = h.passes_at[i];
+    }
   }
+  if (cur_out && stored != h.alive) compact(h.alive);
   GA_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < (all_k ? 0 : nk); ++i) {  // the caller's k values (joined ones went to on_k)
-    const auto at = std::lower_bound(kv.begin(), kv.end(), ks[i]) - kv.begin();
-    if (nnz_at) nnz_at[i] = nz_v[at];
-    if (passes_at) passes_at[i] = ps_v[at];
-  }
+  drain.armed = false;
+  if (account) c.prof_bytes_host += h.bytes;
   if (cur_out) *cur_out = cur;
-  if (nnz_out) *nnz_out = alive;
-  if (kmax_out) *kmax_out = kmax;
+  if (nnz_out) *nnz_out = h.alive;
+  if (kmax_out) *kmax_out = h.kmax;
   if (st) {
-    st->passes = passes;
-    st->steps = steps;
+    st->passes = h.passes;
+    st->steps = h.steps;
   }
 }
 
@@ -989,8 +1299,8 @@ static void truss_loop(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   {
     Context& c = ctx();
     const int mode = c.truss_loop >= 0 ? c.truss_loop : 3;
-    if (mode == 3) {
-      truss_peel(A, all_k, ks, nk, tw, s, cur_out, nnz_out, kmax_out, st, nnz_at, passes_at, on_k);
+    if (mode >= 3) {
+      truss_peel(A, all_k, ks, nk, tw, s, mode == 4, cur_out, nnz_out, kmax_out, st, nnz_at, passes_at, on_k);
       if (st)
         st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
       return;

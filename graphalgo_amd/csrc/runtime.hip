@@ -363,7 +363,7 @@ ga_status ga_init(int device) {
       c->selective_support = std::min(1, std::max(-1, std::atoi(ss)));
     if (const char* sj = std::getenv("GA_SERVE_JOIN")) c->serve_join = std::min(2, std::max(0, std::atoi(sj)));
     if (const char* st = std::getenv("GA_SUPPORT_TRI")) c->support_tri = std::min(1, std::max(-1, std::atoi(st)));
-    if (const char* tl = std::getenv("GA_TRUSS_LOOP")) c->truss_loop = std::min(3, std::max(-1, std::atoi(tl)));
+    if (const char* tl = std::getenv("GA_TRUSS_LOOP")) c->truss_loop = std::min(4, std::max(-1, std::atoi(tl)));
     if (const char* iw = std::getenv("GA_ITEM_WEDGES")) c->item_wedges = std::max(1LL, std::atoll(iw));
     if (const char* rc = std::getenv("GA_ROW_CHUNK")) c->row_chunk = std::max(0, std::atoi(rc));
     if (const char* bu = std::getenv("GA_BC_BOTTOM_UP")) c->bc_bottom_up = std::atoi(bu) != 0;
@@ -419,8 +419,8 @@ ga_status ga_set_option(int option, double value) {
         c.support_tri = (int)value;
         break;
       case GA_OPT_TRUSS_LOOP:
-        GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0, GA_ERR_ARG,
-                   "truss_loop is -1, 0, 1, 2 or 3");
+        GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0 || value == 4.0,
+                   GA_ERR_ARG, "truss_loop is -1, 0, 1, 2, 3 or 4");
         c.truss_loop = (int)value;
         break;
       default:

@@ -347,8 +347,11 @@ enum {
                                    replayed from a captured HIP graph; 3 = supports computed
                                    once and then updated: each round lists the triangles
                                    through the entries it removes and decrements the supports
-                                   of their surviving edges; -1 (default) = 3.  Selective
-                                   support passes run with 0. */
+                                   of their surviving edges, rounds queued on the device;
+                                   4 = the same, the host reading each round's size (always
+                                   the case for served runs that answer each k as it is
+                                   reached); -1 (default) = 3.  Selective support passes run
+                                   with 0. */
   GA_OPT_SUPPORT_TRI = 8,       /* k-truss support passes as triangle listing, C<L> = L*L over
                                    the strict lower half with each hit added to its three edges
                                    (GA_SUPPORT_TRI): 1 = on, 0 = off, -1 (default) = on for

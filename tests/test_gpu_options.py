@@ -135,8 +135,9 @@ def _truss_cases(G):
 @pytest.mark.parametrize("name", ["rmat12", "rmat14", "er1000", "path40", "empty16"])
 def test_truss_loops_identical(gpu, G, oracle, name):
     """Host-driven peeling (0), device-driven steps (1), steps replayed
-    from a captured HIP graph (2), peeling by support updates (3) and the
-    default (-1): the oracle's truss numbers and support
+    from a captured HIP graph (2), peeling by support updates with its rounds
+    queued on the device (3) or steered by the host (4) and the default (-1):
+    the oracle's truss numbers and support
     pass count, and between the loops the same filter steps, k-truss sizes,
     pass counts and k-truss patterns (resident graph queried twice, so the
     captured step is reused)."""
@@ -144,7 +145,7 @@ def test_truss_loops_identical(gpu, G, oracle, name):
     want, kmax, opasses, _, _ = oracle.truss_decomposition(g)
     ks = sorted({3, 4, 5, max(kmax, 3), kmax + 1})
     res = {}
-    for loop in (0, 1, 2, 3, -1):
+    for loop in (0, 1, 2, 3, 4, -1):
         print(f"case truss loops {name} truss_loop={loop}", flush=True)
         with gpu.options(truss_loop=loop):
             t1, k1, s1 = gpu.truss_decomposition(g)
@@ -161,7 +162,7 @@ def test_truss_loops_identical(gpu, G, oracle, name):
         for k, (nnz, _) in zip(ks + ks, kt):
             assert nnz == int((want >= k).sum()), (k, nnz)
         res[loop] = (s1["steps"], kt, it, pat.rowptr.tolist(), pat.col.tolist())
-    assert res[0] == res[1] == res[2] == res[3] == res[-1]
+    assert res[0] == res[1] == res[2] == res[3] == res[4] == res[-1]
 
 
 @pytest.mark.parametrize("name", ["rmat12", "rmat14", "er1000", "path40", "empty16"])
@@ -200,7 +201,7 @@ def test_truss_support_tri_rmat16(gpu, G, golden):
 
 def test_option_validation(gpu):
     for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
-                      ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 4), ("truss_loop", -2),
+                      ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 5), ("truss_loop", -2),
                       ("support_tri", 2), ("serve_join", 3)):
         with pytest.raises(Exception):
             gpu.set_option(name, bad)
