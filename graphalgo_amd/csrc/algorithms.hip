@@ -602,30 +602,37 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
 // (when they are at least half of the stored entries).
 constexpr int32_t PEEL_ALIVE = 0x7fffffff;
 
-// The peeling state in device memory.  Every kernel of a round reads it when
-// it starts; the last workgroup of the kernel to finish (every other one has
-// read it by then) applies the kernel's changes.
-struct PeelCtl {
-  long long bytes;   // algorithmic bytes of the destroy launches (accounting on)
+// Counters of a peeling run that change from round to round.
+struct PeelDyn {
   long long alive;   // alive entries (both directions)
   long long passes;  // support products of the reference's schedule (1 + rounds that removed)
   long long steps;   // filter steps (removal rounds + the step that finds a k-truss stable)
-  long long ticks;   // device-driven: advance kernels run (flow control)
-  int cnt[2];        // sizes of the two alternating removal lists
-  int adv_stable;    // device-driven: the last advance found the k-truss stable
-  int scnt;          // device-driven: entries the advance's scan listed (the post moves it into cnt)
-  int ic;            // list of the next round
   int r;             // rounds run (a round r removes the edges stamped r)
   int k;             // current k (threshold k - 2)
-  int kidx, nk, all_k, fin, kmax;
-  int err;           // a list outgrew its capacity: the run stopped
+  int kidx, fin, kmax, err;  // k list position; run ended; largest nonempty k; a list outgrew its capacity
+};
+
+// The peeling state in device memory.  Host-steered runs keep their counters
+// in `h` and two removal lists (cnt); device-driven runs alternate `dyn[2]`
+// (tick t reads dyn[(t - 1) & 1] and stores its own in dyn[t & 1]) and
+// rotate four lists (qcnt): tick t consumes list t & 3, fills (t + 1) & 3 and
+// clears (t + 2) & 3, which no kernel in flight reads.
+struct PeelCtl {
+  long long bytes;  // algorithmic bytes of the round launches (accounting on)
+  PeelDyn h;
+  int cnt[2];
+  int ic;           // host-steered: the list of the next round
+  PeelDyn dyn[2];
+  int qcnt[4];
+  int cr, clist, ccnt;  // compaction view: rounds run, the pending list and its size
+  int nk, all_k;
   int ks[TRUSS_MAX_K];
   int passes_at[TRUSS_MAX_K];
   long long nnz_at[TRUSS_MAX_K];
 };
 
 // Wavefront-aggregated append; entries past cap are dropped (the count still
-// grows, and the host refuses a count above cap).
+// grows, and the run stops with an error).
 __device__ inline void peel_push(bool sel, int32_t pos, int32_t* __restrict__ list, int* __restrict__ cnt, int cap,
                                  int lane) {
   const unsigned long long m = __ballot(sel);
@@ -640,13 +647,14 @@ __device__ inline void peel_push(bool sel, int32_t pos, int32_t* __restrict__ li
   }
 }
 
-// host words: [0] next list size, [1] alive, [2] fin, [3] ticks, [4] sequence (release)
-__device__ inline void peel_publish(const PeelCtl* __restrict__ ctl, long long* __restrict__ host, long long seq) {
+// host words: [0] list size, [1] alive, [2] run ended, [3] ticks, [4] sequence (release)
+__device__ inline void peel_publish(long long* __restrict__ host, long long size, long long alive, int fin,
+                                    long long ticks, long long seq) {
   if (!host) return;
-  __hip_atomic_store(&host[0], (long long)ctl->cnt[ctl->ic], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&host[1], ctl->alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&host[2], (long long)ctl->fin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&host[3], ctl->ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[0], size, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[1], alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[2], (long long)fin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[3], ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(&host[4], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
@@ -671,75 +679,60 @@ __device__ inline void peel_scan_body(const int32_t* __restrict__ mirror, const 
   }
 }
 
-// The scan of a new threshold (host-steered runs): k and the list from the
-// state; the list's count starts at zero (its round cleared it).
-__global__ __launch_bounds__(256) void k_peel_scan(const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
-                                                   int32_t* __restrict__ stamp, int64_t nnz,
-                                                   int32_t* __restrict__ list0, int32_t* __restrict__ list1,
-                                                   PeelCtl* __restrict__ ctl, int cap) {
-  const int ic = ctl->ic;
-  peel_scan_body(mirror, sup, stamp, nnz, ctl->k - 2, ctl->r + 1, ic ? list1 : list0, &ctl->cnt[ic], cap);
+// The k after a stable k-truss (decomposition: k + 1 until the graph is
+// empty; k list: the next k until the list ends).
+__device__ inline void peel_next_k(const PeelCtl* __restrict__ ctl, long long alive, int& k, int& kidx, int& fin) {
+  if (ctl->all_k) {
+    if (alive == 0) fin = 1;
+    else ++k;
+  } else {
+    if (++kidx == ctl->nk) fin = 1;
+    else k = ctl->ks[kidx];
+  }
 }
 
-// The state after a round that removed nc edges (the list it consumed is
-// emptied by k_peel_post).
-struct PeelView {
-  int ic, r, fin;
-  long long alive, passes, steps;
-};
-__device__ inline PeelView peel_after_round(const PeelCtl* __restrict__ ctl, int nc) {
-  PeelView v{ctl->ic, ctl->r, ctl->fin, ctl->alive, ctl->passes, ctl->steps};
-  if (nc > 0) {
-    v.alive -= 2ll * nc;
-    v.steps += 1;
-    if (!ctl->all_k || v.alive > 0) v.passes += 1;  // a decomposition stops at an empty graph
-    v.r += 1;
-    v.ic ^= 1;
-  }
-  return v;
-}
-
-// Host-steered runs: the k to scan next (and the filter step that found the
-// previous one stable).
-__global__ void k_peel_set_k(PeelCtl* __restrict__ ctl, int k, int stable_step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    ctl->k = k;
-    ctl->steps += stable_step;
-  }
+// The counters after a round that removed nc edges.
+__device__ inline void peel_count_round(PeelDyn& d, int all_k, int nc) {
+  d.alive -= 2ll * nc;
+  d.steps += 1;
+  if (!all_k || d.alive > 0) d.passes += 1;  // a decomposition stops without a product at an empty graph
+  d.r += 1;
 }
 
 __global__ void k_peel_ctl_init(PeelCtl* __restrict__ ctl, long long alive, int all_k, int nk, int k0,
                                 long long passes) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    PeelDyn d{};
+    d.alive = alive;
+    d.passes = passes;
+    d.steps = 0;
+    d.r = 0;
+    d.k = k0;
+    d.kidx = 0;
+    d.fin = alive == 0 && all_k;
+    d.kmax = 2;
+    d.err = 0;
     ctl->bytes = 0;
-    ctl->alive = alive;
-    ctl->passes = passes;
-    ctl->steps = 0;
-    ctl->ticks = 0;
+    ctl->h = d;
+    ctl->dyn[0] = ctl->dyn[1] = d;
     ctl->cnt[0] = ctl->cnt[1] = 0;
-    ctl->adv_stable = 0;
-    ctl->scnt = 0;
     ctl->ic = 0;
-    ctl->r = 0;
-    ctl->k = k0;
-    ctl->kidx = 0;
+    for (int i = 0; i < 4; ++i) ctl->qcnt[i] = 0;
+    ctl->cr = ctl->clist = ctl->ccnt = 0;
     ctl->nk = nk;
     ctl->all_k = all_k;
-    ctl->fin = alive == 0 && all_k;
-    ctl->kmax = 2;
-    ctl->err = 0;
   }
 }
 
-// Round r = ctl->r + 1 removes the edges of list ic: the triangles through
-// each edge (i, j) of the list (i > j: row i is the column of the transposed
-// entry).  The shorter row is probed against the longer one: the longer row,
-// or every stride-th entry of it when it is longer than the LDS sample, is
-// staged in LDS; a probed column is found by a binary search of the sample
-// and, between two sampled entries, of at most stride - 1 entries in global
-// memory.  Each triangle the edge owns decrements the supports of its
-// surviving edges, and an edge pushed below thr is stamped r + 1 and joins the
-// next list.  The edge's own truss number is k - 1 (decomposition).
+// A round removes the edges of a list: the triangles through each edge
+// (i, j) (i > j: row i is the column of the transposed entry).  The shorter
+// row is probed against the longer one: the longer row, or every stride-th
+// entry of it when it is longer than the LDS sample, is staged in LDS; a
+// probed column is found by a binary search of the sample and, between two
+// sampled entries, of at most stride - 1 entries in global memory.  Each
+// triangle the edge owns decrements the supports of its surviving edges, and
+// an edge pushed below thr is stamped r + 1 and joins the next list.  The
+// edge's own truss number is tval (decomposition).
 struct PeelEdge {
   int c, mc, ab, ae, arow, bb, be, brow, stride, m;
 };
@@ -823,51 +816,48 @@ constexpr int PEEL_WSTAGE = 1024;         // sample entries per wavefront (large
 constexpr int PEEL_BSTAGE = 4096;         // sample entries per workgroup (small rounds)
 constexpr int PEEL_SPLIT = 8;             // workgroups per edge in small rounds
 constexpr int PEEL_SPLIT_MAX_EDGES = 64;  // rounds of at most this many edges split them
-constexpr int PEEL_GRID = PEEL_SPLIT * PEEL_SPLIT_MAX_EDGES;  // device-driven destroy grid
 
-// A round.  Large rounds: a wavefront per edge (a wavefront probing one chunk
-// searches the target row in global memory, a longer probe row stages the
-// sample first).  Small rounds (few edges, usually long rows): PEEL_SPLIT
-// workgroups per edge, each staging the sample once and probing every
-// PEEL_SPLIT-th run of 256 probe entries.  The last workgroup applies the
-// round to the state.
-__global__ __launch_bounds__(256) void k_peel_destroy(const int32_t* __restrict__ list0, int32_t* __restrict__ list1,
-                                                      PeelCtl* __restrict__ ctl, const int32_t* __restrict__ rp,
-                                                      const int32_t* __restrict__ col,
-                                                      const int32_t* __restrict__ mirror,
-                                                      const int32_t* __restrict__ eid, int32_t* __restrict__ sup,
-                                                      int32_t* __restrict__ stamp, int32_t* __restrict__ truss, int cap,
-                                                      int account, long long* __restrict__ host, long long seq) {
-  __shared__ int32_t s_stage[PEEL_BSTAGE];
+// The round itself.  Large rounds: a wavefront per edge (a wavefront probing
+// one chunk searches the target row in global memory, a longer probe row
+// stages the sample first).  Small rounds (few edges, usually long rows):
+// PEEL_SPLIT workgroups per edge, each staging the sample once and probing
+// every PEEL_SPLIT-th run of 256 probe entries.
+struct PeelGraph {
+  const int32_t* rp;
+  const int32_t* col;
+  const int32_t* mirror;
+  const int32_t* eid;
+  int32_t* sup;
+  int32_t* stamp;
+  int32_t* truss;
+};
+
+__device__ inline void peel_round_body(const int32_t* __restrict__ cur, int nc, int32_t* __restrict__ next,
+                                       int* __restrict__ ncnt, int r, int thr, int tval, const PeelGraph& g, int cap,
+                                       long long* __restrict__ bytes_ctr, int32_t* __restrict__ s_stage) {
   const int lane = threadIdx.x & 63;
-  const int ic = ctl->ic, nc = ctl->fin ? 0 : min(ctl->cnt[ic], cap);
-  (void)host;
-  (void)seq;
-  const int r = ctl->r + 1, thr = ctl->k - 2, tval = ctl->k - 1;
-  const int32_t* cur = ic ? list1 : list0;
-  int32_t* next = ic ? const_cast<int32_t*>(list0) : list1;
-  int* ncnt = &ctl->cnt[ic ^ 1];
   long long bytes = 0, found = 0;
   if (nc > PEEL_SPLIT_MAX_EDGES) {
     int32_t* st = s_stage + (threadIdx.x >> 6) * PEEL_WSTAGE;
     const int64_t wpb = blockDim.x >> 6;
     for (int64_t t = blockIdx.x * wpb + (threadIdx.x >> 6); t < nc; t += (int64_t)gridDim.x * wpb) {
-      const PeelEdge e = peel_edge(cur[t], rp, col, mirror, PEEL_WSTAGE);
-      if (truss && lane == 0) {
-        truss[eid[e.c]] = tval;
-        truss[eid[e.mc]] = tval;
+      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, g.mirror, PEEL_WSTAGE);
+      if (g.truss && lane == 0) {
+        g.truss[g.eid[e.c]] = tval;
+        g.truss[g.eid[e.mc]] = tval;
       }
-      if (account && lane == 0) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
+      if (bytes_ctr && lane == 0) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
       const bool staged = e.ae - e.ab > 64;
-      const PeelEdge g = staged ? e : peel_direct(e);
+      const PeelEdge pe = staged ? e : peel_direct(e);
       __threadfence_block();  // the previous edge's sample reads are done
       if (staged)
-        for (int u = lane; u < e.m; u += 64) st[u] = col[e.bb + u * e.stride];
+        for (int u = lane; u < e.m; u += 64) st[u] = g.col[e.bb + u * e.stride];
       __threadfence_block();
       for (int q0 = e.ab; q0 < e.ae; q0 += 64) {
         const int q = q0 + lane;
         int pa = -1, pb = -1;
-        if (q < e.ae) peel_probe(g, q, staged ? st : col + e.bb, col, mirror, sup, stamp, thr, r, pa, pb, found);
+        if (q < e.ae)
+          peel_probe(pe, q, staged ? st : g.col + e.bb, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
         peel_push(pa >= 0, pa, next, ncnt, cap, lane);
         peel_push(pb >= 0, pb, next, ncnt, cap, lane);
       }
@@ -875,127 +865,207 @@ __global__ __launch_bounds__(256) void k_peel_destroy(const int32_t* __restrict_
   } else {
     for (int64_t b = blockIdx.x; b < (int64_t)nc * PEEL_SPLIT; b += gridDim.x) {
       const int t = (int)(b / PEEL_SPLIT), part = (int)(b % PEEL_SPLIT);
-      const PeelEdge e = peel_edge(cur[t], rp, col, mirror, PEEL_BSTAGE);
+      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, g.mirror, PEEL_BSTAGE);
       if (part == 0 && threadIdx.x == 0) {
-        if (truss) {
-          truss[eid[e.c]] = tval;
-          truss[eid[e.mc]] = tval;
+        if (g.truss) {
+          g.truss[g.eid[e.c]] = tval;
+          g.truss[g.eid[e.mc]] = tval;
         }
-        if (account) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
+        if (bytes_ctr) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
       }
       __syncthreads();  // the previous edge's sample reads are done
-      for (int u = threadIdx.x; u < e.m; u += blockDim.x) s_stage[u] = col[e.bb + u * e.stride];
+      for (int u = threadIdx.x; u < e.m; u += blockDim.x) s_stage[u] = g.col[e.bb + u * e.stride];
       __syncthreads();
       for (int q0 = e.ab + part * 256; q0 < e.ae; q0 += PEEL_SPLIT * 256) {
         const int q = q0 + threadIdx.x;
         int pa = -1, pb = -1;
-        if (q < e.ae) peel_probe(e, q, s_stage, col, mirror, sup, stamp, thr, r, pa, pb, found);
+        if (q < e.ae) peel_probe(e, q, s_stage, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
         peel_push(pa >= 0, pa, next, ncnt, cap, lane);
         peel_push(pb >= 0, pb, next, ncnt, cap, lane);
       }
     }
   }
-  if (account) {
+  if (bytes_ctr) {
     bytes = wave_reduce_sum(bytes + 16 * found);
-    if (lane == 0 && bytes) atomicAdd((unsigned long long*)&ctl->bytes, (unsigned long long)bytes);
+    if (lane == 0 && bytes) atomicAdd((unsigned long long*)bytes_ctr, (unsigned long long)bytes);
   }
 }
 
-// Device-driven runs: after each round, a list that came back empty means the
-// current k-truss is stable -- the new threshold is scanned here (every
-// workgroup derives the same next k from the state as the round left it);
-// k_peel_post then records the step and moves k on.  With a list still
-// pending this kernel does nothing.
-__device__ inline void peel_next_k(const PeelCtl* __restrict__ ctl, long long alive, int& k, int& kidx, int& fin) {
-  if (ctl->all_k) {
-    if (alive == 0) fin = 1;
-    else ++k;
-  } else {
-    if (++kidx == ctl->nk) fin = 1;
-    else k = ctl->ks[kidx];
-  }
+// ---- host-steered runs: a kernel per round or scan, then k_peel_post ------
+__global__ __launch_bounds__(256) void k_peel_scan(int32_t* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
+                                                   const int32_t* __restrict__ mirror,
+                                                   const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
+                                                   int64_t nnz, int cap) {
+  const int ic = ctl->ic;
+  peel_scan_body(mirror, sup, stamp, nnz, ctl->h.k - 2, ctl->h.r + 1, lists + ic * lcap, &ctl->cnt[ic], cap);
 }
 
-__global__ __launch_bounds__(256) void k_peel_advance(const int32_t* __restrict__ mirror,
-                                                      const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
-                                                      int64_t nnz, int32_t* __restrict__ list0,
-                                                      int32_t* __restrict__ list1, PeelCtl* __restrict__ ctl, int cap) {
-  const int nc = ctl->fin ? 0 : min(ctl->cnt[ctl->ic], cap);
-  const PeelView v = peel_after_round(ctl, nc);
-  const bool stable = !v.fin && ctl->cnt[v.ic] == 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->adv_stable = stable ? 1 : 0;
-  if (!stable) return;
-  int k = ctl->k, kidx = ctl->kidx, fin = 0;
-  peel_next_k(ctl, v.alive, k, kidx, fin);
-  // the stability test above reads cnt[v.ic]: the scan counts into scnt
-  if (!fin && v.alive > 0)
-    peel_scan_body(mirror, sup, stamp, nnz, k - 2, v.r + 1, v.ic ? list1 : list0, &ctl->scnt, cap);
+__global__ __launch_bounds__(256) void k_peel_destroy(int32_t* __restrict__ lists, int64_t lcap,
+                                                      PeelCtl* __restrict__ ctl, PeelGraph g, int cap, int account) {
+  __shared__ int32_t s_stage[PEEL_BSTAGE];
+  const int ic = ctl->ic;
+  const int nc = ctl->h.fin ? 0 : min(ctl->cnt[ic], cap);
+  if (nc == 0) return;
+  peel_round_body(lists + ic * lcap, nc, lists + (ic ^ 1) * lcap, &ctl->cnt[ic ^ 1], ctl->h.r + 1, ctl->h.k - 2,
+                  ctl->h.k - 1, g, cap, account ? &ctl->bytes : nullptr, s_stage);
 }
 
-// One workgroup after the big kernels of a round: applies the round to the
-// state (mode 0), the round and the advance (mode 1: device-driven tick), or
-// only checks a host-steered scan (mode 2); then publishes to the host.
+// mode 0: a round ran -- counted, its list emptied, the lists swap; mode 1:
+// a scan ran.  Then the list size of the next round to the host.
 __global__ void k_peel_post(PeelCtl* __restrict__ ctl, int mode, int cap, long long* __restrict__ host,
                             long long seq) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  if (mode != 2) {
-    const int nc = ctl->fin ? 0 : min(ctl->cnt[ctl->ic], cap);
-    const PeelView v = peel_after_round(ctl, nc);
-    if (nc > 0) ctl->cnt[ctl->ic] = 0;  // consumed; the round after next fills it
-    ctl->alive = v.alive;
-    ctl->passes = v.passes;
-    ctl->steps = v.steps;
-    ctl->r = v.r;
-    ctl->ic = v.ic;
-    if (mode == 1 && ctl->adv_stable) {
-      if (ctl->all_k) {
-        if (v.alive > 0) {
-          ctl->kmax = ctl->k;
-          ctl->steps += 1;
-        }
-      } else {
-        ctl->nnz_at[ctl->kidx] = v.alive;
-        ctl->passes_at[ctl->kidx] = (int)v.passes;
-        ctl->steps += 1;
-      }
-      int k = ctl->k, kidx = ctl->kidx, fin = 0;
-      peel_next_k(ctl, v.alive, k, kidx, fin);
-      ctl->k = k;
-      ctl->kidx = kidx;
-      ctl->fin = fin;
-      ctl->cnt[v.ic] = ctl->scnt;  // the scanned list (cnt[v.ic] was empty: the truss was stable)
-    }
-    if (mode == 1) {
-      ctl->scnt = 0;
-      ctl->ticks += 1;
+  if (mode == 0 && !ctl->h.fin) {
+    const int nc = min(ctl->cnt[ctl->ic], cap);
+    if (nc > 0) {
+      peel_count_round(ctl->h, ctl->all_k, nc);
+      ctl->cnt[ctl->ic] = 0;  // consumed; the round after next fills it
+      ctl->ic ^= 1;
     }
   }
-  if (ctl->cnt[0] > cap || ctl->cnt[1] > cap) ctl->err = ctl->fin = 1;
-  peel_publish(ctl, host, seq);
+  if (ctl->cnt[0] > cap || ctl->cnt[1] > cap) ctl->h.err = ctl->h.fin = 1;
+  peel_publish(host, ctl->cnt[ctl->ic], ctl->h.alive, ctl->h.fin, 0, seq);
 }
 
-// Entries of no round still pending or to come (stamp <= r) lose their
+// Host-steered runs: the k to scan next, and the filter step that found the
+// previous k-truss stable.
+__global__ void k_peel_set_k(PeelCtl* __restrict__ ctl, int k, int stable_step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->h.k = k;
+    ctl->h.steps += stable_step;
+  }
+}
+
+// ---- device-driven runs: one kernel per tick --------------------------------
+// The state of tick t: the state of tick t - 1 moved on by what that tick did
+// -- a round (its list was not empty: counted) or the stable step (its list
+// was empty: recorded, k moved on; that tick scanned the new threshold).
+// Tick 0 is the first threshold's scan.
+__device__ inline PeelDyn peel_tick_state(const PeelCtl* __restrict__ ctl, int64_t t, int cap, int& rec_kidx,
+                                          long long& rec_alive, int& rec_passes) {
+  PeelDyn d = ctl->dyn[(t - 1) & 1];
+  rec_kidx = -1;
+  if (t <= 1 || d.fin) return d;
+  const int np = min(ctl->qcnt[(t - 1) & 3], cap);
+  if (np > 0) {
+    peel_count_round(d, ctl->all_k, np);
+  } else {
+    if (ctl->all_k) {
+      if (d.alive > 0) {
+        d.kmax = d.k;
+        d.steps += 1;
+      }
+    } else {
+      rec_kidx = d.kidx;
+      rec_alive = d.alive;
+      rec_passes = (int)d.passes;
+      d.steps += 1;
+    }
+    int k = d.k, kidx = d.kidx, fin = 0;
+    peel_next_k(ctl, d.alive, k, kidx, fin);
+    d.k = k;
+    d.kidx = kidx;
+    d.fin = fin;
+  }
+  return d;
+}
+
+__global__ __launch_bounds__(256) void k_peel_tick(int64_t t, int32_t* __restrict__ lists, int64_t lcap,
+                                                   PeelCtl* __restrict__ ctl, PeelGraph g, int64_t nnz, int cap,
+                                                   int account, long long* __restrict__ host) {
+  __shared__ int32_t s_stage[PEEL_BSTAGE];
+  int rec_kidx;
+  long long rec_alive = 0;
+  int rec_passes = 0;
+  PeelDyn d = peel_tick_state(ctl, t, cap, rec_kidx, rec_alive, rec_passes);
+  const int qc = ctl->qcnt[t & 3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (qc > cap || ctl->qcnt[(t - 1) & 3] > cap) d.err = d.fin = 1;
+    ctl->dyn[t & 1] = d;
+    ctl->qcnt[(t + 2) & 3] = 0;
+    if (rec_kidx >= 0) {
+      ctl->nnz_at[rec_kidx] = rec_alive;
+      ctl->passes_at[rec_kidx] = rec_passes;
+    }
+    if ((t & 3) == 0 || d.fin) peel_publish(host, qc, d.alive, d.fin, t, 0);
+  }
+  if (d.fin) return;
+  const int nc = min(qc, cap);
+  int32_t* next = lists + ((t + 1) & 3) * lcap;
+  int* ncnt = &ctl->qcnt[(t + 1) & 3];
+  if (nc > 0) {
+    peel_round_body(lists + (t & 3) * lcap, nc, next, ncnt, d.r + 1, d.k - 2, d.k - 1, g, cap,
+                    account ? &ctl->bytes : nullptr, s_stage);
+  } else {
+    int k = d.k, kidx = d.kidx, fin = 0;  // the k-truss is stable: scan the next threshold
+    peel_next_k(ctl, d.alive, k, kidx, fin);
+    if (!fin && d.alive > 0) peel_scan_body(g.mirror, g.sup, g.stamp, nnz, k - 2, d.r + 1, next, ncnt, cap);
+  }
+}
+
+// The device-driven queue drained after tick t: the state tick t + 1 will
+// start from, to the host (sequence seq) and as the compaction view.
+__global__ void k_peel_drain(int64_t t, PeelCtl* __restrict__ ctl, int cap, long long* __restrict__ host,
+                             long long seq) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int rk;
+  long long ra;
+  int rp_;
+  const PeelDyn d = peel_tick_state(ctl, t + 1, cap, rk, ra, rp_);
+  ctl->cr = d.r;
+  ctl->clist = (int)((t + 1) & 3);
+  ctl->ccnt = d.fin ? 0 : min(ctl->qcnt[(t + 1) & 3], cap);
+  peel_publish(host, ctl->ccnt, d.alive, d.fin, t, seq);
+}
+
+// Host-steered runs before a compaction (a k boundary: no list pending).
+__global__ void k_peel_cview_host(PeelCtl* __restrict__ ctl) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->cr = ctl->h.r;
+    ctl->clist = ctl->ic;
+    ctl->ccnt = 0;
+  }
+}
+
+// Entries of no round still pending or to come (stamp <= cr) lose their
 // support value (-1), so a compaction at threshold 0 keeps exactly the alive
-// ones, the ones the pending round removes included.
+// ones, the ones of the pending list included.
 __global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __restrict__ sup, int64_t nnz,
                                  const PeelCtl* __restrict__ ctl) {
-  const int r = ctl->r;
+  const int r = ctl->cr;
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
     if (stamp[p] <= r) sup[p] = -1;
 }
 
 // After a compaction: the pending list's entries move to their new positions
-// and keep their stamp r + 1 (both directions); every other stamp is reset.
-__global__ void k_peel_remap(int32_t* __restrict__ list0, int32_t* __restrict__ list1,
-                             const PeelCtl* __restrict__ ctl, const int32_t* __restrict__ newpos,
-                             const int32_t* __restrict__ mirror_new, int32_t* __restrict__ stamp) {
-  const int ic = ctl->ic, n = ctl->cnt[ic], r1 = ctl->r + 1;
-  int32_t* list = ic ? list1 : list0;
+// and keep their stamp cr + 1 (both directions); every other stamp was reset.
+__global__ void k_peel_remap(int32_t* __restrict__ lists, int64_t lcap, const PeelCtl* __restrict__ ctl,
+                             const int32_t* __restrict__ newpos, const int32_t* __restrict__ mirror_new,
+                             int32_t* __restrict__ stamp) {
+  const int n = ctl->ccnt, r1 = ctl->cr + 1;
+  int32_t* list = lists + ctl->clist * lcap;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const int c = newpos[list[t]];
     list[t] = c;
     stamp[c] = r1;
     stamp[mirror_new[c]] = r1;
+  }
+}
+
+// Tick 0 of a device-driven run: the first threshold's scan into list 1.
+__global__ __launch_bounds__(256) void k_peel_tick0(int32_t* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
+                                                    const int32_t* __restrict__ mirror,
+                                                    const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
+                                                    int64_t nnz, int cap) {
+  peel_scan_body(mirror, sup, stamp, nnz, ctl->dyn[0].k - 2, ctl->dyn[0].r + 1, lists + lcap, &ctl->qcnt[1], cap);
+}
+
+// The final compaction of a finished run (no list pending).
+__global__ void k_peel_cview_end(PeelCtl* __restrict__ ctl, int r) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->cr = r;
+    ctl->clist = 0;
+    ctl->ccnt = 0;
   }
 }
 
@@ -1014,7 +1084,7 @@ static hipEvent_t peel_event(Context& c) {
 // the first graph are set up by truss_loop; arguments and results as there.
 // host_steer (always with a k callback): the host reads each round's list
 // size, answers each k as it is reached and takes on the callback's extra k
-// values.  Otherwise the rounds are queued on the device a few ahead of it.
+// values.  Otherwise the rounds run as device ticks queued a few ahead.
 static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, TrussWork& tw, hipStream_t s,
                        bool host_steer, int* cur_out, int64_t* nnz_out, int32_t* kmax_out, ga_truss_stats* st,
                        int64_t* nnz_at, int32_t* passes_at, const KDone* on_k) {
@@ -1023,11 +1093,10 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   host_steer = host_steer || on_k != nullptr;
   GA_REQUIRE(all_k || host_steer || nk <= TRUSS_MAX_K, GA_ERR_LIMIT, "at most 64 k values in one k-truss run");
   const int64_t lcap = nnz0 / 2 + 64;  // canonical entries: every list fits
-  tw.plist.ensure(sizeof(int32_t) * 2 * lcap);
+  tw.plist.ensure(sizeof(int32_t) * 4 * lcap);
   tw.pcnt.ensure(sizeof(PeelCtl));
   tw.stamp.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
-  int32_t* list0 = tw.plist.as<int32_t>();
-  int32_t* list1 = list0 + lcap;
+  int32_t* lists = tw.plist.as<int32_t>();
   auto* ctl = tw.pcnt.as<PeelCtl>();
   int32_t* stamp = tw.stamp.as<int32_t>();
   const int64_t passes0 = (all_k && nnz0 == 0) ? 0 : 1;  // a decomposition of no entries runs no product
@@ -1046,17 +1115,17 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     support_symmetric_dev(C, tw.mirror[0].as<int32_t>(), tw.sup[0].as<int32_t>(), tw.mw, s, false);
   }
   int64_t stored = nnz0;  // entries in the current buffers (alive or not)
+  auto graph = [&]() {
+    return PeelGraph{tw.cur[cur].rp.as<int32_t>(), tw.cur[cur].col.as<int32_t>(), tw.mirror[cur].as<int32_t>(),
+                     tw.eid[cur].as<int32_t>(), tw.sup[cur].as<int32_t>(), stamp,
+                     all_k ? tw.truss.as<int32_t>() : nullptr};
+  };
   long long* hw_dev = nullptr;
   long long* hw = progress_words(&hw_dev) + 16;  // words 16..20 (see peel_publish)
   hw_dev += 16;
   const unsigned cap = (unsigned)(8 * (int64_t)c.num_cu);
   const unsigned gscan = std::min<unsigned>(grid_for(std::max<int64_t>(stored, 1), 256, 1 << 20), cap);
-  auto list_size = [&]() -> int64_t {
-    const int64_t v = __atomic_load_n(&hw[0], __ATOMIC_RELAXED);
-    GA_REQUIRE(v >= 0 && v <= lcap, GA_ERR_STATE, "peeling list overflow");
-    return v;
-  };
-  auto destroy = [&](unsigned grid, long long* host, long long seq) {
+  auto timed = [&](auto&& launch) {  // a round launch, bracketed by events when profiling
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.prof_events_on) {
       e0 = peel_event(c);
@@ -1064,11 +1133,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       GA_HIP(hipEventRecord(e0, s));
     }
     if (account) c.prof_bytes_launches += 1;
-    hipLaunchKernelGGL(k_peel_destroy, dim3(grid), dim3(256), 0, s, (const int32_t*)list0, list1, ctl,
-                       (const int32_t*)tw.cur[cur].rp.as<int32_t>(), (const int32_t*)tw.cur[cur].col.as<int32_t>(),
-                       (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.eid[cur].as<int32_t>(),
-                       tw.sup[cur].as<int32_t>(), stamp, all_k ? tw.truss.as<int32_t>() : nullptr, (int)lcap,
-                       account ? 1 : 0, host, seq);
+    launch();
     GA_HIP(hipGetLastError());
     if (c.prof_events_on) {
       GA_HIP(hipEventRecord(e1, s));
@@ -1076,17 +1141,8 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       c.prof_launches += 1;
     }
   };
-  auto scan = [&]() {
-    hipLaunchKernelGGL(k_peel_scan, dim3(gscan), dim3(256), 0, s, (const int32_t*)tw.mirror[cur].as<int32_t>(),
-                       (const int32_t*)tw.sup[cur].as<int32_t>(), stamp, stored, list0, list1, ctl, (int)lcap);
-    GA_HIP(hipGetLastError());
-  };
-  auto post = [&](int mode, long long* host, long long seq) {
-    hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, mode, (int)lcap, host, seq);
-    GA_HIP(hipGetLastError());
-  };
-  // compaction (the device idle behind it in stream order): the dead entries
-  // go, supports carried; the pending list moves with its entries
+  // compaction (every kernel before it in stream order is done): the dead
+  // entries go, supports carried; the pending list moves with its entries
   auto compact = [&](int64_t alive) {
     hipLaunchKernelGGL(k_peel_drop_dead, dim3(grid_for(stored, 256, 8192)), dim3(256), 0, s, (const int32_t*)stamp,
                        tw.sup[cur].as<int32_t>(), stored, (const PeelCtl*)ctl);
@@ -1110,7 +1166,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                        (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
                        tw.mirror[cur ^ 1].as<int32_t>(), (const TrussDevState*)nullptr);
     fill_u32(stamp, (uint32_t)PEEL_ALIVE, alive, s);
-    hipLaunchKernelGGL(k_peel_remap, dim3(grid_for(lcap, 256, 4096)), dim3(256), 0, s, list0, list1,
+    hipLaunchKernelGGL(k_peel_remap, dim3(grid_for(lcap, 256, 4096)), dim3(256), 0, s, lists, lcap,
                        (const PeelCtl*)ctl, (const int32_t*)tw.newpos.as<int32_t>(),
                        (const int32_t*)tw.mirror[cur ^ 1].as<int32_t>(), stamp);
     GA_HIP(hipGetLastError());
@@ -1124,7 +1180,13 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       if (armed) (void)hipStreamSynchronize(s);
     }
   } drain{s};
-  PeelCtl h{};
+  auto list_size = [&]() -> int64_t {
+    const int64_t v = __atomic_load_n(&hw[0], __ATOMIC_RELAXED);
+    GA_REQUIRE(v >= 0 && v <= lcap && !__atomic_load_n(&hw[2], __ATOMIC_RELAXED), GA_ERR_STATE,
+               "peeling list overflow");
+    return v;
+  };
+  PeelDyn out{};
   if (host_steer) {
     std::vector<int32_t> kv(ks, ks + (all_k ? 0 : nk));
     std::vector<int64_t> nz_v(kv.size(), 0);
@@ -1132,13 +1194,17 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     int kidx = 0, k = all_k ? 3 : kv[0];
     int32_t kmax = 2;
     int64_t alive = nnz0, passes = passes0, ncur = 0;
-    if (alive > 0) {
+    auto scan = [&]() {
       const long long seq = next_host_seq();
-      scan();
-      post(2, hw_dev, seq);
+      hipLaunchKernelGGL(k_peel_scan, dim3(gscan), dim3(256), 0, s, lists, lcap, ctl,
+                         (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.sup[cur].as<int32_t>(),
+                         stamp, stored, (int)lcap);
+      hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, 1, (int)lcap, hw_dev, seq);
+      GA_HIP(hipGetLastError());
       wait_host_word(&hw[4], seq, s);
       ncur = list_size();
-    }
+    };
+    if (alive > 0) scan();
     const int64_t max_rounds = nnz0 / 2 + n + 4 * (int64_t)kv.size() + 64;  // a round removes an edge or moves k on
     int64_t rounds = 0;
     while (true) {
@@ -1170,91 +1236,128 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
         hipLaunchKernelGGL(k_peel_set_k, dim3(1), dim3(64), 0, s, ctl, k, 1);
         if (last) break;
         if (alive == 0) continue;  // (a k list past an empty truss: every later k is empty too)
-        if (2 * alive <= stored) compact(alive);
-        const long long seq = next_host_seq();
+        if (2 * alive <= stored) {
+          hipLaunchKernelGGL(k_peel_cview_host, dim3(1), dim3(64), 0, s, ctl);
+          compact(alive);
+        }
         scan();
-        post(2, hw_dev, seq);
-        wait_host_word(&hw[4], seq, s);
-        ncur = list_size();
         continue;
       }
       const long long seq = next_host_seq();
-      destroy(ncur <= PEEL_SPLIT_MAX_EDGES ? (unsigned)(ncur * PEEL_SPLIT)
-                                           : std::min<unsigned>(grid_for(ncur, 4, 1 << 20), cap),
-              nullptr, 0);
-      post(0, hw_dev, seq);
+      const PeelGraph g = graph();
+      const unsigned grid = ncur <= PEEL_SPLIT_MAX_EDGES ? (unsigned)(ncur * PEEL_SPLIT)
+                                                          : std::min<unsigned>(grid_for(ncur, 4, 1 << 20), cap);
+      timed([&] {
+        hipLaunchKernelGGL(k_peel_destroy, dim3(grid), dim3(256), 0, s, lists, lcap, ctl, g, (int)lcap,
+                           account ? 1 : 0);
+      });
+      hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, 0, (int)lcap, hw_dev, seq);
+      GA_HIP(hipGetLastError());
       wait_host_word(&hw[4], seq, s);
       alive -= 2 * ncur;
       if (!all_k || alive > 0) ++passes;
       ncur = list_size();
     }
+    PeelCtl h{};
     GA_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
-    GA_REQUIRE(!h.err && h.alive == alive && h.passes == passes, GA_ERR_STATE, "peeling state disagrees with the host");
+    GA_REQUIRE(!h.h.err && h.h.alive == alive && h.h.passes == passes, GA_ERR_STATE,
+               "peeling state disagrees with the host");
     for (int i = 0; i < (all_k ? 0 : nk); ++i) {  // the caller's k values (joined ones went to on_k)
       const auto at = std::lower_bound(kv.begin(), kv.end(), ks[i]) - kv.begin();
       if (nnz_at) nnz_at[i] = nz_v[at];
       if (passes_at) passes_at[i] = ps_v[at];
     }
-    h.kmax = kmax;
+    out = h.h;
+    out.kmax = kmax;
   } else {
-    // device-driven: ticks of (round, advance) queued a few ahead of the
-    // device, which reports through the progress words; the host drains the
-    // queue only to compact, when the dead entries are at least half of the
-    // stored ones
-    constexpr int64_t kInFlight = 4;
+    // device-driven: ticks queued a few ahead of the device, which reports
+    // every fourth tick and when the run ends; to compact, the host drains
+    // the queue through k_peel_drain and compacts when the dead entries are
+    // at least half of the stored ones
+    constexpr int64_t kInFlight = 12;
     const int64_t max_ticks = nnz0 / 2 + n + nk + 64;  // a tick removes an edge or moves k on
     __atomic_store_n(&hw[1], (long long)nnz0, __ATOMIC_RELAXED);
     __atomic_store_n(&hw[2], 0ll, __ATOMIC_RELAXED);
     __atomic_store_n(&hw[3], 0ll, __ATOMIC_RELEASE);
-    if (nnz0 > 0) scan();  // the first threshold
-    const unsigned gdes = std::min<unsigned>(cap, PEEL_GRID);
-    int64_t queued = 0;
+    if (nnz0 > 0)  // tick 0: the first threshold, into list 1
+      hipLaunchKernelGGL(k_peel_tick0, dim3(gscan), dim3(256), 0, s, lists, lcap, ctl,
+                         (const int32_t*)tw.mirror[0].as<int32_t>(), (const int32_t*)tw.sup[0].as<int32_t>(), stamp,
+                         stored, (int)lcap);
+    const unsigned gtick = std::min<unsigned>(gscan, 4 * (unsigned)c.num_cu);
+    int64_t t = 0;  // the last tick queued
     auto t_seen = std::chrono::steady_clock::now();
     int64_t seen = -1;
     while (true) {
-      const int64_t done_ticks = __atomic_load_n(&hw[3], __ATOMIC_ACQUIRE);
-      if (__atomic_load_n(&hw[2], __ATOMIC_RELAXED)) break;
+      if (__atomic_load_n(&hw[2], __ATOMIC_ACQUIRE)) break;
+      const int64_t done_t = __atomic_load_n(&hw[3], __ATOMIC_ACQUIRE);
       const int64_t alive_seen = __atomic_load_n(&hw[1], __ATOMIC_RELAXED);
-      const bool want_compact = alive_seen > 0 && 2 * alive_seen <= stored;
-      if (want_compact && queued == done_ticks) {
-        compact(alive_seen);  // every queued tick ran: the state is final up to here
-      } else if (!want_compact && queued - done_ticks < kInFlight) {
-        GA_REQUIRE(queued < max_ticks, GA_ERR_STATE, "peeling did not converge");
-        destroy(gdes, nullptr, 0);
-        hipLaunchKernelGGL(k_peel_advance, dim3(gscan), dim3(256), 0, s, (const int32_t*)tw.mirror[cur].as<int32_t>(),
-                           (const int32_t*)tw.sup[cur].as<int32_t>(), stamp, stored, list0, list1, ctl, (int)lcap);
-        post(1, hw_dev, 0);
-        ++queued;
+      if (alive_seen > 0 && 2 * alive_seen <= stored) {
+        const long long seq = next_host_seq();
+        hipLaunchKernelGGL(k_peel_drain, dim3(1), dim3(64), 0, s, t, ctl, (int)lcap, hw_dev, seq);
+        GA_HIP(hipGetLastError());
+        wait_host_word(&hw[4], seq, s);
+        if (__atomic_load_n(&hw[2], __ATOMIC_RELAXED)) break;
+        const int64_t alive = __atomic_load_n(&hw[1], __ATOMIC_RELAXED);
+        if (alive > 0 && 2 * alive <= stored) compact(alive);
+        continue;
+      }
+      if (t - done_t < kInFlight) {
+        GA_REQUIRE(t < max_ticks, GA_ERR_STATE, "peeling did not converge");
+        ++t;
+        const PeelGraph g = graph();
+        timed([&] {
+          hipLaunchKernelGGL(k_peel_tick, dim3(gtick), dim3(256), 0, s, t, lists, lcap, ctl, g, stored, (int)lcap,
+                             account ? 1 : 0, hw_dev);
+        });
         continue;
       }
       const auto now = std::chrono::steady_clock::now();
-      if (done_ticks != seen) {
-        seen = done_ticks;
+      if (done_t != seen) {
+        seen = done_t;
         t_seen = now;
       } else if (now - t_seen > std::chrono::seconds(60)) {
         GA_HIP(hipStreamSynchronize(s));  // a tick that ran for a minute: report, do not spin on
-        GA_REQUIRE(__atomic_load_n(&hw[3], __ATOMIC_ACQUIRE) != seen, GA_ERR_STATE, "peeling made no progress");
+        GA_REQUIRE(__atomic_load_n(&hw[3], __ATOMIC_ACQUIRE) != seen || __atomic_load_n(&hw[2], __ATOMIC_ACQUIRE),
+                   GA_ERR_STATE, "peeling made no progress");
       }
     }
+    // one more tick stores the final state (and the last k's record) when the
+    // end was seen by a drain; after a tick that ended the run it changes nothing
+    ++t;
+    {
+      const PeelGraph g = graph();
+      hipLaunchKernelGGL(k_peel_tick, dim3(gtick), dim3(256), 0, s, t, lists, lcap, ctl, g, stored, (int)lcap, 0,
+                         (long long*)nullptr);
+      GA_HIP(hipGetLastError());
+    }
+    PeelCtl h{};
     GA_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
-    GA_REQUIRE(h.fin && !h.err, GA_ERR_STATE, "peeling did not finish (a removal list outgrew its capacity)");
+    out = h.dyn[t & 1];
+    GA_REQUIRE(out.fin && !out.err, GA_ERR_STATE, "peeling did not finish (a removal list outgrew its capacity)");
     for (int i = 0; i < (all_k ? 0 : nk); ++i) {
       if (nnz_at) nnz_at[i] = h.nnz_at[i];
       if (passes_at) passes_at[i] = h.passes_at[i];
     }
   }
-  if (cur_out && stored != h.alive) compact(h.alive);
+  if (cur_out && stored != out.alive) {
+    hipLaunchKernelGGL(k_peel_cview_end, dim3(1), dim3(64), 0, s, ctl, out.r);
+    compact(out.alive);
+  }
   GA_HIP(hipStreamSynchronize(s));
   drain.armed = false;
-  if (account) c.prof_bytes_host += h.bytes;
+  if (account) {
+    long long bytes = 0;
+    GA_HIP(hipMemcpy(&bytes, &ctl->bytes, sizeof(bytes), hipMemcpyDeviceToHost));
+    c.prof_bytes_host += bytes;
+  }
   if (cur_out) *cur_out = cur;
-  if (nnz_out) *nnz_out = h.alive;
-  if (kmax_out) *kmax_out = h.kmax;
+  if (nnz_out) *nnz_out = out.alive;
+  if (kmax_out) *kmax_out = out.kmax;
   if (st) {
-    st->passes = h.passes;
-    st->steps = h.steps;
+    st->passes = out.passes;
+    st->steps = out.steps;
   }
 }
 
