@@ -225,9 +225,10 @@ def bench_ktruss(args) -> dict:
                    "n": g.n, "m_undirected": m, "nnz": g.nnz, "kmax": kmax,
                    "support_passes": stats.get("passes"), "filter_steps": stats.get("steps"),
                    "truss_sha256": digest},
-        "roofline": _roofline(timing, acct, "k_mxm_items (support passes: symmetric half pass; triangle "
-                                            "listing while the graph holds >= 2^20 entries)",
-                              "k_mxm_items/truss_rmat16"),
+        "roofline": _roofline(timing, acct, "k_peel_tick (a peeling round: the triangles through the removed "
+                                            "edges, supports of the survivors decremented; or the scan of the next "
+                                            "threshold); the one full support product per step is the tc workload's "
+                                            "kernel", "k_peel_tick/truss_rmat16"),
     }
     golden = _golden_truss_rmat16()
     if golden:

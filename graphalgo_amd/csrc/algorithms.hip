@@ -633,8 +633,9 @@ struct PeelCtl {
 
 // Wavefront-aggregated append; entries past cap are dropped (the count still
 // grows, and the run stops with an error).
-__device__ inline void peel_push(bool sel, int32_t pos, int32_t* __restrict__ list, int* __restrict__ cnt, int cap,
-                                 int lane) {
+// A list item is an edge: its canonical (lower) entry and the transposed one.
+__device__ inline void peel_push(bool sel, int32_t pos, int32_t other, int2* __restrict__ list, int* __restrict__ cnt,
+                                 int cap, int lane) {
   const unsigned long long m = __ballot(sel);
   if (!m) return;
   int base = 0;
@@ -643,7 +644,7 @@ __device__ inline void peel_push(bool sel, int32_t pos, int32_t* __restrict__ li
   if (sel) {
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int at = base + __popcll(m & lt);
-    if (at < cap) list[at] = pos;
+    if (at < cap) list[at] = make_int2(pos, other);
   }
 }
 
@@ -658,24 +659,61 @@ __device__ inline void peel_publish(long long* __restrict__ host, long long size
   __hip_atomic_store(&host[4], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// alive canonical entries with support below thr: stamped r, listed
+// alive canonical entries with support below thr: stamped r, listed.  A
+// workgroup takes 1024 entries at a time (4 per thread, coalesced) and
+// claims their list slots with one atomic (the list counter is a single
+// word: per-wavefront claims queued ~28 k atomics on it for R-MAT 16's first
+// threshold).  The body holds __syncthreads: every thread of the workgroup
+// calls it.
+constexpr int PEEL_SCAN_PER = 4;
 __device__ inline void peel_scan_body(const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
                                       int32_t* __restrict__ stamp, int64_t nnz, int thr, int r,
-                                      int32_t* __restrict__ list, int* __restrict__ cnt, int cap) {
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nnz; base += stride) {
-    const int64_t p = base + threadIdx.x;
-    bool sel = false;
-    if (p < nnz) {
-      const int m = mirror[p], sv = sup[p], sp = stamp[p];  // independent loads
-      if (m < p && sv < thr && sp == PEEL_ALIVE) {
-        sel = true;
-        stamp[p] = r;
-        stamp[m] = r;
+                                      int2* __restrict__ list, int* __restrict__ cnt, int cap) {
+  __shared__ int s_wcnt[4];
+  __shared__ int s_base;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t tile = (int64_t)blockDim.x * PEEL_SCAN_PER;
+  for (int64_t base = blockIdx.x * tile; base < nnz; base += (int64_t)gridDim.x * tile) {
+    unsigned long long mk[PEEL_SCAN_PER];
+    int mm[PEEL_SCAN_PER];
+    int wc = 0;
+#pragma unroll
+    for (int u = 0; u < PEEL_SCAN_PER; ++u) {
+      const int64_t p = base + u * blockDim.x + threadIdx.x;
+      bool sel = false;
+      int m = 0;
+      if (p < nnz) {
+        m = mirror[p];
+        const int sv = sup[p], sp = stamp[p];  // independent loads
+        sel = m < p && sv < thr && sp == PEEL_ALIVE;
       }
+      mm[u] = m;
+      mk[u] = __ballot(sel);
+      wc += __popcll(mk[u]);
     }
-    peel_push(sel, (int32_t)p, list, cnt, cap, lane);
+    if (lane == 0) s_wcnt[wv] = wc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += s_wcnt[w];
+      s_base = tot ? atomicAdd(cnt, tot) : 0;
+    }
+    __syncthreads();
+    int at = s_base;
+    for (int w = 0; w < wv; ++w) at += s_wcnt[w];
+#pragma unroll
+    for (int u = 0; u < PEEL_SCAN_PER; ++u) {
+      const int64_t p = base + u * blockDim.x + threadIdx.x;
+      if ((mk[u] >> lane) & 1ull) {
+        const int slot = at + __popcll(mk[u] & lt);
+        stamp[p] = r;
+        stamp[mm[u]] = r;
+        if (slot < cap) list[slot] = make_int2((int32_t)p, mm[u]);
+      }
+      at += __popcll(mk[u]);
+    }
+    __syncthreads();  // s_wcnt and s_base are reused by the next tile
   }
 }
 
@@ -737,12 +775,12 @@ struct PeelEdge {
   int c, mc, ab, ae, arow, bb, be, brow, stride, m;
 };
 
-__device__ inline PeelEdge peel_edge(int c, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                     const int32_t* __restrict__ mirror, int cap) {
+__device__ inline PeelEdge peel_edge(int2 item, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                     int cap) {
   PeelEdge e;
-  e.c = c;
-  e.mc = mirror[c];
-  const int j = col[c], i = col[e.mc];
+  e.c = item.x;
+  e.mc = item.y;
+  const int j = col[e.c], i = col[e.mc];
   const int ib = rp[i], ie = rp[i + 1], jb = rp[j], je = rp[j + 1];
   const bool pi = (ie - ib) <= (je - jb);  // probe row i
   e.ab = pi ? ib : jb;
@@ -765,14 +803,13 @@ __device__ inline PeelEdge peel_direct(PeelEdge e) {
 }
 
 // One probed entry q of the edge's probe row: the decrements it owns and the
-// canonical positions to push (-1: none).
+// edges to push (canonical entry, transposed entry; x = -1: none).
 __device__ inline void peel_probe(const PeelEdge& e, int q, const int32_t* __restrict__ st,
                                   const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
                                   int32_t* __restrict__ sup, int32_t* __restrict__ stamp, int thr, int r,
-                                  int& push_a, int& push_b, long long& found) {
-  push_a = push_b = -1;
-  const int w = col[q];
-  const int sa = stamp[q];
+                                  int2& push_a, int2& push_b, long long& found) {
+  push_a = push_b = make_int2(-1, -1);
+  const int w = col[q], sa = stamp[q], mq = mirror[q];  // independent loads
   if (sa < r || w == e.brow) return;
   int lo = 0, hi = e.m;
   while (lo < hi) {
@@ -796,19 +833,20 @@ __device__ inline void peel_probe(const PeelEdge& e, int q, const int32_t* __res
   if (pos < 0) return;
   const int sb = stamp[pos];
   if (sb < r) return;
-  const int cid_a = w < e.arow ? q : mirror[q];
-  const int cid_b = w < e.brow ? pos : mirror[pos];
+  const int mp = mirror[pos];
+  const int cid_a = w < e.arow ? q : mq;
+  const int cid_b = w < e.brow ? pos : mp;
   if ((sa == r && cid_a < e.c) || (sb == r && cid_b < e.c)) return;  // another removed edge owns it
   ++found;
   if (sa > r && atomicSub(&sup[cid_a], 1) == thr) {
-    push_a = cid_a;
+    push_a = make_int2(cid_a, cid_a == q ? mq : q);
     stamp[q] = r + 1;
-    stamp[mirror[q]] = r + 1;
+    stamp[mq] = r + 1;
   }
   if (sb > r && atomicSub(&sup[cid_b], 1) == thr) {
-    push_b = cid_b;
+    push_b = make_int2(cid_b, cid_b == pos ? mp : pos);
     stamp[pos] = r + 1;
-    stamp[mirror[pos]] = r + 1;
+    stamp[mp] = r + 1;
   }
 }
 
@@ -816,6 +854,7 @@ constexpr int PEEL_WSTAGE = 1024;         // sample entries per wavefront (large
 constexpr int PEEL_BSTAGE = 4096;         // sample entries per workgroup (small rounds)
 constexpr int PEEL_SPLIT = 8;             // workgroups per edge in small rounds
 constexpr int PEEL_SPLIT_MAX_EDGES = 64;  // rounds of at most this many edges split them
+constexpr int PEEL_STAGE_ROUND = 4096;    // rounds up to this size stage every long target row
 
 // The round itself.  Large rounds: a wavefront per edge (a wavefront probing
 // one chunk searches the target row in global memory, a longer probe row
@@ -832,7 +871,7 @@ struct PeelGraph {
   int32_t* truss;
 };
 
-__device__ inline void peel_round_body(const int32_t* __restrict__ cur, int nc, int32_t* __restrict__ next,
+__device__ inline void peel_round_body(const int2* __restrict__ cur, int nc, int2* __restrict__ next,
                                        int* __restrict__ ncnt, int r, int thr, int tval, const PeelGraph& g, int cap,
                                        long long* __restrict__ bytes_ctr, int32_t* __restrict__ s_stage) {
   const int lane = threadIdx.x & 63;
@@ -841,13 +880,16 @@ __device__ inline void peel_round_body(const int32_t* __restrict__ cur, int nc, 
     int32_t* st = s_stage + (threadIdx.x >> 6) * PEEL_WSTAGE;
     const int64_t wpb = blockDim.x >> 6;
     for (int64_t t = blockIdx.x * wpb + (threadIdx.x >> 6); t < nc; t += (int64_t)gridDim.x * wpb) {
-      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, g.mirror, PEEL_WSTAGE);
+      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, PEEL_WSTAGE);
       if (g.truss && lane == 0) {
         g.truss[g.eid[e.c]] = tval;
         g.truss[g.eid[e.mc]] = tval;
       }
       if (bytes_ctr && lane == 0) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
-      const bool staged = e.ae - e.ab > 64;
+      // a long target row is staged when several chunks probe it or the
+      // round is small enough that the staging loads are cheaper than the
+      // chains of a binary search in global memory
+      const bool staged = e.be - e.bb > 64 && (e.ae - e.ab > 64 || nc <= PEEL_STAGE_ROUND);
       const PeelEdge pe = staged ? e : peel_direct(e);
       __threadfence_block();  // the previous edge's sample reads are done
       if (staged)
@@ -855,17 +897,17 @@ __device__ inline void peel_round_body(const int32_t* __restrict__ cur, int nc, 
       __threadfence_block();
       for (int q0 = e.ab; q0 < e.ae; q0 += 64) {
         const int q = q0 + lane;
-        int pa = -1, pb = -1;
+        int2 pa = make_int2(-1, -1), pb = make_int2(-1, -1);
         if (q < e.ae)
           peel_probe(pe, q, staged ? st : g.col + e.bb, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
-        peel_push(pa >= 0, pa, next, ncnt, cap, lane);
-        peel_push(pb >= 0, pb, next, ncnt, cap, lane);
+        peel_push(pa.x >= 0, pa.x, pa.y, next, ncnt, cap, lane);
+        peel_push(pb.x >= 0, pb.x, pb.y, next, ncnt, cap, lane);
       }
     }
   } else {
     for (int64_t b = blockIdx.x; b < (int64_t)nc * PEEL_SPLIT; b += gridDim.x) {
       const int t = (int)(b / PEEL_SPLIT), part = (int)(b % PEEL_SPLIT);
-      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, g.mirror, PEEL_BSTAGE);
+      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, PEEL_BSTAGE);
       if (part == 0 && threadIdx.x == 0) {
         if (g.truss) {
           g.truss[g.eid[e.c]] = tval;
@@ -878,10 +920,10 @@ __device__ inline void peel_round_body(const int32_t* __restrict__ cur, int nc, 
       __syncthreads();
       for (int q0 = e.ab + part * 256; q0 < e.ae; q0 += PEEL_SPLIT * 256) {
         const int q = q0 + threadIdx.x;
-        int pa = -1, pb = -1;
+        int2 pa = make_int2(-1, -1), pb = make_int2(-1, -1);
         if (q < e.ae) peel_probe(e, q, s_stage, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
-        peel_push(pa >= 0, pa, next, ncnt, cap, lane);
-        peel_push(pb >= 0, pb, next, ncnt, cap, lane);
+        peel_push(pa.x >= 0, pa.x, pa.y, next, ncnt, cap, lane);
+        peel_push(pb.x >= 0, pb.x, pb.y, next, ncnt, cap, lane);
       }
     }
   }
@@ -892,7 +934,7 @@ __device__ inline void peel_round_body(const int32_t* __restrict__ cur, int nc, 
 }
 
 // ---- host-steered runs: a kernel per round or scan, then k_peel_post ------
-__global__ __launch_bounds__(256) void k_peel_scan(int32_t* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
+__global__ __launch_bounds__(256) void k_peel_scan(int2* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
                                                    const int32_t* __restrict__ mirror,
                                                    const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
                                                    int64_t nnz, int cap) {
@@ -900,7 +942,7 @@ __global__ __launch_bounds__(256) void k_peel_scan(int32_t* __restrict__ lists, 
   peel_scan_body(mirror, sup, stamp, nnz, ctl->h.k - 2, ctl->h.r + 1, lists + ic * lcap, &ctl->cnt[ic], cap);
 }
 
-__global__ __launch_bounds__(256) void k_peel_destroy(int32_t* __restrict__ lists, int64_t lcap,
+__global__ __launch_bounds__(256) void k_peel_destroy(int2* __restrict__ lists, int64_t lcap,
                                                       PeelCtl* __restrict__ ctl, PeelGraph g, int cap, int account) {
   __shared__ int32_t s_stage[PEEL_BSTAGE];
   const int ic = ctl->ic;
@@ -970,7 +1012,7 @@ __device__ inline PeelDyn peel_tick_state(const PeelCtl* __restrict__ ctl, int64
   return d;
 }
 
-__global__ __launch_bounds__(256) void k_peel_tick(int64_t t, int32_t* __restrict__ lists, int64_t lcap,
+__global__ __launch_bounds__(256) void k_peel_tick(int64_t t, int2* __restrict__ lists, int64_t lcap,
                                                    PeelCtl* __restrict__ ctl, PeelGraph g, int64_t nnz, int cap,
                                                    int account, long long* __restrict__ host) {
   __shared__ int32_t s_stage[PEEL_BSTAGE];
@@ -991,7 +1033,7 @@ __global__ __launch_bounds__(256) void k_peel_tick(int64_t t, int32_t* __restric
   }
   if (d.fin) return;
   const int nc = min(qc, cap);
-  int32_t* next = lists + ((t + 1) & 3) * lcap;
+  int2* next = lists + ((t + 1) & 3) * lcap;
   int* ncnt = &ctl->qcnt[(t + 1) & 3];
   if (nc > 0) {
     peel_round_body(lists + (t & 3) * lcap, nc, next, ncnt, d.r + 1, d.k - 2, d.k - 1, g, cap,
@@ -999,7 +1041,11 @@ __global__ __launch_bounds__(256) void k_peel_tick(int64_t t, int32_t* __restric
   } else {
     int k = d.k, kidx = d.kidx, fin = 0;  // the k-truss is stable: scan the next threshold
     peel_next_k(ctl, d.alive, k, kidx, fin);
-    if (!fin && d.alive > 0) peel_scan_body(g.mirror, g.sup, g.stamp, nnz, k - 2, d.r + 1, next, ncnt, cap);
+    if (!fin && d.alive > 0) {
+      peel_scan_body(g.mirror, g.sup, g.stamp, nnz, k - 2, d.r + 1, next, ncnt, cap);
+      // accounting: the scan reads mirror, support and stamp of every stored entry
+      if (account && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&ctl->bytes, 12ull * nnz);
+    }
   }
 }
 
@@ -1039,21 +1085,21 @@ __global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __r
 
 // After a compaction: the pending list's entries move to their new positions
 // and keep their stamp cr + 1 (both directions); every other stamp was reset.
-__global__ void k_peel_remap(int32_t* __restrict__ lists, int64_t lcap, const PeelCtl* __restrict__ ctl,
-                             const int32_t* __restrict__ newpos, const int32_t* __restrict__ mirror_new,
-                             int32_t* __restrict__ stamp) {
+__global__ void k_peel_remap(int2* __restrict__ lists, int64_t lcap, const PeelCtl* __restrict__ ctl,
+                             const int32_t* __restrict__ newpos, int32_t* __restrict__ stamp) {
   const int n = ctl->ccnt, r1 = ctl->cr + 1;
-  int32_t* list = lists + ctl->clist * lcap;
+  int2* list = lists + ctl->clist * lcap;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-    const int c = newpos[list[t]];
-    list[t] = c;
+    const int2 it = list[t];
+    const int c = newpos[it.x], m = newpos[it.y];
+    list[t] = make_int2(c, m);
     stamp[c] = r1;
-    stamp[mirror_new[c]] = r1;
+    stamp[m] = r1;
   }
 }
 
 // Tick 0 of a device-driven run: the first threshold's scan into list 1.
-__global__ __launch_bounds__(256) void k_peel_tick0(int32_t* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
+__global__ __launch_bounds__(256) void k_peel_tick0(int2* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
                                                     const int32_t* __restrict__ mirror,
                                                     const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
                                                     int64_t nnz, int cap) {
@@ -1093,10 +1139,10 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   host_steer = host_steer || on_k != nullptr;
   GA_REQUIRE(all_k || host_steer || nk <= TRUSS_MAX_K, GA_ERR_LIMIT, "at most 64 k values in one k-truss run");
   const int64_t lcap = nnz0 / 2 + 64;  // canonical entries: every list fits
-  tw.plist.ensure(sizeof(int32_t) * 4 * lcap);
+  tw.plist.ensure(sizeof(int2) * 4 * lcap);
   tw.pcnt.ensure(sizeof(PeelCtl));
   tw.stamp.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
-  int32_t* lists = tw.plist.as<int32_t>();
+  int2* lists = tw.plist.as<int2>();
   auto* ctl = tw.pcnt.as<PeelCtl>();
   int32_t* stamp = tw.stamp.as<int32_t>();
   const int64_t passes0 = (all_k && nnz0 == 0) ? 0 : 1;  // a decomposition of no entries runs no product
@@ -1107,11 +1153,22 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                           hipMemcpyHostToDevice, s));
   fill_u32(stamp, (uint32_t)PEEL_ALIVE, nnz0, s);
   const bool account = c.prof_bytes;
-  // the first supports: one full product of the input graph
+  // the first supports: one full product of the input graph (outside the
+  // kernel timing and byte accounting, which cover the peeling launches)
   int cur = 0;
   {
     const DCsr C{n, n, nnz0, tw.cur[0].rp.as<int32_t>(), tw.cur[0].col.as<int32_t>(), nullptr};
     tw.mw.balance = false;
+    struct Hooks {
+      Context& c;
+      bool ev, by;
+      ~Hooks() {
+        c.prof_events_on = ev;
+        c.prof_bytes = by;
+      }
+    } hooks{c, c.prof_events_on, c.prof_bytes};
+    c.prof_events_on = false;
+    c.prof_bytes = false;
     support_symmetric_dev(C, tw.mirror[0].as<int32_t>(), tw.sup[0].as<int32_t>(), tw.mw, s, false);
   }
   int64_t stored = nnz0;  // entries in the current buffers (alive or not)
@@ -1167,8 +1224,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                        tw.mirror[cur ^ 1].as<int32_t>(), (const TrussDevState*)nullptr);
     fill_u32(stamp, (uint32_t)PEEL_ALIVE, alive, s);
     hipLaunchKernelGGL(k_peel_remap, dim3(grid_for(lcap, 256, 4096)), dim3(256), 0, s, lists, lcap,
-                       (const PeelCtl*)ctl, (const int32_t*)tw.newpos.as<int32_t>(),
-                       (const int32_t*)tw.mirror[cur ^ 1].as<int32_t>(), stamp);
+                       (const PeelCtl*)ctl, (const int32_t*)tw.newpos.as<int32_t>(), stamp);
     GA_HIP(hipGetLastError());
     cur ^= 1;
     stored = alive;
@@ -1585,7 +1641,7 @@ size_t ktruss_workspace_bytes(int64_t n, int64_t nnz) {
   b += lease_bytes(sizeof(long long) * std::max<int64_t>(ceil_div(std::max<int64_t>(nnz, 1), KEEP_TILE), 1));
   b += mxm_workspace_bytes(n, nnz, true);
   b += sel_workspace_bytes(n, nnz);
-  b += lease_bytes(sizeof(int32_t) * 2 * (nnz / 2 + 64)) + lease_bytes(sizeof(PeelCtl));  // peeling lists
+  b += lease_bytes(sizeof(int2) * 4 * (nnz / 2 + 64)) + lease_bytes(sizeof(PeelCtl));  // peeling lists
   b += e;                                                                                   // stamps
   return b;
 }

@@ -1594,9 +1594,39 @@ __global__ void k_mirror_index(const int32_t* __restrict__ rp, const int32_t* __
   }
 }
 
+// Entry-parallel: each lower entry t = (i, j), j < i, finds its row by a
+// binary search of the row pointers and its transposed entry in row j, and
+// writes both directions' mirror indices (a wavefront per row serialised the
+// searches of hub rows: 0.70 ms on R-MAT 16 against 0.04 ms).
+__global__ void k_mirror_lower(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int64_t n,
+                               int64_t nnz, int32_t* __restrict__ mirror) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nnz; t += (int64_t)gridDim.x * blockDim.x) {
+    const int j = col[t];
+    int64_t lo = 0, hi = n;  // row of t: the last i with rp[i] <= t
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (rp[mid + 1] <= t) lo = mid + 1;
+      else hi = mid;
+    }
+    const int i = (int)lo;
+    if (j >= i) continue;
+    int a = rp[j], b = rp[j + 1];
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (col[mid] < i) a = mid + 1;
+      else b = mid;
+    }
+    if (a < rp[j + 1] && col[a] == i) {
+      mirror[t] = a;
+      mirror[a] = (int32_t)t;
+    }
+  }
+}
+
 void mirror_index_dev(const DCsr& C, int32_t* mirror, hipStream_t s) {
   if (C.nrows == 0 || C.nnz == 0) return;
-  hipLaunchKernelGGL(k_mirror_index, dim3(grid_for(C.nrows, 4, 16384)), dim3(256), 0, s, C.rp, C.col, C.nrows,
+  fill_u32(mirror, 0xffffffffu, C.nnz, s);  // -1 where a pattern has no transposed entry
+  hipLaunchKernelGGL(k_mirror_lower, dim3(grid_for(C.nnz, 256, 8192)), dim3(256), 0, s, C.rp, C.col, C.nrows, C.nnz,
                      mirror);
   GA_HIP(hipGetLastError());
 }
