@@ -23,16 +23,16 @@
 // the other slots' queries (continuous batching at the granularity of the
 // paths' launches: a short triangle count proceeds while a k-truss waits on
 // its filter step, and slots refill the moment a query ends).  Admission is
-// in arrival order (no overtaking), so a large query is never starved.  The
-// one exception takes nothing from the others: a queued k-truss query joins
-// a peeling run on its graph that is already going when that run reaches a
-// k at or below the query's (it uses the run's memory and slot), and every
-// k-truss query is answered as soon as its own k is reached.  Both hold for
-// runs of the host-driven peeling loop (graphs above 2^18 stored entries by
-// default); on smaller graphs the device-driven steps run the whole k list
-// before the host sees any k, so their queries are answered when the run
-// ends and queries arriving meanwhile start a run of their own (such runs
-// take about a millisecond).
+// in arrival order (no overtaking), so a large query is never starved.  A
+// worker that admits a query also takes the queued queries of the same kind
+// on the same graph: triangle counts share one count, k-truss queries one
+// peeling run over their k values in ascending order, BC queries of at most
+// 64 sources one multi-source pass.  The one exception to arrival order
+// takes nothing from the others: a queued k-truss query joins a peeling run
+// on its graph that is already going when that run reaches a k at or below
+// the query's (it uses the run's memory and slot), and every k-truss query
+// is answered as soon as its own k is reached (served runs peel with the
+// host reading each round's size, GA_OPT_TRUSS_LOOP 4, so that it can).
 //
 // Graphs may be added while queries run: a query holds a pointer to its
 // graph (taken under the server lock at submit), never an index into the
@@ -177,7 +177,7 @@ size_t ga_server::footprint(const ga_query& q) const {
 }
 
 size_t ga_server::batch_footprint(const std::vector<Job*>& set) const {
-  if (set.size() == 1) return set[0]->footprint;
+  if (set.size() == 1 || set[0]->q.kind == GA_Q_TRIANGLES) return set[0]->footprint;  // one count serves all
   const ServerGraph& g = *set[0]->g;
   if (set[0]->q.kind == GA_Q_BC) {
     std::vector<int64_t> sizes;
@@ -188,11 +188,12 @@ size_t ga_server::batch_footprint(const std::vector<Job*>& set) const {
 }
 
 // Queued jobs that can share the head's run: same graph and kind, BC sets of
-// at most 64 sources (one lane group each), any k-truss.
+// at most 64 sources (one lane group each), any k-truss, any triangle count
+// (one count answers them all).
 std::vector<Job*> ga_server::merge_set(Job* h) const {
   std::vector<Job*> set{h};
   const bool bc = h->q.kind == GA_Q_BC && h->src.size() <= 64;
-  if (max_batch <= 1 || !(bc || h->q.kind == GA_Q_KTRUSS)) return set;
+  if (max_batch <= 1 || !(bc || h->q.kind == GA_Q_KTRUSS || h->q.kind == GA_Q_TRIANGLES)) return set;
   for (size_t i = 1; i < queue.size() && (int)set.size() < max_batch; ++i) {
     Job* c = queue[i];
     if (c->q.graph != h->q.graph || c->q.kind != h->q.kind) continue;
@@ -223,7 +224,8 @@ void ga_server::run_batch(std::vector<Job*>& set, std::vector<char>& early, int 
       w.max_mask_row = w.max_b_row = g.max_row;
       const DCsr vL = g.L.view();
       mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, sums.as<unsigned long long>(), nullptr, w, s);
-      j.res.value = (int64_t)publish_sum_and_wait(w, sums.as<unsigned long long>(), s);
+      const int64_t t = (int64_t)publish_sum_and_wait(w, sums.as<unsigned long long>(), s);
+      for (Job* x : set) x->res.value = t;  // queued counts of the same graph share this one
       break;
     }
     case GA_Q_KTRUSS: {

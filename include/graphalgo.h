@@ -236,11 +236,11 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, dou
  * and the arena holds a block of the footprint; each worker runs its query
  * on a HIP stream of its own, so queries of every kind and size overlap on
  * the device and a new query starts as soon as a slot and memory free up.
- * When a worker admits a k-truss or BC query it also takes the queued
- * queries of the same kind on the same graph (up to max_batch): k-truss
- * queries share one peeling run that stops at each requested k in ascending
- * order; BC queries of at most 64 sources share one multi-source pass, one
- * 64-lane group per query.  Results do not depend on what runs beside or
+ * When a worker admits a query it also takes the queued queries of the same
+ * kind on the same graph (up to max_batch): triangle counts share one count;
+ * k-truss queries share one peeling run that stops at each requested k in
+ * ascending order; BC queries of at most 64 sources share one multi-source
+ * pass, one 64-lane group per query.  Results do not depend on what runs beside or
  * with a query (BC scores are bit-identical to the query run alone). */
 typedef struct ga_server ga_server;
 typedef struct {
