@@ -576,6 +576,13 @@ def bench_serve(args) -> dict:
     plain_res = api.plain_replay(graphs, specs, arr)
     plain = S.latency_summary(plain_res, arr)
     _log(f"plain replay: p50 {plain['p50_ms']:.2f} ms p99 {plain['p99_ms']:.2f} ms")
+    # a second operating point that both paths sustain: 0.8 x the plain capacity
+    qps_low = round(0.8 * cap_plain, 1)
+    specs_low, arr_low = S.make_stream(pool, nq, qps_low, seed=4)
+    low_served = S.latency_summary(srv.replay(specs_low, arr_low), arr_low)
+    low_plain = S.latency_summary(api.plain_replay(graphs, specs_low, arr_low), arr_low)
+    _log(f"at {qps_low} QPS: served p50 {low_served['p50_ms']:.2f} p99 {low_served['p99_ms']:.2f} ms, "
+         f"plain p50 {low_plain['p50_ms']:.2f} p99 {low_plain['p99_ms']:.2f} ms")
     for q, a, b in zip(specs, res, plain_res):
         if q["kind"] == "bc":
             assert np.allclose(a["bc"], b["bc"], rtol=1e-9), "served and plain BC differ"
@@ -605,8 +612,15 @@ def bench_serve(args) -> dict:
                           "busy_ms_top_kind_at_scale": {k: round(v, 1) for k, v in top}},
            "served_vs_plain": {"throughput": last["throughput_qps"] / plain["throughput_qps"],
                                "capacity": cap_served / cap_plain,
+                               "capacity_without_shared_runs": cap_nobatch / cap_plain,
                                "p50_latency": plain["p50_ms"] / last["p50_ms"],
                                "p99_latency": plain["p99_ms"] / last["p99_ms"]},
+           "at_plain_sustainable_load": {
+               "qps_offered": qps_low, "stream_seed": 4,
+               "served": {k: round(v, 3) for k, v in low_served.items()},
+               "plain": {k: round(v, 3) for k, v in low_plain.items()},
+               "p50_latency_ratio": low_plain["p50_ms"] / low_served["p50_ms"],
+               "p99_latency_ratio": low_plain["p99_ms"] / low_served["p99_ms"]},
            "roofline_note": "queries of every kind run concurrently on 'slots' streams; per-kernel roofline "
                             "is reported by the ktruss, tc, bc and batch workloads"}
     if not args.no_cpu:

@@ -198,9 +198,11 @@ struct Context {
   // rows per claim of the short-row kernels (0 = automatic; GA_ROW_CHUNK
   // overrides it at ga_init, MxmWork::row_chunk per workspace)
   int row_chunk = 0;
-  // short rows of pattern triangle counts: 0 = 16-lane wedge runs
-  // (k_sum_subrows), 8/16/32/64 = lanes per row walking its entries
-  // (k_sum_entries); GA_SUM_SUB overrides it at ga_init
+  // short rows of pattern triangle counts: 0 / 2 = rows packed per
+  // wavefront with rank-located wedges, 256 / 512 wedges per trip
+  // (k_sum_wave), 1 = 16-lane wedge runs (k_sum_subrows), 8/16/32/64 =
+  // lanes per row walking its entries (k_sum_entries); GA_SUM_SUB overrides
+  // it at ga_init
   int sum_sub = 0;
   // kernel timing (events around the masked-product launches) and
   // algorithmic byte accounting (extra counting kernels; kept out of timed runs)
@@ -292,7 +294,7 @@ struct MxmWork {
                                  // -1 = the context default
   int threads = 256;             // workgroup width of the item kernel (64, 128 or 256)
   int row_chunk = 0;             // rows per claim of the short-row kernels (0 = automatic)
-  int sum_sub = -1;              // short pattern sum rows: 0 = wedge runs, 8..64 = lanes per row
+  int sum_sub = -1;              // short pattern sum rows: as Context::sum_sub
                                  // walking entries (-1 = the context default)
   int64_t max_mask_row = INT32_MAX;  // longest mask row (host-known bound); < 65536 enables the rank lookup
   int64_t max_b_row = INT32_MAX;     // longest B row (host-known bound; B.nnz is used when tighter)

@@ -370,7 +370,7 @@ ga_status ga_init(int device) {
     if (const char* ba = std::getenv("GA_BC_ALPHA")) c->bc_alpha = std::max(0.01, std::atof(ba));
     if (const char* ss = std::getenv("GA_SUM_SUB")) {
       const int v = std::atoi(ss);
-      c->sum_sub = (v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
+      c->sum_sub = (v == 1 || v == 2 || v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
     }
     c->prof_dev.ensure(64);
     GA_HIP(hipMemsetAsync(c->prof_dev.p, 0, 64, c->stream));
@@ -398,8 +398,9 @@ ga_status ga_set_option(int option, double value) {
         c.row_chunk = (int)value;
         break;
       case GA_OPT_SUM_SUB:
-        GA_REQUIRE(value == 0.0 || value == 8.0 || value == 16.0 || value == 32.0 || value == 64.0, GA_ERR_ARG,
-                   "sum_sub is 0, 8, 16, 32 or 64");
+        GA_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0 || value == 8.0 || value == 16.0 || value == 32.0 ||
+                       value == 64.0,
+                   GA_ERR_ARG, "sum_sub is 0, 1, 2, 8, 16, 32 or 64");
         c.sum_sub = (int)value;
         break;
       case GA_OPT_ITEM_WEDGES:

@@ -1037,6 +1037,252 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_sum_entries(
   if (wedge_counter && wedges) atomicAdd(wedge_counter, wedges);
 }
 
+// ---- short rows of pattern sum-mode products, rows packed per wavefront ---
+// A wavefront takes consecutive claimed rows of one graph -- up to SW_SLOTS
+// rows holding at most 64 A entries together, or a single longer row -- and
+// runs their A entries as one flattened chunk: lane L stages entry L (its B
+// row bounds, truncated at its own row's last mask column), a wave scan lays
+// the entries' wedges end to end, and trips of 64 x K wedges cover the
+// chunk, lane L taking wedge W0 + 64u + L at step u.  A wedge finds its entry
+// by rank, not by a search or a walk: the entries of non-zero length start at
+// distinct wedge positions, so with S the 64-bit mask of entry starts inside
+// the step's 64-wedge window (set in LDS once per trip), the wedge belongs to
+// non-empty entry number (starts before the window) + (starts at or below
+// the lane) - 1, which is two mbcnt instructions on a wave-uniform mask.  The
+// entry's record (B offset, bitmap slot of its row) is one LDS read.  Each
+// row's mask is an exact bitmap in its own slot.  Every lane runs the same
+// trips, so rows of different lengths do not leave lanes behind a divergent
+// group, and only the last trip of a chunk checks for wedges past its end
+// (those lanes read the chunk's last wedge and test an all-zero bitmap).
+constexpr int SW_SLOTS = 8;
+
+template <bool BATCHED, int K>
+__global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void k_sum_wave(
+    const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const int32_t* __restrict__ Brp,
+    const int32_t* __restrict__ Bcol, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol,
+    const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
+    const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
+    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
+  // the slot bitmaps of every wave, then WR_BMW words that are never set
+  // (tested by lanes past a chunk's last wedge); records address them by
+  // word offset from the array's start
+  constexpr int ZERO_OFF = WR_WAVES * SW_SLOTS * WR_BMW;
+  __shared__ uint32_t s_bm[ZERO_OFF + WR_BMW];
+  __shared__ int2 s_rec[WR_WAVES][65];  // non-empty entries from 1: (B position - first wedge, bitmap offset)
+  __shared__ __align__(16) uint32_t s_win[WR_WAVES][2 * K];  // entry starts among the trip's 64 x K wedges
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int bm_off = wv * SW_SLOTS * WR_BMW;
+  uint32_t* bm = s_bm + bm_off;
+  int2* rec = s_rec[wv];
+  uint32_t* win = s_win[wv];
+  for (int q = threadIdx.x; q < ZERO_OFF + WR_BMW; q += blockDim.x) s_bm[q] = 0u;
+  __syncthreads();
+  const int n_rows = *n_rows_ptr;
+  // segmented claims as in k_mxm_rows: `chunk` rows at a time from the
+  // wave's current segment, the next live segment once it is drained
+  const int seg_len = (n_rows + MXM_NSEG - 1) / MXM_NSEG;
+  int seg = blockIdx.x % MXM_NSEG;
+  bool drained = false;
+  int c_lo = 0, c_hi = 0;
+  long long lsum = 0;  // this lane's hits for graph cur_g
+  int cur_g = -1;
+  unsigned long long wedges = 0;
+  auto flush = [&]() {
+    const long long t = wave_reduce_sum(lsum);
+    if (lane == 0 && t) atomicAdd(&sums[cur_g], (unsigned long long)t);
+    lsum = 0;
+  };
+  auto try_claim = [&](int sj) {  // wave-uniform
+    const int lo = sj * seg_len;
+    const int hi = min(lo + seg_len, n_rows);
+    if (lo >= hi) return false;
+    int k = 0;
+    if (lane == 0) k = (int)atomicAdd(&counter[sj * MXM_CTR_STRIDE], (unsigned)chunk);
+    k = __shfl(k, 0, 64);
+    if (lo + k >= hi) return false;
+    c_lo = lo + k;
+    c_hi = min(lo + k + chunk, hi);
+    return true;
+  };
+  while (true) {
+    bool got = !drained && try_claim(seg);
+    while (!got && !drained) {
+      bool live = false;
+      if (lane < MXM_NSEG) {
+        const int sj = (seg + 1 + lane) % MXM_NSEG;
+        const int lo = sj * seg_len;
+        const int hi = min(lo + seg_len, n_rows);
+        if (lo < hi) {
+          const unsigned taken =
+              __hip_atomic_load(&counter[sj * MXM_CTR_STRIDE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          live = lo + (long long)taken < hi;
+        }
+      }
+      const unsigned long long lm = __ballot(live);
+      if (lm == 0ull) {
+        drained = true;
+        break;
+      }
+      seg = (seg + 1 + (__ffsll((long long)lm) - 1)) % MXM_NSEG;
+      got = try_claim(seg);
+    }
+    if (!got) break;
+    // the claim's rows, one per lane (chunk <= 64)
+    const int nq = c_hi - c_lo;
+    int qal = 0, qmnz = 0, qg = 0, qmaxc = 0x7FFFFFFF, qa_adj = 0, qm_adj = 0;
+    if (lane < nq) {
+      const int qi = rows[c_lo + lane];
+      const int a0 = Arp[qi];
+      qal = Arp[qi + 1] - a0;
+      const int ms = Mrp[qi];
+      qmnz = (Mend ? Mend[qi] : Mrp[qi + 1]) - ms;
+      qg = row_group ? row_group[qi] : 0;
+      if (trunc && qmnz > 0) qmaxc = Mcol[ms + qmnz - 1];
+      qa_adj = a0;
+      qm_adj = ms;
+    }
+    const int qa_incl = wave_incl_scan(qal);
+    const int qm_incl = wave_incl_scan(qmnz);
+    // flattened index f of a batch row's entries = position in the claim's
+    // concatenation - the batch's first; qa_adj + f' is the entry's A index
+    qa_adj -= qa_incl - qal;
+    qm_adj -= qm_incl - qmnz;
+    for (int q0 = 0; q0 < nq;) {
+      const int g0 = __builtin_amdgcn_readlane(qg, q0);
+      const int a_before = q0 ? __builtin_amdgcn_readlane(qa_incl, q0 - 1) : 0;
+      const int m_before = q0 ? __builtin_amdgcn_readlane(qm_incl, q0 - 1) : 0;
+      const bool ok = lane >= q0 && lane < nq && lane - q0 < SW_SLOTS && qg == g0 &&
+                      (lane == q0 || qa_incl - a_before <= 64);
+      const unsigned long long bad = __ballot(!ok && lane > q0);
+      const int q1 = bad ? (__ffsll((long long)bad) - 1) : 64;
+      const int nb = q1 - q0;
+      const int ea = __builtin_amdgcn_readlane(qa_incl, q1 - 1) - a_before;  // A entries of the batch
+      const int em = __builtin_amdgcn_readlane(qm_incl, q1 - 1) - m_before;  // mask entries of the batch
+      if (g0 != cur_g) {
+        if (cur_g >= 0) flush();
+        cur_g = g0;
+      }
+      // batch row holding flattened index f (of the entries or of the mask)
+      auto slot_of = [&](int f, int incl, int before) {
+        int sl = 0;
+        for (int t = 1; t < nb; ++t)
+          if (f >= __builtin_amdgcn_readlane(incl, q0 + t - 1) - before) sl = t;
+        return sl;
+      };
+      if (ea > 0) {
+        for (int f0 = 0; f0 < em; f0 += 64) {  // mask bitmaps
+          const int f = f0 + lane;
+          const int sl = slot_of(min(f, em - 1), qm_incl, m_before);
+          const int mb = __shfl(qm_adj, q0 + sl, 64);
+          if (f < em) {
+            const int c = Mcol[mb + m_before + f];
+            atomicOr(&bm[sl * WR_BMW + (c >> 5)], 1u << (c & 31));
+          }
+        }
+        const int bbase = BATCHED ? gbase[g0] : 0;
+        uint32_t hits = 0;
+        for (int F = 0; F < ea; F += 64) {
+          const int f = F + lane;
+          const int sl = slot_of(min(f, ea - 1), qa_incl, a_before);
+          const int ab = __shfl(qa_adj, q0 + sl, 64);
+          const int maxc = __shfl(qmaxc, q0 + sl, 64);
+          int len = 0, b0 = 0;
+          if (f < ea) {
+            const int k = bbase + Acol[ab + a_before + f];
+            b0 = Brp[k];
+            int b1 = Brp[k + 1];
+            if (trunc) {
+              int lo = b0, hi = b1;
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (Bcol[mid] <= maxc) lo = mid + 1;
+                else hi = mid;
+              }
+              b1 = lo;
+            }
+            len = b1 - b0;
+          }
+          const int incl = wave_incl_scan(len);
+          const int excl = incl - len;
+          const int T = __builtin_amdgcn_readlane(incl, 63);
+          wedges += (unsigned long long)T;
+          if (T == 0) continue;
+          const unsigned long long ne = __ballot(len > 0);
+          const int rk = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ne >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)ne, 0u));
+          if (len > 0) rec[rk + 1] = make_int2(b0 - excl, bm_off + sl * WR_BMW);
+          for (int W0 = 0; W0 < T; W0 += 64 * K) {
+            if (lane < 2 * K) win[lane] = 0u;
+            __threadfence_block();
+            const int rel = excl - W0;
+            if (len > 0 && rel >= 0 && rel < 64 * K) atomicOr(&win[rel >> 5], 1u << (rel & 31));
+            __threadfence_block();
+            // non-empty entries that start before the trip's first wedge
+            const int rb0 = __popcll(__ballot(len > 0 && excl < W0));
+            auto trip = [&](auto full_c) {
+              constexpr bool FULL = decltype(full_c)::value;
+              uint32_t wl[2 * K];
+#pragma unroll
+              for (int j = 0; j < 2 * K; j += 4) {
+                const uint4 v = *reinterpret_cast<const uint4*>(win + j);
+                wl[j] = v.x;
+                wl[j + 1] = v.y;
+                wl[j + 2] = v.z;
+                wl[j + 3] = v.w;
+              }
+              const int2* rq[K];
+              int ri[K];
+              int rb = rb0;
+#pragma unroll
+              for (int u = 0; u < K; ++u) {
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)wl[2 * u]);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)wl[2 * u + 1]);
+                const unsigned long long S1 = (((unsigned long long)hi << 32) | lo) >> 1;
+                // entries starting at positions 0..lane of the window: bit 0
+                // of S (a uniform part of the base) plus the bits of S >> 1
+                // below the lane; the records are numbered from 1
+                rq[u] = rec + (rb + (int)(lo & 1u));
+                ri[u] = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(S1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)S1, 0u));
+                rb += __popc(lo) + __popc(hi);
+              }
+              int2 r[K];
+#pragma unroll
+              for (int u = 0; u < K; ++u) r[u] = rq[u][ri[u]];
+              int cc[K];
+#pragma unroll
+              for (int u = 0; u < K; ++u) {
+                const int w = W0 + 64 * u + lane;
+                cc[u] = Bcol[(unsigned)(r[u].x + (FULL ? w : min(w, T - 1)))];
+              }
+#pragma unroll
+              for (int u = 0; u < K; ++u) {
+                int off = r[u].y;
+                if (!FULL && W0 + 64 * u + lane >= T) off = ZERO_OFF;
+                hits += __builtin_amdgcn_ubfe(s_bm[off + (cc[u] >> 5)], (uint32_t)cc[u], 1u);
+              }
+            };
+            if (W0 + 64 * K <= T) trip(std::true_type{});
+            else trip(std::false_type{});
+          }
+          __threadfence_block();  // records and window are rewritten by the next chunk
+        }
+        lsum += hits;
+        for (int f0 = 0; f0 < em; f0 += 64) {  // clear the bitmaps
+          const int f = f0 + lane;
+          const int sl = slot_of(min(f, em - 1), qm_incl, m_before);
+          const int mb = __shfl(qm_adj, q0 + sl, 64);
+          if (f < em) bm[sl * WR_BMW + (Mcol[mb + m_before + f] >> 5)] = 0u;
+        }
+        __threadfence_block();
+      }
+      q0 = q1;
+    }
+  }
+  if (cur_g >= 0) flush();
+  if (wedge_counter && lane == 0 && wedges) atomicAdd(wedge_counter, wedges);
+}
+
 // Row binning: rows with an empty A row or empty mask get no work; rows
 // whose mask fits WR_MCAP and whose (untruncated) product count is at most
 // wave_flops go to the wavefront kernel; the rest are split into items of
@@ -1407,7 +1653,26 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   }
   unsigned long long* wc = c.prof_bytes ? c.prof_dev.as<unsigned long long>() : nullptr;
   const int sum_sub = w.sum_sub >= 0 ? w.sum_sub : c.sum_sub;
-  if (run_rows && MODE == 1 && PA && PB && sum_sub > 0) {
+  if (run_rows && MODE == 1 && PA && PB && (sum_sub == 0 || sum_sub == 2)) {
+    // pattern sum mode: rows packed per wavefront, 64 x K wedges per trip
+    auto launch_wave = [&](auto kern) {
+      int occ = 0;
+      GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WR_WAVES, 0));
+      const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ, 1),
+                                                                 ceil_div(m, WR_WAVES)));
+      // rows per claim: at most 64 (one lane each); by default about four
+      // claims per resident wave, at most 16 rows
+      const int chunk = std::min(64, chunk_fix > 0 ? chunk_fix
+                                                   : (int)std::max<int64_t>(
+                                                         1, std::min<int64_t>(16, m / (grid * WR_WAVES * 4))));
+      hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
+                         M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
+                         (const int32_t*)(w.rstart.as<int32_t>() + m),
+                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, trunc, chunk);
+    };
+    if (sum_sub == 2) launch_wave(k_sum_wave<BATCHED, 8>);
+    else launch_wave(k_sum_wave<BATCHED, 4>);
+  } else if (run_rows && MODE == 1 && PA && PB && sum_sub >= 8) {
     // pattern sum mode, entry by entry: sum_sub lanes per short row
     auto launch_entries = [&](auto kern, int groups) {
       int occ = 0;

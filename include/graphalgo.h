@@ -332,8 +332,10 @@ enum {
                                    of the unvisited vertices' edges (GA_BC_ALPHA) */
   GA_OPT_ROW_CLAIM = 3,         /* >= 0: rows per claim of the short-row kernels, 0 = automatic
                                    (GA_ROW_CHUNK) */
-  GA_OPT_SUM_SUB = 4,           /* 0 = 16-lane wedge runs, 8/16/32/64 = lanes per row walking its
-                                   entries, for short rows of pattern sums (GA_SUM_SUB) */
+  GA_OPT_SUM_SUB = 4,           /* short rows of pattern sums (GA_SUM_SUB): 0 = rows packed per
+                                   wavefront, 256 wedges per trip; 2 = the same, 512 per trip;
+                                   1 = 16-lane wedge runs; 8/16/32/64 = lanes per row walking its
+                                   entries */
   GA_OPT_ITEM_WEDGES = 5,       /* >= 1: target wedges per work item of repeated products
                                    (GA_ITEM_WEDGES) */
   GA_OPT_SELECTIVE_SUPPORT = 6, /* k-truss passes after the first recompute only the supports an

@@ -60,7 +60,7 @@ def tc_cases(G, oracle):
     return graphs, np.array([oracle.tc(g) for g in graphs], dtype=np.int64), er, oracle.tc(er)
 
 
-@pytest.mark.parametrize("sub", [0, 8, 16, 32, 64])
+@pytest.mark.parametrize("sub", [0, 1, 2, 8, 16, 32, 64])
 @pytest.mark.parametrize("claim", [0, 1, 3, 16])
 def test_tc_short_row_kernels_and_claims(gpu, tc_cases, sub, claim):
     graphs, want, er, want_er = tc_cases
