@@ -306,7 +306,7 @@ def bench_tc_er(args) -> dict:
            "config": {"workload": "triangle count C<L>=L*L on Erdos-Renyi n=1000 m=9990 (seed 1)", "n": g.n,
                       "m_undirected": g.m_undirected, "triangles": t, "latency_us": round(dt * 1e6, 2),
                       "note": "latency-bound: one call moves ~0.43 MB of algorithmic bytes (54 ns at 8 TB/s)"},
-           "roofline": _roofline(timing, acct, "k_sum_subrows (C<L> = L*L, sum mode)")}
+           "roofline": _roofline(timing, acct, "k_sum_wave (C<L> = L*L, sum mode)")}
     if not args.no_cpu:
         O = _oracle()
         want = O.tc(g)
@@ -439,7 +439,7 @@ def bench_batch(args) -> dict:
         h.close()
     B.close()
     assert plain == counts.tolist(), "batched and per-graph counts differ"
-    roof = _roofline(timing, acct, "k_sum_subrows + k_mxm_items (batched)", "k_sum_subrows/batched_tc_knn4096")
+    roof = _roofline(timing, acct, "k_sum_wave + k_mxm_items (batched)", "k_sum_wave/batched_tc_knn4096")
     out = {"value": m_total / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": f"batched triangle count over {count} kNN graphs (k=30) in one launch",
                       "graphs": count, "m_total": m_total, "triangles_total": int(counts.sum()),

@@ -304,7 +304,7 @@ class Batch:
         return out[: len(self.graphs)]
 
     TUNE_WIDTHS = (128, 256)
-    TUNE_CLAIMS = (2, 4, 8, 16)
+    TUNE_CLAIMS = (8, 16, 32, 64)
 
     def autotune(self) -> tuple[tuple[int, int], dict]:
         """Times every (item width, rows per claim) pair on the resident batch

@@ -120,7 +120,7 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, dou
     GA_REQUIRE(b != nullptr, GA_ERR_ARG, "null batch");
     hipStream_t s = ctx().stream;
     const int widths[2] = {128, 256};
-    const int claims[4] = {2, 4, 8, 16};
+    const int claims[4] = {8, 16, 32, 64};
     static_assert(GA_BATCH_TUNE_CANDIDATES == 8, "candidate grid");
     hipEvent_t e0, e1;
     GA_HIP(hipEventCreate(&e0));

@@ -199,7 +199,7 @@ struct Context {
   // overrides it at ga_init, MxmWork::row_chunk per workspace)
   int row_chunk = 0;
   // short rows of pattern triangle counts: 0 / 2 = rows packed per
-  // wavefront with rank-located wedges, 256 / 512 wedges per trip
+  // wavefront with rank-located wedges, 512 / 256 wedges per trip
   // (k_sum_wave), 1 = 16-lane wedge runs (k_sum_subrows), 8/16/32/64 =
   // lanes per row walking its entries (k_sum_entries); GA_SUM_SUB overrides
   // it at ga_init

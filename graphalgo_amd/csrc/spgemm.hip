@@ -1039,37 +1039,40 @@ __global__ __launch_bounds__(64 * WR_WAVES) void k_sum_entries(
 
 // ---- short rows of pattern sum-mode products, rows packed per wavefront ---
 // A wavefront takes consecutive claimed rows of one graph -- up to SW_SLOTS
-// rows holding at most 64 A entries together, or a single longer row -- and
-// runs their A entries as one flattened chunk: lane L stages entry L (its B
-// row bounds, truncated at its own row's last mask column), a wave scan lays
-// the entries' wedges end to end, and trips of 64 x K wedges cover the
-// chunk, lane L taking wedge W0 + 64u + L at step u.  A wedge finds its entry
-// by rank, not by a search or a walk: the entries of non-zero length start at
-// distinct wedge positions, so with S the 64-bit mask of entry starts inside
-// the step's 64-wedge window (set in LDS once per trip), the wedge belongs to
-// non-empty entry number (starts before the window) + (starts at or below
-// the lane) - 1, which is two mbcnt instructions on a wave-uniform mask.  The
-// entry's record (B offset, bitmap slot of its row) is one LDS read.  Each
-// row's mask is an exact bitmap in its own slot.  Every lane runs the same
-// trips, so rows of different lengths do not leave lanes behind a divergent
-// group, and only the last trip of a chunk checks for wedges past its end
-// (those lanes read the chunk's last wedge and test an all-zero bitmap).
-constexpr int SW_SLOTS = 8;
+// rows holding at most SW_CH A entries together, or a single longer row --
+// and runs their A entries as flattened chunks of SW_CH: lane L stages
+// entries 2L and 2L+1 (their B row bounds, truncated at their own row's last
+// mask column), a wave scan lays the entries' wedges end to end, and trips
+// of 64 x K wedges cover the chunk, lane L taking wedge W0 + 64u + L at step
+// u.  A wedge finds its entry by rank, not by a search or a walk: the entries
+// of non-zero length start at distinct wedge positions, so with S the 64-bit
+// mask of entry starts inside the step's 64-wedge window (start bits are set
+// in LDS once per 32 x SW_WINW wedges), the wedge belongs to non-empty entry
+// number (starts before the window) + (starts at or below the lane), which is
+// two mbcnt instructions on a wave-uniform mask.  The entry's record (B
+// offset, bitmap slot of its row) is one LDS read.  Each row's mask is an
+// exact bitmap in its own slot.  Every lane runs the same trips, so rows of
+// different lengths do not leave lanes behind a divergent group, and only the
+// last trip of a chunk checks for wedges past its end (those lanes read the
+// chunk's last wedge and test an all-zero bitmap).
+constexpr int SW_SLOTS = 6;   // rows per batch (one bitmap each)
+constexpr int SW_CH = 128;    // A entries per chunk (two per lane)
+constexpr int SW_WINW = 64;   // words of entry-start bits per wave (2048 wedges, one word per lane)
 
-template <bool BATCHED, int K>
+template <bool BATCHED, int K, bool TRUNC>
 __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8))) void k_sum_wave(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const int32_t* __restrict__ Brp,
     const int32_t* __restrict__ Bcol, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol,
     const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
-    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int trunc, int chunk) {
+    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int chunk) {
   // the slot bitmaps of every wave, then WR_BMW words that are never set
   // (tested by lanes past a chunk's last wedge); records address them by
   // word offset from the array's start
   constexpr int ZERO_OFF = WR_WAVES * SW_SLOTS * WR_BMW;
   __shared__ uint32_t s_bm[ZERO_OFF + WR_BMW];
-  __shared__ int2 s_rec[WR_WAVES][65];  // non-empty entries from 1: (B position - first wedge, bitmap offset)
-  __shared__ __align__(16) uint32_t s_win[WR_WAVES][2 * K];  // entry starts among the trip's 64 x K wedges
+  __shared__ int2 s_rec[WR_WAVES][SW_CH + 1];  // non-empty entries from 1: (B position - first wedge, bitmap offset)
+  __shared__ __align__(16) uint32_t s_win[WR_WAVES][SW_WINW];  // entry starts among 32 x SW_WINW wedges
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int bm_off = wv * SW_SLOTS * WR_BMW;
@@ -1138,7 +1141,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
       const int ms = Mrp[qi];
       qmnz = (Mend ? Mend[qi] : Mrp[qi + 1]) - ms;
       qg = row_group ? row_group[qi] : 0;
-      if (trunc && qmnz > 0) qmaxc = Mcol[ms + qmnz - 1];
+      if (TRUNC && qmnz > 0) qmaxc = Mcol[ms + qmnz - 1];
       qa_adj = a0;
       qm_adj = ms;
     }
@@ -1153,7 +1156,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
       const int a_before = q0 ? __builtin_amdgcn_readlane(qa_incl, q0 - 1) : 0;
       const int m_before = q0 ? __builtin_amdgcn_readlane(qm_incl, q0 - 1) : 0;
       const bool ok = lane >= q0 && lane < nq && lane - q0 < SW_SLOTS && qg == g0 &&
-                      (lane == q0 || qa_incl - a_before <= 64);
+                      (lane == q0 || qa_incl - a_before <= SW_CH);
       const unsigned long long bad = __ballot(!ok && lane > q0);
       const int q1 = bad ? (__ffsll((long long)bad) - 1) : 64;
       const int nb = q1 - q0;
@@ -1182,90 +1185,108 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
         }
         const int bbase = BATCHED ? gbase[g0] : 0;
         uint32_t hits = 0;
-        for (int F = 0; F < ea; F += 64) {
-          const int f = F + lane;
-          const int sl = slot_of(min(f, ea - 1), qa_incl, a_before);
-          const int ab = __shfl(qa_adj, q0 + sl, 64);
-          const int maxc = __shfl(qmaxc, q0 + sl, 64);
-          int len = 0, b0 = 0;
-          if (f < ea) {
-            const int k = bbase + Acol[ab + a_before + f];
-            b0 = Brp[k];
-            int b1 = Brp[k + 1];
-            if (trunc) {
-              int lo = b0, hi = b1;
-              while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (Bcol[mid] <= maxc) lo = mid + 1;
-                else hi = mid;
+        for (int F = 0; F < ea; F += SW_CH) {
+          // lane L stages entries F + 2L and F + 2L + 1
+          int len[2] = {0, 0}, b0[2] = {0, 0}, sl[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int f = F + 2 * lane + h;
+            sl[h] = slot_of(min(f, ea - 1), qa_incl, a_before);
+            const int ab = __shfl(qa_adj, q0 + sl[h], 64);
+            const int maxc = TRUNC ? __shfl(qmaxc, q0 + sl[h], 64) : 0;
+            if (f < ea) {
+              const int k = bbase + Acol[ab + a_before + f];
+              b0[h] = Brp[k];
+              int b1 = Brp[k + 1];
+              if (TRUNC) {
+                int lo = b0[h], hi = b1;
+                while (lo < hi) {
+                  const int mid = (lo + hi) >> 1;
+                  if (Bcol[mid] <= maxc) lo = mid + 1;
+                  else hi = mid;
+                }
+                b1 = lo;
               }
-              b1 = lo;
+              len[h] = b1 - b0[h];
             }
-            len = b1 - b0;
           }
-          const int incl = wave_incl_scan(len);
-          const int excl = incl - len;
+          const int incl = wave_incl_scan(len[0] + len[1]);
+          const int excl[2] = {incl - len[0] - len[1], incl - len[1]};
           const int T = __builtin_amdgcn_readlane(incl, 63);
           wedges += (unsigned long long)T;
           if (T == 0) continue;
-          const unsigned long long ne = __ballot(len > 0);
-          const int rk = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ne >> 32),
-                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)ne, 0u));
-          if (len > 0) rec[rk + 1] = make_int2(b0 - excl, bm_off + sl * WR_BMW);
-          for (int W0 = 0; W0 < T; W0 += 64 * K) {
-            if (lane < 2 * K) win[lane] = 0u;
+          // records of the non-empty entries, numbered from 1 in entry order
+          const unsigned long long ne0 = __ballot(len[0] > 0), ne1 = __ballot(len[1] > 0);
+          const int rk = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ne0 >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)ne0, 0u)) +
+                         (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(ne1 >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)ne1, 0u));
+          if (len[0] > 0) rec[rk + 1] = make_int2(b0[0] - excl[0], bm_off + sl[0] * WR_BMW);
+          if (len[1] > 0) rec[rk + 1 + (len[0] > 0)] = make_int2(b0[1] - excl[1], bm_off + sl[1] * WR_BMW);
+          for (int Wb = 0; Wb < T; Wb += 32 * SW_WINW) {
+            // entry starts among wedges Wb .. Wb + 32 * SW_WINW - 1
+            win[lane] = 0u;
             __threadfence_block();
-            const int rel = excl - W0;
-            if (len > 0 && rel >= 0 && rel < 64 * K) atomicOr(&win[rel >> 5], 1u << (rel & 31));
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int rel = excl[h] - Wb;
+              if (len[h] > 0 && rel >= 0 && rel < 32 * SW_WINW) atomicOr(&win[rel >> 5], 1u << (rel & 31));
+            }
             __threadfence_block();
-            // non-empty entries that start before the trip's first wedge
-            const int rb0 = __popcll(__ballot(len > 0 && excl < W0));
-            auto trip = [&](auto full_c) {
-              constexpr bool FULL = decltype(full_c)::value;
-              uint32_t wl[2 * K];
+            // non-empty entries that start before the window
+            int rb = __popcll(__ballot(len[0] > 0 && excl[0] < Wb)) + __popcll(__ballot(len[1] > 0 && excl[1] < Wb));
+            const int wend = min(T, Wb + 32 * SW_WINW);
+            for (int W0 = Wb; W0 < wend; W0 += 64 * K) {
+              auto trip = [&](auto full_c) {
+                constexpr bool FULL = decltype(full_c)::value;
+                uint32_t wl[2 * K];
 #pragma unroll
-              for (int j = 0; j < 2 * K; j += 4) {
-                const uint4 v = *reinterpret_cast<const uint4*>(win + j);
-                wl[j] = v.x;
-                wl[j + 1] = v.y;
-                wl[j + 2] = v.z;
-                wl[j + 3] = v.w;
-              }
-              const int2* rq[K];
-              int ri[K];
-              int rb = rb0;
+                for (int j = 0; j < 2 * K; j += 4) {
+                  const uint4 v = *reinterpret_cast<const uint4*>(win + ((W0 - Wb) >> 5) + j);
+                  wl[j] = v.x;
+                  wl[j + 1] = v.y;
+                  wl[j + 2] = v.z;
+                  wl[j + 3] = v.w;
+                }
+                const int2* rq[K];
+                int ri[K];
 #pragma unroll
-              for (int u = 0; u < K; ++u) {
-                const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)wl[2 * u]);
-                const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)wl[2 * u + 1]);
-                const unsigned long long S1 = (((unsigned long long)hi << 32) | lo) >> 1;
-                // entries starting at positions 0..lane of the window: bit 0
-                // of S (a uniform part of the base) plus the bits of S >> 1
-                // below the lane; the records are numbered from 1
-                rq[u] = rec + (rb + (int)(lo & 1u));
-                ri[u] = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(S1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)S1, 0u));
-                rb += __popc(lo) + __popc(hi);
-              }
-              int2 r[K];
+                for (int u = 0; u < K; ++u) {
+                  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)wl[2 * u]);
+                  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)wl[2 * u + 1]);
+                  const unsigned long long S1 = (((unsigned long long)hi << 32) | lo) >> 1;
+                  // entries starting at positions 0..lane of the step's
+                  // window: bit 0 of S (a uniform part of the base) plus the
+                  // bits of S >> 1 below the lane
+                  rq[u] = rec + (rb + (int)(lo & 1u));
+                  ri[u] = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(S1 >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)S1, 0u));
+                  rb += __popc(lo) + __popc(hi);
+                }
+                int2 r[K];
 #pragma unroll
-              for (int u = 0; u < K; ++u) r[u] = rq[u][ri[u]];
-              int cc[K];
+                for (int u = 0; u < K; ++u) r[u] = rq[u][ri[u]];
+                int cc[K];
 #pragma unroll
-              for (int u = 0; u < K; ++u) {
-                const int w = W0 + 64 * u + lane;
-                cc[u] = Bcol[(unsigned)(r[u].x + (FULL ? w : min(w, T - 1)))];
-              }
+                for (int u = 0; u < K; ++u) {
+                  const int w = W0 + 64 * u + lane;
+                  // 32-bit byte offset from the array base (B has < 2^29
+                  // entries): one address register per gather
+                  const uint32_t boff = 4u * (uint32_t)(r[u].x + (FULL ? w : min(w, T - 1)));
+                  cc[u] = *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(Bcol) + boff);
+                }
 #pragma unroll
-              for (int u = 0; u < K; ++u) {
-                int off = r[u].y;
-                if (!FULL && W0 + 64 * u + lane >= T) off = ZERO_OFF;
-                hits += __builtin_amdgcn_ubfe(s_bm[off + (cc[u] >> 5)], (uint32_t)cc[u], 1u);
-              }
-            };
-            if (W0 + 64 * K <= T) trip(std::true_type{});
-            else trip(std::false_type{});
+                for (int u = 0; u < K; ++u) {
+                  int off = r[u].y;
+                  if (!FULL && W0 + 64 * u + lane >= T) off = ZERO_OFF;
+                  hits += __builtin_amdgcn_ubfe(s_bm[off + (cc[u] >> 5)], (uint32_t)cc[u], 1u);
+                }
+              };
+              if (W0 + 64 * K <= T) trip(std::true_type{});
+              else trip(std::false_type{});
+            }
+            __threadfence_block();  // the window is rewritten next
           }
-          __threadfence_block();  // records and window are rewritten by the next chunk
         }
         lsum += hits;
         for (int f0 = 0; f0 < em; f0 += 64) {  // clear the bitmaps
@@ -1655,7 +1676,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   const int sum_sub = w.sum_sub >= 0 ? w.sum_sub : c.sum_sub;
   if (run_rows && MODE == 1 && PA && PB && (sum_sub == 0 || sum_sub == 2)) {
     // pattern sum mode: rows packed per wavefront, 64 x K wedges per trip
-    auto launch_wave = [&](auto kern) {
+    auto launch_wave = [&](auto kern) {  // (trunc is compiled into the kernel)
       int occ = 0;
       GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WR_WAVES, 0));
       const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ, 1),
@@ -1668,10 +1689,15 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
                          M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
                          (const int32_t*)(w.rstart.as<int32_t>() + m),
-                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, trunc, chunk);
+                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, chunk);
     };
-    if (sum_sub == 2) launch_wave(k_sum_wave<BATCHED, 8>);
-    else launch_wave(k_sum_wave<BATCHED, 4>);
+    if (sum_sub == 2) {
+      if (trunc) launch_wave(k_sum_wave<BATCHED, 4, true>);
+      else launch_wave(k_sum_wave<BATCHED, 4, false>);
+    } else {
+      if (trunc) launch_wave(k_sum_wave<BATCHED, 8, true>);
+      else launch_wave(k_sum_wave<BATCHED, 8, false>);
+    }
   } else if (run_rows && MODE == 1 && PA && PB && sum_sub >= 8) {
     // pattern sum mode, entry by entry: sum_sub lanes per short row
     auto launch_entries = [&](auto kern, int groups) {

@@ -215,10 +215,10 @@ ga_status ga_batch_destroy(ga_batch* b);
  * claim (automatic before tuning). */
 ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts);
 /* Times the launch shapes on the resident batch -- item width 128 or 256 x
- * rows per claim of the short-row kernel 2, 4, 8 or 16 (the rows in flight
+ * rows per claim of the short-row kernel 8, 16, 32 or 64 (the rows in flight
  * on an XCD share its L2) -- and keeps the fastest for later calls.
  * ms_per_candidate (may be NULL) receives GA_BATCH_TUNE_CANDIDATES times,
- * width-major: (128, 2), (128, 4), ..., (256, 16). */
+ * width-major: (128, 8), (128, 16), ..., (256, 64). */
 #define GA_BATCH_TUNE_CANDIDATES 8
 ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, double* ms_per_candidate);
 
@@ -333,7 +333,7 @@ enum {
   GA_OPT_ROW_CLAIM = 3,         /* >= 0: rows per claim of the short-row kernels, 0 = automatic
                                    (GA_ROW_CHUNK) */
   GA_OPT_SUM_SUB = 4,           /* short rows of pattern sums (GA_SUM_SUB): 0 = rows packed per
-                                   wavefront, 256 wedges per trip; 2 = the same, 512 per trip;
+                                   wavefront, 512 wedges per trip; 2 = the same, 256 per trip;
                                    1 = 16-lane wedge runs; 8/16/32/64 = lanes per row walking its
                                    entries */
   GA_OPT_ITEM_WEDGES = 5,       /* >= 1: target wedges per work item of repeated products

@@ -92,7 +92,7 @@ def test_batch_tc_knn(gpu, G, oracle):
         print(f"case batch tc threads={threads}", flush=True)
         assert np.array_equal(B.triangle_counts(threads), want)
     (width, claim), ms = B.autotune()
-    assert width in (128, 256) and claim in (2, 4, 8, 16) and len(ms) == 8
+    assert width in (128, 256) and claim in (8, 16, 32, 64) and len(ms) == 8
     assert np.array_equal(B.triangle_counts(), want)
     B.close()
 
