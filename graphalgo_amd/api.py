@@ -460,7 +460,7 @@ def device_info() -> dict:
 # process-wide tuning options (include/graphalgo.h, GA_OPT_*): every value
 # gives the same results, only the speed differs
 OPTIONS = {"bc_bottom_up": 1, "bc_alpha": 2, "row_claim": 3, "sum_sub": 4, "item_wedges": 5,
-           "selective_support": 6, "truss_loop": 7, "support_tri": 8, "serve_join": 9}
+           "selective_support": 6, "truss_loop": 7, "support_tri": 8, "serve_join": 9, "bc_loop": 10}
 
 
 def set_option(name: str, value: float) -> None:

@@ -46,6 +46,37 @@ namespace ga {
 template <typename DT>
 __host__ __device__ constexpr DT bc_unvisited() { return (DT)~DT(0); }
 
+// State of a pass whose level lists are sized on the device (device-driven
+// loop): the host queues level steps without reading any size, and each
+// kernel of a step takes its level, its lists and their sizes from here; a
+// step queued after the last level finds `done` and does nothing.  The
+// kernels below take it as `ctl` (null on the host-steered path, which
+// passes levels and sizes as arguments).
+struct BcCtl {
+  int32_t* cur;   // level-d list (nf vertices, ascending)
+  int32_t* next;  // level-(d+1) list being built
+  int d, nf, done, dmax, bu, over, allow_bu, bk;  // bk: level of the backward step
+  double alpha;
+  int next_tot[3];  // list builder totals: vertices, items, hub slots
+  int cand_tot[3];  // the same for the bottom-up candidates
+  int back_tot[3];  // the same for a backward level
+  unsigned long long edges[2];
+  long long seq;
+};
+// which steps of the forward sweep run (every one only while !done)
+enum { BG_ANY = 0, BG_CAND = 1, BG_DECIDE = 2, BG_BU = 3, BG_TD = 4, BG_POST = 5 };
+__device__ inline bool bc_live(const BcCtl* c, int g) {
+  if (g == BG_POST) return true;  // after the sweep
+  if (c->done) return false;
+  switch (g) {
+    case BG_CAND: return c->d >= 1 && (c->bu || c->allow_bu);
+    case BG_DECIDE: return c->d >= 1 && !c->bu && c->allow_bu;
+    case BG_BU: return c->bu != 0;
+    case BG_TD: return c->bu == 0;
+    default: return true;
+  }
+}
+
 template <typename DT>
 __global__ void k_bc_init(DT* __restrict__ depth, double* __restrict__ sigma, int64_t total) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
@@ -93,7 +124,8 @@ __global__ void k_flag_emit(uint8_t* __restrict__ flag, int64_t n, const int32_t
 // flag[v] = 1 when some lane of vertex v is at depth k
 template <typename DT>
 __global__ __launch_bounds__(256) void k_level_flags(const DT* __restrict__ depth, int64_t n, int nsp, int k,
-                                                     uint8_t* __restrict__ flag) {
+                                                     uint8_t* __restrict__ flag, const BcCtl* __restrict__ ctl) {
+  if (ctl) k = ctl->bk;
   const int lane = threadIdx.x & 63;
   const int64_t wpb = blockDim.x >> 6;
   for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += (int64_t)gridDim.x * wpb) {
@@ -106,7 +138,13 @@ __global__ __launch_bounds__(256) void k_level_flags(const DT* __restrict__ dept
 template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_last(const int32_t* __restrict__ front, int nfront, int nsp, int k,
                                                  const DT* __restrict__ depth,
-                                                 const double* __restrict__ sigma, double* __restrict__ q) {
+                                                 const double* __restrict__ sigma, double* __restrict__ q,
+                                                 const BcCtl* __restrict__ ctl) {
+  if (ctl) {  // the deepest level's list
+    front = ctl->cur;
+    nfront = ctl->nf;
+    k = ctl->dmax;
+  }
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nfront; f += gridDim.x * wpb) {
@@ -172,8 +210,12 @@ template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_discover_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ n_items_ptr, int nsp, int d,
-    DT* __restrict__ depth, uint8_t* __restrict__ next_flag) {
+    DT* __restrict__ depth, uint8_t* __restrict__ next_flag, const BcCtl* __restrict__ ctl, int guard) {
   __shared__ int32_t s_nb[4][BC_CH];
+  if (ctl) {
+    if (!bc_live(ctl, guard)) return;
+    d = ctl->d;
+  }
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
@@ -220,8 +262,13 @@ template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_sigma_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
-    int nsp, int d, const DT* __restrict__ depth, double* __restrict__ sigma, double* __restrict__ partial) {
+    int nsp, int d, const DT* __restrict__ depth, double* __restrict__ sigma, double* __restrict__ partial,
+    const BcCtl* __restrict__ ctl, int guard) {
   __shared__ int32_t s_nb[4][BC_CH];
+  if (ctl) {  // the level being added: d + 1
+    if (!bc_live(ctl, guard)) return;
+    d = ctl->d + 1;
+  }
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
@@ -281,8 +328,12 @@ __global__ __launch_bounds__(256) void k_bc_bottom_up_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
     int nsp, int d, DT* __restrict__ depth, double* __restrict__ sigma, double* __restrict__ partial,
-    uint8_t* __restrict__ next_flag) {
+    uint8_t* __restrict__ next_flag, const BcCtl* __restrict__ ctl, int guard) {
   __shared__ int32_t s_nb[4][BC_CH];
+  if (ctl) {
+    if (!bc_live(ctl, guard)) return;
+    d = ctl->d;
+  }
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
@@ -357,7 +408,13 @@ __global__ __launch_bounds__(256) void k_bc_bottom_up_combine(const int32_t* __r
                                                               const int32_t* __restrict__ hstart, int nsp, int d,
                                                               DT* __restrict__ depth, double* __restrict__ sigma,
                                                               const double* __restrict__ partial,
-                                                              uint8_t* __restrict__ next_flag) {
+                                                              uint8_t* __restrict__ next_flag,
+                                                              const BcCtl* __restrict__ ctl, int guard) {
+  if (ctl) {
+    if (!bc_live(ctl, guard)) return;
+    nc = ctl->cand_tot[0];
+    d = ctl->d;
+  }
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const DT dn = (DT)(d + 1);
@@ -386,16 +443,20 @@ __global__ __launch_bounds__(256) void k_bc_bottom_up_combine(const int32_t* __r
 
 // flag[v] = 1 when vertex v has an edge and an unvisited lane (the
 // candidates of a bottom-up step)
+// (with ctl every flag is written, 0 or 1, so the array needs no clearing)
 template <typename DT>
 __global__ __launch_bounds__(256) void k_unvisited_flags(const int32_t* __restrict__ rp, const DT* __restrict__ depth,
-                                                         int64_t n, int nsp, uint8_t* __restrict__ flag) {
+                                                         int64_t n, int nsp, uint8_t* __restrict__ flag,
+                                                         const BcCtl* __restrict__ ctl, int guard) {
+  if (ctl && !bc_live(ctl, guard)) return;
   const int lane = threadIdx.x & 63;
   const int64_t wpb = blockDim.x >> 6;
   for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < n; v += (int64_t)gridDim.x * wpb) {
-    if (rp[v + 1] == rp[v]) continue;
     bool hit = false;
-    for (int c = lane; c < nsp; c += 64) hit |= depth[v * nsp + c] == bc_unvisited<DT>();
-    if (__ballot(hit) != 0ull && lane == 0) flag[v] = 1;
+    if (rp[v + 1] != rp[v])
+      for (int c = lane; c < nsp; c += 64) hit |= depth[v * nsp + c] == bc_unvisited<DT>();
+    const bool any = __ballot(hit) != 0ull;
+    if (lane == 0 && (any || ctl)) flag[v] = any ? 1 : 0;
   }
 }
 
@@ -403,7 +464,12 @@ __global__ __launch_bounds__(256) void k_unvisited_flags(const int32_t* __restri
 // (*nb_ptr entries, a device count)
 __global__ void k_list_edges(const int32_t* __restrict__ a, int na, const int32_t* __restrict__ b,
                              const int32_t* __restrict__ nb_ptr, const int32_t* __restrict__ rp,
-                             unsigned long long* __restrict__ out) {
+                             unsigned long long* __restrict__ out, const BcCtl* __restrict__ ctl, int guard) {
+  if (ctl) {  // list a: the level-d list
+    if (!bc_live(ctl, guard)) return;
+    a = ctl->cur;
+    na = ctl->nf;
+  }
   const int nb = *nb_ptr;
   unsigned long long ea = 0, eb = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < max(na, nb); i += gridDim.x * blockDim.x) {
@@ -424,7 +490,14 @@ __global__ __launch_bounds__(256) void k_bc_sigma_combine(const int32_t* __restr
                                                           const int32_t* __restrict__ rp,
                                                           const int32_t* __restrict__ hstart, int nsp, int d,
                                                           const DT* __restrict__ depth, double* __restrict__ sigma,
-                                                          const double* __restrict__ partial) {
+                                                          const double* __restrict__ partial,
+                                                          const BcCtl* __restrict__ ctl, int guard) {
+  if (ctl) {  // the list being added (level d + 1)
+    if (!bc_live(ctl, guard)) return;
+    front = ctl->next;
+    nf = ctl->next_tot[0];
+    d = ctl->d + 1;
+  }
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const DT dd = (DT)d;
@@ -455,8 +528,9 @@ __global__ __launch_bounds__(256) void k_bc_backward_items(
     const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
     const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
     int nsp, int k, const DT* __restrict__ depth, const double* __restrict__ sigma, double* __restrict__ q,
-    double* __restrict__ bc, double* __restrict__ partial, int64_t bc_stride) {
+    double* __restrict__ bc, double* __restrict__ partial, int64_t bc_stride, const BcCtl* __restrict__ ctl) {
   __shared__ int32_t s_nb[4][BC_CH];
+  if (ctl) k = ctl->bk;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
@@ -543,7 +617,11 @@ __global__ __launch_bounds__(256) void k_bc_combine(const int32_t* __restrict__ 
                                                     const DT* __restrict__ depth,
                                                     const double* __restrict__ sigma, double* __restrict__ q,
                                                     double* __restrict__ bc, const double* __restrict__ partial,
-                                                    int64_t bc_stride) {
+                                                    int64_t bc_stride, const BcCtl* __restrict__ ctl) {
+  if (ctl) {  // the backward level's list (built into `front`)
+    nf = ctl->back_tot[0];
+    k = ctl->bk;
+  }
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const DT dk = (DT)k;
@@ -614,6 +692,211 @@ __global__ void k_bc_account(const int32_t* __restrict__ rp, const DT* __restric
   }
 }
 
+// ---- level lists sized on the device ------------------------------------
+// Flags -> ascending vertex list + its neighbour-chunk items + hub slots, in
+// two launches whose grids depend only on n.  A group is 64 consecutive
+// vertices, one per lane of a wavefront; each workgroup owns a contiguous
+// range of groups (its four wavefronts a quarter each, walked in order).
+// The count pass writes every group's vertices, items and hub slots and its
+// workgroup's sums; the emit pass adds the sums of the workgroups before its
+// own and of its earlier wavefronts, then walks its groups placing each
+// lane's vertex by a ballot count and its items and hub slots by wave scans:
+// the list (hstart[f] = first hub slot of list vertex f), the items (same
+// layout as bc_items) and, from the last wavefront, the three totals.  Flags
+// are cleared.
+constexpr int BCB_T = 256;
+constexpr int BCB_MAX_BLOCKS = 1024;
+__device__ inline int64_t ceil_div_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+struct BcGroupRange {
+  int64_t g0, g1;  // the wavefront's groups
+};
+__device__ inline BcGroupRange bc_wave_groups(int64_t ng, int64_t per_block) {
+  const int64_t b0 = (int64_t)blockIdx.x * per_block, b1 = min(ng, b0 + per_block);
+  const int64_t per_wave = ceil_div_dev(per_block, BCB_T / 64);
+  const int64_t w0 = b0 + (int64_t)(threadIdx.x >> 6) * per_wave;
+  return {min(w0, b1), min(w0 + per_wave, b1)};
+}
+// lane's vertex of group g: flagged?, its items, its hub slots
+__device__ inline void bc_group_lane(const uint8_t* flag, const int32_t* rp, int64_t n, int64_t g, bool& f, int& nc,
+                                     int& hub) {
+  const int64_t v = g * BC_FLAG_CHUNK + (threadIdx.x & 63);
+  f = v < n && flag[v] != 0;
+  nc = 0;
+  if (f) {
+    const int deg = rp[v + 1] - rp[v];
+    nc = deg > BC_CH ? (deg + BC_CH - 1) / BC_CH : 1;
+  }
+  hub = nc > 1 ? nc : 0;
+}
+
+__global__ __launch_bounds__(BCB_T) void k_bc_build_count(const uint8_t* __restrict__ flag, int64_t n,
+                                                          const int32_t* __restrict__ rp, int32_t* __restrict__ gcnt,
+                                                          long long* __restrict__ bsum, int64_t per_block,
+                                                          const BcCtl* __restrict__ ctl, int guard) {
+  if (ctl && !bc_live(ctl, guard)) return;
+  __shared__ long long red[3][BCB_T / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t ng = ceil_div_dev(n, BC_FLAG_CHUNK);
+  const BcGroupRange r = bc_wave_groups(ng, per_block);
+  long long t[3] = {0, 0, 0};
+  for (int64_t g = r.g0; g < r.g1; ++g) {
+    bool f;
+    int nc, hub;
+    bc_group_lane(flag, rp, n, g, f, nc, hub);
+    const int cv = __popcll(__ballot(f));
+    const int ci = wave_reduce_sum(nc), ch = wave_reduce_sum(hub);
+    if (lane == 0) {
+      gcnt[g] = cv;
+      gcnt[ng + g] = ci;
+      gcnt[2 * ng + g] = ch;
+    }
+    t[0] += cv;
+    t[1] += ci;
+    t[2] += ch;
+  }
+  if (lane == 0)
+    for (int k = 0; k < 3; ++k) red[k][wv] = t[k];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    long long u = 0;
+    for (int w = 0; w < BCB_T / 64; ++w) u += red[threadIdx.x][w];
+    bsum[3 * (int64_t)blockIdx.x + threadIdx.x] = u;
+  }
+}
+
+// to_next: the list goes to ctl->next (the level being built); otherwise to `list`
+__global__ __launch_bounds__(BCB_T) void k_bc_build_emit(uint8_t* __restrict__ flag, int64_t n,
+                                                         const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ gcnt,
+                                                         const long long* __restrict__ bsum, int64_t per_block,
+                                                         int32_t* list, int32_t* __restrict__ hstart,
+                                                         int32_t* __restrict__ it_v, int32_t* __restrict__ it_t0,
+                                                         int32_t* __restrict__ it_slot, int32_t* __restrict__ tot,
+                                                         const BcCtl* __restrict__ ctl, int guard, int to_next) {
+  if (ctl && !bc_live(ctl, guard)) return;
+  if (ctl && to_next) list = ctl->next;
+  __shared__ long long prior[3][BCB_T / 64];
+  __shared__ long long wtot[3][BCB_T / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t ng = ceil_div_dev(n, BC_FLAG_CHUNK);
+  const BcGroupRange r = bc_wave_groups(ng, per_block);
+  long long p[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += BCB_T)
+    for (int k = 0; k < 3; ++k) p[k] += bsum[3 * (int64_t)b + k];
+  for (int64_t g = r.g0 + lane; g < r.g1; g += 64)
+    for (int k = 0; k < 3; ++k) q[k] += gcnt[k * ng + g];
+  for (int k = 0; k < 3; ++k) {
+    p[k] = wave_reduce_sum(p[k]);
+    q[k] = wave_reduce_sum(q[k]);
+    if (lane == 0) {
+      prior[k][wv] = p[k];
+      wtot[k][wv] = q[k];
+    }
+  }
+  __syncthreads();
+  long long run[3];
+  for (int k = 0; k < 3; ++k) {
+    run[k] = 0;
+    for (int w = 0; w < BCB_T / 64; ++w) {
+      run[k] += prior[k][w];
+      if (w < wv) run[k] += wtot[k][w];
+    }
+  }
+  for (int64_t g = r.g0; g < r.g1; ++g) {
+    bool f;
+    int nc, hub;
+    bc_group_lane(flag, rp, n, g, f, nc, hub);
+    const unsigned long long m = __ballot(f);
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const int ci = wave_incl_scan(nc), ch = wave_incl_scan(hub);
+    if (f) {
+      const int64_t v = g * BC_FLAG_CHUNK + lane;
+      const int ov = (int)run[0] + below, oi = (int)run[1] + ci - nc, oh = (int)run[2] + ch - hub;
+      const int r0 = rp[v];
+      flag[v] = 0;
+      list[ov] = (int32_t)v;
+      hstart[ov] = oh;
+      for (int j = 0; j < nc; ++j) {
+        it_v[oi + j] = (int32_t)v;
+        it_t0[oi + j] = r0 + j * BC_CH;
+        it_slot[oi + j] = nc > 1 ? oh + j : -1;
+      }
+    }
+    run[0] += __popcll(m);
+    run[1] += __shfl(ci, 63, 64);
+    run[2] += __shfl(ch, 63, 64);
+  }
+  if (blockIdx.x == gridDim.x - 1 && wv == BCB_T / 64 - 1 && lane == 0)
+    for (int k = 0; k < 3; ++k) tot[k] = (int32_t)run[k];
+}
+
+// One thread steers the sweep between the launches of a step.
+enum { BC_OP_INIT = 0, BC_OP_START = 1, BC_OP_DECIDE = 2, BC_OP_NEXT = 3, BC_OP_END = 4, BC_OP_BACK = 5 };
+__global__ void k_bc_ctl(BcCtl* __restrict__ c, int op, int32_t* l0, int32_t* l1, int allow_bu, double alpha,
+                         int limit, long long* __restrict__ host) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  switch (op) {
+    case BC_OP_INIT:  // before the source list is built into `next`
+      c->cur = l1;
+      c->next = l0;
+      c->d = c->nf = c->done = c->dmax = c->bu = c->over = 0;
+      c->allow_bu = allow_bu;
+      c->alpha = alpha;
+      c->bk = 0;
+      c->edges[0] = c->edges[1] = 0ull;
+      c->seq = 0;
+      break;
+    case BC_OP_START: {  // level 0 = the source list
+      int32_t* t = c->cur;
+      c->cur = c->next;
+      c->next = t;
+      c->nf = c->next_tot[0];
+      if (c->nf == 0) c->done = 1;
+      break;
+    }
+    case BC_OP_DECIDE:  // after the candidates and the edges of both lists
+      if (!c->done && c->d >= 1 && !c->bu && c->allow_bu)
+        c->bu = (double)c->edges[0] * c->alpha > (double)c->edges[1] ? 1 : 0;
+      c->edges[0] = c->edges[1] = 0ull;
+      if (!c->done && c->bu && c->cand_tot[0] == 0) {  // no vertex can be reached any more
+        c->done = 1;
+        c->dmax = c->d;
+      }
+      break;
+    case BC_OP_NEXT:  // after the level-(d+1) list
+      if (!c->done && c->next_tot[0] == 0) {
+        c->done = 1;
+        c->dmax = c->d;
+      }
+      break;
+    case BC_OP_END:  // level d + 1 complete
+      if (!c->done) {
+        int32_t* t = c->cur;
+        c->cur = c->next;
+        c->next = t;
+        c->nf = c->next_tot[0];
+        c->d += 1;
+        if (c->d + 1 >= limit) {  // the next level would reach the unvisited marker
+          c->over = 1;
+          c->done = 1;
+          c->dmax = c->d;
+        }
+      }
+      c->seq += 1;
+      if (host) {  // the last word (released after the others): 2 x steps + done
+        __hip_atomic_store(&host[0], (long long)c->d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&host[1], (long long)c->dmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&host[2], (long long)c->over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&host[3], 2 * c->seq + (c->done ? 1 : 0), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+      break;
+    case BC_OP_BACK:  // a backward level done: the next one up (the first is set by the host)
+      c->bk = limit > 0 ? limit : c->bk - 1;
+      break;
+  }
+}
+
 static hipEvent_t bc_event(Context& c) {
   hipEvent_t ev;
   if (!c.event_pool.empty()) {
@@ -679,6 +962,15 @@ static void bc_reserve(BcWork& w, int64_t n, const BcSizes& z, int64_t ns) {
   w.hstart.ensure(sizeof(int32_t) * (n1 + 1));
   // largest scan: a level list (at most n entries) or the flag groups
   w.scan_tmp.ensure(scan_tmp_bytes(std::max(n1, z.ng)));
+  // device-driven passes
+  w.cflag.ensure(4 * ceil_div(n1, 4));
+  w.cit_v.ensure(sizeof(int32_t) * z.item_cap);
+  w.cit_t0.ensure(sizeof(int32_t) * z.item_cap);
+  w.cit_slot.ensure(sizeof(int32_t) * z.item_cap);
+  w.chstart.ensure(sizeof(int32_t) * (n1 + 1));
+  w.gcnt.ensure(sizeof(int32_t) * 3 * z.ng);
+  w.gsum.ensure(sizeof(long long) * 3 * BCB_MAX_BLOCKS);
+  w.ctl.ensure(sizeof(BcCtl));
 }
 
 static size_t bc_bytes(int64_t n, const BcSizes& z, int64_t ns) {
@@ -691,6 +983,9 @@ static size_t bc_bytes(int64_t n, const BcSizes& z, int64_t ns) {
   b += lease_bytes(sizeof(int32_t) * z.ng) + lease_bytes(sizeof(int32_t) * (z.ng + 1));
   b += 2 * lease_bytes(sizeof(int32_t) * n1) + 2 * lease_bytes(sizeof(int32_t) * (n1 + 1));
   b += lease_bytes(scan_tmp_bytes(std::max(n1, z.ng)));
+  b += lease_bytes(4 * ceil_div(n1, 4)) + 3 * lease_bytes(sizeof(int32_t) * z.item_cap) +
+       lease_bytes(sizeof(int32_t) * (n1 + 1)) + lease_bytes(sizeof(int32_t) * 3 * z.ng) +
+       lease_bytes(sizeof(long long) * 3 * BCB_MAX_BLOCKS) + lease_bytes(sizeof(BcCtl));
   return b;
 }
 
@@ -711,8 +1006,17 @@ size_t bc_groups_workspace_bytes(int64_t n, int64_t nnz, const std::vector<int64
 template <typename DT>
 static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp, int64_t outs,
                    double* bc_h, BcWork& w, hipStream_t s);
+template <typename DT>
+static bool bc_run_dev(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp_,
+                       int64_t outs, double* bc_h, BcWork& w, hipStream_t s, bool replay);
 static void bc_run_any(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp,
                        int64_t outs, double* bc_h, BcWork& w, hipStream_t s) {
+  const int mode = ctx().bc_loop;
+  if (mode >= 1 && A.nrows > 0) {
+    if (!bc_run_dev<uint8_t>(A, src_h, lane_h, ns, nsp, outs, bc_h, w, s, mode == 2))
+      (void)bc_run_dev<uint16_t>(A, src_h, lane_h, ns, nsp, outs, bc_h, w, s, mode == 2);
+    return;
+  }
   if (!bc_run<uint8_t>(A, src_h, lane_h, ns, nsp, outs, bc_h, w, s))
     (void)bc_run<uint16_t>(A, src_h, lane_h, ns, nsp, outs, bc_h, w, s);
 }
@@ -820,13 +1124,13 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
   auto candidates = [&](const int32_t* front, int32_t nfront, bool costs) {
     ev_open();
     hipLaunchKernelGGL(k_unvisited_flags<DT>, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
-                       (const DT*)w.depth.as<DT>(), n, nsp, w.flag.as<uint8_t>());
+                       (const DT*)w.depth.as<DT>(), n, nsp, w.flag.as<uint8_t>(), (const BcCtl*)nullptr, 0);
     frontier_from_flags(w.flag.as<uint8_t>(), n, w, cand, count_h, s, false);
     if (costs) {
       fill_u64(w.dsum.p, 0ull, 2, s);
       hipLaunchKernelGGL(k_list_edges, dim3(grid_for(std::max<int64_t>(n, 1), 256, 1024)), dim3(256), 0, s, front,
                          nfront, (const int32_t*)cand, (const int32_t*)(w.off.as<int32_t>() + ng), A.rp,
-                         w.dsum.as<unsigned long long>());
+                         w.dsum.as<unsigned long long>(), (const BcCtl*)nullptr, 0);
       GA_HIP(hipMemcpyAsync(edges_h, w.dsum.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     }
     GA_HIP(hipGetLastError());
@@ -858,11 +1162,12 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
       hipLaunchKernelGGL(k_bc_bottom_up_items<DT>, dim3(grid_for(nc + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0,
                          s, A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
                          (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nc), nsp,
-                         d, w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(), w.flag.as<uint8_t>());
+                         d, w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(), w.flag.as<uint8_t>(),
+                         (const BcCtl*)nullptr, 0);
       hipLaunchKernelGGL(k_bc_bottom_up_combine<DT>, dim3(grid_for(nc, 4, 65536)), dim3(256), 0, s,
                          (const int32_t*)cand, nc, A.rp, (const int32_t*)w.hstart.as<int32_t>(), nsp, d,
                          w.depth.as<DT>(), w.sigma.as<double>(), (const double*)w.partial.as<double>(),
-                         w.flag.as<uint8_t>());
+                         w.flag.as<uint8_t>(), (const BcCtl*)nullptr, 0);
       GA_HIP(hipGetLastError());
       ev_close();
       ++timed;
@@ -878,7 +1183,7 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
     hipLaunchKernelGGL(k_bc_discover_items<DT>, dim3(grid_for(nf + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s,
                        A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
                        (const int32_t*)(w.istart.as<int32_t>() + nf), nsp, d, w.depth.as<DT>(),
-                       w.flag.as<uint8_t>());
+                       w.flag.as<uint8_t>(), (const BcCtl*)nullptr, 0);
     GA_HIP(hipGetLastError());
     ev_close();
     ++timed;
@@ -890,10 +1195,11 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
     hipLaunchKernelGGL(k_bc_sigma_items<DT>, dim3(grid_for(nn + A.nnz / BC_CH + 1, 4, 65536)), dim3(256), 0, s, A.rp,
                        A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
                        (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nn), nsp,
-                       d + 1, (const DT*)w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>());
+                       d + 1, (const DT*)w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(),
+                       (const BcCtl*)nullptr, 0);
     hipLaunchKernelGGL(k_bc_sigma_combine<DT>, dim3(grid_for(nn, 4, 65536)), dim3(256), 0, s, (const int32_t*)next, nn,
                        A.rp, (const int32_t*)w.hstart.as<int32_t>(), nsp, d + 1, (const DT*)w.depth.as<DT>(),
-                       w.sigma.as<double>(), (const double*)w.partial.as<double>());
+                       w.sigma.as<double>(), (const double*)w.partial.as<double>(), (const BcCtl*)nullptr, 0);
     GA_HIP(hipGetLastError());
     ev_close();
     ++timed;
@@ -912,12 +1218,12 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
   if (dmax >= 1) {
     hipLaunchKernelGGL(k_bc_last<DT>, dim3(grid_for(nf, 4, 65536)), dim3(256), 0, s, (const int32_t*)lists[dmax & 1], nf,
                        nsp, dmax, (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(),
-                       w.q.as<double>());
+                       w.q.as<double>(), (const BcCtl*)nullptr);
     GA_HIP(hipGetLastError());
     for (int k = dmax - 1; k >= 1; --k) {
       // the level-k list again, from the depths (vertices with a lane at depth k)
       hipLaunchKernelGGL(k_level_flags<DT>, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s,
-                         (const DT*)w.depth.as<DT>(), n, nsp, k, w.flag.as<uint8_t>());
+                         (const DT*)w.depth.as<DT>(), n, nsp, k, w.flag.as<uint8_t>(), (const BcCtl*)nullptr);
       int32_t* front = lists[0];
       frontier_from_flags(w.flag.as<uint8_t>(), n, w, front, count_h, s);
       const int32_t nk = *count_h;
@@ -927,13 +1233,238 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
                          A.rp, A.col, (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
                          (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(w.istart.as<int32_t>() + nk), nsp,
                          k, (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(),
-                         w.q.as<double>(), w.bc.as<double>(), w.partial.as<double>(), bc_stride);
+                         w.q.as<double>(), w.bc.as<double>(), w.partial.as<double>(), bc_stride,
+                         (const BcCtl*)nullptr);
       hipLaunchKernelGGL(k_bc_combine<DT>, dim3(grid_for(nk, 4, 65536)), dim3(256), 0, s, (const int32_t*)front, nk, A.rp,
                          (const int32_t*)w.hstart.as<int32_t>(), nsp, k, (const DT*)w.depth.as<DT>(),
                          (const double*)w.sigma.as<double>(), w.q.as<double>(), w.bc.as<double>(),
-                         (const double*)w.partial.as<double>(), bc_stride);
+                         (const double*)w.partial.as<double>(), bc_stride, (const BcCtl*)nullptr);
       GA_HIP(hipGetLastError());
     }
+  }
+  ev_close();
+  if (n) GA_HIP(hipMemcpyAsync(bc_h, w.bc.p, sizeof(double) * n * outs, hipMemcpyDeviceToHost, s));
+  GA_HIP(hipStreamSynchronize(s));
+  return true;
+}
+
+// The pass with level lists sized on the device (GA_BC_LOOP 1 and 2).  The
+// same kernels as bc_run, in the same order within a level, on the same
+// lists (ascending vertex order) and items, so the scores are bit-identical
+// to it; what changes is who knows the sizes.  A forward step is one fixed
+// sequence of launches: candidates and the direction decision, the
+// bottom-up sweep or the top-down discovery (each skipped by the other
+// direction), the next level's list, the path counts; the kernels read the
+// level and the list sizes from the control block, and a step queued after
+// the last level does nothing.  The host keeps two steps queued ahead of the
+// sequence number the last kernel of each step writes to pinned host memory,
+// and waits once, for the end of the sweep (it needs the deepest level to
+// size the backward loop).  The backward steps then run with no host wait.
+// With `replay`, each step is captured once as a HIP graph (kept in the
+// workspace while the launch arguments stay the same) and replayed per level.
+template <typename DT>
+static bool bc_run_dev(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, int64_t ns, int64_t nsp_,
+                       int64_t outs, double* bc_h, BcWork& w, hipStream_t s, bool replay) {
+  const int64_t n = A.nrows;
+  const int64_t n1 = std::max<int64_t>(n, 1);
+  const BcSizes z = bc_sizes(n, A.nnz, ns, nsp_, outs);
+  const int nsp = (int)z.nsp;
+  const int64_t total = z.total;
+  const int64_t bc_stride = outs > 1 ? n : 0;
+  bc_reserve(w, n, z, ns);
+  Context& cx = ctx();
+  GA_HIP(hipMemcpyAsync(w.src.p, src_h, sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  GA_HIP(hipMemcpyAsync(w.src.as<int32_t>() + ns, lane_h, sizeof(int32_t) * ns, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_bc_init<DT>, dim3(grid_for(total, 256, 16384)), dim3(256), 0, s, w.depth.as<DT>(),
+                     w.sigma.as<double>(), total);
+  fill_u64(w.bc.p, 0ull, n * outs, s);
+  fill_u32(w.flag.p, 0u, ceil_div(n1, 4), s);
+  hipLaunchKernelGGL(k_bc_sources<DT>, dim3(grid_for(ns, 256)), dim3(256), 0, s, (const int32_t*)w.src.as<int32_t>(),
+                     (const int32_t*)(w.src.as<int32_t>() + ns), (int)ns, nsp, w.depth.as<DT>(),
+                     w.sigma.as<double>(), w.flag.as<uint8_t>());
+  GA_HIP(hipGetLastError());
+  BcCtl* ctl = w.ctl.as<BcCtl>();
+  int32_t* lists[2] = {w.levels.as<int32_t>(), w.levels.as<int32_t>() + n1};
+  int32_t* cand = w.levels.as<int32_t>() + 2 * n1;
+  long long* hw_dev = nullptr;
+  long long* hw = progress_words(&hw_dev) + 24;  // words 24..27: d, dmax, over, 2 x steps + done
+  hw_dev += 24;
+  const int limit = (int)bc_unvisited<DT>();
+  // grids: sized for the largest list (one item per wavefront, as bc_run
+  // sizes them for the list at hand): the dispatcher hands workgroups to
+  // CUs as they free up, which balances the uneven hub items; workgroups
+  // past the list's items exit at once
+  const unsigned g_items = grid_for(n1 + A.nnz / BC_CH + 1, 4, 65536);
+  const unsigned g_vert = grid_for(n1, 4, 16384);
+  // list builder: groups of 64 vertices, one per wavefront, at most
+  // BCB_MAX_BLOCKS workgroups each walking a contiguous range
+  const int64_t ngroups = ceil_div(n1, BC_FLAG_CHUNK);
+  const int64_t g_build = std::min<int64_t>(BCB_MAX_BLOCKS, ceil_div(ngroups, BCB_T / 64));
+  const int64_t per_block = ceil_div(ngroups, g_build);
+  auto build = [&](uint8_t* flag, int32_t* list, int32_t* hst, int32_t* iv, int32_t* it0, int32_t* isl, int32_t* tot,
+                   const BcCtl* c, int guard, int to_next) {
+    hipLaunchKernelGGL(k_bc_build_count, dim3((unsigned)g_build), dim3(BCB_T), 0, s, (const uint8_t*)flag, n, A.rp,
+                       w.gcnt.as<int32_t>(), w.gsum.as<long long>(), per_block, c, guard);
+    hipLaunchKernelGGL(k_bc_build_emit, dim3((unsigned)g_build), dim3(BCB_T), 0, s, flag, n, A.rp,
+                       (const int32_t*)w.gcnt.as<int32_t>(), (const long long*)w.gsum.as<long long>(), per_block,
+                       list, hst, iv, it0, isl, tot, c, guard, to_next);
+  };
+  auto op = [&](int code, int lim, long long* host) {
+    hipLaunchKernelGGL(k_bc_ctl, dim3(1), dim3(64), 0, s, ctl, code, (int32_t*)nullptr, (int32_t*)nullptr, 0, 0.0,
+                       lim, host);
+  };
+  auto fwd_step = [&]() {
+    // candidates of a bottom-up step and, while top-down, the decision
+    hipLaunchKernelGGL(k_unvisited_flags<DT>, dim3(g_vert), dim3(256), 0, s, A.rp, (const DT*)w.depth.as<DT>(), n,
+                       nsp, w.cflag.as<uint8_t>(), (const BcCtl*)ctl, (int)BG_CAND);
+    build(w.cflag.as<uint8_t>(), cand, w.chstart.as<int32_t>(), w.cit_v.as<int32_t>(), w.cit_t0.as<int32_t>(),
+          w.cit_slot.as<int32_t>(), ctl->cand_tot, ctl, BG_CAND, 0);
+    hipLaunchKernelGGL(k_list_edges, dim3(grid_for(n1, 256, 1024)), dim3(256), 0, s, (const int32_t*)nullptr, 0,
+                       (const int32_t*)cand, (const int32_t*)ctl->cand_tot, A.rp, ctl->edges, (const BcCtl*)ctl,
+                       (int)BG_DECIDE);
+    op(BC_OP_DECIDE, 0, nullptr);
+    // bottom-up: the candidates pull from level d
+    hipLaunchKernelGGL(k_bc_bottom_up_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
+                       (const int32_t*)w.cit_v.as<int32_t>(), (const int32_t*)w.cit_t0.as<int32_t>(),
+                       (const int32_t*)w.cit_slot.as<int32_t>(), (const int32_t*)(ctl->cand_tot + 1), nsp, 0,
+                       w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(), w.flag.as<uint8_t>(),
+                       (const BcCtl*)ctl, (int)BG_BU);
+    hipLaunchKernelGGL(k_bc_bottom_up_combine<DT>, dim3(g_items), dim3(256), 0, s, (const int32_t*)cand, 0, A.rp,
+                       (const int32_t*)w.chstart.as<int32_t>(), nsp, 0, w.depth.as<DT>(), w.sigma.as<double>(),
+                       (const double*)w.partial.as<double>(), w.flag.as<uint8_t>(), (const BcCtl*)ctl, (int)BG_BU);
+    // top-down: discovery from level d (its items were built with its list)
+    hipLaunchKernelGGL(k_bc_discover_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
+                       (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+                       (const int32_t*)(ctl->next_tot + 1), nsp, 0, w.depth.as<DT>(), w.flag.as<uint8_t>(),
+                       (const BcCtl*)ctl, (int)BG_TD);
+    // the level-(d+1) list and its items
+    build(w.flag.as<uint8_t>(), nullptr, w.hstart.as<int32_t>(), w.it_v.as<int32_t>(), w.it_t0.as<int32_t>(),
+          w.it_slot.as<int32_t>(), ctl->next_tot, ctl, BG_ANY, 1);
+    op(BC_OP_NEXT, 0, nullptr);
+    // top-down: path counts of level d + 1
+    hipLaunchKernelGGL(k_bc_sigma_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
+                       (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+                       (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(ctl->next_tot + 1), nsp, 0,
+                       (const DT*)w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(), (const BcCtl*)ctl,
+                       (int)BG_TD);
+    hipLaunchKernelGGL(k_bc_sigma_combine<DT>, dim3(g_items), dim3(256), 0, s, (const int32_t*)nullptr, 0, A.rp,
+                       (const int32_t*)w.hstart.as<int32_t>(), nsp, 0, (const DT*)w.depth.as<DT>(),
+                       w.sigma.as<double>(), (const double*)w.partial.as<double>(), (const BcCtl*)ctl, (int)BG_TD);
+    op(BC_OP_END, limit, hw_dev);
+  };
+  auto bwd_step = [&]() {
+    hipLaunchKernelGGL(k_level_flags<DT>, dim3(g_vert), dim3(256), 0, s, (const DT*)w.depth.as<DT>(), n, nsp, 0,
+                       w.flag.as<uint8_t>(), (const BcCtl*)ctl);
+    build(w.flag.as<uint8_t>(), lists[0], w.hstart.as<int32_t>(), w.it_v.as<int32_t>(), w.it_t0.as<int32_t>(),
+          w.it_slot.as<int32_t>(), ctl->back_tot, nullptr, BG_ANY, 0);
+    hipLaunchKernelGGL(k_bc_backward_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
+                       (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
+                       (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(ctl->back_tot + 1), nsp, 0,
+                       (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(), w.q.as<double>(),
+                       w.bc.as<double>(), w.partial.as<double>(), bc_stride, (const BcCtl*)ctl);
+    hipLaunchKernelGGL(k_bc_combine<DT>, dim3(g_items), dim3(256), 0, s, (const int32_t*)lists[0], 0, A.rp,
+                       (const int32_t*)w.hstart.as<int32_t>(), nsp, 0, (const DT*)w.depth.as<DT>(),
+                       (const double*)w.sigma.as<double>(), w.q.as<double>(), w.bc.as<double>(),
+                       (const double*)w.partial.as<double>(), bc_stride, (const BcCtl*)ctl);
+    op(BC_OP_BACK, 0, nullptr);
+  };
+  // captured steps, kept while every launch argument they hold is the same
+  if (replay) {
+    const std::vector<uintptr_t> key = {
+        (uintptr_t)A.rp, (uintptr_t)A.col, (uintptr_t)n, (uintptr_t)A.nnz, (uintptr_t)nsp, (uintptr_t)bc_stride,
+        sizeof(DT), (uintptr_t)w.depth.p, (uintptr_t)w.sigma.p, (uintptr_t)w.q.p, (uintptr_t)w.bc.p,
+        (uintptr_t)w.flag.p, (uintptr_t)w.cflag.p, (uintptr_t)w.levels.p, (uintptr_t)w.it_v.p, (uintptr_t)w.it_t0.p,
+        (uintptr_t)w.it_slot.p, (uintptr_t)w.hstart.p, (uintptr_t)w.cit_v.p, (uintptr_t)w.cit_t0.p,
+        (uintptr_t)w.cit_slot.p, (uintptr_t)w.chstart.p, (uintptr_t)w.partial.p, (uintptr_t)w.gcnt.p,
+        (uintptr_t)w.gsum.p, (uintptr_t)ctl, (uintptr_t)hw_dev, (uintptr_t)cx.num_cu};
+    if (w.graphs.key != key || !w.graphs.fwd) {
+      w.graphs.reset();
+      auto capture = [&](auto&& step) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        GA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        step();
+        GA_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        GA_HIP(ie);
+        return e;
+      };
+      w.graphs.fwd = capture(fwd_step);
+      w.graphs.bwd = capture(bwd_step);
+      w.graphs.key = key;
+    }
+  }
+  hipEvent_t ev0 = nullptr;
+  auto ev_open = [&] {
+    if (!cx.prof_events_on) return;
+    ev0 = bc_event(cx);
+    GA_HIP(hipEventRecord(ev0, s));
+  };
+  auto ev_close = [&] {
+    if (!cx.prof_events_on) return;
+    hipEvent_t ev1 = bc_event(cx);
+    GA_HIP(hipEventRecord(ev1, s));
+    cx.prof_events.emplace_back(ev0, ev1);
+    cx.prof_launches += 1;
+  };
+  // level 0: the sources
+  hipLaunchKernelGGL(k_bc_ctl, dim3(1), dim3(64), 0, s, ctl, (int)BC_OP_INIT, lists[0], lists[1],
+                     cx.bc_bottom_up ? 1 : 0, cx.bc_alpha, limit, (long long*)nullptr);
+  build(w.flag.as<uint8_t>(), nullptr, w.hstart.as<int32_t>(), w.it_v.as<int32_t>(), w.it_t0.as<int32_t>(),
+        w.it_slot.as<int32_t>(), ctl->next_tot, ctl, BG_ANY, 1);
+  op(BC_OP_START, 0, nullptr);
+  GA_HIP(hipGetLastError());
+  // forward sweep: steps queued two ahead of the published sequence number
+  for (int i = 0; i < 4; ++i) __atomic_store_n(&hw[i], 0ll, __ATOMIC_RELAXED);
+  ev_open();
+  long long queued = 0, dmax = 0, over = 0;
+  const long long max_steps = (long long)limit + 2;
+  while (true) {
+    const long long word = __atomic_load_n(&hw[3], __ATOMIC_ACQUIRE);
+    const long long seq = word >> 1;
+    if (word & 1) {  // the sweep is over; dmax and over were stored before this word
+      dmax = __atomic_load_n(&hw[1], __ATOMIC_RELAXED);
+      over = __atomic_load_n(&hw[2], __ATOMIC_RELAXED);
+      break;
+    }
+    if (queued - seq < 2) {
+      GA_REQUIRE(queued < max_steps, GA_ERR_STATE, "BC forward sweep did not finish within its level bound");
+      if (replay) GA_HIP(hipGraphLaunch(w.graphs.fwd, s));
+      else fwd_step();
+      GA_HIP(hipGetLastError());
+      ++queued;
+      continue;
+    }
+    // wait for the device; a failed launch surfaces here instead of a hang
+    const hipError_t q = hipStreamQuery(s);
+    if (q != hipSuccess && q != hipErrorNotReady) GA_HIP(q);
+  }
+  ev_close();
+  if (over) {  // deeper than DT can hold: the caller redoes the pass with 16-bit depths
+    GA_REQUIRE(sizeof(DT) < sizeof(uint16_t), GA_ERR_LIMIT, "BFS deeper than 65533 levels");
+    GA_HIP(hipStreamSynchronize(s));
+    return false;
+  }
+  if (cx.prof_bytes) {
+    hipLaunchKernelGGL(k_bc_account<DT>, dim3(grid_for(n, 4, 16384)), dim3(256), 0, s, A.rp,
+                       (const DT*)w.depth.as<DT>(), n, nsp, cx.prof_dev.as<unsigned long long>());
+    cx.prof_bytes_host += (dmax + 1) * (4 * (n + 1) + 4 * A.nnz);
+    cx.prof_bytes_launches += 2;  // the forward sweep and the backward sweep
+  }
+  ev_open();
+  if (dmax >= 1) {
+    hipLaunchKernelGGL(k_bc_last<DT>, dim3(g_items), dim3(256), 0, s, (const int32_t*)nullptr, 0, nsp, 0,
+                       (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(), w.q.as<double>(),
+                       (const BcCtl*)ctl);
+    if (dmax >= 2) {
+      op(BC_OP_BACK, (int)dmax - 1, nullptr);  // first backward level
+      for (long long k = dmax - 1; k >= 1; --k) {
+        if (replay) GA_HIP(hipGraphLaunch(w.graphs.bwd, s));
+        else bwd_step();
+      }
+    }
+    GA_HIP(hipGetLastError());
   }
   ev_close();
   if (n) GA_HIP(hipMemcpyAsync(bc_h, w.bc.p, sizeof(double) * n * outs, hipMemcpyDeviceToHost, s));

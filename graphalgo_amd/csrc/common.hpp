@@ -192,6 +192,11 @@ struct Context {
   // x 256 sources: 14.9 ms per pass at alpha 1, 12.7 at 2, 12.3 at 4 and 14.
   bool bc_bottom_up = true;
   double bc_alpha = 14.0;
+  // BC level loop (GA_BC_LOOP): 0 = host-steered (the host reads every
+  // level list's size), 1 = device-driven (level lists sized on the device,
+  // one host wait at the end of the forward sweep), 2 = the device-driven
+  // steps replayed from captured HIP graphs
+  int bc_loop = 1;
   // target wedges per work item when rows are split by their measured work
   // (GA_ITEM_WEDGES overrides it at ga_init)
   long long item_wedges = 32768;
@@ -426,10 +431,42 @@ void ktruss_multi_dev(const DCsr& A, const int32_t* ks, int nk, TrussWork& tw, h
 size_t ktruss_workspace_bytes(int64_t n, int64_t nnz);
 
 // Batched betweenness centrality workspace and driver (bc.hip).
+// Instantiated HIP graphs of a BC pass's steps (captured once, replayed per
+// level), with the launch arguments they were captured with.
+struct BcGraphs {
+  hipGraphExec_t fwd = nullptr, bwd = nullptr;
+  std::vector<uintptr_t> key;
+  BcGraphs() = default;
+  BcGraphs(const BcGraphs&) = delete;
+  BcGraphs& operator=(const BcGraphs&) = delete;
+  BcGraphs(BcGraphs&& o) noexcept : fwd(o.fwd), bwd(o.bwd), key(std::move(o.key)) { o.fwd = o.bwd = nullptr; }
+  BcGraphs& operator=(BcGraphs&& o) noexcept {
+    if (this != &o) {
+      reset();
+      fwd = o.fwd;
+      bwd = o.bwd;
+      key = std::move(o.key);
+      o.fwd = o.bwd = nullptr;
+    }
+    return *this;
+  }
+  ~BcGraphs() { reset(); }
+  void reset() {
+    if (fwd) (void)hipGraphExecDestroy(fwd);
+    if (bwd) (void)hipGraphExecDestroy(bwd);
+    fwd = bwd = nullptr;
+    key.clear();
+  }
+};
+
 struct BcWork {
   DevBuf depth, sigma, q, bc, flag, levels, cnt, off, scan_tmp, src;
   DevBuf icnt, hcnt, istart, hstart, it_v, it_t0, it_slot, partial;  // neighbour-chunked items
   DevBuf dsum;                                                        // edge counts of two lists
+  // device-driven passes: candidate flags, list and items, list builder
+  // counts, the control block
+  DevBuf cflag, cit_v, cit_t0, cit_slot, chstart, gcnt, gsum, ctl;
+  BcGraphs graphs;
 };
 void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h, BcWork& w, hipStream_t s);
 size_t bc_workspace_bytes(int64_t n, int64_t nnz, int64_t ns);

@@ -362,6 +362,7 @@ ga_status ga_init(int device) {
     if (const char* ss = std::getenv("GA_SELECTIVE_SUPPORT"))
       c->selective_support = std::min(1, std::max(-1, std::atoi(ss)));
     if (const char* sj = std::getenv("GA_SERVE_JOIN")) c->serve_join = std::min(2, std::max(0, std::atoi(sj)));
+    if (const char* bl = std::getenv("GA_BC_LOOP")) c->bc_loop = std::min(2, std::max(0, std::atoi(bl)));
     if (const char* st = std::getenv("GA_SUPPORT_TRI")) c->support_tri = std::min(1, std::max(-1, std::atoi(st)));
     if (const char* tl = std::getenv("GA_TRUSS_LOOP")) c->truss_loop = std::min(4, std::max(-1, std::atoi(tl)));
     if (const char* iw = std::getenv("GA_ITEM_WEDGES")) c->item_wedges = std::max(1LL, std::atoll(iw));
@@ -415,6 +416,10 @@ ga_status ga_set_option(int option, double value) {
         GA_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0, GA_ERR_ARG, "serve_join is 0, 1 or 2");
         c.serve_join = (int)value;
         break;
+      case GA_OPT_BC_LOOP:
+        GA_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0, GA_ERR_ARG, "bc_loop is 0, 1 or 2");
+        c.bc_loop = (int)value;
+        break;
       case GA_OPT_SUPPORT_TRI:
         GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0, GA_ERR_ARG, "support_tri is -1, 0 or 1");
         c.support_tri = (int)value;
@@ -444,6 +449,7 @@ ga_status ga_get_option(int option, double* value) {
       case GA_OPT_TRUSS_LOOP: *value = c.truss_loop; break;
       case GA_OPT_SUPPORT_TRI: *value = c.support_tri; break;
       case GA_OPT_SERVE_JOIN: *value = c.serve_join; break;
+      case GA_OPT_BC_LOOP: *value = c.bc_loop; break;
       default: GA_REQUIRE(false, GA_ERR_ARG, "unknown option " + std::to_string(option));
     }
   });

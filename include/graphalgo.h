@@ -358,10 +358,15 @@ enum {
                                    the strict lower half with each hit added to its three edges
                                    (GA_SUPPORT_TRI): 1 = on, 0 = off, -1 (default) = on for
                                    every pass whose graph holds at least 2^20 stored entries */
-  GA_OPT_SERVE_JOIN = 9         /* served k-truss queries joining a peeling run on their graph
+  GA_OPT_SERVE_JOIN = 9,        /* served k-truss queries joining a peeling run on their graph
                                    that is already going (GA_SERVE_JOIN): 0 = never, 1 = when
                                    the query's k lies within the run's k range, 2 = also above
                                    it (the run goes on to it) */
+  GA_OPT_BC_LOOP = 10           /* BC level loop (GA_BC_LOOP): 0 = host-steered, the host reading
+                                   the size of every level list; 1 (default) = level lists sized
+                                   on the device, the host waiting once per pass (end of the
+                                   forward sweep); 2 = the same steps replayed from captured HIP
+                                   graphs */
 };
 ga_status ga_set_option(int option, double value);
 ga_status ga_get_option(int option, double* value);

@@ -1,6 +1,7 @@
 """GPU tests: every tuning option gives the same results.  BC with top-down
 forward steps only, with bottom-up steps from the first level on and with the
-default switch point (bit-identical scores, and the oracle); triangle counts
+default switch point (bit-identical scores, and the oracle); the host-steered,
+device-sized and graph-replayed BC level loops (bit-identical); triangle counts
 with every short-row kernel and claim size (exact); truss numbers with tiny
 work items (rows split over many items, whose partial supports meet in global
 atomics) and with the selective support passes (exact); the host-driven,
@@ -51,6 +52,52 @@ def test_bc_directions_deep_and_disconnected(gpu, G, oracle):
         with gpu.options(**opts):
             got = gpu.betweenness_centrality_batch(g, src)
         assert _bc_close(got, want), opts
+
+
+@pytest.mark.parametrize("scale,ns", [(12, 64), (14, 256), (16, 200)])
+def test_bc_loop_modes_bit_identical(gpu, G, oracle, scale, ns):
+    """Host-steered levels, levels sized on the device, and those steps
+    replayed from captured HIP graphs: the same scores bit for bit, in both
+    forward directions, on repeated calls of one resident graph (the captured
+    steps are reused) and against the oracle."""
+    g = G.rmat(scale, 16, seed=60 + scale)
+    src = G.pick_sources(g, ns, 5)
+    src2 = G.pick_sources(g, ns, 6)
+    R = gpu.Graph(g)
+    got = {}
+    for loop in (0, 1, 2):
+        for bu in ({"bc_bottom_up": 0}, {"bc_bottom_up": 1}, {"bc_bottom_up": 1, "bc_alpha": 1e6}):
+            print(f"case bc loop={loop} {bu} rmat{scale} ns={ns}", flush=True)
+            with gpu.options(bc_loop=loop, **bu):
+                got[loop, tuple(bu.items())] = (R.bc(src), R.bc(src2), R.bc(src))
+    R.close()
+    ref = got[0, (("bc_bottom_up", 0),)]
+    for key, (a, b, c) in got.items():
+        assert np.array_equal(a, ref[0]) and np.array_equal(c, ref[0]), key
+        assert np.array_equal(b, ref[1]), key
+    if scale <= 14:
+        assert _bc_close(ref[0], oracle.bc(g, src))
+
+
+def test_bc_loop_modes_deep_and_disconnected(gpu, G, oracle):
+    """Levels beyond the 8-bit depth range (the pass is redone with 16-bit
+    depths), a disjoint star no source reaches, isolated vertices, and a
+    graph whose sources have no edges."""
+    n_path, n_star = 600, 50
+    rows = list(range(n_path - 1)) + [n_path] * (n_star - 1)
+    cols = list(range(1, n_path)) + list(range(n_path + 1, n_path + n_star))
+    g = G.CSR.from_edges(n_path + n_star + 5, rows, cols)
+    src = np.array([0, 7, 300, n_path + 3], dtype=np.int32)
+    lonely = np.array([n_path + n_star, n_path + n_star + 4], dtype=np.int32)
+    want, want_lonely = oracle.bc(g, src), oracle.bc(g, lonely)
+    for loop in (0, 1, 2):
+        for opts in ({"bc_bottom_up": 0}, {"bc_bottom_up": 1, "bc_alpha": 1e6}):
+            print(f"case bc deep loop={loop} {opts}", flush=True)
+            with gpu.options(bc_loop=loop, **opts):
+                got = gpu.betweenness_centrality_batch(g, src)
+                got_lonely = gpu.betweenness_centrality_batch(g, lonely)
+            assert _bc_close(got, want), (loop, opts)
+            assert np.array_equal(got_lonely, want_lonely) and not got_lonely.any()
 
 
 @pytest.fixture(scope="module")
@@ -202,7 +249,7 @@ def test_truss_support_tri_rmat16(gpu, G, golden):
 def test_option_validation(gpu):
     for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
                       ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 5), ("truss_loop", -2),
-                      ("support_tri", 2), ("serve_join", 3)):
+                      ("support_tri", 2), ("serve_join", 3), ("bc_loop", 3), ("bc_loop", -1)):
         with pytest.raises(Exception):
             gpu.set_option(name, bad)
     before = gpu.get_option("row_claim")
