@@ -57,9 +57,9 @@ struct BcCtl {
   int32_t* next;  // level-(d+1) list being built
   int d, nf, done, dmax, bu, over, allow_bu, bk;  // bk: level of the backward step
   double alpha;
-  int next_tot[3];  // list builder totals: vertices, items, hub slots
-  int cand_tot[3];  // the same for the bottom-up candidates
-  int back_tot[3];  // the same for a backward level
+  int next_tot[4];  // list builder totals: vertices, items, hub slots, hub vertices
+  int cand_tot[4];  // the same for the bottom-up candidates
+  int back_tot[4];  // the same for a backward level
   unsigned long long edges[2];
   long long seq;
 };
@@ -206,22 +206,16 @@ __global__ void k_bc_item_fill(const int32_t* __restrict__ front, int nf, const 
 // neighbour depths are in flight per lane.  A neighbour pair that is still
 // unvisited gets depth d+1 (every writer stores the same byte) and its vertex
 // is flagged for the level-(d+1) list.
+// (nb: the wavefront's BC_CH-entry LDS slice)
 template <typename DT>
-__global__ __launch_bounds__(256) void k_bc_discover_items(
-    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
-    const int32_t* __restrict__ it_t0, const int32_t* __restrict__ n_items_ptr, int nsp, int d,
-    DT* __restrict__ depth, uint8_t* __restrict__ next_flag, const BcCtl* __restrict__ ctl, int guard) {
-  __shared__ int32_t s_nb[4][BC_CH];
-  if (ctl) {
-    if (!bc_live(ctl, guard)) return;
-    d = ctl->d;
-  }
+__device__ __forceinline__ void bc_discover_body(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                 const int32_t* __restrict__ it_v, const int32_t* __restrict__ it_t0,
+                                                 int n_items, int nsp, int d, DT* __restrict__ depth,
+                                                 uint8_t* __restrict__ next_flag, int32_t* nb) {
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  const int n_items = *n_items_ptr;
   const DT dd = (DT)d, dn = (DT)(d + 1);
-  int32_t* nb = s_nb[wv];
   for (int q = blockIdx.x * wpb + wv; q < n_items; q += gridDim.x * wpb) {
     const int u = it_v[q];
     const int t0 = it_t0[q];
@@ -250,6 +244,19 @@ __global__ __launch_bounds__(256) void k_bc_discover_items(
     }
     __threadfence_block();  // the neighbour slots are rewritten by the next item
   }
+}
+
+template <typename DT>
+__global__ __launch_bounds__(256) void k_bc_discover_items(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
+    const int32_t* __restrict__ it_t0, const int32_t* __restrict__ n_items_ptr, int nsp, int d,
+    DT* __restrict__ depth, uint8_t* __restrict__ next_flag, const BcCtl* __restrict__ ctl, int guard) {
+  __shared__ int32_t s_nb[4][BC_CH];
+  if (ctl) {
+    if (!bc_live(ctl, guard)) return;
+    d = ctl->d;
+  }
+  bc_discover_body<DT>(rp, col, it_v, it_t0, *n_items_ptr, nsp, d, depth, next_flag, s_nb[threadIdx.x >> 6]);
 }
 
 // Path counts of level d (pull along the edges, the frontier product):
@@ -324,22 +331,16 @@ __global__ __launch_bounds__(256) void k_bc_sigma_items(
 // partial sums; k_bc_bottom_up_combine decides their lanes.  Reads of
 // depth = d never see a lane written here (those become d+1).
 template <typename DT>
-__global__ __launch_bounds__(256) void k_bc_bottom_up_items(
-    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
-    const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
-    int nsp, int d, DT* __restrict__ depth, double* __restrict__ sigma, double* __restrict__ partial,
-    uint8_t* __restrict__ next_flag, const BcCtl* __restrict__ ctl, int guard) {
-  __shared__ int32_t s_nb[4][BC_CH];
-  if (ctl) {
-    if (!bc_live(ctl, guard)) return;
-    d = ctl->d;
-  }
+__device__ __forceinline__ void bc_bottom_up_body(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ it_v, const int32_t* __restrict__ it_t0,
+                                                  const int32_t* __restrict__ it_slot, int n_items, int nsp, int d,
+                                                  DT* __restrict__ depth, double* __restrict__ sigma,
+                                                  double* __restrict__ partial, uint8_t* __restrict__ next_flag,
+                                                  int32_t* nb) {
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  const int n_items = *n_items_ptr;
   const DT dd = (DT)d, dn = (DT)(d + 1);
-  int32_t* nb = s_nb[wv];
   for (int it = blockIdx.x * wpb + wv; it < n_items; it += gridDim.x * wpb) {
     const int v = it_v[it];
     const int t0 = it_t0[it];
@@ -401,6 +402,40 @@ __global__ __launch_bounds__(256) void k_bc_bottom_up_items(
   }
 }
 
+template <typename DT>
+__global__ __launch_bounds__(256) void k_bc_bottom_up_items(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
+    const int32_t* __restrict__ it_t0, const int32_t* __restrict__ it_slot, const int32_t* __restrict__ n_items_ptr,
+    int nsp, int d, DT* __restrict__ depth, double* __restrict__ sigma, double* __restrict__ partial,
+    uint8_t* __restrict__ next_flag, const BcCtl* __restrict__ ctl, int guard) {
+  __shared__ int32_t s_nb[4][BC_CH];
+  if (ctl) {
+    if (!bc_live(ctl, guard)) return;
+    d = ctl->d;
+  }
+  bc_bottom_up_body<DT>(rp, col, it_v, it_t0, it_slot, *n_items_ptr, nsp, d, depth, sigma, partial, next_flag,
+                        s_nb[threadIdx.x >> 6]);
+}
+
+// Device-driven forward step, first sweep: the bottom-up sweep of the
+// candidates or the top-down discovery from the level-d list, whichever
+// direction the step has taken (one launch instead of one per direction).
+template <typename DT>
+__global__ __launch_bounds__(256) void k_bc_forward_items(
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ col, const int32_t* __restrict__ it_v,
+    const int32_t* __restrict__ it_t0, const int32_t* __restrict__ cit_v, const int32_t* __restrict__ cit_t0,
+    const int32_t* __restrict__ cit_slot, int nsp, DT* __restrict__ depth, double* __restrict__ sigma,
+    double* __restrict__ partial, uint8_t* __restrict__ next_flag, const BcCtl* __restrict__ ctl) {
+  __shared__ int32_t s_nb[4][BC_CH];
+  if (ctl->done) return;
+  if (ctl->bu)
+    bc_bottom_up_body<DT>(rp, col, cit_v, cit_t0, cit_slot, ctl->cand_tot[1], nsp, ctl->d, depth, sigma, partial,
+                          next_flag, s_nb[threadIdx.x >> 6]);
+  else
+    bc_discover_body<DT>(rp, col, it_v, it_t0, ctl->next_tot[1], nsp, ctl->d, depth, next_flag,
+                         s_nb[threadIdx.x >> 6]);
+}
+
 // the lanes of the candidate hubs: partial sums in chunk order, then as above
 template <typename DT>
 __global__ __launch_bounds__(256) void k_bc_bottom_up_combine(const int32_t* __restrict__ cand, int nc,
@@ -409,16 +444,18 @@ __global__ __launch_bounds__(256) void k_bc_bottom_up_combine(const int32_t* __r
                                                               DT* __restrict__ depth, double* __restrict__ sigma,
                                                               const double* __restrict__ partial,
                                                               uint8_t* __restrict__ next_flag,
+                                                              const int32_t* __restrict__ hpos,
                                                               const BcCtl* __restrict__ ctl, int guard) {
-  if (ctl) {
+  if (ctl) {  // the candidate hubs only (hpos: their list positions)
     if (!bc_live(ctl, guard)) return;
-    nc = ctl->cand_tot[0];
+    nc = ctl->cand_tot[3];
     d = ctl->d;
   }
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const DT dn = (DT)(d + 1);
-  for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nc; f += gridDim.x * wpb) {
+  for (int j = blockIdx.x * wpb + (threadIdx.x >> 6); j < nc; j += gridDim.x * wpb) {
+    const int f = ctl ? hpos[j] : j;
     const int v = cand[f];
     const int deg = rp[v + 1] - rp[v];
     if (deg <= BC_CH) continue;  // single-item vertex: finished in the item pass
@@ -491,17 +528,19 @@ __global__ __launch_bounds__(256) void k_bc_sigma_combine(const int32_t* __restr
                                                           const int32_t* __restrict__ hstart, int nsp, int d,
                                                           const DT* __restrict__ depth, double* __restrict__ sigma,
                                                           const double* __restrict__ partial,
+                                                          const int32_t* __restrict__ hpos,
                                                           const BcCtl* __restrict__ ctl, int guard) {
-  if (ctl) {  // the list being added (level d + 1)
+  if (ctl) {  // the hubs of the list being added (level d + 1)
     if (!bc_live(ctl, guard)) return;
     front = ctl->next;
-    nf = ctl->next_tot[0];
+    nf = ctl->next_tot[3];
     d = ctl->d + 1;
   }
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const DT dd = (DT)d;
-  for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nf; f += gridDim.x * wpb) {
+  for (int j = blockIdx.x * wpb + (threadIdx.x >> 6); j < nf; j += gridDim.x * wpb) {
+    const int f = ctl ? hpos[j] : j;
     const int v = front[f];
     const int deg = rp[v + 1] - rp[v];
     if (deg <= BC_CH) continue;  // single-item vertex: finished in the item pass
@@ -617,15 +656,17 @@ __global__ __launch_bounds__(256) void k_bc_combine(const int32_t* __restrict__ 
                                                     const DT* __restrict__ depth,
                                                     const double* __restrict__ sigma, double* __restrict__ q,
                                                     double* __restrict__ bc, const double* __restrict__ partial,
-                                                    int64_t bc_stride, const BcCtl* __restrict__ ctl) {
-  if (ctl) {  // the backward level's list (built into `front`)
-    nf = ctl->back_tot[0];
+                                                    int64_t bc_stride, const int32_t* __restrict__ hpos,
+                                                    const BcCtl* __restrict__ ctl) {
+  if (ctl) {  // the hubs of the backward level's list (built into `front`)
+    nf = ctl->back_tot[3];
     k = ctl->bk;
   }
   const int lane = threadIdx.x & 63;
   const int wpb = blockDim.x >> 6;
   const DT dk = (DT)k;
-  for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nf; f += gridDim.x * wpb) {
+  for (int j = blockIdx.x * wpb + (threadIdx.x >> 6); j < nf; j += gridDim.x * wpb) {
+    const int f = ctl ? hpos[j] : j;
     const int v = front[f];
     const int deg = rp[v + 1] - rp[v];
     if (deg <= BC_CH) continue;  // single-item vertex: finished in the item pass
@@ -702,8 +743,9 @@ __global__ void k_bc_account(const int32_t* __restrict__ rp, const DT* __restric
 // own and of its earlier wavefronts, then walks its groups placing each
 // lane's vertex by a ballot count and its items and hub slots by wave scans:
 // the list (hstart[f] = first hub slot of list vertex f), the items (same
-// layout as bc_items) and, from the last wavefront, the three totals.  Flags
-// are cleared.
+// layout as bc_items), the list positions of its hub vertices (more than
+// BC_CH neighbours, the only ones the combine passes touch) and, from the
+// last wavefront, the four totals.  Flags are cleared.
 constexpr int BCB_T = 256;
 constexpr int BCB_MAX_BLOCKS = 1024;
 __device__ inline int64_t ceil_div_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -719,7 +761,7 @@ __device__ inline BcGroupRange bc_wave_groups(int64_t ng, int64_t per_block) {
 }
 // lane's vertex of group g: flagged?, its items, its hub slots
 __device__ inline void bc_group_lane(const uint8_t* flag, const int32_t* rp, int64_t n, int64_t g, bool& f, int& nc,
-                                     int& hub) {
+                                     int& hub) {  // hub: hub slots of the lane's vertex
   const int64_t v = g * BC_FLAG_CHUNK + (threadIdx.x & 63);
   f = v < n && flag[v] != 0;
   nc = 0;
@@ -735,33 +777,35 @@ __global__ __launch_bounds__(BCB_T) void k_bc_build_count(const uint8_t* __restr
                                                           long long* __restrict__ bsum, int64_t per_block,
                                                           const BcCtl* __restrict__ ctl, int guard) {
   if (ctl && !bc_live(ctl, guard)) return;
-  __shared__ long long red[3][BCB_T / 64];
+  __shared__ long long red[4][BCB_T / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t ng = ceil_div_dev(n, BC_FLAG_CHUNK);
   const BcGroupRange r = bc_wave_groups(ng, per_block);
-  long long t[3] = {0, 0, 0};
+  long long t[4] = {0, 0, 0, 0};
   for (int64_t g = r.g0; g < r.g1; ++g) {
     bool f;
     int nc, hub;
     bc_group_lane(flag, rp, n, g, f, nc, hub);
-    const int cv = __popcll(__ballot(f));
+    const int cv = __popcll(__ballot(f)), cn = __popcll(__ballot(hub > 0));
     const int ci = wave_reduce_sum(nc), ch = wave_reduce_sum(hub);
     if (lane == 0) {
       gcnt[g] = cv;
       gcnt[ng + g] = ci;
       gcnt[2 * ng + g] = ch;
+      gcnt[3 * ng + g] = cn;
     }
     t[0] += cv;
     t[1] += ci;
     t[2] += ch;
+    t[3] += cn;
   }
   if (lane == 0)
-    for (int k = 0; k < 3; ++k) red[k][wv] = t[k];
+    for (int k = 0; k < 4; ++k) red[k][wv] = t[k];
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < 4) {
     long long u = 0;
     for (int w = 0; w < BCB_T / 64; ++w) u += red[threadIdx.x][w];
-    bsum[3 * (int64_t)blockIdx.x + threadIdx.x] = u;
+    bsum[4 * (int64_t)blockIdx.x + threadIdx.x] = u;
   }
 }
 
@@ -772,21 +816,22 @@ __global__ __launch_bounds__(BCB_T) void k_bc_build_emit(uint8_t* __restrict__ f
                                                          const long long* __restrict__ bsum, int64_t per_block,
                                                          int32_t* list, int32_t* __restrict__ hstart,
                                                          int32_t* __restrict__ it_v, int32_t* __restrict__ it_t0,
-                                                         int32_t* __restrict__ it_slot, int32_t* __restrict__ tot,
-                                                         const BcCtl* __restrict__ ctl, int guard, int to_next) {
+                                                         int32_t* __restrict__ it_slot, int32_t* __restrict__ hpos,
+                                                         int32_t* __restrict__ tot, const BcCtl* __restrict__ ctl,
+                                                         int guard, int to_next) {
   if (ctl && !bc_live(ctl, guard)) return;
   if (ctl && to_next) list = ctl->next;
-  __shared__ long long prior[3][BCB_T / 64];
-  __shared__ long long wtot[3][BCB_T / 64];
+  __shared__ long long prior[4][BCB_T / 64];
+  __shared__ long long wtot[4][BCB_T / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t ng = ceil_div_dev(n, BC_FLAG_CHUNK);
   const BcGroupRange r = bc_wave_groups(ng, per_block);
-  long long p[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+  long long p[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
   for (int b = threadIdx.x; b < (int)blockIdx.x; b += BCB_T)
-    for (int k = 0; k < 3; ++k) p[k] += bsum[3 * (int64_t)b + k];
+    for (int k = 0; k < 4; ++k) p[k] += bsum[4 * (int64_t)b + k];
   for (int64_t g = r.g0 + lane; g < r.g1; g += 64)
-    for (int k = 0; k < 3; ++k) q[k] += gcnt[k * ng + g];
-  for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) q[k] += gcnt[k * ng + g];
+  for (int k = 0; k < 4; ++k) {
     p[k] = wave_reduce_sum(p[k]);
     q[k] = wave_reduce_sum(q[k]);
     if (lane == 0) {
@@ -795,8 +840,8 @@ __global__ __launch_bounds__(BCB_T) void k_bc_build_emit(uint8_t* __restrict__ f
     }
   }
   __syncthreads();
-  long long run[3];
-  for (int k = 0; k < 3; ++k) {
+  long long run[4];
+  for (int k = 0; k < 4; ++k) {
     run[k] = 0;
     for (int w = 0; w < BCB_T / 64; ++w) {
       run[k] += prior[k][w];
@@ -807,8 +852,10 @@ __global__ __launch_bounds__(BCB_T) void k_bc_build_emit(uint8_t* __restrict__ f
     bool f;
     int nc, hub;
     bc_group_lane(flag, rp, n, g, f, nc, hub);
-    const unsigned long long m = __ballot(f);
+    const unsigned long long m = __ballot(f), mh = __ballot(hub > 0);
     const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const int hbelow =
+        (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mh >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mh, 0u));
     const int ci = wave_incl_scan(nc), ch = wave_incl_scan(hub);
     if (f) {
       const int64_t v = g * BC_FLAG_CHUNK + lane;
@@ -817,6 +864,7 @@ __global__ __launch_bounds__(BCB_T) void k_bc_build_emit(uint8_t* __restrict__ f
       flag[v] = 0;
       list[ov] = (int32_t)v;
       hstart[ov] = oh;
+      if (hub > 0) hpos[(int)run[3] + hbelow] = ov;
       for (int j = 0; j < nc; ++j) {
         it_v[oi + j] = (int32_t)v;
         it_t0[oi + j] = r0 + j * BC_CH;
@@ -826,9 +874,10 @@ __global__ __launch_bounds__(BCB_T) void k_bc_build_emit(uint8_t* __restrict__ f
     run[0] += __popcll(m);
     run[1] += __shfl(ci, 63, 64);
     run[2] += __shfl(ch, 63, 64);
+    run[3] += __popcll(mh);
   }
   if (blockIdx.x == gridDim.x - 1 && wv == BCB_T / 64 - 1 && lane == 0)
-    for (int k = 0; k < 3; ++k) tot[k] = (int32_t)run[k];
+    for (int k = 0; k < 4; ++k) tot[k] = (int32_t)run[k];
 }
 
 // One thread steers the sweep between the launches of a step.
@@ -968,8 +1017,10 @@ static void bc_reserve(BcWork& w, int64_t n, const BcSizes& z, int64_t ns) {
   w.cit_t0.ensure(sizeof(int32_t) * z.item_cap);
   w.cit_slot.ensure(sizeof(int32_t) * z.item_cap);
   w.chstart.ensure(sizeof(int32_t) * (n1 + 1));
-  w.gcnt.ensure(sizeof(int32_t) * 3 * z.ng);
-  w.gsum.ensure(sizeof(long long) * 3 * BCB_MAX_BLOCKS);
+  w.gcnt.ensure(sizeof(int32_t) * 4 * z.ng);
+  w.gsum.ensure(sizeof(long long) * 4 * BCB_MAX_BLOCKS);
+  w.hpos.ensure(sizeof(int32_t) * n1);
+  w.chpos.ensure(sizeof(int32_t) * n1);
   w.ctl.ensure(sizeof(BcCtl));
 }
 
@@ -984,8 +1035,9 @@ static size_t bc_bytes(int64_t n, const BcSizes& z, int64_t ns) {
   b += 2 * lease_bytes(sizeof(int32_t) * n1) + 2 * lease_bytes(sizeof(int32_t) * (n1 + 1));
   b += lease_bytes(scan_tmp_bytes(std::max(n1, z.ng)));
   b += lease_bytes(4 * ceil_div(n1, 4)) + 3 * lease_bytes(sizeof(int32_t) * z.item_cap) +
-       lease_bytes(sizeof(int32_t) * (n1 + 1)) + lease_bytes(sizeof(int32_t) * 3 * z.ng) +
-       lease_bytes(sizeof(long long) * 3 * BCB_MAX_BLOCKS) + lease_bytes(sizeof(BcCtl));
+       lease_bytes(sizeof(int32_t) * (n1 + 1)) + lease_bytes(sizeof(int32_t) * 4 * z.ng) +
+       lease_bytes(sizeof(long long) * 4 * BCB_MAX_BLOCKS) + lease_bytes(sizeof(BcCtl)) +
+       2 * lease_bytes(sizeof(int32_t) * n1);
   return b;
 }
 
@@ -1167,7 +1219,7 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
       hipLaunchKernelGGL(k_bc_bottom_up_combine<DT>, dim3(grid_for(nc, 4, 65536)), dim3(256), 0, s,
                          (const int32_t*)cand, nc, A.rp, (const int32_t*)w.hstart.as<int32_t>(), nsp, d,
                          w.depth.as<DT>(), w.sigma.as<double>(), (const double*)w.partial.as<double>(),
-                         w.flag.as<uint8_t>(), (const BcCtl*)nullptr, 0);
+                         w.flag.as<uint8_t>(), (const int32_t*)nullptr, (const BcCtl*)nullptr, 0);
       GA_HIP(hipGetLastError());
       ev_close();
       ++timed;
@@ -1199,7 +1251,8 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
                        (const BcCtl*)nullptr, 0);
     hipLaunchKernelGGL(k_bc_sigma_combine<DT>, dim3(grid_for(nn, 4, 65536)), dim3(256), 0, s, (const int32_t*)next, nn,
                        A.rp, (const int32_t*)w.hstart.as<int32_t>(), nsp, d + 1, (const DT*)w.depth.as<DT>(),
-                       w.sigma.as<double>(), (const double*)w.partial.as<double>(), (const BcCtl*)nullptr, 0);
+                       w.sigma.as<double>(), (const double*)w.partial.as<double>(), (const int32_t*)nullptr,
+                       (const BcCtl*)nullptr, 0);
     GA_HIP(hipGetLastError());
     ev_close();
     ++timed;
@@ -1238,7 +1291,8 @@ static bool bc_run(const DCsr& A, const int32_t* src_h, const int32_t* lane_h, i
       hipLaunchKernelGGL(k_bc_combine<DT>, dim3(grid_for(nk, 4, 65536)), dim3(256), 0, s, (const int32_t*)front, nk, A.rp,
                          (const int32_t*)w.hstart.as<int32_t>(), nsp, k, (const DT*)w.depth.as<DT>(),
                          (const double*)w.sigma.as<double>(), w.q.as<double>(), w.bc.as<double>(),
-                         (const double*)w.partial.as<double>(), bc_stride, (const BcCtl*)nullptr);
+                         (const double*)w.partial.as<double>(), bc_stride, (const int32_t*)nullptr,
+                         (const BcCtl*)nullptr);
       GA_HIP(hipGetLastError());
     }
   }
@@ -1301,45 +1355,45 @@ static bool bc_run_dev(const DCsr& A, const int32_t* src_h, const int32_t* lane_
   const int64_t ngroups = ceil_div(n1, BC_FLAG_CHUNK);
   const int64_t g_build = std::min<int64_t>(BCB_MAX_BLOCKS, ceil_div(ngroups, BCB_T / 64));
   const int64_t per_block = ceil_div(ngroups, g_build);
-  auto build = [&](uint8_t* flag, int32_t* list, int32_t* hst, int32_t* iv, int32_t* it0, int32_t* isl, int32_t* tot,
-                   const BcCtl* c, int guard, int to_next) {
+  auto build = [&](uint8_t* flag, int32_t* list, int32_t* hst, int32_t* iv, int32_t* it0, int32_t* isl, int32_t* hp,
+                   int32_t* tot, const BcCtl* c, int guard, int to_next) {
     hipLaunchKernelGGL(k_bc_build_count, dim3((unsigned)g_build), dim3(BCB_T), 0, s, (const uint8_t*)flag, n, A.rp,
                        w.gcnt.as<int32_t>(), w.gsum.as<long long>(), per_block, c, guard);
     hipLaunchKernelGGL(k_bc_build_emit, dim3((unsigned)g_build), dim3(BCB_T), 0, s, flag, n, A.rp,
                        (const int32_t*)w.gcnt.as<int32_t>(), (const long long*)w.gsum.as<long long>(), per_block,
-                       list, hst, iv, it0, isl, tot, c, guard, to_next);
+                       list, hst, iv, it0, isl, hp, tot, c, guard, to_next);
   };
   auto op = [&](int code, int lim, long long* host) {
     hipLaunchKernelGGL(k_bc_ctl, dim3(1), dim3(64), 0, s, ctl, code, (int32_t*)nullptr, (int32_t*)nullptr, 0, 0.0,
                        lim, host);
   };
+  // combine passes: one wavefront per hub vertex (more than BC_CH
+  // neighbours, so at most nnz / (BC_CH + 1) of them)
+  const unsigned g_hub = grid_for(std::min<int64_t>(n1, A.nnz / (BC_CH + 1) + 1), 4, 65536);
   auto fwd_step = [&]() {
     // candidates of a bottom-up step and, while top-down, the decision
     hipLaunchKernelGGL(k_unvisited_flags<DT>, dim3(g_vert), dim3(256), 0, s, A.rp, (const DT*)w.depth.as<DT>(), n,
                        nsp, w.cflag.as<uint8_t>(), (const BcCtl*)ctl, (int)BG_CAND);
     build(w.cflag.as<uint8_t>(), cand, w.chstart.as<int32_t>(), w.cit_v.as<int32_t>(), w.cit_t0.as<int32_t>(),
-          w.cit_slot.as<int32_t>(), ctl->cand_tot, ctl, BG_CAND, 0);
+          w.cit_slot.as<int32_t>(), w.chpos.as<int32_t>(), ctl->cand_tot, ctl, BG_CAND, 0);
     hipLaunchKernelGGL(k_list_edges, dim3(grid_for(n1, 256, 1024)), dim3(256), 0, s, (const int32_t*)nullptr, 0,
                        (const int32_t*)cand, (const int32_t*)ctl->cand_tot, A.rp, ctl->edges, (const BcCtl*)ctl,
                        (int)BG_DECIDE);
     op(BC_OP_DECIDE, 0, nullptr);
-    // bottom-up: the candidates pull from level d
-    hipLaunchKernelGGL(k_bc_bottom_up_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
-                       (const int32_t*)w.cit_v.as<int32_t>(), (const int32_t*)w.cit_t0.as<int32_t>(),
-                       (const int32_t*)w.cit_slot.as<int32_t>(), (const int32_t*)(ctl->cand_tot + 1), nsp, 0,
-                       w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(), w.flag.as<uint8_t>(),
-                       (const BcCtl*)ctl, (int)BG_BU);
-    hipLaunchKernelGGL(k_bc_bottom_up_combine<DT>, dim3(g_items), dim3(256), 0, s, (const int32_t*)cand, 0, A.rp,
-                       (const int32_t*)w.chstart.as<int32_t>(), nsp, 0, w.depth.as<DT>(), w.sigma.as<double>(),
-                       (const double*)w.partial.as<double>(), w.flag.as<uint8_t>(), (const BcCtl*)ctl, (int)BG_BU);
-    // top-down: discovery from level d (its items were built with its list)
-    hipLaunchKernelGGL(k_bc_discover_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
+    // bottom-up: the candidates pull from level d; top-down: discovery from
+    // the level-d list (its items were built with it)
+    hipLaunchKernelGGL(k_bc_forward_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
                        (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
-                       (const int32_t*)(ctl->next_tot + 1), nsp, 0, w.depth.as<DT>(), w.flag.as<uint8_t>(),
-                       (const BcCtl*)ctl, (int)BG_TD);
+                       (const int32_t*)w.cit_v.as<int32_t>(), (const int32_t*)w.cit_t0.as<int32_t>(),
+                       (const int32_t*)w.cit_slot.as<int32_t>(), nsp, w.depth.as<DT>(), w.sigma.as<double>(),
+                       w.partial.as<double>(), w.flag.as<uint8_t>(), (const BcCtl*)ctl);
+    hipLaunchKernelGGL(k_bc_bottom_up_combine<DT>, dim3(g_hub), dim3(256), 0, s, (const int32_t*)cand, 0, A.rp,
+                       (const int32_t*)w.chstart.as<int32_t>(), nsp, 0, w.depth.as<DT>(), w.sigma.as<double>(),
+                       (const double*)w.partial.as<double>(), w.flag.as<uint8_t>(),
+                       (const int32_t*)w.chpos.as<int32_t>(), (const BcCtl*)ctl, (int)BG_BU);
     // the level-(d+1) list and its items
     build(w.flag.as<uint8_t>(), nullptr, w.hstart.as<int32_t>(), w.it_v.as<int32_t>(), w.it_t0.as<int32_t>(),
-          w.it_slot.as<int32_t>(), ctl->next_tot, ctl, BG_ANY, 1);
+          w.it_slot.as<int32_t>(), w.hpos.as<int32_t>(), ctl->next_tot, ctl, BG_ANY, 1);
     op(BC_OP_NEXT, 0, nullptr);
     // top-down: path counts of level d + 1
     hipLaunchKernelGGL(k_bc_sigma_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
@@ -1347,25 +1401,27 @@ static bool bc_run_dev(const DCsr& A, const int32_t* src_h, const int32_t* lane_
                        (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(ctl->next_tot + 1), nsp, 0,
                        (const DT*)w.depth.as<DT>(), w.sigma.as<double>(), w.partial.as<double>(), (const BcCtl*)ctl,
                        (int)BG_TD);
-    hipLaunchKernelGGL(k_bc_sigma_combine<DT>, dim3(g_items), dim3(256), 0, s, (const int32_t*)nullptr, 0, A.rp,
+    hipLaunchKernelGGL(k_bc_sigma_combine<DT>, dim3(g_hub), dim3(256), 0, s, (const int32_t*)nullptr, 0, A.rp,
                        (const int32_t*)w.hstart.as<int32_t>(), nsp, 0, (const DT*)w.depth.as<DT>(),
-                       w.sigma.as<double>(), (const double*)w.partial.as<double>(), (const BcCtl*)ctl, (int)BG_TD);
+                       w.sigma.as<double>(), (const double*)w.partial.as<double>(),
+                       (const int32_t*)w.hpos.as<int32_t>(), (const BcCtl*)ctl, (int)BG_TD);
     op(BC_OP_END, limit, hw_dev);
   };
   auto bwd_step = [&]() {
     hipLaunchKernelGGL(k_level_flags<DT>, dim3(g_vert), dim3(256), 0, s, (const DT*)w.depth.as<DT>(), n, nsp, 0,
                        w.flag.as<uint8_t>(), (const BcCtl*)ctl);
     build(w.flag.as<uint8_t>(), lists[0], w.hstart.as<int32_t>(), w.it_v.as<int32_t>(), w.it_t0.as<int32_t>(),
-          w.it_slot.as<int32_t>(), ctl->back_tot, nullptr, BG_ANY, 0);
+          w.it_slot.as<int32_t>(), w.hpos.as<int32_t>(), ctl->back_tot, nullptr, BG_ANY, 0);
     hipLaunchKernelGGL(k_bc_backward_items<DT>, dim3(g_items), dim3(256), 0, s, A.rp, A.col,
                        (const int32_t*)w.it_v.as<int32_t>(), (const int32_t*)w.it_t0.as<int32_t>(),
                        (const int32_t*)w.it_slot.as<int32_t>(), (const int32_t*)(ctl->back_tot + 1), nsp, 0,
                        (const DT*)w.depth.as<DT>(), (const double*)w.sigma.as<double>(), w.q.as<double>(),
                        w.bc.as<double>(), w.partial.as<double>(), bc_stride, (const BcCtl*)ctl);
-    hipLaunchKernelGGL(k_bc_combine<DT>, dim3(g_items), dim3(256), 0, s, (const int32_t*)lists[0], 0, A.rp,
+    hipLaunchKernelGGL(k_bc_combine<DT>, dim3(g_hub), dim3(256), 0, s, (const int32_t*)lists[0], 0, A.rp,
                        (const int32_t*)w.hstart.as<int32_t>(), nsp, 0, (const DT*)w.depth.as<DT>(),
                        (const double*)w.sigma.as<double>(), w.q.as<double>(), w.bc.as<double>(),
-                       (const double*)w.partial.as<double>(), bc_stride, (const BcCtl*)ctl);
+                       (const double*)w.partial.as<double>(), bc_stride, (const int32_t*)w.hpos.as<int32_t>(),
+                       (const BcCtl*)ctl);
     op(BC_OP_BACK, 0, nullptr);
   };
   // captured steps, kept while every launch argument they hold is the same
@@ -1376,7 +1432,8 @@ static bool bc_run_dev(const DCsr& A, const int32_t* src_h, const int32_t* lane_
         (uintptr_t)w.flag.p, (uintptr_t)w.cflag.p, (uintptr_t)w.levels.p, (uintptr_t)w.it_v.p, (uintptr_t)w.it_t0.p,
         (uintptr_t)w.it_slot.p, (uintptr_t)w.hstart.p, (uintptr_t)w.cit_v.p, (uintptr_t)w.cit_t0.p,
         (uintptr_t)w.cit_slot.p, (uintptr_t)w.chstart.p, (uintptr_t)w.partial.p, (uintptr_t)w.gcnt.p,
-        (uintptr_t)w.gsum.p, (uintptr_t)ctl, (uintptr_t)hw_dev, (uintptr_t)cx.num_cu};
+        (uintptr_t)w.gsum.p, (uintptr_t)w.hpos.p, (uintptr_t)w.chpos.p, (uintptr_t)ctl, (uintptr_t)hw_dev,
+        (uintptr_t)cx.num_cu};
     if (w.graphs.key != key || !w.graphs.fwd) {
       w.graphs.reset();
       auto capture = [&](auto&& step) {
@@ -1412,7 +1469,7 @@ static bool bc_run_dev(const DCsr& A, const int32_t* src_h, const int32_t* lane_
   hipLaunchKernelGGL(k_bc_ctl, dim3(1), dim3(64), 0, s, ctl, (int)BC_OP_INIT, lists[0], lists[1],
                      cx.bc_bottom_up ? 1 : 0, cx.bc_alpha, limit, (long long*)nullptr);
   build(w.flag.as<uint8_t>(), nullptr, w.hstart.as<int32_t>(), w.it_v.as<int32_t>(), w.it_t0.as<int32_t>(),
-        w.it_slot.as<int32_t>(), ctl->next_tot, ctl, BG_ANY, 1);
+        w.it_slot.as<int32_t>(), w.hpos.as<int32_t>(), ctl->next_tot, ctl, BG_ANY, 1);
   op(BC_OP_START, 0, nullptr);
   GA_HIP(hipGetLastError());
   // forward sweep: steps queued two ahead of the published sequence number

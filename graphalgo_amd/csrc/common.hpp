@@ -465,7 +465,7 @@ struct BcWork {
   DevBuf dsum;                                                        // edge counts of two lists
   // device-driven passes: candidate flags, list and items, list builder
   // counts, the control block
-  DevBuf cflag, cit_v, cit_t0, cit_slot, chstart, gcnt, gsum, ctl;
+  DevBuf cflag, cit_v, cit_t0, cit_slot, chstart, gcnt, gsum, ctl, hpos, chpos;
   BcGraphs graphs;
 };
 void bc_batch_dev(const DCsr& A, const int32_t* src_h, int64_t ns, double* bc_h, BcWork& w, hipStream_t s);
