@@ -488,10 +488,10 @@ def bench_bc(args) -> dict:
     dt, bc, timing, acct = _measure(lambda: R.bc(src), args.steps, args.warmup)
     R.close()
     m = g.m_undirected
-    roof = _roofline(timing, acct, "k_bc_bottom_up_items / k_bc_discover_items + k_bc_sigma_items + "
-                     "k_bc_backward_items (per level)", "k_bc/bc_rmat18")
-    roof["launches_note"] = ("one event pair per device segment between host waits: candidate lists, "
-                             "forward steps, the backward sweep")
+    roof = _roofline(timing, acct, "k_bc_forward_items (bottom-up or top-down per level) + k_bc_sigma_items + "
+                     "k_bc_backward_items", "k_bc/bc_rmat18")
+    roof["launches_note"] = ("one event pair per timed segment of a pass: the forward sweep (levels queued "
+                             "on the device, one host wait at its end) and the backward sweep")
     out = {"value": m * len(src) / dt, "unit": "edges*sources/s", "ms_per_step": dt * 1e3, "dtype": "f64",
            "config": {"workload": "batched BC, 256 sources, R-MAT scale 18", "n": g.n, "m_undirected": m,
                       "sources": len(src), "source_seed": 7, "bc_sum": float(bc.sum())},
