@@ -426,7 +426,8 @@ def bench_batch(args) -> dict:
     gen_s = time.perf_counter() - t0
     m_total = sum(g.m_undirected for g in graphs)
     B = api.Batch(graphs)
-    (best_w, best_c), tune_ms = B.autotune()
+    _, tune_ms = B.autotune()
+    shape = B.shape()
     dt, counts, timing, acct = _measure(B.triangle_counts, args.steps, args.warmup)
     # plain path: every graph resident, one triangle-count call per graph
     handles = [api.Graph(g) for g in graphs]
@@ -443,8 +444,9 @@ def bench_batch(args) -> dict:
     out = {"value": m_total / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
            "config": {"workload": f"batched triangle count over {count} kNN graphs (k=30) in one launch",
                       "graphs": count, "m_total": m_total, "triangles_total": int(counts.sum()),
-                      "autotuned": {"item_width": best_w, "rows_per_claim": best_c},
-                      "autotune_ms": {f"{w}x{c}": round(v, 3) for (w, c), v in tune_ms.items()},
+                      "autotuned": shape,
+                      "autotune_ms": {(f"wave_flops={c}" if w == "wave_flops" else f"{w}x{c}"): round(v, 3)
+                                      for (w, c), v in tune_ms.items()},
                       "generation_s": round(gen_s, 1)},
            "roofline": roof,
            "plain_path": {"value": m_total / dt_plain, "unit": "edges/s", "ms": dt_plain * 1e3,

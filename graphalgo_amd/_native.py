@@ -54,6 +54,7 @@ class GaServerConfig(C.Structure):
 
 GA_Q_TRIANGLES, GA_Q_KTRUSS, GA_Q_BC = 0, 1, 2
 GA_EW_ADD, GA_EW_MULT, GA_EW_ADD_DISJOINT = 0, 1, 2
+GA_BATCH_TUNE_CANDIDATES = 12  # include/graphalgo.h
 
 
 class GaQuery(C.Structure):
@@ -131,6 +132,7 @@ GA_SIGNATURES = {
     "ga_batch_destroy": (C.c_int, [C.c_void_p]),
     "ga_batch_triangle_count": (C.c_int, [C.c_void_p, C.c_int, I64P]),
     "ga_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), F64P]),
+    "ga_batch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
     "ga_set_option": (C.c_int, [C.c_int, C.c_double]),
     "ga_get_option": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "ga_prof_enable": (C.c_int, [C.c_int]),

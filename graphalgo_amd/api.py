@@ -305,16 +305,27 @@ class Batch:
 
     TUNE_WIDTHS = (128, 256)
     TUNE_CLAIMS = (8, 16, 32, 64)
+    TUNE_WAVE_FLOPS = (2048, 16384, 65536, 1 << 20)
 
     def autotune(self) -> tuple[tuple[int, int], dict]:
-        """Times every (item width, rows per claim) pair on the resident batch
-        and keeps the fastest; returns it and the time of each pair in ms."""
+        """Times every (item width, rows per claim) pair on the resident batch,
+        then the short-row product threshold at the best pair, and keeps the
+        fastest; returns the (width, claim) kept and the time of every
+        candidate in ms -- keys (width, claim) and ("wave_flops", threshold).
+        `shape()` gives the threshold kept."""
         best_t, best_c = C.c_int(), C.c_int()
-        ms = np.zeros(len(self.TUNE_WIDTHS) * len(self.TUNE_CLAIMS), dtype=np.float64)
+        ms = np.zeros(N.GA_BATCH_TUNE_CANDIDATES, dtype=np.float64)
         N.check(N.galib().ga_batch_autotune(self._h, C.byref(best_t), C.byref(best_c), ms.ctypes.data_as(N.F64P)))
         self.threads = best_t.value
-        pairs = [(t, c) for t in self.TUNE_WIDTHS for c in self.TUNE_CLAIMS]
-        return (best_t.value, best_c.value), dict(zip(pairs, ms.tolist()))
+        keys = [(t, c) for t in self.TUNE_WIDTHS for c in self.TUNE_CLAIMS]
+        keys += [("wave_flops", f) for f in self.TUNE_WAVE_FLOPS]
+        return (best_t.value, best_c.value), dict(zip(keys, ms.tolist()))
+
+    def shape(self) -> dict:
+        """The launch shape the batch uses: item width, rows per claim, short-row threshold."""
+        t, c, f = C.c_int(), C.c_int(), C.c_int64()
+        N.check(N.galib().ga_batch_shape(self._h, C.byref(t), C.byref(c), C.byref(f)))
+        return {"item_width": t.value, "rows_per_claim": c.value, "wave_flops": f.value}
 
 
 _KINDS = {"tc": N.GA_Q_TRIANGLES, "ktruss": N.GA_Q_KTRUSS, "bc": N.GA_Q_BC}
@@ -460,7 +471,8 @@ def device_info() -> dict:
 # process-wide tuning options (include/graphalgo.h, GA_OPT_*): every value
 # gives the same results, only the speed differs
 OPTIONS = {"bc_bottom_up": 1, "bc_alpha": 2, "row_claim": 3, "sum_sub": 4, "item_wedges": 5,
-           "selective_support": 6, "truss_loop": 7, "support_tri": 8, "serve_join": 9, "bc_loop": 10}
+           "selective_support": 6, "truss_loop": 7, "support_tri": 8, "serve_join": 9, "bc_loop": 10,
+           "wave_flops": 11}
 
 
 def set_option(name: str, value: float) -> None:

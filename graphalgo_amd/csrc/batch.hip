@@ -9,9 +9,13 @@
 // graphs and one launch replaces ngraphs launches of the per-graph path
 // (triangle_counting_masked_sandia, src/main.cpp:65-86, once per graph).
 // The launch shape -- workgroup width of the long-row items (A entries per
-// work item) and rows per claim of the short-row kernel (the rows in flight
-// on an XCD, whose graphs share its L2) -- is chosen by timing the
-// candidates on the resident batch (ga_batch_autotune).
+// work item), rows per claim of the short-row kernel (the rows in flight on
+// an XCD, whose graphs share its L2) and the product count up to which a row
+// goes to the short-row kernel (LDS bitmap per row, rows packed per
+// wavefront) rather than to the item kernel (LDS mask accumulator of the
+// workgroup): the per-row choice of accumulator that replaces the
+// reference's global msa / mca / heap switch (src/main.cpp:206-218) -- is
+// chosen by timing the candidates on the resident batch (ga_batch_autotune).
 #include "common.hpp"
 
 #include <algorithm>
@@ -27,6 +31,7 @@ struct ga_batch {
   MxmWork mw;
   int threads = 0;  // 0 = not tuned (256)
   int claim = 0;    // rows per claim of the short-row kernel (0 = automatic)
+  long long wave_flops = -2;  // short-row threshold (products per row); -2 = not tuned (library option)
 };
 
 extern "C" {
@@ -93,11 +98,12 @@ ga_status ga_batch_destroy(ga_batch* b) {
   });
 }
 
-static void batch_run(ga_batch* b, int threads, int claim, hipStream_t s) {
+static void batch_run(ga_batch* b, int threads, int claim, long long wave_flops, hipStream_t s) {
   fill_u64(b->sums.p, 0ull, b->ngraphs, s);
   const DCsr L = b->L.view();
   b->mw.threads = threads;
   b->mw.row_chunk = claim;
+  b->mw.wave_flops = wave_flops >= 0 ? wave_flops : -1;  // -1: the library option
   mxm_batched_sum_dev(L, b->row_group.as<int32_t>(), b->gbase.as<int32_t>(), b->max_n,
                       b->sums.as<unsigned long long>(), b->mw, s);
 }
@@ -108,7 +114,7 @@ ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts) {
     if (threads == 0) threads = b->threads ? b->threads : 256;
     GA_REQUIRE(threads == 64 || threads == 128 || threads == 256, GA_ERR_ARG, "threads must be 64, 128 or 256");
     hipStream_t s = ctx().stream;
-    batch_run(b, threads, b->claim, s);
+    batch_run(b, threads, b->claim, b->wave_flops, s);
     if (counts && b->ngraphs)
       GA_HIP(hipMemcpyAsync(counts, b->sums.p, sizeof(int64_t) * b->ngraphs, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
@@ -121,19 +127,25 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, dou
     hipStream_t s = ctx().stream;
     const int widths[2] = {128, 256};
     const int claims[4] = {8, 16, 32, 64};
-    static_assert(GA_BATCH_TUNE_CANDIDATES == 8, "candidate grid");
+    const long long flops[4] = {2048, 16384, 65536, 1 << 20};
+    static_assert(GA_BATCH_TUNE_CANDIDATES == 12, "candidate grid");
     hipEvent_t e0, e1;
     GA_HIP(hipEventCreate(&e0));
     GA_HIP(hipEventCreate(&e1));
     double best = 1e300;
     int arg_t = 256, arg_c = 0;
+    long long arg_f = -1;
     try {
+      // stage 1: width x claim at the library's threshold; stage 2: the
+      // threshold at the best of stage 1
       for (int ci = 0; ci < GA_BATCH_TUNE_CANDIDATES; ++ci) {
-        const int th = widths[ci / 4], cl = claims[ci % 4];
-        batch_run(b, th, cl, s);  // warm-up
+        const bool st2 = ci >= 8;
+        const int th = st2 ? arg_t : widths[ci / 4], cl = st2 ? arg_c : claims[ci % 4];
+        const long long wf = st2 ? flops[ci - 8] : -1;
+        batch_run(b, th, cl, wf, s);  // warm-up
         GA_HIP(hipEventRecord(e0, s));
         const int reps = 3;
-        for (int r = 0; r < reps; ++r) batch_run(b, th, cl, s);
+        for (int r = 0; r < reps; ++r) batch_run(b, th, cl, wf, s);
         GA_HIP(hipEventRecord(e1, s));
         GA_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -144,6 +156,7 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, dou
           best = per;
           arg_t = th;
           arg_c = cl;
+          arg_f = wf;
         }
       }
     } catch (...) {
@@ -155,8 +168,18 @@ ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, dou
     GA_HIP(hipEventDestroy(e1));
     b->threads = arg_t;
     b->claim = arg_c;
+    b->wave_flops = arg_f;
     if (best_threads) *best_threads = arg_t;
     if (best_claim) *best_claim = arg_c;
+  });
+}
+
+ga_status ga_batch_shape(const ga_batch* b, int* threads, int* claim, int64_t* wave_flops) {
+  return guarded([&] {
+    GA_REQUIRE(b != nullptr, GA_ERR_ARG, "null batch");
+    if (threads) *threads = b->threads;
+    if (claim) *claim = b->claim;
+    if (wave_flops) *wave_flops = b->wave_flops >= 0 ? b->wave_flops : ctx().wave_flops;
   });
 }
 

@@ -420,6 +420,10 @@ ga_status ga_set_option(int option, double value) {
         GA_REQUIRE(value == 0.0 || value == 1.0 || value == 2.0, GA_ERR_ARG, "bc_loop is 0, 1 or 2");
         c.bc_loop = (int)value;
         break;
+      case GA_OPT_WAVE_FLOPS:
+        GA_REQUIRE(integral && value >= -1.0 && value <= 1e12, GA_ERR_ARG, "wave_flops is -1 .. 1e12");
+        c.wave_flops = (long long)value;
+        break;
       case GA_OPT_SUPPORT_TRI:
         GA_REQUIRE(value == -1.0 || value == 0.0 || value == 1.0, GA_ERR_ARG, "support_tri is -1, 0 or 1");
         c.support_tri = (int)value;
@@ -450,6 +454,7 @@ ga_status ga_get_option(int option, double* value) {
       case GA_OPT_SUPPORT_TRI: *value = c.support_tri; break;
       case GA_OPT_SERVE_JOIN: *value = c.serve_join; break;
       case GA_OPT_BC_LOOP: *value = c.bc_loop; break;
+      case GA_OPT_WAVE_FLOPS: *value = (double)c.wave_flops; break;
       default: GA_REQUIRE(false, GA_ERR_ARG, "unknown option " + std::to_string(option));
     }
   });

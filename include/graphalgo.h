@@ -214,13 +214,21 @@ ga_status ga_batch_destroy(ga_batch* b);
  * (256 before tuning); the short-row kernel takes the autotuned rows per
  * claim (automatic before tuning). */
 ga_status ga_batch_triangle_count(ga_batch* b, int threads, int64_t* counts);
-/* Times the launch shapes on the resident batch -- item width 128 or 256 x
- * rows per claim of the short-row kernel 8, 16, 32 or 64 (the rows in flight
- * on an XCD share its L2) -- and keeps the fastest for later calls.
- * ms_per_candidate (may be NULL) receives GA_BATCH_TUNE_CANDIDATES times,
- * width-major: (128, 8), (128, 16), ..., (256, 64). */
-#define GA_BATCH_TUNE_CANDIDATES 8
+/* Times the launch shapes on the resident batch and keeps the fastest for
+ * later calls: first item width 128 or 256 x rows per claim of the short-row
+ * kernel 8, 16, 32 or 64 (the rows in flight on an XCD share its L2), then,
+ * at the best of those, the product count up to which a row takes the
+ * short-row kernel (LDS bitmap, rows packed per wavefront) instead of the
+ * item kernel (LDS mask accumulator): 2048, 16384, 65536 or 2^20 -- the
+ * per-row accumulator choice that replaces the reference's global
+ * msa / mca / heap switch (src/main.cpp:206-218).  ms_per_candidate (may be
+ * NULL) receives GA_BATCH_TUNE_CANDIDATES times: (128, 8), (128, 16), ...,
+ * (256, 64) width-major, then the four thresholds. */
+#define GA_BATCH_TUNE_CANDIDATES 12
 ga_status ga_batch_autotune(ga_batch* b, int* best_threads, int* best_claim, double* ms_per_candidate);
+/* The launch shape the batch uses: item width (0 = not tuned, 256), rows per
+ * claim (0 = automatic) and the short-row product threshold. */
+ga_status ga_batch_shape(const ga_batch* b, int* threads, int* claim, int64_t* wave_flops);
 
 /* ---- serving ----------------------------------------------------------- */
 /* A request loop over graphs resident in HBM.  The reference has no serving
@@ -362,11 +370,16 @@ enum {
                                    that is already going (GA_SERVE_JOIN): 0 = never, 1 = when
                                    the query's k lies within the run's k range, 2 = also above
                                    it (the run goes on to it) */
-  GA_OPT_BC_LOOP = 10           /* BC level loop (GA_BC_LOOP): 0 = host-steered, the host reading
+  GA_OPT_BC_LOOP = 10,          /* BC level loop (GA_BC_LOOP): 0 = host-steered, the host reading
                                    the size of every level list; 1 (default) = level lists sized
                                    on the device, the host waiting once per pass (end of the
                                    forward sweep); 2 = the same steps replayed from captured HIP
                                    graphs */
+  GA_OPT_WAVE_FLOPS = 11        /* masked products on matrices of at most 4096 columns: rows with a
+                                   mask of at most 128 entries and at most this many products go
+                                   to the short-row kernels, the rest to the item kernel
+                                   (GA_WAVE_FLOPS; default 65536; -1 = every row to the item
+                                   kernel); a batch's autotuner sets its own */
 };
 ga_status ga_set_option(int option, double value);
 ga_status ga_get_option(int option, double* value);

@@ -92,7 +92,10 @@ def test_batch_tc_knn(gpu, G, oracle):
         print(f"case batch tc threads={threads}", flush=True)
         assert np.array_equal(B.triangle_counts(threads), want)
     (width, claim), ms = B.autotune()
-    assert width in (128, 256) and claim in (8, 16, 32, 64) and len(ms) == 8
+    assert width in (128, 256) and claim in (8, 16, 32, 64) and len(ms) == 12
+    shape = B.shape()
+    assert (shape["item_width"], shape["rows_per_claim"]) == (width, claim)
+    assert shape["wave_flops"] in (2048, 16384, 65536, 1 << 20, gpu.get_option("wave_flops"))
     assert np.array_equal(B.triangle_counts(), want)
     B.close()
 

@@ -4,7 +4,8 @@
   random non-isolated sources the benchmark uses, FP64, 1e-9 relative to the
   oracle (include/betweenness_centrality.h:125-237);
 * triangle counts of all 4096 kNN graphs (k = 30) of the benchmark's batch in
-  one persistent launch, before and after the autotuner, bit-exact per graph
+  one persistent launch, before and after the autotuner and at every short-row
+  threshold it tries, bit-exact per graph
   against the oracle (src/main.cpp:65-86);
 * the served stream over the resident pool of eleven R-MAT graphs of scale
   10..20 (60% TC / 25% k-truss / 15% 64-source BC, Poisson arrivals): every
@@ -51,12 +52,18 @@ def test_batch_tc_4096_knn_graphs(gpu, G, oracle):
     before = B.triangle_counts()
     assert np.array_equal(before, want)
     (width, claim), ms = B.autotune()
-    print(f"  autotuned width={width} claim={claim}", flush=True)
+    print(f"  autotuned {B.shape()}", flush=True)
     after = B.triangle_counts()
     assert np.array_equal(after, want)
     for threads in (64, 128, 256):
         assert np.array_equal(B.triangle_counts(threads), want), threads
     B.close()
+    # every short-row threshold the autotuner tries, on untuned batches
+    for wf in gpu.Batch.TUNE_WAVE_FLOPS:
+        with gpu.options(wave_flops=wf, row_claim=claim):
+            B = gpu.Batch(graphs)
+            assert np.array_equal(B.triangle_counts(width), want), wf
+            B.close()
 
 
 @pytest.fixture(scope="module")

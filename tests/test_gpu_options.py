@@ -123,6 +123,24 @@ def test_tc_short_row_kernels_and_claims(gpu, tc_cases, sub, claim):
     assert got_er == want_er
 
 
+@pytest.mark.parametrize("wave_flops", [-1, 0, 2048, 16384, 1 << 20])
+def test_tc_short_row_threshold(gpu, tc_cases, wave_flops):
+    """Every candidate of the batch autotuner's short-row product threshold
+    (and the extremes: no short-row kernel, every row that fits it) gives the
+    exact counts, batched and per graph."""
+    graphs, want, er, want_er = tc_cases
+    print(f"case tc wave_flops={wave_flops}", flush=True)
+    with gpu.options(wave_flops=wave_flops):
+        B = gpu.Batch(graphs)
+        got = B.triangle_counts()
+        B.close()
+        R = gpu.Graph(er)
+        got_er = R.triangle_count()
+        R.close()
+    assert np.array_equal(got, want)
+    assert got_er == want_er
+
+
 @pytest.mark.parametrize("item_wedges", [64, 4096])
 def test_truss_small_items(gpu, G, oracle, item_wedges):
     g = G.rmat(12, 16, seed=8)
@@ -249,7 +267,8 @@ def test_truss_support_tri_rmat16(gpu, G, golden):
 def test_option_validation(gpu):
     for name, bad in (("bc_bottom_up", 2), ("bc_alpha", 0), ("row_claim", -1), ("sum_sub", 12),
                       ("item_wedges", 0), ("selective_support", 0.5), ("truss_loop", 5), ("truss_loop", -2),
-                      ("support_tri", 2), ("serve_join", 3), ("bc_loop", 3), ("bc_loop", -1)):
+                      ("support_tri", 2), ("serve_join", 3), ("bc_loop", 3), ("bc_loop", -1),
+                      ("wave_flops", -2), ("wave_flops", 1.5)):
         with pytest.raises(Exception):
             gpu.set_option(name, bad)
     before = gpu.get_option("row_claim")
