@@ -37,6 +37,7 @@ from __future__ import annotations
 
 import os
 import argparse
+import gc
 import json
 import subprocess
 import sys
@@ -150,19 +151,26 @@ def _measure(step, steps: int, warmup: int):
     for _ in range(warmup):
         step()
     _sync()
-    t0 = time.perf_counter()
-    out = None
-    for _ in range(steps):
-        out = step()
-    _sync()
-    dt = (time.perf_counter() - t0) / steps
-    api.prof_enable(api.PROF_EVENTS)
-    api.prof_read(reset=True)
-    t1 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    _sync()
-    dt_events = (time.perf_counter() - t1) / steps
+    # no collector pass inside the timed loops: a full collection over the
+    # objects earlier workloads left (thousands of graphs) took ~20 ms
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        out = None
+        for _ in range(steps):
+            out = step()
+        _sync()
+        dt = (time.perf_counter() - t0) / steps
+        api.prof_enable(api.PROF_EVENTS)
+        api.prof_read(reset=True)
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        _sync()
+        dt_events = (time.perf_counter() - t1) / steps
+    finally:
+        gc.enable()
     timing = api.prof_read(reset=True)
     api.prof_enable(api.PROF_BYTES)
     step()
