@@ -1682,10 +1682,13 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
       const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ, 1),
                                                                  ceil_div(m, WR_WAVES)));
       // rows per claim: at most 64 (one lane each); by default about four
-      // claims per resident wave, at most 16 rows
+      // claims per resident wave, 2 to 16 rows (a wavefront that packs two
+      // rows of a small graph finishes sooner than two waves of one row each
+      // that then drain the claim segments: Erdos-Renyi n = 1000 triangle
+      // count 18.8 -> 14.6 us per launch)
       const int chunk = std::min(64, chunk_fix > 0 ? chunk_fix
                                                    : (int)std::max<int64_t>(
-                                                         1, std::min<int64_t>(16, m / (grid * WR_WAVES * 4))));
+                                                         2, std::min<int64_t>(16, m / (grid * WR_WAVES * 4))));
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
                          M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
                          (const int32_t*)(w.rstart.as<int32_t>() + m),
