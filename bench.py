@@ -314,7 +314,7 @@ def bench_tc_er(args) -> dict:
            "config": {"workload": "triangle count C<L>=L*L on Erdos-Renyi n=1000 m=9990 (seed 1)", "n": g.n,
                       "m_undirected": g.m_undirected, "triangles": t, "latency_us": round(dt * 1e6, 2),
                       "note": "latency-bound: one call moves ~0.43 MB of algorithmic bytes (54 ns at 8 TB/s)"},
-           "roofline": _roofline(timing, acct, "k_sum_wave (C<L> = L*L, sum mode)")}
+           "roofline": _roofline(timing, acct, "k_sum_wave (C<L> = L*L, sum mode)", "k_sum_wave/tc_er_n1000")}
     if not args.no_cpu:
         O = _oracle()
         want = O.tc(g)
@@ -355,7 +355,7 @@ def bench_mxm(args) -> dict:
            "config": {"workload": "C<A>=A*A on R-MAT scale 16, int32 values", "n": g.n, "nnz": g.nnz,
                       "sum_C": int(C.astype(np.int64).sum()), "host_inclusive_ms_per_call": dt * 1e3,
                       "host_inclusive_edges_per_s": g.m_undirected / dt},
-           "roofline": _roofline(timing, acct, "k_mxm_items (C<A> = A*A, int32 values per entry)")}
+           "roofline": _roofline(timing, acct, "k_mxm_items (C<A> = A*A, int32 values per entry)", "k_mxm_items/mxm_rmat16")}
     if not args.no_cpu:
         O = _oracle()
         want = O.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
