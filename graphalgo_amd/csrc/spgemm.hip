@@ -1509,6 +1509,31 @@ static hipEvent_t take_event(Context& c) {
   return ev;
 }
 
+// Resident workgroups per CU of a kernel at a launch shape.  The answer
+// depends only on the kernel's resources and the device, so it is asked of
+// the runtime once per (kernel, block, dynamic LDS, device) and kept per host
+// thread; small products (a triangle count of a 1000-vertex graph is ~15 us
+// of GPU time) are launched often enough for the query to show.
+static int occupancy_of(const void* kern, int block, size_t dyn, int dev) {
+  struct Entry {
+    const void* kern;
+    int block;
+    size_t dyn;
+    int device, occ;
+  };
+  thread_local std::vector<Entry> cache;
+  for (const Entry& e : cache)
+    if (e.kern == kern && e.block == block && e.dyn == dyn && e.device == dev) return e.occ;
+  int occ = 0;
+  GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, dyn));
+  cache.push_back(Entry{kern, block, dyn, dev, occ});
+  return occ;
+}
+template <typename K>
+static int occupancy_of(K* kern, int block, size_t dyn, int dev) {
+  return occupancy_of(reinterpret_cast<const void*>(kern), block, dyn, dev);
+}
+
 // Bins the rows, then runs the wavefront kernel on the short rows and the
 // workgroup kernel on the items of the long rows.  With profiling on, the
 // pair is bracketed by one event pair (one "launch" of the masked product).
@@ -1652,10 +1677,9 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
                      : k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, false>;
   const int bm_words = bitmap_words + (rank ? 1 : 0);  // rank: one zero word for dropped wedges
   const size_t dyn = sizeof(uint32_t) * (size_t)bm_words + (rank ? sizeof(uint16_t) * ((bm_words + 1) & ~1) : 0);
-  int occ_r = 0, occ_i = 0;
-  GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, krows, 64 * WR_WAVES, 0));
-  GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_i, kitems, THREADS, dyn));
   Context& c = ctx();
+  const int occ_r = occupancy_of(krows, 64 * WR_WAVES, 0, c.device);
+  const int occ_i = occupancy_of(kitems, THREADS, dyn, c.device);
   const int64_t grid_r = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ_r, 1),
                                                                ceil_div(m, WR_WAVES)));
   const int64_t grid_i = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ_i, 1), max_items));
@@ -1677,8 +1701,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   if (run_rows && MODE == 1 && PA && PB && (sum_sub == 0 || sum_sub == 2)) {
     // pattern sum mode: rows packed per wavefront, 64 x K wedges per trip
     auto launch_wave = [&](auto kern) {  // (trunc is compiled into the kernel)
-      int occ = 0;
-      GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WR_WAVES, 0));
+      const int occ = occupancy_of(kern, 64 * WR_WAVES, 0, c.device);
       const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ, 1),
                                                                  ceil_div(m, WR_WAVES)));
       // rows per claim: at most 64 (one lane each); by default about four
@@ -1704,8 +1727,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   } else if (run_rows && MODE == 1 && PA && PB && sum_sub >= 8) {
     // pattern sum mode, entry by entry: sum_sub lanes per short row
     auto launch_entries = [&](auto kern, int groups) {
-      int occ = 0;
-      GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 64 * WR_WAVES, 0));
+      const int occ = occupancy_of(kern, 64 * WR_WAVES, 0, c.device);
       const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ, 1),
                                                                  ceil_div(m, groups)));
       const int chunk = chunk_fix > 0 ? chunk_fix
@@ -1724,8 +1746,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   } else if (run_rows && MODE == 1 && PA && PB) {
     // pattern sum mode: 16 lanes per short row, four rows per wavefront
     auto ksub = k_sum_subrows<BATCHED>;
-    int occ_s = 0;
-    GA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_s, ksub, 64 * WR_WAVES, 0));
+    const int occ_s = occupancy_of(ksub, 64 * WR_WAVES, 0, c.device);
     const int64_t grid_s = std::max<int64_t>(1, std::min<int64_t>((int64_t)c.num_cu * std::max(occ_s, 1),
                                                                  ceil_div(m, SR_GROUPS)));
     // rows per claim: about 8 claims per resident group, at most 4: the rows
