@@ -1065,7 +1065,8 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     const int32_t* __restrict__ Bcol, const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol,
     const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
-    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int chunk) {
+    unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int chunk,
+    int static_rows) {
   // the slot bitmaps of every wave, then WR_BMW words that are never set
   // (tested by lanes past a chunk's last wedge); records address them by
   // word offset from the array's start
@@ -1108,8 +1109,20 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     c_hi = min(lo + k + chunk, hi);
     return true;
   };
+  // static_rows: the launch has a wavefront for every `chunk` rows, so wave
+  // number g takes rows g*chunk .. and no claim counter is touched
+  if (static_rows) {
+    c_lo = (int)min((int64_t)n_rows, ((int64_t)blockIdx.x * WR_WAVES + wv) * chunk);
+    c_hi = (int)min((int64_t)n_rows, (int64_t)c_lo + chunk);
+  }
   while (true) {
-    bool got = !drained && try_claim(seg);
+    bool got;
+    if (static_rows) {
+      got = !drained && c_lo < c_hi;
+      drained = true;
+    } else {
+      got = !drained && try_claim(seg);
+    }
     while (!got && !drained) {
       bool live = false;
       if (lane < MXM_NSEG) {
@@ -1712,10 +1725,16 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
       const int chunk = std::min(64, chunk_fix > 0 ? chunk_fix
                                                    : (int)std::max<int64_t>(
                                                          2, std::min<int64_t>(16, m / (grid * WR_WAVES * 4))));
+      // a kept plan knows its short-row count: when every wavefront of the
+      // launch gets at most one claim anyway, hand the claims out statically
+      // (no claim atomics, no drain of the segment counters: the latency of a
+      // small graph's count)
+      const int static_rows =
+          (keep_plan && w.plan_rows >= 0 && w.plan_rows <= grid * WR_WAVES * (int64_t)chunk) ? 1 : 0;
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
                          M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
                          (const int32_t*)(w.rstart.as<int32_t>() + m),
-                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, chunk);
+                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, chunk, static_rows);
     };
     if (sum_sub == 2) {
       if (trunc) launch_wave(k_sum_wave<BATCHED, 4, true>);
