@@ -612,39 +612,129 @@ struct PeelDyn {
   int kidx, fin, kmax, err;  // k list position; run ended; largest nonempty k; a list outgrew its capacity
 };
 
+// Removal lists.  Every wavefront that pushes edges and every scan tile
+// claims list slots with a device-scope atomic, and claims on one address
+// serialise (about 11 ns each on MI355X: 4096 of them take 48 us, spread over
+// 32 counters 4.5 us).  A list is therefore PEEL_SEGS segments of segcap
+// items, each with its own counter on its own 128-byte line (a workgroup
+// appends to segment blockIdx % PEEL_SEGS), followed by an overflow segment
+// of ovcap items for the claims past a full segment.  The items of a list are
+// numbered segment by segment, the overflow's last.
+constexpr int PEEL_SEGS = 32;
+constexpr int PEEL_CNT_STRIDE = 32;  // ints from one counter to the next (128 bytes)
+struct PeelLists {
+  int4* items;     // list l starts at items + l * stride
+  int64_t stride;  // PEEL_SEGS * segcap + ovcap
+  int segcap, ovcap;
+};
+
 // The peeling state in device memory.  Host-steered runs keep their counters
-// in `h` and two removal lists (cnt); device-driven runs alternate `dyn[2]`
-// (tick t reads dyn[(t - 1) & 1] and stores its own in dyn[t & 1]) and
-// rotate four lists (qcnt): tick t consumes list t & 3, fills (t + 1) & 3 and
-// clears (t + 2) & 3, which no kernel in flight reads.
+// in `h` and use lists 0 and 1 (`ic` the next round's); device-driven runs
+// alternate `dyn[2]` (tick t reads dyn[(t - 1) & 1] and stores its own in
+// dyn[t & 1]) and rotate four lists: tick t consumes list t & 3, fills
+// (t + 1) & 3 and clears (t + 2) & 3, which no kernel in flight reads.
 struct PeelCtl {
   long long bytes;  // algorithmic bytes of the round launches (accounting on)
   PeelDyn h;
-  int cnt[2];
   int ic;           // host-steered: the list of the next round
   PeelDyn dyn[2];
-  int qcnt[4];
   int cr, clist, ccnt;  // compaction view: rounds run, the pending list and its size
   int nk, all_k;
   int ks[TRUSS_MAX_K];
   int passes_at[TRUSS_MAX_K];
   long long nnz_at[TRUSS_MAX_K];
+  // list counters (last: the host reads only the fields above)
+  alignas(128) int lcnt[4][PEEL_SEGS + 1][PEEL_CNT_STRIDE];
+};
+constexpr size_t PEEL_CTL_HOST_BYTES = offsetof(PeelCtl, lcnt);
+
+// Items per list for a graph of nnz stored entries: a round or a scan lists
+// each edge at most once, so nnz / 2 + 64 items always fit the overflow
+// segment alone.
+inline PeelLists peel_lists_geometry(int64_t nnz) {
+  PeelLists L{};
+  const int64_t lcap = nnz / 2 + 64;
+  L.segcap = (int)(lcap / PEEL_SEGS + 64);
+  L.ovcap = (int)lcap;
+  L.stride = (int64_t)PEEL_SEGS * L.segcap + L.ovcap;
+  return L;
+}
+
+__device__ inline int* peel_lc(PeelCtl* __restrict__ ctl, int l) { return &ctl->lcnt[l][0][0]; }
+
+// A wavefront's view of list l: lane s <= PEEL_SEGS holds the number of items
+// before segment s (segment PEEL_SEGS: the overflow); total and over (the
+// overflow outgrew its capacity) are the same in every lane.
+struct PeelView {
+  int pre, total;
+  bool over;
 };
 
-// Wavefront-aggregated append; entries past cap are dropped (the count still
-// grows, and the run stops with an error).
-// A list item is an edge: its canonical (lower) entry and the transposed one.
-__device__ inline void peel_push(bool sel, int32_t pos, int32_t other, int2* __restrict__ list, int* __restrict__ cnt,
-                                 int cap, int lane) {
+__device__ inline PeelView peel_view(const PeelCtl* __restrict__ ctl, int l, const PeelLists& L, int lane) {
+  const int raw = lane <= PEEL_SEGS ? ctl->lcnt[l][lane][0] : 0;
+  const int sz = lane < PEEL_SEGS ? min(raw, L.segcap) : (lane == PEEL_SEGS ? min(raw, L.ovcap) : 0);
+  const int inc = wave_incl_scan(sz);
+  PeelView v;
+  v.pre = inc - sz;
+  v.total = __shfl(inc, 63);
+  v.over = __ballot(lane == PEEL_SEGS && raw > L.ovcap) != 0;
+  return v;
+}
+
+// Item t (0 <= t < total, the same in every lane) of a list.
+__device__ inline int4 peel_item(const int4* __restrict__ list, const PeelView& v, const PeelLists& L, int64_t t,
+                                 int lane) {
+  const unsigned long long b = __ballot(lane <= PEEL_SEGS && v.pre <= t);  // lane 0 always (pre = 0)
+  const int s = 63 - __clzll(b);
+  return list[(int64_t)s * L.segcap + (t - __shfl(v.pre, s))];
+}
+
+// Sums of a list's segment sizes from one thread (one-thread kernels).
+__device__ inline int peel_total1(const PeelCtl* __restrict__ ctl, int l, const PeelLists& L, bool* over) {
+  int tot = 0;
+  for (int s = 0; s < PEEL_SEGS; ++s) tot += min(ctl->lcnt[l][s][0], L.segcap);
+  const int ov = ctl->lcnt[l][PEEL_SEGS][0];
+  if (over) *over = ov > L.ovcap;
+  return tot + min(ov, L.ovcap);
+}
+
+// The slot of the r-th item (0-based) of a claim of n items that got `base`
+// in segment seg and `obase` in the overflow (used only when the segment had
+// fewer than n slots left); -1 past the overflow's capacity.
+__device__ inline int64_t peel_slot(int seg, int base, int obase, int r, const PeelLists& L) {
+  const int room = max(L.segcap - base, 0);
+  if (r < room) return (int64_t)seg * L.segcap + base + r;
+  const int o = obase + r - room;
+  return o < L.ovcap ? (int64_t)PEEL_SEGS * L.segcap + o : -1;
+}
+
+// Claims n slots of list counters lc for the calling workgroup's segment:
+// base in the segment, obase in the overflow (claimed only for the part that
+// does not fit).  One thread calls it.
+__device__ inline void peel_claim(int* __restrict__ lc, int n, const PeelLists& L, int& seg, int& base, int& obase) {
+  seg = blockIdx.x % PEEL_SEGS;
+  base = atomicAdd(&lc[seg * PEEL_CNT_STRIDE], n);
+  const int room = max(L.segcap - base, 0);
+  obase = room < n ? atomicAdd(&lc[PEEL_SEGS * PEEL_CNT_STRIDE], n - room) : 0;
+}
+
+// Wavefront-aggregated append of one item per selected lane.
+// A list item is an edge (i, j), i > j: x = its canonical entry (row i,
+// column j), y = the transposed entry, z = i, w = j.  Carrying the endpoints
+// saves a round the dependent load of both columns before the row bounds.
+__device__ inline void peel_push(bool sel, int4 item, int4* __restrict__ list, int* __restrict__ lc,
+                                 const PeelLists& L, int lane) {
   const unsigned long long m = __ballot(sel);
   if (!m) return;
-  int base = 0;
-  if (lane == 0) base = atomicAdd(cnt, __popcll(m));
+  int seg = 0, base = 0, obase = 0;
+  if (lane == 0) peel_claim(lc, __popcll(m), L, seg, base, obase);
+  seg = __shfl(seg, 0);
   base = __shfl(base, 0);
+  obase = __shfl(obase, 0);
   if (sel) {
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int at = base + __popcll(m & lt);
-    if (at < cap) list[at] = make_int2(pos, other);
+    const int64_t at = peel_slot(seg, base, obase, __popcll(m & lt), L);
+    if (at >= 0) list[at] = item;
   }
 }
 
@@ -661,16 +751,15 @@ __device__ inline void peel_publish(long long* __restrict__ host, long long size
 
 // alive canonical entries with support below thr: stamped r, listed.  A
 // workgroup takes 1024 entries at a time (4 per thread, coalesced) and
-// claims their list slots with one atomic (the list counter is a single
-// word: per-wavefront claims queued ~28 k atomics on it for R-MAT 16's first
-// threshold).  The body holds __syncthreads: every thread of the workgroup
-// calls it.
+// claims their list slots with one claim in its segment.  The body holds
+// __syncthreads: every thread of the workgroup calls it.
 constexpr int PEEL_SCAN_PER = 4;
-__device__ inline void peel_scan_body(const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
-                                      int32_t* __restrict__ stamp, int64_t nnz, int thr, int r,
-                                      int2* __restrict__ list, int* __restrict__ cnt, int cap) {
+__device__ inline void peel_scan_body(const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
+                                      const int32_t* __restrict__ sup, int32_t* __restrict__ stamp, int64_t nnz,
+                                      int thr, int r, int4* __restrict__ list, int* __restrict__ lc,
+                                      const PeelLists& L) {
   __shared__ int s_wcnt[4];
-  __shared__ int s_base;
+  __shared__ int s_claim[3];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t tile = (int64_t)blockDim.x * PEEL_SCAN_PER;
@@ -697,23 +786,29 @@ __device__ inline void peel_scan_body(const int32_t* __restrict__ mirror, const 
     if (threadIdx.x == 0) {
       int tot = 0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += s_wcnt[w];
-      s_base = tot ? atomicAdd(cnt, tot) : 0;
+      int seg = 0, cb = 0, ob = 0;
+      if (tot) peel_claim(lc, tot, L, seg, cb, ob);
+      s_claim[0] = seg;
+      s_claim[1] = cb;
+      s_claim[2] = ob;
     }
     __syncthreads();
-    int at = s_base;
+    const int seg = s_claim[0], cb = s_claim[1], ob = s_claim[2];
+    int at = 0;
     for (int w = 0; w < wv; ++w) at += s_wcnt[w];
 #pragma unroll
     for (int u = 0; u < PEEL_SCAN_PER; ++u) {
       const int64_t p = base + u * blockDim.x + threadIdx.x;
       if ((mk[u] >> lane) & 1ull) {
-        const int slot = at + __popcll(mk[u] & lt);
+        const int64_t slot = peel_slot(seg, cb, ob, at + __popcll(mk[u] & lt), L);
+        const int j = col[p], i = col[mm[u]];  // p: row i, column j
         stamp[p] = r;
         stamp[mm[u]] = r;
-        if (slot < cap) list[slot] = make_int2((int32_t)p, mm[u]);
+        if (slot >= 0) list[slot] = make_int4((int32_t)p, mm[u], i, j);
       }
       at += __popcll(mk[u]);
     }
-    __syncthreads();  // s_wcnt and s_base are reused by the next tile
+    __syncthreads();  // s_wcnt and s_claim are reused by the next tile
   }
 }
 
@@ -739,7 +834,9 @@ __device__ inline void peel_count_round(PeelDyn& d, int all_k, int nc) {
 
 __global__ void k_peel_ctl_init(PeelCtl* __restrict__ ctl, long long alive, int all_k, int nk, int k0,
                                 long long passes) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+  if (blockIdx.x != 0) return;
+  for (int i = threadIdx.x; i < 4 * (PEEL_SEGS + 1); i += blockDim.x) ctl->lcnt[i / (PEEL_SEGS + 1)][i % (PEEL_SEGS + 1)][0] = 0;
+  if (threadIdx.x == 0) {
     PeelDyn d{};
     d.alive = alive;
     d.passes = passes;
@@ -753,9 +850,7 @@ __global__ void k_peel_ctl_init(PeelCtl* __restrict__ ctl, long long alive, int 
     ctl->bytes = 0;
     ctl->h = d;
     ctl->dyn[0] = ctl->dyn[1] = d;
-    ctl->cnt[0] = ctl->cnt[1] = 0;
     ctl->ic = 0;
-    for (int i = 0; i < 4; ++i) ctl->qcnt[i] = 0;
     ctl->cr = ctl->clist = ctl->ccnt = 0;
     ctl->nk = nk;
     ctl->all_k = all_k;
@@ -775,12 +870,11 @@ struct PeelEdge {
   int c, mc, ab, ae, arow, bb, be, brow, stride, m;
 };
 
-__device__ inline PeelEdge peel_edge(int2 item, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                     int cap) {
+__device__ inline PeelEdge peel_edge(int4 item, const int32_t* __restrict__ rp, int cap) {
   PeelEdge e;
   e.c = item.x;
   e.mc = item.y;
-  const int j = col[e.c], i = col[e.mc];
+  const int i = item.z, j = item.w;
   const int ib = rp[i], ie = rp[i + 1], jb = rp[j], je = rp[j + 1];
   const bool pi = (ie - ib) <= (je - jb);  // probe row i
   e.ab = pi ? ib : jb;
@@ -802,14 +896,34 @@ __device__ inline PeelEdge peel_direct(PeelEdge e) {
   return e;
 }
 
+// A probed entry's column, stamp and transposed position, loaded before the
+// target row's sample is staged so that both sets of loads are in flight
+// together.  A lane past the probe row gets stamp -1, which every round
+// (r >= 1) skips.
+struct PeelQ {
+  int w, sa, mq;
+};
+
+__device__ inline PeelQ peel_qload(int q, int qe, const int32_t* __restrict__ col, const int32_t* __restrict__ stamp,
+                                   const int32_t* __restrict__ mirror) {
+  PeelQ v{-1, -1, -1};
+  if (q < qe) {
+    v.w = col[q];
+    v.sa = stamp[q];
+    v.mq = mirror[q];
+  }
+  return v;
+}
+
 // One probed entry q of the edge's probe row: the decrements it owns and the
-// edges to push (canonical entry, transposed entry; x = -1: none).
-__device__ inline void peel_probe(const PeelEdge& e, int q, const int32_t* __restrict__ st,
+// edges to push (x = -1: none).  st: the target row's sample (LDS) or the row
+// itself (global memory).
+__device__ inline void peel_probe(const PeelEdge& e, int q, const PeelQ& v, const int32_t* __restrict__ st,
                                   const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
                                   int32_t* __restrict__ sup, int32_t* __restrict__ stamp, int thr, int r,
-                                  int2& push_a, int2& push_b, long long& found) {
-  push_a = push_b = make_int2(-1, -1);
-  const int w = col[q], sa = stamp[q], mq = mirror[q];  // independent loads
+                                  int4& push_a, int4& push_b, long long& found) {
+  push_a = push_b = make_int4(-1, -1, -1, -1);
+  const int w = v.w, sa = v.sa, mq = v.mq;
   if (sa < r || w == e.brow) return;
   int lo = 0, hi = e.m;
   while (lo < hi) {
@@ -838,13 +952,13 @@ __device__ inline void peel_probe(const PeelEdge& e, int q, const int32_t* __res
   const int cid_b = w < e.brow ? pos : mp;
   if ((sa == r && cid_a < e.c) || (sb == r && cid_b < e.c)) return;  // another removed edge owns it
   ++found;
-  if (sa > r && atomicSub(&sup[cid_a], 1) == thr) {
-    push_a = make_int2(cid_a, cid_a == q ? mq : q);
+  if (sa > r && atomicSub(&sup[cid_a], 1) == thr) {  // edge (arow, w): entry q lies in row arow
+    push_a = make_int4(cid_a, cid_a == q ? mq : q, max(e.arow, w), min(e.arow, w));
     stamp[q] = r + 1;
     stamp[mq] = r + 1;
   }
-  if (sb > r && atomicSub(&sup[cid_b], 1) == thr) {
-    push_b = make_int2(cid_b, cid_b == pos ? mp : pos);
+  if (sb > r && atomicSub(&sup[cid_b], 1) == thr) {  // edge (brow, w): entry pos lies in row brow
+    push_b = make_int4(cid_b, cid_b == pos ? mp : pos, max(e.brow, w), min(e.brow, w));
     stamp[pos] = r + 1;
     stamp[mp] = r + 1;
   }
@@ -852,15 +966,20 @@ __device__ inline void peel_probe(const PeelEdge& e, int q, const int32_t* __res
 
 constexpr int PEEL_WSTAGE = 1024;         // sample entries per wavefront (large rounds)
 constexpr int PEEL_BSTAGE = 4096;         // sample entries per workgroup (small rounds)
-constexpr int PEEL_SPLIT = 8;             // workgroups per edge in small rounds
-constexpr int PEEL_SPLIT_MAX_EDGES = 64;  // rounds of at most this many edges split them
+constexpr int PEEL_SPLIT = 8;             // least workgroups per edge in small rounds
+constexpr int PEEL_SPLIT_MAX = 32;        // most workgroups per edge in small rounds
+constexpr int PEEL_SPLIT_MAX_EDGES = 64;  // rounds of at most this many edges split them by workgroups
 constexpr int PEEL_STAGE_ROUND = 4096;    // rounds up to this size stage every long target row
 
-// The round itself.  Large rounds: a wavefront per edge (a wavefront probing
-// one chunk searches the target row in global memory, a longer probe row
-// stages the sample first).  Small rounds (few edges, usually long rows):
-// PEEL_SPLIT workgroups per edge, each staging the sample once and probing
-// every PEEL_SPLIT-th run of 256 probe entries.
+// The round itself.  Rounds of more than PEEL_SPLIT_MAX_EDGES edges: a
+// wavefront per (edge, part); when the round has fewer edges than the grid has
+// wavefronts, each edge's probe row is split into 64-entry chunks dealt
+// round-robin over nw / nc wavefronts (a hub edge's chunks no longer run one
+// after another on one wavefront).  A wavefront probing a chunk searches the
+// target row in global memory, or stages its sample first when the row is
+// long.  Small rounds (few edges, usually long rows): PEEL_SPLIT to
+// PEEL_SPLIT_MAX workgroups per edge, each staging the sample once and probing
+// every split-th run of 256 probe entries.
 struct PeelGraph {
   const int32_t* rp;
   const int32_t* col;
@@ -871,43 +990,56 @@ struct PeelGraph {
   int32_t* truss;
 };
 
-__device__ inline void peel_round_body(const int2* __restrict__ cur, int nc, int2* __restrict__ next,
-                                       int* __restrict__ ncnt, int r, int thr, int tval, const PeelGraph& g, int cap,
-                                       long long* __restrict__ bytes_ctr, int32_t* __restrict__ s_stage) {
+__device__ inline void peel_round_body(const int4* __restrict__ cur, const PeelView& cv, int4* __restrict__ next,
+                                       int* __restrict__ nlc, const PeelLists& L, int r, int thr, int tval,
+                                       const PeelGraph& g, long long* __restrict__ bytes_ctr,
+                                       int32_t* __restrict__ s_stage) {
   const int lane = threadIdx.x & 63;
+  const int nc = cv.total;
   long long bytes = 0, found = 0;
   if (nc > PEEL_SPLIT_MAX_EDGES) {
     int32_t* st = s_stage + (threadIdx.x >> 6) * PEEL_WSTAGE;
     const int64_t wpb = blockDim.x >> 6;
-    for (int64_t t = blockIdx.x * wpb + (threadIdx.x >> 6); t < nc; t += (int64_t)gridDim.x * wpb) {
-      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, PEEL_WSTAGE);
-      if (g.truss && lane == 0) {
-        g.truss[g.eid[e.c]] = tval;
-        g.truss[g.eid[e.mc]] = tval;
+    const int64_t nw = (int64_t)gridDim.x * wpb;
+    const int wpe = nc < nw ? (int)(nw / nc) : 1;  // wavefronts per edge
+    const int64_t nv = (int64_t)nc * wpe;
+    for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < nv; v += nw) {
+      const int64_t t = v / wpe;
+      const int part = (int)(v - t * wpe);
+      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), g.rp, PEEL_WSTAGE);
+      if (part == 0 && lane == 0) {
+        if (g.truss) {
+          g.truss[g.eid[e.c]] = tval;
+          g.truss[g.eid[e.mc]] = tval;
+        }
+        if (bytes_ctr) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
       }
-      if (bytes_ctr && lane == 0) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
+      const int q0 = e.ab + part * 64;
+      if (q0 >= e.ae) continue;  // no chunk of this edge for this part (uniform in the wavefront)
       // a long target row is staged when several chunks probe it or the
       // round is small enough that the staging loads are cheaper than the
       // chains of a binary search in global memory
       const bool staged = e.be - e.bb > 64 && (e.ae - e.ab > 64 || nc <= PEEL_STAGE_ROUND);
       const PeelEdge pe = staged ? e : peel_direct(e);
+      PeelQ pq = peel_qload(q0 + lane, e.ae, g.col, g.stamp, g.mirror);
       __threadfence_block();  // the previous edge's sample reads are done
       if (staged)
         for (int u = lane; u < e.m; u += 64) st[u] = g.col[e.bb + u * e.stride];
       __threadfence_block();
-      for (int q0 = e.ab; q0 < e.ae; q0 += 64) {
-        const int q = q0 + lane;
-        int2 pa = make_int2(-1, -1), pb = make_int2(-1, -1);
-        if (q < e.ae)
-          peel_probe(pe, q, staged ? st : g.col + e.bb, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
-        peel_push(pa.x >= 0, pa.x, pa.y, next, ncnt, cap, lane);
-        peel_push(pb.x >= 0, pb.x, pb.y, next, ncnt, cap, lane);
+      for (int qb = q0; qb < e.ae; qb += 64 * wpe) {
+        const int q = qb + lane;
+        if (qb != q0) pq = peel_qload(q, e.ae, g.col, g.stamp, g.mirror);
+        int4 pa, pb;
+        peel_probe(pe, q, pq, staged ? st : g.col + e.bb, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
+        peel_push(pa.x >= 0, pa, next, nlc, L, lane);
+        peel_push(pb.x >= 0, pb, next, nlc, L, lane);
       }
     }
   } else {
-    for (int64_t b = blockIdx.x; b < (int64_t)nc * PEEL_SPLIT; b += gridDim.x) {
-      const int t = (int)(b / PEEL_SPLIT), part = (int)(b % PEEL_SPLIT);
-      const PeelEdge e = peel_edge(cur[t], g.rp, g.col, PEEL_BSTAGE);
+    const int split = max(PEEL_SPLIT, min(PEEL_SPLIT_MAX, (int)(gridDim.x / max(nc, 1))));
+    for (int64_t b = blockIdx.x; b < (int64_t)nc * split; b += gridDim.x) {
+      const int t = (int)(b / split), part = (int)(b % split);
+      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), g.rp, PEEL_BSTAGE);
       if (part == 0 && threadIdx.x == 0) {
         if (g.truss) {
           g.truss[g.eid[e.c]] = tval;
@@ -915,15 +1047,19 @@ __device__ inline void peel_round_body(const int2* __restrict__ cur, int nc, int
         }
         if (bytes_ctr) bytes += 28 + 4ll * ((e.ae - e.ab) + (e.be - e.bb));
       }
+      const int q0 = e.ab + part * 256;
+      if (q0 >= e.ae) continue;  // uniform in the workgroup
+      PeelQ pq = peel_qload(q0 + (int)threadIdx.x, e.ae, g.col, g.stamp, g.mirror);
       __syncthreads();  // the previous edge's sample reads are done
       for (int u = threadIdx.x; u < e.m; u += blockDim.x) s_stage[u] = g.col[e.bb + u * e.stride];
       __syncthreads();
-      for (int q0 = e.ab + part * 256; q0 < e.ae; q0 += PEEL_SPLIT * 256) {
-        const int q = q0 + threadIdx.x;
-        int2 pa = make_int2(-1, -1), pb = make_int2(-1, -1);
-        if (q < e.ae) peel_probe(e, q, s_stage, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
-        peel_push(pa.x >= 0, pa.x, pa.y, next, ncnt, cap, lane);
-        peel_push(pb.x >= 0, pb.x, pb.y, next, ncnt, cap, lane);
+      for (int qb = q0; qb < e.ae; qb += split * 256) {
+        const int q = qb + threadIdx.x;
+        if (qb != q0) pq = peel_qload(q, e.ae, g.col, g.stamp, g.mirror);
+        int4 pa, pb;
+        peel_probe(e, q, pq, s_stage, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
+        peel_push(pa.x >= 0, pa, next, nlc, L, lane);
+        peel_push(pb.x >= 0, pb, next, nlc, L, lane);
       }
     }
   }
@@ -934,39 +1070,42 @@ __device__ inline void peel_round_body(const int2* __restrict__ cur, int nc, int
 }
 
 // ---- host-steered runs: a kernel per round or scan, then k_peel_post ------
-__global__ __launch_bounds__(256) void k_peel_scan(int2* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
-                                                   const int32_t* __restrict__ mirror,
+__global__ __launch_bounds__(256) void k_peel_scan(PeelLists L, PeelCtl* __restrict__ ctl,
+                                                   const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
                                                    const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
-                                                   int64_t nnz, int cap) {
+                                                   int64_t nnz) {
   const int ic = ctl->ic;
-  peel_scan_body(mirror, sup, stamp, nnz, ctl->h.k - 2, ctl->h.r + 1, lists + ic * lcap, &ctl->cnt[ic], cap);
+  peel_scan_body(col, mirror, sup, stamp, nnz, ctl->h.k - 2, ctl->h.r + 1, L.items + ic * L.stride,
+                 peel_lc(ctl, ic), L);
 }
 
-__global__ __launch_bounds__(256) void k_peel_destroy(int2* __restrict__ lists, int64_t lcap,
-                                                      PeelCtl* __restrict__ ctl, PeelGraph g, int cap, int account) {
+__global__ __launch_bounds__(256) void k_peel_destroy(PeelLists L, PeelCtl* __restrict__ ctl, PeelGraph g,
+                                                      int account) {
   __shared__ int32_t s_stage[PEEL_BSTAGE];
   const int ic = ctl->ic;
-  const int nc = ctl->h.fin ? 0 : min(ctl->cnt[ic], cap);
-  if (nc == 0) return;
-  peel_round_body(lists + ic * lcap, nc, lists + (ic ^ 1) * lcap, &ctl->cnt[ic ^ 1], ctl->h.r + 1, ctl->h.k - 2,
-                  ctl->h.k - 1, g, cap, account ? &ctl->bytes : nullptr, s_stage);
+  const PeelView cv = peel_view(ctl, ic, L, threadIdx.x & 63);
+  if (ctl->h.fin || cv.total == 0) return;
+  peel_round_body(L.items + ic * L.stride, cv, L.items + (ic ^ 1) * L.stride, peel_lc(ctl, ic ^ 1), L,
+                  ctl->h.r + 1, ctl->h.k - 2, ctl->h.k - 1, g, account ? &ctl->bytes : nullptr, s_stage);
 }
 
 // mode 0: a round ran -- counted, its list emptied, the lists swap; mode 1:
 // a scan ran.  Then the list size of the next round to the host.
-__global__ void k_peel_post(PeelCtl* __restrict__ ctl, int mode, int cap, long long* __restrict__ host,
+__global__ void k_peel_post(PeelCtl* __restrict__ ctl, int mode, PeelLists L, long long* __restrict__ host,
                             long long seq) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  bool over0 = false, over1 = false;
   if (mode == 0 && !ctl->h.fin) {
-    const int nc = min(ctl->cnt[ctl->ic], cap);
+    const int nc = peel_total1(ctl, ctl->ic, L, &over0);
     if (nc > 0) {
       peel_count_round(ctl->h, ctl->all_k, nc);
-      ctl->cnt[ctl->ic] = 0;  // consumed; the round after next fills it
+      for (int s = 0; s <= PEEL_SEGS; ++s) ctl->lcnt[ctl->ic][s][0] = 0;  // consumed; the round after next fills it
       ctl->ic ^= 1;
     }
   }
-  if (ctl->cnt[0] > cap || ctl->cnt[1] > cap) ctl->h.err = ctl->h.fin = 1;
-  peel_publish(host, ctl->cnt[ctl->ic], ctl->h.alive, ctl->h.fin, 0, seq);
+  const int size = peel_total1(ctl, ctl->ic, L, &over1);
+  if (over0 || over1) ctl->h.err = ctl->h.fin = 1;
+  peel_publish(host, size, ctl->h.alive, ctl->h.fin, 0, seq);
 }
 
 // Host-steered runs: the k to scan next, and the filter step that found the
@@ -980,15 +1119,14 @@ __global__ void k_peel_set_k(PeelCtl* __restrict__ ctl, int k, int stable_step) 
 
 // ---- device-driven runs: one kernel per tick --------------------------------
 // The state of tick t: the state of tick t - 1 moved on by what that tick did
-// -- a round (its list was not empty: counted) or the stable step (its list
-// was empty: recorded, k moved on; that tick scanned the new threshold).
-// Tick 0 is the first threshold's scan.
-__device__ inline PeelDyn peel_tick_state(const PeelCtl* __restrict__ ctl, int64_t t, int cap, int& rec_kidx,
+// -- a round (its list, of np items, was not empty: counted) or the stable
+// step (its list was empty: recorded, k moved on; that tick scanned the new
+// threshold).  Tick 0 is the first threshold's scan.
+__device__ inline PeelDyn peel_tick_state(const PeelCtl* __restrict__ ctl, int64_t t, int np, int& rec_kidx,
                                           long long& rec_alive, int& rec_passes) {
   PeelDyn d = ctl->dyn[(t - 1) & 1];
   rec_kidx = -1;
   if (t <= 1 || d.fin) return d;
-  const int np = min(ctl->qcnt[(t - 1) & 3], cap);
   if (np > 0) {
     peel_count_round(d, ctl->all_k, np);
   } else {
@@ -1012,37 +1150,37 @@ __device__ inline PeelDyn peel_tick_state(const PeelCtl* __restrict__ ctl, int64
   return d;
 }
 
-__global__ __launch_bounds__(256) void k_peel_tick(int64_t t, int2* __restrict__ lists, int64_t lcap,
-                                                   PeelCtl* __restrict__ ctl, PeelGraph g, int64_t nnz, int cap,
-                                                   int account, long long* __restrict__ host) {
+__global__ __launch_bounds__(256) void k_peel_tick(int64_t t, PeelLists L, PeelCtl* __restrict__ ctl, PeelGraph g,
+                                                   int64_t nnz, int account, long long* __restrict__ host) {
   __shared__ int32_t s_stage[PEEL_BSTAGE];
+  const int lane = threadIdx.x & 63;
+  const PeelView pv = peel_view(ctl, (int)((t - 1) & 3), L, lane);  // the previous tick's list
+  const PeelView cv = peel_view(ctl, (int)(t & 3), L, lane);        // this tick's
   int rec_kidx;
   long long rec_alive = 0;
   int rec_passes = 0;
-  PeelDyn d = peel_tick_state(ctl, t, cap, rec_kidx, rec_alive, rec_passes);
-  const int qc = ctl->qcnt[t & 3];
+  PeelDyn d = peel_tick_state(ctl, t, pv.total, rec_kidx, rec_alive, rec_passes);
+  if (blockIdx.x == 0 && threadIdx.x <= PEEL_SEGS) ctl->lcnt[(t + 2) & 3][threadIdx.x][0] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (qc > cap || ctl->qcnt[(t - 1) & 3] > cap) d.err = d.fin = 1;
+    if (cv.over || pv.over) d.err = d.fin = 1;
     ctl->dyn[t & 1] = d;
-    ctl->qcnt[(t + 2) & 3] = 0;
     if (rec_kidx >= 0) {
       ctl->nnz_at[rec_kidx] = rec_alive;
       ctl->passes_at[rec_kidx] = rec_passes;
     }
-    if ((t & 3) == 0 || d.fin) peel_publish(host, qc, d.alive, d.fin, t, 0);
+    if ((t & 3) == 0 || d.fin) peel_publish(host, cv.total, d.alive, d.fin, t, 0);
   }
   if (d.fin) return;
-  const int nc = min(qc, cap);
-  int2* next = lists + ((t + 1) & 3) * lcap;
-  int* ncnt = &ctl->qcnt[(t + 1) & 3];
-  if (nc > 0) {
-    peel_round_body(lists + (t & 3) * lcap, nc, next, ncnt, d.r + 1, d.k - 2, d.k - 1, g, cap,
+  const int nl = (int)((t + 1) & 3);
+  int4* next = L.items + nl * L.stride;
+  if (cv.total > 0) {
+    peel_round_body(L.items + (t & 3) * L.stride, cv, next, peel_lc(ctl, nl), L, d.r + 1, d.k - 2, d.k - 1, g,
                     account ? &ctl->bytes : nullptr, s_stage);
   } else {
     int k = d.k, kidx = d.kidx, fin = 0;  // the k-truss is stable: scan the next threshold
     peel_next_k(ctl, d.alive, k, kidx, fin);
     if (!fin && d.alive > 0) {
-      peel_scan_body(g.mirror, g.sup, g.stamp, nnz, k - 2, d.r + 1, next, ncnt, cap);
+      peel_scan_body(g.col, g.mirror, g.sup, g.stamp, nnz, k - 2, d.r + 1, next, peel_lc(ctl, nl), L);
       // accounting: the scan reads mirror, support and stamp of every stored entry
       if (account && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&ctl->bytes, 12ull * nnz);
     }
@@ -1050,18 +1188,23 @@ __global__ __launch_bounds__(256) void k_peel_tick(int64_t t, int2* __restrict__
 }
 
 // The device-driven queue drained after tick t: the state tick t + 1 will
-// start from, to the host (sequence seq) and as the compaction view.
-__global__ void k_peel_drain(int64_t t, PeelCtl* __restrict__ ctl, int cap, long long* __restrict__ host,
+// start from, to the host (sequence seq) and as the compaction view.  One
+// wavefront.
+__global__ void k_peel_drain(int64_t t, PeelCtl* __restrict__ ctl, PeelLists L, long long* __restrict__ host,
                              long long seq) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (blockIdx.x != 0 || threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  const PeelView pv = peel_view(ctl, (int)(t & 3), L, lane);
+  const PeelView cv = peel_view(ctl, (int)((t + 1) & 3), L, lane);
   int rk;
   long long ra;
   int rp_;
-  const PeelDyn d = peel_tick_state(ctl, t + 1, cap, rk, ra, rp_);
+  const PeelDyn d = peel_tick_state(ctl, t + 1, pv.total, rk, ra, rp_);
+  if (lane != 0) return;
   ctl->cr = d.r;
   ctl->clist = (int)((t + 1) & 3);
-  ctl->ccnt = d.fin ? 0 : min(ctl->qcnt[(t + 1) & 3], cap);
-  peel_publish(host, ctl->ccnt, d.alive, d.fin, t, seq);
+  ctl->ccnt = d.fin ? 0 : cv.total;
+  peel_publish(host, ctl->ccnt, d.alive, d.fin || pv.over || cv.over, t, seq);
 }
 
 // Host-steered runs before a compaction (a k boundary: no list pending).
@@ -1085,25 +1228,34 @@ __global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __r
 
 // After a compaction: the pending list's entries move to their new positions
 // and keep their stamp cr + 1 (both directions); every other stamp was reset.
-__global__ void k_peel_remap(int2* __restrict__ lists, int64_t lcap, const PeelCtl* __restrict__ ctl,
-                             const int32_t* __restrict__ newpos, int32_t* __restrict__ stamp) {
-  const int n = ctl->ccnt, r1 = ctl->cr + 1;
-  int2* list = lists + ctl->clist * lcap;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-    const int2 it = list[t];
-    const int c = newpos[it.x], m = newpos[it.y];
-    list[t] = make_int2(c, m);
-    stamp[c] = r1;
-    stamp[m] = r1;
+// Segment by segment (a segment's items are its first min(count, capacity)
+// slots).
+__global__ void k_peel_remap(PeelLists L, const PeelCtl* __restrict__ ctl, const int32_t* __restrict__ newpos,
+                             int32_t* __restrict__ stamp) {
+  if (ctl->ccnt == 0) return;
+  const int l = ctl->clist, r1 = ctl->cr + 1;
+  int4* list = L.items + l * L.stride;
+  for (int s = 0; s <= PEEL_SEGS; ++s) {
+    const int raw = ctl->lcnt[l][s][0];
+    const int n = s < PEEL_SEGS ? min(raw, L.segcap) : min(raw, L.ovcap);
+    int4* seg = list + (int64_t)s * L.segcap;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+      const int4 it = seg[t];
+      const int c = newpos[it.x], m = newpos[it.y];
+      seg[t] = make_int4(c, m, it.z, it.w);  // the endpoints do not move
+      stamp[c] = r1;
+      stamp[m] = r1;
+    }
   }
 }
 
 // Tick 0 of a device-driven run: the first threshold's scan into list 1.
-__global__ __launch_bounds__(256) void k_peel_tick0(int2* __restrict__ lists, int64_t lcap, PeelCtl* __restrict__ ctl,
-                                                    const int32_t* __restrict__ mirror,
+__global__ __launch_bounds__(256) void k_peel_tick0(PeelLists L, PeelCtl* __restrict__ ctl,
+                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
                                                     const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
-                                                    int64_t nnz, int cap) {
-  peel_scan_body(mirror, sup, stamp, nnz, ctl->dyn[0].k - 2, ctl->dyn[0].r + 1, lists + lcap, &ctl->qcnt[1], cap);
+                                                    int64_t nnz) {
+  peel_scan_body(col, mirror, sup, stamp, nnz, ctl->dyn[0].k - 2, ctl->dyn[0].r + 1, L.items + L.stride,
+                 peel_lc(ctl, 1), L);
 }
 
 // The final compaction of a finished run (no list pending).
@@ -1138,11 +1290,12 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   Context& c = ctx();
   host_steer = host_steer || on_k != nullptr;
   GA_REQUIRE(all_k || host_steer || nk <= TRUSS_MAX_K, GA_ERR_LIMIT, "at most 64 k values in one k-truss run");
-  const int64_t lcap = nnz0 / 2 + 64;  // canonical entries: every list fits
-  tw.plist.ensure(sizeof(int2) * 4 * lcap);
+  PeelLists L = peel_lists_geometry(nnz0);
+  const int64_t lcap = nnz0 / 2 + 64;  // canonical entries: every list holds at most this many
+  tw.plist.ensure(sizeof(int4) * 4 * L.stride);
   tw.pcnt.ensure(sizeof(PeelCtl));
   tw.stamp.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
-  int2* lists = tw.plist.as<int2>();
+  L.items = tw.plist.as<int4>();
   auto* ctl = tw.pcnt.as<PeelCtl>();
   int32_t* stamp = tw.stamp.as<int32_t>();
   const int64_t passes0 = (all_k && nnz0 == 0) ? 0 : 1;  // a decomposition of no entries runs no product
@@ -1182,6 +1335,11 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   hw_dev += 16;
   const unsigned cap = (unsigned)(8 * (int64_t)c.num_cu);
   const unsigned gscan = std::min<unsigned>(grid_for(std::max<int64_t>(stored, 1), 256, 1 << 20), cap);
+  // round launches: every workgroup of a tick is queued before its list size
+  // is known; rounds smaller than the grid spread each edge over several
+  // wavefronts (peel_round_body)
+  // (4 workgroups per CU measured best on R-MAT 16 against 2, 6 and 8)
+  const unsigned gtick = std::min<unsigned>(gscan, 4 * (unsigned)c.num_cu);
   auto timed = [&](auto&& launch) {  // a round launch, bracketed by events when profiling
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.prof_events_on) {
@@ -1223,7 +1381,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                        (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.nold.as<int32_t>(),
                        tw.mirror[cur ^ 1].as<int32_t>(), (const TrussDevState*)nullptr);
     fill_u32(stamp, (uint32_t)PEEL_ALIVE, alive, s);
-    hipLaunchKernelGGL(k_peel_remap, dim3(grid_for(lcap, 256, 4096)), dim3(256), 0, s, lists, lcap,
+    hipLaunchKernelGGL(k_peel_remap, dim3(grid_for(lcap, 256, 4096)), dim3(256), 0, s, L,
                        (const PeelCtl*)ctl, (const int32_t*)tw.newpos.as<int32_t>(), stamp);
     GA_HIP(hipGetLastError());
     cur ^= 1;
@@ -1252,10 +1410,11 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     int64_t alive = nnz0, passes = passes0, ncur = 0;
     auto scan = [&]() {
       const long long seq = next_host_seq();
-      hipLaunchKernelGGL(k_peel_scan, dim3(gscan), dim3(256), 0, s, lists, lcap, ctl,
-                         (const int32_t*)tw.mirror[cur].as<int32_t>(), (const int32_t*)tw.sup[cur].as<int32_t>(),
-                         stamp, stored, (int)lcap);
-      hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, 1, (int)lcap, hw_dev, seq);
+      hipLaunchKernelGGL(k_peel_scan, dim3(gscan), dim3(256), 0, s, L, ctl,
+                         (const int32_t*)tw.cur[cur].col.as<int32_t>(), (const int32_t*)tw.mirror[cur].as<int32_t>(),
+                         (const int32_t*)tw.sup[cur].as<int32_t>(),
+                         stamp, stored);
+      hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, 1, L, hw_dev, seq);
       GA_HIP(hipGetLastError());
       wait_host_word(&hw[4], seq, s);
       ncur = list_size();
@@ -1301,13 +1460,12 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       }
       const long long seq = next_host_seq();
       const PeelGraph g = graph();
-      const unsigned grid = ncur <= PEEL_SPLIT_MAX_EDGES ? (unsigned)(ncur * PEEL_SPLIT)
-                                                          : std::min<unsigned>(grid_for(ncur, 4, 1 << 20), cap);
+      const unsigned grid = ncur <= PEEL_SPLIT_MAX_EDGES ? (unsigned)(ncur * PEEL_SPLIT) : gtick;
       timed([&] {
-        hipLaunchKernelGGL(k_peel_destroy, dim3(grid), dim3(256), 0, s, lists, lcap, ctl, g, (int)lcap,
+        hipLaunchKernelGGL(k_peel_destroy, dim3(grid), dim3(256), 0, s, L, ctl, g,
                            account ? 1 : 0);
       });
-      hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, 0, (int)lcap, hw_dev, seq);
+      hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, 0, L, hw_dev, seq);
       GA_HIP(hipGetLastError());
       wait_host_word(&hw[4], seq, s);
       alive -= 2 * ncur;
@@ -1315,7 +1473,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       ncur = list_size();
     }
     PeelCtl h{};
-    GA_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipMemcpyAsync(&h, ctl, PEEL_CTL_HOST_BYTES, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
     GA_REQUIRE(!h.h.err && h.h.alive == alive && h.h.passes == passes, GA_ERR_STATE,
                "peeling state disagrees with the host");
@@ -1337,10 +1495,9 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     __atomic_store_n(&hw[2], 0ll, __ATOMIC_RELAXED);
     __atomic_store_n(&hw[3], 0ll, __ATOMIC_RELEASE);
     if (nnz0 > 0)  // tick 0: the first threshold, into list 1
-      hipLaunchKernelGGL(k_peel_tick0, dim3(gscan), dim3(256), 0, s, lists, lcap, ctl,
-                         (const int32_t*)tw.mirror[0].as<int32_t>(), (const int32_t*)tw.sup[0].as<int32_t>(), stamp,
-                         stored, (int)lcap);
-    const unsigned gtick = std::min<unsigned>(gscan, 4 * (unsigned)c.num_cu);
+      hipLaunchKernelGGL(k_peel_tick0, dim3(gscan), dim3(256), 0, s, L, ctl,
+                         (const int32_t*)tw.cur[0].col.as<int32_t>(), (const int32_t*)tw.mirror[0].as<int32_t>(),
+                         (const int32_t*)tw.sup[0].as<int32_t>(), stamp, stored);
     int64_t t = 0;  // the last tick queued
     auto t_seen = std::chrono::steady_clock::now();
     int64_t seen = -1;
@@ -1350,7 +1507,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
       const int64_t alive_seen = __atomic_load_n(&hw[1], __ATOMIC_RELAXED);
       if (alive_seen > 0 && 2 * alive_seen <= stored) {
         const long long seq = next_host_seq();
-        hipLaunchKernelGGL(k_peel_drain, dim3(1), dim3(64), 0, s, t, ctl, (int)lcap, hw_dev, seq);
+        hipLaunchKernelGGL(k_peel_drain, dim3(1), dim3(64), 0, s, t, ctl, L, hw_dev, seq);
         GA_HIP(hipGetLastError());
         wait_host_word(&hw[4], seq, s);
         if (__atomic_load_n(&hw[2], __ATOMIC_RELAXED)) break;
@@ -1363,7 +1520,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
         ++t;
         const PeelGraph g = graph();
         timed([&] {
-          hipLaunchKernelGGL(k_peel_tick, dim3(gtick), dim3(256), 0, s, t, lists, lcap, ctl, g, stored, (int)lcap,
+          hipLaunchKernelGGL(k_peel_tick, dim3(gtick), dim3(256), 0, s, t, L, ctl, g, stored,
                              account ? 1 : 0, hw_dev);
         });
         continue;
@@ -1383,12 +1540,12 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     ++t;
     {
       const PeelGraph g = graph();
-      hipLaunchKernelGGL(k_peel_tick, dim3(gtick), dim3(256), 0, s, t, lists, lcap, ctl, g, stored, (int)lcap, 0,
+      hipLaunchKernelGGL(k_peel_tick, dim3(gtick), dim3(256), 0, s, t, L, ctl, g, stored, 0,
                          (long long*)nullptr);
       GA_HIP(hipGetLastError());
     }
     PeelCtl h{};
-    GA_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s));
+    GA_HIP(hipMemcpyAsync(&h, ctl, PEEL_CTL_HOST_BYTES, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
     out = h.dyn[t & 1];
     GA_REQUIRE(out.fin && !out.err, GA_ERR_STATE, "peeling did not finish (a removal list outgrew its capacity)");
@@ -1641,7 +1798,7 @@ size_t ktruss_workspace_bytes(int64_t n, int64_t nnz) {
   b += lease_bytes(sizeof(long long) * std::max<int64_t>(ceil_div(std::max<int64_t>(nnz, 1), KEEP_TILE), 1));
   b += mxm_workspace_bytes(n, nnz, true);
   b += sel_workspace_bytes(n, nnz);
-  b += lease_bytes(sizeof(int2) * 4 * (nnz / 2 + 64)) + lease_bytes(sizeof(PeelCtl));  // peeling lists
+  b += lease_bytes(sizeof(int4) * 4 * peel_lists_geometry(nnz).stride) + lease_bytes(sizeof(PeelCtl));  // peeling lists
   b += e;                                                                                   // stamps
   return b;
 }
