@@ -622,17 +622,28 @@ struct PeelDyn {
 // numbered segment by segment, the overflow's last.
 constexpr int PEEL_SEGS = 32;
 constexpr int PEEL_CNT_STRIDE = 32;  // ints from one counter to the next (128 bytes)
+
+// A list item is an edge (i, j), i > j: e = (its canonical entry (row i,
+// column j), the transposed entry, i, j), b = the row bounds of i and j
+// (rp[i], rp[i + 1], rp[j], rp[j + 1]).  Carrying the endpoints and their
+// bounds saves a round two dependent loads (both columns, then the row
+// pointers) before it can touch the rows.
+struct alignas(16) PeelItem {
+  int4 e, b;
+};
+
 struct PeelLists {
-  int4* items;     // list l starts at items + l * stride
+  PeelItem* items; // list l starts at items + l * stride
   int64_t stride;  // PEEL_SEGS * segcap + ovcap
   int segcap, ovcap;
 };
 
 // The peeling state in device memory.  Host-steered runs keep their counters
 // in `h` and use lists 0 and 1 (`ic` the next round's); device-driven runs
-// alternate `dyn[2]` (tick t reads dyn[(t - 1) & 1] and stores its own in
-// dyn[t & 1]) and rotate four lists: tick t consumes list t & 3, fills
-// (t + 1) & 3 and clears (t + 2) & 3, which no kernel in flight reads.
+// alternate `dyn[2]` (tick t starts from dyn[t & 1] and stores the next
+// tick's start in dyn[(t + 1) & 1]) and rotate four lists: tick t consumes
+// list t & 3, fills (t + 1) & 3 and clears (t + 2) & 3, which no kernel in
+// flight reads.
 struct PeelCtl {
   long long bytes;  // algorithmic bytes of the round launches (accounting on)
   PeelDyn h;
@@ -654,7 +665,7 @@ constexpr size_t PEEL_CTL_HOST_BYTES = offsetof(PeelCtl, lcnt);
 inline PeelLists peel_lists_geometry(int64_t nnz) {
   PeelLists L{};
   const int64_t lcap = nnz / 2 + 64;
-  L.segcap = (int)(lcap / PEEL_SEGS + 64);
+  L.segcap = (int)(lcap / (2 * PEEL_SEGS) + 64);
   L.ovcap = (int)lcap;
   L.stride = (int64_t)PEEL_SEGS * L.segcap + L.ovcap;
   return L;
@@ -682,8 +693,8 @@ __device__ inline PeelView peel_view(const PeelCtl* __restrict__ ctl, int l, con
 }
 
 // Item t (0 <= t < total, the same in every lane) of a list.
-__device__ inline int4 peel_item(const int4* __restrict__ list, const PeelView& v, const PeelLists& L, int64_t t,
-                                 int lane) {
+__device__ inline PeelItem peel_item(const PeelItem* __restrict__ list, const PeelView& v, const PeelLists& L,
+                                     int64_t t, int lane) {
   const unsigned long long b = __ballot(lane <= PEEL_SEGS && v.pre <= t);  // lane 0 always (pre = 0)
   const int s = 63 - __clzll(b);
   return list[(int64_t)s * L.segcap + (t - __shfl(v.pre, s))];
@@ -719,10 +730,7 @@ __device__ inline void peel_claim(int* __restrict__ lc, int n, const PeelLists& 
 }
 
 // Wavefront-aggregated append of one item per selected lane.
-// A list item is an edge (i, j), i > j: x = its canonical entry (row i,
-// column j), y = the transposed entry, z = i, w = j.  Carrying the endpoints
-// saves a round the dependent load of both columns before the row bounds.
-__device__ inline void peel_push(bool sel, int4 item, int4* __restrict__ list, int* __restrict__ lc,
+__device__ inline void peel_push(bool sel, const PeelItem& item, PeelItem* __restrict__ list, int* __restrict__ lc,
                                  const PeelLists& L, int lane) {
   const unsigned long long m = __ballot(sel);
   if (!m) return;
@@ -754,10 +762,10 @@ __device__ inline void peel_publish(long long* __restrict__ host, long long size
 // claims their list slots with one claim in its segment.  The body holds
 // __syncthreads: every thread of the workgroup calls it.
 constexpr int PEEL_SCAN_PER = 4;
-__device__ inline void peel_scan_body(const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
-                                      const int32_t* __restrict__ sup, int32_t* __restrict__ stamp, int64_t nnz,
-                                      int thr, int r, int4* __restrict__ list, int* __restrict__ lc,
-                                      const PeelLists& L) {
+__device__ inline void peel_scan_body(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                      const int32_t* __restrict__ mirror, const int32_t* __restrict__ sup,
+                                      int32_t* __restrict__ stamp, int64_t nnz, int thr, int r,
+                                      PeelItem* __restrict__ list, int* __restrict__ lc, const PeelLists& L) {
   __shared__ int s_wcnt[4];
   __shared__ int s_claim[3];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -804,7 +812,8 @@ __device__ inline void peel_scan_body(const int32_t* __restrict__ col, const int
         const int j = col[p], i = col[mm[u]];  // p: row i, column j
         stamp[p] = r;
         stamp[mm[u]] = r;
-        if (slot >= 0) list[slot] = make_int4((int32_t)p, mm[u], i, j);
+        if (slot >= 0)
+          list[slot] = PeelItem{make_int4((int32_t)p, mm[u], i, j), make_int4(rp[i], rp[i + 1], rp[j], rp[j + 1])};
       }
       at += __popcll(mk[u]);
     }
@@ -870,12 +879,12 @@ struct PeelEdge {
   int c, mc, ab, ae, arow, bb, be, brow, stride, m;
 };
 
-__device__ inline PeelEdge peel_edge(int4 item, const int32_t* __restrict__ rp, int cap) {
+__device__ inline PeelEdge peel_edge(const PeelItem& it, int cap) {
   PeelEdge e;
-  e.c = item.x;
-  e.mc = item.y;
-  const int i = item.z, j = item.w;
-  const int ib = rp[i], ie = rp[i + 1], jb = rp[j], je = rp[j + 1];
+  e.c = it.e.x;
+  e.mc = it.e.y;
+  const int i = it.e.z, j = it.e.w;
+  const int ib = it.b.x, ie = it.b.y, jb = it.b.z, je = it.b.w;
   const bool pi = (ie - ib) <= (je - jb);  // probe row i
   e.ab = pi ? ib : jb;
   e.ae = pi ? ie : je;
@@ -919,10 +928,11 @@ __device__ inline PeelQ peel_qload(int q, int qe, const int32_t* __restrict__ co
 // edges to push (x = -1: none).  st: the target row's sample (LDS) or the row
 // itself (global memory).
 __device__ inline void peel_probe(const PeelEdge& e, int q, const PeelQ& v, const int32_t* __restrict__ st,
-                                  const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
-                                  int32_t* __restrict__ sup, int32_t* __restrict__ stamp, int thr, int r,
-                                  int4& push_a, int4& push_b, long long& found) {
-  push_a = push_b = make_int4(-1, -1, -1, -1);
+                                  const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                  const int32_t* __restrict__ mirror, int32_t* __restrict__ sup,
+                                  int32_t* __restrict__ stamp, int thr, int r, PeelItem& push_a, PeelItem& push_b,
+                                  long long& found) {
+  push_a.e.x = push_b.e.x = -1;
   const int w = v.w, sa = v.sa, mq = v.mq;
   if (sa < r || w == e.brow) return;
   int lo = 0, hi = e.m;
@@ -952,13 +962,19 @@ __device__ inline void peel_probe(const PeelEdge& e, int q, const PeelQ& v, cons
   const int cid_b = w < e.brow ? pos : mp;
   if ((sa == r && cid_a < e.c) || (sb == r && cid_b < e.c)) return;  // another removed edge owns it
   ++found;
-  if (sa > r && atomicSub(&sup[cid_a], 1) == thr) {  // edge (arow, w): entry q lies in row arow
-    push_a = make_int4(cid_a, cid_a == q ? mq : q, max(e.arow, w), min(e.arow, w));
+  const bool xa = sa > r && atomicSub(&sup[cid_a], 1) == thr;  // edge (arow, w): entry q lies in row arow
+  const bool xb = sb > r && atomicSub(&sup[cid_b], 1) == thr;  // edge (brow, w): entry pos lies in row brow
+  if (!xa && !xb) return;
+  const int wb = rp[w], we = rp[w + 1];  // in flight with the list claim that follows
+  if (xa) {
+    push_a.e = make_int4(cid_a, cid_a == q ? mq : q, max(e.arow, w), min(e.arow, w));
+    push_a.b = e.arow > w ? make_int4(e.ab, e.ae, wb, we) : make_int4(wb, we, e.ab, e.ae);
     stamp[q] = r + 1;
     stamp[mq] = r + 1;
   }
-  if (sb > r && atomicSub(&sup[cid_b], 1) == thr) {  // edge (brow, w): entry pos lies in row brow
-    push_b = make_int4(cid_b, cid_b == pos ? mp : pos, max(e.brow, w), min(e.brow, w));
+  if (xb) {
+    push_b.e = make_int4(cid_b, cid_b == pos ? mp : pos, max(e.brow, w), min(e.brow, w));
+    push_b.b = e.brow > w ? make_int4(e.bb, e.be, wb, we) : make_int4(wb, we, e.bb, e.be);
     stamp[pos] = r + 1;
     stamp[mp] = r + 1;
   }
@@ -990,7 +1006,8 @@ struct PeelGraph {
   int32_t* truss;
 };
 
-__device__ inline void peel_round_body(const int4* __restrict__ cur, const PeelView& cv, int4* __restrict__ next,
+__device__ inline void peel_round_body(const PeelItem* __restrict__ cur, const PeelView& cv,
+                                       PeelItem* __restrict__ next,
                                        int* __restrict__ nlc, const PeelLists& L, int r, int thr, int tval,
                                        const PeelGraph& g, long long* __restrict__ bytes_ctr,
                                        int32_t* __restrict__ s_stage) {
@@ -1006,7 +1023,7 @@ __device__ inline void peel_round_body(const int4* __restrict__ cur, const PeelV
     for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < nv; v += nw) {
       const int64_t t = v / wpe;
       const int part = (int)(v - t * wpe);
-      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), g.rp, PEEL_WSTAGE);
+      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), PEEL_WSTAGE);
       if (part == 0 && lane == 0) {
         if (g.truss) {
           g.truss[g.eid[e.c]] = tval;
@@ -1021,25 +1038,27 @@ __device__ inline void peel_round_body(const int4* __restrict__ cur, const PeelV
       // chains of a binary search in global memory
       const bool staged = e.be - e.bb > 64 && (e.ae - e.ab > 64 || nc <= PEEL_STAGE_ROUND);
       const PeelEdge pe = staged ? e : peel_direct(e);
+      __threadfence_block();  // the previous edge's sample reads are done (waits for every load in flight)
+      // the first chunk's probe loads, in flight with the staging loads
       PeelQ pq = peel_qload(q0 + lane, e.ae, g.col, g.stamp, g.mirror);
-      __threadfence_block();  // the previous edge's sample reads are done
       if (staged)
         for (int u = lane; u < e.m; u += 64) st[u] = g.col[e.bb + u * e.stride];
       __threadfence_block();
       for (int qb = q0; qb < e.ae; qb += 64 * wpe) {
         const int q = qb + lane;
         if (qb != q0) pq = peel_qload(q, e.ae, g.col, g.stamp, g.mirror);
-        int4 pa, pb;
-        peel_probe(pe, q, pq, staged ? st : g.col + e.bb, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
-        peel_push(pa.x >= 0, pa, next, nlc, L, lane);
-        peel_push(pb.x >= 0, pb, next, nlc, L, lane);
+        PeelItem pa, pb;
+        peel_probe(pe, q, pq, staged ? st : g.col + e.bb, g.rp, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb,
+                   found);
+        peel_push(pa.e.x >= 0, pa, next, nlc, L, lane);
+        peel_push(pb.e.x >= 0, pb, next, nlc, L, lane);
       }
     }
   } else {
     const int split = max(PEEL_SPLIT, min(PEEL_SPLIT_MAX, (int)(gridDim.x / max(nc, 1))));
     for (int64_t b = blockIdx.x; b < (int64_t)nc * split; b += gridDim.x) {
       const int t = (int)(b / split), part = (int)(b % split);
-      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), g.rp, PEEL_BSTAGE);
+      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), PEEL_BSTAGE);
       if (part == 0 && threadIdx.x == 0) {
         if (g.truss) {
           g.truss[g.eid[e.c]] = tval;
@@ -1049,17 +1068,17 @@ __device__ inline void peel_round_body(const int4* __restrict__ cur, const PeelV
       }
       const int q0 = e.ab + part * 256;
       if (q0 >= e.ae) continue;  // uniform in the workgroup
-      PeelQ pq = peel_qload(q0 + (int)threadIdx.x, e.ae, g.col, g.stamp, g.mirror);
-      __syncthreads();  // the previous edge's sample reads are done
+      __syncthreads();  // the previous edge's sample reads are done (waits for every load in flight)
+      PeelQ pq = peel_qload(q0 + (int)threadIdx.x, e.ae, g.col, g.stamp, g.mirror);  // in flight with the staging
       for (int u = threadIdx.x; u < e.m; u += blockDim.x) s_stage[u] = g.col[e.bb + u * e.stride];
       __syncthreads();
       for (int qb = q0; qb < e.ae; qb += split * 256) {
         const int q = qb + threadIdx.x;
         if (qb != q0) pq = peel_qload(q, e.ae, g.col, g.stamp, g.mirror);
-        int4 pa, pb;
-        peel_probe(e, q, pq, s_stage, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
-        peel_push(pa.x >= 0, pa, next, nlc, L, lane);
-        peel_push(pb.x >= 0, pb, next, nlc, L, lane);
+        PeelItem pa, pb;
+        peel_probe(e, q, pq, s_stage, g.rp, g.col, g.mirror, g.sup, g.stamp, thr, r, pa, pb, found);
+        peel_push(pa.e.x >= 0, pa, next, nlc, L, lane);
+        peel_push(pb.e.x >= 0, pb, next, nlc, L, lane);
       }
     }
   }
@@ -1071,11 +1090,12 @@ __device__ inline void peel_round_body(const int4* __restrict__ cur, const PeelV
 
 // ---- host-steered runs: a kernel per round or scan, then k_peel_post ------
 __global__ __launch_bounds__(256) void k_peel_scan(PeelLists L, PeelCtl* __restrict__ ctl,
-                                                   const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
+                                                   const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                   const int32_t* __restrict__ mirror,
                                                    const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
                                                    int64_t nnz) {
   const int ic = ctl->ic;
-  peel_scan_body(col, mirror, sup, stamp, nnz, ctl->h.k - 2, ctl->h.r + 1, L.items + ic * L.stride,
+  peel_scan_body(rp, col, mirror, sup, stamp, nnz, ctl->h.k - 2, ctl->h.r + 1, L.items + ic * L.stride,
                  peel_lc(ctl, ic), L);
 }
 
@@ -1118,17 +1138,18 @@ __global__ void k_peel_set_k(PeelCtl* __restrict__ ctl, int k, int stable_step) 
 }
 
 // ---- device-driven runs: one kernel per tick --------------------------------
-// The state of tick t: the state of tick t - 1 moved on by what that tick did
-// -- a round (its list, of np items, was not empty: counted) or the stable
-// step (its list was empty: recorded, k moved on; that tick scanned the new
-// threshold).  Tick 0 is the first threshold's scan.
-__device__ inline PeelDyn peel_tick_state(const PeelCtl* __restrict__ ctl, int64_t t, int np, int& rec_kidx,
+// dyn[t & 1] is the state tick t starts from.  Tick t's workgroup 0 stores
+// the state tick t + 1 starts from -- tick t's start state moved on by what
+// tick t does with its list of nc items: a round (nc > 0: counted) or the
+// stable step (nc == 0: recorded, k moved on; tick t scans the new
+// threshold) -- so a tick reads one state and one list's counters.  Tick 0
+// is the first threshold's scan; ticks 0 and 1 start from the initial state.
+__device__ inline PeelDyn peel_after_tick(const PeelCtl* __restrict__ ctl, PeelDyn d, int nc, int& rec_kidx,
                                           long long& rec_alive, int& rec_passes) {
-  PeelDyn d = ctl->dyn[(t - 1) & 1];
   rec_kidx = -1;
-  if (t <= 1 || d.fin) return d;
-  if (np > 0) {
-    peel_count_round(d, ctl->all_k, np);
+  if (d.fin) return d;
+  if (nc > 0) {
+    peel_count_round(d, ctl->all_k, nc);
   } else {
     if (ctl->all_k) {
       if (d.alive > 0) {
@@ -1154,25 +1175,25 @@ __global__ __launch_bounds__(256) void k_peel_tick(int64_t t, PeelLists L, PeelC
                                                    int64_t nnz, int account, long long* __restrict__ host) {
   __shared__ int32_t s_stage[PEEL_BSTAGE];
   const int lane = threadIdx.x & 63;
-  const PeelView pv = peel_view(ctl, (int)((t - 1) & 3), L, lane);  // the previous tick's list
-  const PeelView cv = peel_view(ctl, (int)(t & 3), L, lane);        // this tick's
-  int rec_kidx;
-  long long rec_alive = 0;
-  int rec_passes = 0;
-  PeelDyn d = peel_tick_state(ctl, t, pv.total, rec_kidx, rec_alive, rec_passes);
+  const PeelDyn d = ctl->dyn[t & 1];
+  const PeelView cv = peel_view(ctl, (int)(t & 3), L, lane);
   if (blockIdx.x == 0 && threadIdx.x <= PEEL_SEGS) ctl->lcnt[(t + 2) & 3][threadIdx.x][0] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (cv.over || pv.over) d.err = d.fin = 1;
-    ctl->dyn[t & 1] = d;
+    int rec_kidx;
+    long long rec_alive = 0;
+    int rec_passes = 0;
+    PeelDyn dn = peel_after_tick(ctl, d, cv.total, rec_kidx, rec_alive, rec_passes);
+    if (cv.over) dn.err = dn.fin = 1;
+    ctl->dyn[(t + 1) & 1] = dn;
     if (rec_kidx >= 0) {
       ctl->nnz_at[rec_kidx] = rec_alive;
       ctl->passes_at[rec_kidx] = rec_passes;
     }
     if ((t & 3) == 0 || d.fin) peel_publish(host, cv.total, d.alive, d.fin, t, 0);
   }
-  if (d.fin) return;
+  if (d.fin || cv.over) return;
   const int nl = (int)((t + 1) & 3);
-  int4* next = L.items + nl * L.stride;
+  PeelItem* next = L.items + nl * L.stride;
   if (cv.total > 0) {
     peel_round_body(L.items + (t & 3) * L.stride, cv, next, peel_lc(ctl, nl), L, d.r + 1, d.k - 2, d.k - 1, g,
                     account ? &ctl->bytes : nullptr, s_stage);
@@ -1180,7 +1201,7 @@ __global__ __launch_bounds__(256) void k_peel_tick(int64_t t, PeelLists L, PeelC
     int k = d.k, kidx = d.kidx, fin = 0;  // the k-truss is stable: scan the next threshold
     peel_next_k(ctl, d.alive, k, kidx, fin);
     if (!fin && d.alive > 0) {
-      peel_scan_body(g.col, g.mirror, g.sup, g.stamp, nnz, k - 2, d.r + 1, next, peel_lc(ctl, nl), L);
+      peel_scan_body(g.rp, g.col, g.mirror, g.sup, g.stamp, nnz, k - 2, d.r + 1, next, peel_lc(ctl, nl), L);
       // accounting: the scan reads mirror, support and stamp of every stored entry
       if (account && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&ctl->bytes, 12ull * nnz);
     }
@@ -1193,18 +1214,13 @@ __global__ __launch_bounds__(256) void k_peel_tick(int64_t t, PeelLists L, PeelC
 __global__ void k_peel_drain(int64_t t, PeelCtl* __restrict__ ctl, PeelLists L, long long* __restrict__ host,
                              long long seq) {
   if (blockIdx.x != 0 || threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
-  const PeelView pv = peel_view(ctl, (int)(t & 3), L, lane);
-  const PeelView cv = peel_view(ctl, (int)((t + 1) & 3), L, lane);
-  int rk;
-  long long ra;
-  int rp_;
-  const PeelDyn d = peel_tick_state(ctl, t + 1, pv.total, rk, ra, rp_);
-  if (lane != 0) return;
+  const PeelView cv = peel_view(ctl, (int)((t + 1) & 3), L, threadIdx.x);
+  if (threadIdx.x != 0) return;
+  const PeelDyn d = ctl->dyn[(t + 1) & 1];
   ctl->cr = d.r;
   ctl->clist = (int)((t + 1) & 3);
   ctl->ccnt = d.fin ? 0 : cv.total;
-  peel_publish(host, ctl->ccnt, d.alive, d.fin || pv.over || cv.over, t, seq);
+  peel_publish(host, ctl->ccnt, d.alive, d.fin || cv.over, t, seq);
 }
 
 // Host-steered runs before a compaction (a k boundary: no list pending).
@@ -1227,22 +1243,22 @@ __global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __r
 }
 
 // After a compaction: the pending list's entries move to their new positions
-// and keep their stamp cr + 1 (both directions); every other stamp was reset.
-// Segment by segment (a segment's items are its first min(count, capacity)
-// slots).
+// (rows of the compacted graph, row pointers nrp) and keep their stamp cr + 1
+// (both directions); every other stamp was reset.  Segment by segment (a
+// segment's items are its first min(count, capacity) slots).
 __global__ void k_peel_remap(PeelLists L, const PeelCtl* __restrict__ ctl, const int32_t* __restrict__ newpos,
-                             int32_t* __restrict__ stamp) {
+                             const int32_t* __restrict__ nrp, int32_t* __restrict__ stamp) {
   if (ctl->ccnt == 0) return;
   const int l = ctl->clist, r1 = ctl->cr + 1;
-  int4* list = L.items + l * L.stride;
+  PeelItem* list = L.items + l * L.stride;
   for (int s = 0; s <= PEEL_SEGS; ++s) {
     const int raw = ctl->lcnt[l][s][0];
     const int n = s < PEEL_SEGS ? min(raw, L.segcap) : min(raw, L.ovcap);
-    int4* seg = list + (int64_t)s * L.segcap;
+    PeelItem* seg = list + (int64_t)s * L.segcap;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-      const int4 it = seg[t];
-      const int c = newpos[it.x], m = newpos[it.y];
-      seg[t] = make_int4(c, m, it.z, it.w);  // the endpoints do not move
+      const PeelItem it = seg[t];
+      const int c = newpos[it.e.x], m = newpos[it.e.y], i = it.e.z, j = it.e.w;  // the endpoints do not move
+      seg[t] = PeelItem{make_int4(c, m, i, j), make_int4(nrp[i], nrp[i + 1], nrp[j], nrp[j + 1])};
       stamp[c] = r1;
       stamp[m] = r1;
     }
@@ -1251,10 +1267,11 @@ __global__ void k_peel_remap(PeelLists L, const PeelCtl* __restrict__ ctl, const
 
 // Tick 0 of a device-driven run: the first threshold's scan into list 1.
 __global__ __launch_bounds__(256) void k_peel_tick0(PeelLists L, PeelCtl* __restrict__ ctl,
-                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ mirror,
+                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ mirror,
                                                     const int32_t* __restrict__ sup, int32_t* __restrict__ stamp,
                                                     int64_t nnz) {
-  peel_scan_body(col, mirror, sup, stamp, nnz, ctl->dyn[0].k - 2, ctl->dyn[0].r + 1, L.items + L.stride,
+  peel_scan_body(rp, col, mirror, sup, stamp, nnz, ctl->dyn[0].k - 2, ctl->dyn[0].r + 1, L.items + L.stride,
                  peel_lc(ctl, 1), L);
 }
 
@@ -1292,10 +1309,10 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   GA_REQUIRE(all_k || host_steer || nk <= TRUSS_MAX_K, GA_ERR_LIMIT, "at most 64 k values in one k-truss run");
   PeelLists L = peel_lists_geometry(nnz0);
   const int64_t lcap = nnz0 / 2 + 64;  // canonical entries: every list holds at most this many
-  tw.plist.ensure(sizeof(int4) * 4 * L.stride);
+  tw.plist.ensure(sizeof(PeelItem) * 4 * L.stride);
   tw.pcnt.ensure(sizeof(PeelCtl));
   tw.stamp.ensure(sizeof(int32_t) * std::max<int64_t>(nnz0, 1));
-  L.items = tw.plist.as<int4>();
+  L.items = tw.plist.as<PeelItem>();
   auto* ctl = tw.pcnt.as<PeelCtl>();
   int32_t* stamp = tw.stamp.as<int32_t>();
   const int64_t passes0 = (all_k && nnz0 == 0) ? 0 : 1;  // a decomposition of no entries runs no product
@@ -1382,7 +1399,8 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
                        tw.mirror[cur ^ 1].as<int32_t>(), (const TrussDevState*)nullptr);
     fill_u32(stamp, (uint32_t)PEEL_ALIVE, alive, s);
     hipLaunchKernelGGL(k_peel_remap, dim3(grid_for(lcap, 256, 4096)), dim3(256), 0, s, L,
-                       (const PeelCtl*)ctl, (const int32_t*)tw.newpos.as<int32_t>(), stamp);
+                       (const PeelCtl*)ctl, (const int32_t*)tw.newpos.as<int32_t>(),
+                       (const int32_t*)tw.cur[cur ^ 1].rp.as<int32_t>(), stamp);
     GA_HIP(hipGetLastError());
     cur ^= 1;
     stored = alive;
@@ -1411,7 +1429,8 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     auto scan = [&]() {
       const long long seq = next_host_seq();
       hipLaunchKernelGGL(k_peel_scan, dim3(gscan), dim3(256), 0, s, L, ctl,
-                         (const int32_t*)tw.cur[cur].col.as<int32_t>(), (const int32_t*)tw.mirror[cur].as<int32_t>(),
+                         (const int32_t*)tw.cur[cur].rp.as<int32_t>(), (const int32_t*)tw.cur[cur].col.as<int32_t>(),
+                         (const int32_t*)tw.mirror[cur].as<int32_t>(),
                          (const int32_t*)tw.sup[cur].as<int32_t>(),
                          stamp, stored);
       hipLaunchKernelGGL(k_peel_post, dim3(1), dim3(64), 0, s, ctl, 1, L, hw_dev, seq);
@@ -1496,7 +1515,8 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     __atomic_store_n(&hw[3], 0ll, __ATOMIC_RELEASE);
     if (nnz0 > 0)  // tick 0: the first threshold, into list 1
       hipLaunchKernelGGL(k_peel_tick0, dim3(gscan), dim3(256), 0, s, L, ctl,
-                         (const int32_t*)tw.cur[0].col.as<int32_t>(), (const int32_t*)tw.mirror[0].as<int32_t>(),
+                         (const int32_t*)tw.cur[0].rp.as<int32_t>(), (const int32_t*)tw.cur[0].col.as<int32_t>(),
+                         (const int32_t*)tw.mirror[0].as<int32_t>(),
                          (const int32_t*)tw.sup[0].as<int32_t>(), stamp, stored);
     int64_t t = 0;  // the last tick queued
     auto t_seen = std::chrono::steady_clock::now();
@@ -1547,7 +1567,7 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     PeelCtl h{};
     GA_HIP(hipMemcpyAsync(&h, ctl, PEEL_CTL_HOST_BYTES, hipMemcpyDeviceToHost, s));
     GA_HIP(hipStreamSynchronize(s));
-    out = h.dyn[t & 1];
+    out = h.dyn[(t + 1) & 1];  // the state after the last tick
     GA_REQUIRE(out.fin && !out.err, GA_ERR_STATE, "peeling did not finish (a removal list outgrew its capacity)");
     for (int i = 0; i < (all_k ? 0 : nk); ++i) {
       if (nnz_at) nnz_at[i] = h.nnz_at[i];
@@ -1798,7 +1818,7 @@ size_t ktruss_workspace_bytes(int64_t n, int64_t nnz) {
   b += lease_bytes(sizeof(long long) * std::max<int64_t>(ceil_div(std::max<int64_t>(nnz, 1), KEEP_TILE), 1));
   b += mxm_workspace_bytes(n, nnz, true);
   b += sel_workspace_bytes(n, nnz);
-  b += lease_bytes(sizeof(int4) * 4 * peel_lists_geometry(nnz).stride) + lease_bytes(sizeof(PeelCtl));  // peeling lists
+  b += lease_bytes(sizeof(PeelItem) * 4 * peel_lists_geometry(nnz).stride) + lease_bytes(sizeof(PeelCtl));  // peeling lists
   b += e;                                                                                   // stamps
   return b;
 }
