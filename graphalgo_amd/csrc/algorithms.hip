@@ -700,6 +700,19 @@ __device__ inline PeelItem peel_item(const PeelItem* __restrict__ list, const Pe
   return list[(int64_t)s * L.segcap + (t - __shfl(v.pre, s))];
 }
 
+// Item t of a list, t different per lane (0 <= t < total): the segment by a
+// binary search over the wavefront's prefix (every lane runs the same steps).
+__device__ inline PeelItem peel_item_lane(const PeelItem* __restrict__ list, const PeelView& v, const PeelLists& L,
+                                          int64_t t) {
+  int lo = 0;  // the last segment s with pre[s] <= t (segment PEEL_SEGS: the overflow)
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    const int cand = min(lo + step, PEEL_SEGS);
+    if (__shfl(v.pre, cand) <= t) lo = cand;
+  }
+  return list[(int64_t)lo * L.segcap + (t - __shfl(v.pre, lo))];
+}
+
 // Sums of a list's segment sizes from one thread (one-thread kernels).
 __device__ inline int peel_total1(const PeelCtl* __restrict__ ctl, int l, const PeelLists& L, bool* over) {
   int tot = 0;
@@ -1014,7 +1027,23 @@ __device__ inline void peel_round_body(const PeelItem* __restrict__ cur, const P
   const int lane = threadIdx.x & 63;
   const int nc = cv.total;
   long long bytes = 0, found = 0;
-  if (nc > PEEL_SPLIT_MAX_EDGES) {
+  if (thr <= 1) {
+    // supports below 1: no triangle holds an edge of the round, so there is
+    // nothing to intersect or decrement (the first round of k = 3); only the
+    // truss numbers are written, an edge per thread
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x; b < nc; b += stride) {  // uniform in the wavefront
+      const int64_t t = b + threadIdx.x;
+      const PeelItem it = peel_item_lane(cur, cv, L, min<int64_t>(t, nc - 1));
+      if (t < nc) {
+        if (g.truss) {
+          g.truss[g.eid[it.e.x]] = tval;
+          g.truss[g.eid[it.e.y]] = tval;
+        }
+        if (bytes_ctr) bytes += 28;
+      }
+    }
+  } else if (nc > PEEL_SPLIT_MAX_EDGES) {
     int32_t* st = s_stage + (threadIdx.x >> 6) * PEEL_WSTAGE;
     const int64_t wpb = blockDim.x >> 6;
     const int64_t nw = (int64_t)gridDim.x * wpb;
