@@ -700,17 +700,22 @@ __device__ inline PeelItem peel_item(const PeelItem* __restrict__ list, const Pe
   return list[(int64_t)s * L.segcap + (t - __shfl(v.pre, s))];
 }
 
-// Item t of a list, t different per lane (0 <= t < total): the segment by a
-// binary search over the wavefront's prefix (every lane runs the same steps).
-__device__ inline PeelItem peel_item_lane(const PeelItem* __restrict__ list, const PeelView& v, const PeelLists& L,
-                                          int64_t t) {
+// The slot of item t of a list, t different per lane (0 <= t < total): the
+// segment by a binary search over the wavefront's prefix (every lane runs the
+// same steps).
+__device__ inline int64_t peel_slot_lane(const PeelView& v, const PeelLists& L, int64_t t) {
   int lo = 0;  // the last segment s with pre[s] <= t (segment PEEL_SEGS: the overflow)
 #pragma unroll
   for (int step = 32; step >= 1; step >>= 1) {
     const int cand = min(lo + step, PEEL_SEGS);
     if (__shfl(v.pre, cand) <= t) lo = cand;
   }
-  return list[(int64_t)lo * L.segcap + (t - __shfl(v.pre, lo))];
+  return (int64_t)lo * L.segcap + (t - __shfl(v.pre, lo));
+}
+
+__device__ inline PeelItem peel_item_lane(const PeelItem* __restrict__ list, const PeelView& v, const PeelLists& L,
+                                          int64_t t) {
+  return list[peel_slot_lane(v, L, t)];
 }
 
 // Sums of a list's segment sizes from one thread (one-thread kernels).
@@ -1273,21 +1278,22 @@ __global__ void k_peel_drop_dead(const int32_t* __restrict__ stamp, int32_t* __r
 
 // After a compaction: the pending list's entries move to their new positions
 // (rows of the compacted graph, row pointers nrp) and keep their stamp cr + 1
-// (both directions); every other stamp was reset.  Segment by segment (a
-// segment's items are its first min(count, capacity) slots).
+// (both directions); every other stamp was reset.  An item per thread.
 __global__ void k_peel_remap(PeelLists L, const PeelCtl* __restrict__ ctl, const int32_t* __restrict__ newpos,
                              const int32_t* __restrict__ nrp, int32_t* __restrict__ stamp) {
-  if (ctl->ccnt == 0) return;
+  const int n = ctl->ccnt;
+  if (n == 0) return;
   const int l = ctl->clist, r1 = ctl->cr + 1;
   PeelItem* list = L.items + l * L.stride;
-  for (int s = 0; s <= PEEL_SEGS; ++s) {
-    const int raw = ctl->lcnt[l][s][0];
-    const int n = s < PEEL_SEGS ? min(raw, L.segcap) : min(raw, L.ovcap);
-    PeelItem* seg = list + (int64_t)s * L.segcap;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-      const PeelItem it = seg[t];
+  const PeelView v = peel_view(ctl, l, L, threadIdx.x & 63);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b = blockIdx.x * (int64_t)blockDim.x; b < n; b += stride) {  // uniform in the wavefront
+    const int64_t t = b + threadIdx.x;
+    const int64_t slot = peel_slot_lane(v, L, min<int64_t>(t, n - 1));
+    if (t < n) {
+      const PeelItem it = list[slot];
       const int c = newpos[it.e.x], m = newpos[it.e.y], i = it.e.z, j = it.e.w;  // the endpoints do not move
-      seg[t] = PeelItem{make_int4(c, m, i, j), make_int4(nrp[i], nrp[i + 1], nrp[j], nrp[j + 1])};
+      list[slot] = PeelItem{make_int4(c, m, i, j), make_int4(nrp[i], nrp[i + 1], nrp[j], nrp[j + 1])};
       stamp[c] = r1;
       stamp[m] = r1;
     }
