@@ -1313,7 +1313,22 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
       q0 = q1;
     }
   }
-  if (cur_g >= 0) flush();
+  if (BATCHED || row_group) {  // several targets (graphs of a batch, or per-row groups)
+    if (cur_g >= 0) flush();
+  } else {
+    // one target: the workgroup's waves add up in LDS and one atomic goes to
+    // sums[0] (atomics on one address serialise, ~11 ns each: a small graph's
+    // count had one per wavefront)
+    __shared__ unsigned long long s_red[WR_WAVES];
+    const long long t = wave_reduce_sum(lsum);
+    if (lane == 0) s_red[wv] = (unsigned long long)t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long tot = 0;
+      for (int w = 0; w < WR_WAVES; ++w) tot += s_red[w];
+      if (tot) atomicAdd(&sums[0], tot);
+    }
+  }
   if (wedge_counter && lane == 0 && wedges) atomicAdd(wedge_counter, wedges);
 }
 
