@@ -693,7 +693,8 @@ def bench_ktruss_one(args) -> dict:
         c = pool[sc]
         graphs.append((f"rmat{sc}_ef{c['edge_factor']}_s{c['seed']}",
                        dict(scale=sc, edge_factor=c["edge_factor"], seed=c["seed"])))
-    variants = {"default": {}, "host_loop": {"truss_loop": 0}, "device_steps": {"truss_loop": 1},
+    variants = {"default": {}, "peel_host_steered": {"truss_loop": 4}, "host_loop": {"truss_loop": 0},
+                "device_steps": {"truss_loop": 1},
                 "graph_replay": {"truss_loop": 2}, "full_passes": {"selective_support": 0, "support_tri": 0},
                 "no_tri": {"support_tri": 0}, "tri_full_passes": {"selective_support": 0, "support_tri": 1}}
     if args.ktruss_variants:
