@@ -105,6 +105,8 @@ GA_SIGNATURES = {
                            C.POINTER(I32P), C.POINTER(I32P), C.POINTER(C.c_void_p), I64P]),
     "ga_scale": (C.c_int, [C.POINTER(GaCsr), C.c_int, C.c_double, C.c_void_p]),
     "ga_free_host": (None, [C.c_void_p]),
+    "ga_host_alloc": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p)]),
+    "ga_host_free": (C.c_int, [C.c_void_p]),
     "ga_triangle_count": (C.c_int, [C.POINTER(GaCsr), I64P]),
     "ga_triangle_count_lu": (C.c_int, [C.POINTER(GaCsr), I64P]),
     "ga_triangle_count_vertex": (C.c_int, [C.POINTER(GaCsr), I64P]),

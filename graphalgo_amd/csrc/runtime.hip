@@ -482,6 +482,21 @@ ga_status ga_check_csr(const ga_csr* A) {
 
 void ga_free_host(void* p) { std::free(p); }
 
+ga_status ga_host_alloc(int64_t bytes, void** ptr) {
+  return guarded([&] {
+    GA_REQUIRE(ptr && bytes > 0, GA_ERR_ARG, "ga_host_alloc needs a positive size and an output pointer");
+    *ptr = nullptr;
+    ctx();  // the device is initialised
+    GA_HIP(hipHostMalloc(ptr, (size_t)bytes, hipHostMallocDefault));
+  });
+}
+
+ga_status ga_host_free(void* ptr) {
+  return guarded([&] {
+    if (ptr) GA_HIP(hipHostFree(ptr));
+  });
+}
+
 ga_status ga_stream_copy(int64_t bytes, int reps, double* ms_per_copy) {
   return guarded([&] {
     GA_REQUIRE(bytes > 0 && bytes % 16 == 0 && reps > 0, GA_ERR_ARG, "bytes must be a positive multiple of 16");

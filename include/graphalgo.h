@@ -127,6 +127,15 @@ ga_status ga_dense_mult_masked(const ga_csr* M, ga_dtype dtype, const void* A, c
 
 void ga_free_host(void* p);
 
+/* Page-locked host memory for result buffers a caller keeps across calls:
+ * device-to-host copies into it run at DMA speed with no page faults (a truss
+ * decomposition of R-MAT 16 reads back 7 MB of truss numbers: 0.5 ms into
+ * fresh pageable memory).  No counterpart in the reference, whose results are
+ * host arrays already; for callers that read results back often (the Python
+ * mirror's resident graphs).  Free with ga_host_free. */
+ga_status ga_host_alloc(int64_t bytes, void** ptr);
+ga_status ga_host_free(void* ptr);
+
 /* ---- triangle counting ------------------------------------------------- */
 /* A: symmetric pattern, sorted rows.  T = sum of C<L> = L*L, L = tril(A,-1).
  * Replaces triangle_counting_masked_sandia (src/main.cpp:65-86). */

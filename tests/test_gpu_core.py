@@ -178,3 +178,21 @@ def test_truss_edge_cases(gpu, G):
     assert (truss == 2).all() and kmax == 2
     truss, kmax, _ = gpu.truss_decomposition(_complete(G, 9))
     assert (truss == 9).all() and kmax == 9
+
+
+def test_truss_numbers_in_page_locked_buffers(gpu, G, oracle):
+    """Resident-graph truss numbers come back in page-locked buffers
+    (ga_host_alloc): a buffer is reused only once the array handed out from it
+    is gone, and arrays that outlive the graph stay valid."""
+    g = G.rmat(10, 8, seed=3)
+    want = oracle.truss_decomposition(g)[0]
+    R = gpu.Graph(g)
+    t1, _, _ = R.truss()
+    t2, _, _ = R.truss()  # t1 still alive: a second buffer
+    assert t1.ctypes.data != t2.ctypes.data
+    addr = t2.ctypes.data
+    del t2
+    t3, _, _ = R.truss()  # t2's buffer again
+    assert t3.ctypes.data == addr
+    R.close()
+    assert np.array_equal(t1, want) and np.array_equal(t3, want)
