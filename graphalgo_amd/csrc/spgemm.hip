@@ -38,6 +38,10 @@ constexpr int64_t MXM_BITMAP_MAX_COLS = int64_t(1) << 18;  // 32 KB bitmap
 constexpr int MXM_G = GA_MXM_G;  // lanes that read adjacent B columns (one 64-byte run)
 constexpr int MXM_K = GA_MXM_K;  // wedges per lane per trip (gathers in flight)
 constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up to 2^17 columns
+// wider matrices: a filter of this many bits (column c sets bit c mod it),
+// small enough to keep eight wavefronts per SIMD; a search of the mask row
+// confirms its hits
+constexpr int64_t MXM_FILTER_COLS = int64_t(1) << 16;
 // Items of the support pass and of pattern products: 512 A entries and an
 // LDS mask accumulator of 4096 entries.  On R-MAT 16 the support kernel takes
 // 344 us per pass at 256 / 1024, 233 us at 512 / 2048 and 212 us at
@@ -68,7 +72,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
     unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ item_row, const int32_t* __restrict__ item_t0,
     const int32_t* __restrict__ n_items_ptr, unsigned int* __restrict__ work_counter,
-    unsigned long long* __restrict__ wedge_counter, int bitmap_words, const int32_t* __restrict__ Mend,
+    unsigned long long* __restrict__ wedge_counter, int bitmap_words, int bm_and, const int32_t* __restrict__ Mend,
     int trunc, int chunk, int direct_rows, const int32_t* __restrict__ mirror, const int32_t* __restrict__ opos,
     unsigned long long* __restrict__ row_work, const int32_t* __restrict__ skip) {
   // skip: a device flag that turns the launch into a no-op (device-driven
@@ -100,6 +104,10 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
   const int lane = tid & 63;
   const int wv = tid >> 6;
   const bool use_bm = RANK || bitmap_words > 0;
+  // bm_and = -1: the bitmap holds a bit per column (exact membership);
+  // otherwise column c owns bit c & bm_and of a smaller bitmap, a filter
+  // whose hits a search of the mask row confirms
+  const bool bm_exact = bm_and == -1;
   for (int q = tid; q < bitmap_words + BM_EXTRA; q += MXM_THREADS) s_bitmap[q] = 0u;
   // direct mode: items 0..n_list-1 are the listed extra chunks of long rows
   // (the heaviest work goes first), then item n_list + r is row r's first
@@ -268,7 +276,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
         s_mcol[p] = c;
       }
       if (MODE != 1 && local) s_acc[p] = T(0);
-      if (use_bm) atomicOr(&s_bitmap[c >> 5], 1u << (c & 31));
+      if (use_bm) atomicOr(&s_bitmap[(c & bm_and) >> 5], 1u << (c & 31));
     }
     // stage the item's entries: B row start and length, A value
     const int ne = t1 - t0;
@@ -409,7 +417,11 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
           for (int u = 0; u < MXM_K; ++u) {
             const int c = cc[u];
             if (c < 0) continue;
-            if (use_bm && !((s_bitmap[c >> 5] >> (c & 31)) & 1u)) continue;
+            if (use_bm && !((s_bitmap[(c & bm_and) >> 5] >> (c & 31)) & 1u)) continue;
+            if (MODE == 1 && use_bm && bm_exact) {  // a sum needs only membership, which the bit is
+              accumulate(0, (PAT_A ? T(1) : s_aval[ee[u]]) * (PAT_B ? T(1) : Bval[jj[u]]));
+              continue;
+            }
             int lo = 0, hi = mnz;
             bool found;
             if (LOCAL) {
@@ -485,7 +497,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
     if (use_bm) {
       for (int p = tid; p < mnz; p += MXM_THREADS) {
         const int c = (!RANK && local) ? s_mcol[p] : Mcol[ms + p];
-        s_bitmap[c >> 5] = 0u;
+        s_bitmap[(c & bm_and) >> 5] = 0u;
       }
     }
     if (tid == 0 && wedge_counter) atomicAdd(wedge_counter, (unsigned long long)total);
@@ -1700,7 +1712,11 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   auto krows = k_mxm_rows<T, PA, PB, MODE, BATCHED>;
   // rank lookup when the bitmap is at most 16 KB and every mask row fits
   // the 16-bit word rank bases
-  const bool rank = bitmap_words > 0 && bitmap_words <= MXM_RANK_MAX_WORDS && w.max_mask_row < 65536;
+  // a bitmap narrower than the mask's columns (MXM_BITMAP_MAX_COLS bits for
+  // wider matrices) is a filter: column c sets bit c & bm_and
+  const int bm_and = (int64_t)M.ncols <= 32 * (int64_t)bitmap_words ? -1 : 32 * bitmap_words - 1;
+  const bool rank = bitmap_words > 0 && bitmap_words <= MXM_RANK_MAX_WORDS && w.max_mask_row < 65536 &&
+                    bm_and == -1;
   auto kitems = rank ? k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, true>
                      : k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, false>;
   const int bm_words = bitmap_words + (rank ? 1 : 0);  // rank: one zero word for dropped wedges
@@ -1805,7 +1821,7 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
                      static_cast<const T*>(A.val), B.rp, B.col, static_cast<const T*>(B.val), M.rp, M.col,
                      static_cast<T*>(Cval), sums, row_group, gbase, w.item_row.as<int32_t>(), w.item_t0.as<int32_t>(),
                      direct ? (const int32_t*)n_extra : (const int32_t*)(w.start.as<int32_t>() + m),
-                     w.counter.as<unsigned int>(), wc, bitmap_words, Mend, trunc, chunk_i, direct ? (int)m : 0,
+                     w.counter.as<unsigned int>(), wc, bitmap_words, bm_and, Mend, trunc, chunk_i, direct ? (int)m : 0,
                      mirror, opos, rw_cur, w.skip);
   GA_HIP(hipGetLastError());
   if (c.prof_events_on) {
@@ -1900,7 +1916,7 @@ __global__ void k_count_wedges(const int32_t* __restrict__ Arp, const int32_t* _
 void mxm_batched_sum_dev(const DCsr& L, const int32_t* row_group, const int32_t* gbase, int64_t max_cols,
                          unsigned long long* sums, MxmWork& w, hipStream_t s) {
   if (L.nrows == 0 || L.nnz == 0) return;
-  const int bm = (max_cols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(max_cols, 32) : 0;
+  const int bm = (max_cols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(max_cols, 32) : (int)(MXM_FILTER_COLS / 32);
   if (ctx().prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(L.nrows, 4, 4096)), dim3(256), 0, s, L.rp, L.col, L.rp,
                        L.nrows, ctx().prof_dev.as<unsigned long long>() + 1, row_group, gbase);
@@ -2014,7 +2030,7 @@ void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, Mxm
   if (n == 0 || C.nnz == 0) return;
   w.mend.ensure(sizeof(int32_t) * n);
   int32_t* split = w.mend.as<int32_t>();
-  const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : 0;
+  const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : (int)(MXM_FILTER_COLS / 32);
   Context& c = ctx();
   if (c.prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
@@ -2117,7 +2133,7 @@ void support_selected_dev(const DCsr& C, const int32_t* mirror, const uint8_t* d
   // M'.nnz is only a bound here (the exact count stays on the device); the
   // algorithmic accounting is the reference's full product, M = C
   const DCsr M{n, n, C.nnz, sw.mrp.as<int32_t>(), sw.mcol.as<int32_t>(), nullptr};
-  const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : 0;
+  const int bm = (C.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(C.ncols, 32) : (int)(MXM_FILTER_COLS / 32);
   Context& c = ctx();
   if (c.prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(n, 4, 4096)), dim3(256), 0, s, C.rp, C.col, C.rp, n,
@@ -2146,7 +2162,7 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
   if (ctx().prof_bytes)
     hipLaunchKernelGGL(k_count_wedges, dim3(grid_for(m, 4, 4096)), dim3(256), 0, s, A.rp, A.col, B.rp, m,
                        ctx().prof_dev.as<unsigned long long>() + 1);
-  const int bm = (M.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(M.ncols, 32) : 0;
+  const int bm = (M.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(M.ncols, 32) : (int)(MXM_FILTER_COLS / 32);
   if (mode == 1) {
     dispatch_pat<int64_t, 1>(A, B, M, nullptr, sums, row_group, w, bm, s);
     return;

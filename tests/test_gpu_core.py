@@ -196,3 +196,32 @@ def test_truss_numbers_in_page_locked_buffers(gpu, G, oracle):
     assert t3.ctypes.data == addr
     R.close()
     assert np.array_equal(t1, want) and np.array_equal(t3, want)
+
+
+def _wide_graph(G, n_wide=(1 << 19) + 1000, seed=11):
+    """R-MAT 13 plus a hub of 5000 neighbours, its vertices spread over
+    n_wide > 2^18 ids: mask rows wider than the exact LDS bitmap (the filter
+    path, with columns that alias in it) and one mask row longer than the
+    LDS-staged capacity (the global search)."""
+    rng = np.random.default_rng(seed)
+    g = G.rmat(13, 16, seed=seed)
+    rows = np.repeat(np.arange(g.n), np.diff(g.rowptr))
+    ids = np.sort(rng.choice(n_wide - 1, size=g.n, replace=False))  # an order-keeping spread
+    hub_nb = ids[rng.choice(g.n, size=5000, replace=False)]  # the hub is the last vertex: a long lower row
+    u = np.concatenate([ids[rows], np.full(5000, n_wide - 1)])
+    v = np.concatenate([ids[g.col], hub_nb])
+    return G.CSR.from_edges(n_wide, u, v)
+
+
+def test_wide_matrix_filter_paths(gpu, G, oracle):
+    g = _wide_graph(G)
+    print(f"case wide n={g.n} nnz={g.nnz} maxdeg={g.degrees().max()}", flush=True)
+    got = gpu.mxm_masked(g, g, g, dtype=np.int32)
+    want = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
+    assert np.array_equal(got.astype(np.int64), want)
+    R = gpu.Graph(g)
+    assert R.triangle_count() == oracle.tc(g)
+    truss, kmax, _ = R.truss()
+    want_t, want_kmax, *_ = oracle.truss_decomposition(g)
+    R.close()
+    assert kmax == want_kmax and np.array_equal(truss, want_t)
