@@ -213,8 +213,11 @@ def _wide_graph(G, n_wide=(1 << 19) + 1000, seed=11):
     return G.CSR.from_edges(n_wide, u, v)
 
 
-def test_wide_matrix_filter_paths(gpu, G, oracle):
-    g = _wide_graph(G)
+@pytest.mark.parametrize("n_wide", [1 << 18, (1 << 19) + 1000])
+def test_wide_matrix_filter_paths(gpu, G, oracle, n_wide):
+    """2^18 columns: the exact bitmap (sums count its bits, products with
+    positions take the rank lookup); wider: the filter and a search."""
+    g = _wide_graph(G, n_wide)
     print(f"case wide n={g.n} nnz={g.nnz} maxdeg={g.degrees().max()}", flush=True)
     got = gpu.mxm_masked(g, g, g, dtype=np.int32)
     want = oracle.mxm_masked(g.n, g.n, g.rowptr, g.col, g.rowptr, g.col, g.rowptr, g.col)
