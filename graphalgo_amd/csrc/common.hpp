@@ -135,6 +135,7 @@ struct Context {
   hipStream_t stream = nullptr;
   int num_cu = 0;
   int64_t hbm_bytes = 0;
+  int64_t lds_per_block = 65536;  // the device's LDS limit per workgroup
   std::string name;
   // row binning default: rows with at most this many products (and a mask
   // of at most 128 entries) go to the wavefront-per-row kernel; GA_WAVE_FLOPS

@@ -355,6 +355,7 @@ ga_status ga_init(int device) {
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
     c->hbm_bytes = (int64_t)prop.totalGlobalMem;
+    c->lds_per_block = (int64_t)prop.sharedMemPerBlock;
     c->name = prop.gcnArchName;
     GA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* wf = std::getenv("GA_WAVE_FLOPS")) c->wave_flops = std::atoll(wf);

@@ -39,6 +39,9 @@ constexpr int MXM_G = GA_MXM_G;  // lanes that read adjacent B columns (one 64-b
 constexpr int MXM_K = GA_MXM_K;  // wedges per lane per trip (gathers in flight)
 constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up to 2^17 columns
 constexpr int MXM_RANK_MAX_WORDS_POS = 8192;                 // products that need positions: up to 2^18
+// sums (triangle counts) need membership only: an exact bitmap up to 2^20
+// columns (128 KB of LDS; no rank bases)
+constexpr int64_t MXM_SUM_BITMAP_MAX_COLS = int64_t(1) << 20;
 // wider matrices: a filter of this many bits (column c sets bit c mod it),
 // small enough to keep eight wavefronts per SIMD; a search of the mask row
 // confirms its hits
@@ -272,7 +275,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
     for (int p = tid; p < mnz; p += MXM_THREADS) {
       const int c = Mcol[ms + p];
       if (RANK) {
-        if (p == 0 || (Mcol[ms + p - 1] >> 5) != (c >> 5)) s_wpre[c >> 5] = (uint16_t)p;
+        if (MODE != 1 && (p == 0 || (Mcol[ms + p - 1] >> 5) != (c >> 5))) s_wpre[c >> 5] = (uint16_t)p;
       } else if (local) {
         s_mcol[p] = c;
       }
@@ -400,7 +403,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
             wd[u] = s_bitmap[wi[u]];
           }
 #pragma unroll
-          for (int u = 0; u < MXM_K; ++u) pre[u] = s_wpre[wi[u]];
+          for (int u = 0; u < MXM_K; ++u) pre[u] = MODE == 1 ? 0 : s_wpre[wi[u]];  // a sum needs no positions
 #pragma unroll
           for (int u = 0; u < MXM_K; ++u) {
             const uint32_t bit = 1u << (cc[u] & 31);
@@ -1719,12 +1722,17 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // (a sum needs no positions: it tests the exact bitmap alone, in less LDS;
   // products with 4-byte accumulators -- supports -- take the rank lookup up
   // to 2^18 columns: R-MAT 18 k-truss k = 3 12.1 -> 7.1 ms)
-  const int rank_words = (MODE != 1 && sizeof(T) == 4) ? MXM_RANK_MAX_WORDS_POS : MXM_RANK_MAX_WORDS;
-  const bool rank = bitmap_words > 0 && bitmap_words <= rank_words && w.max_mask_row < 65536 && bm_and == -1;
+  // a sum takes the rank instantiation for any exact bitmap: its bitmap
+  // test is the same, without rank bases
+  const int rank_words = MODE == 1 ? (int)(MXM_SUM_BITMAP_MAX_COLS / 32)
+                                   : (sizeof(T) == 4 ? MXM_RANK_MAX_WORDS_POS : MXM_RANK_MAX_WORDS);
+  const bool rank = bitmap_words > 0 && bitmap_words <= rank_words && (MODE == 1 || w.max_mask_row < 65536) &&
+                    bm_and == -1;
   auto kitems = rank ? k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, true>
                      : k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, false>;
   const int bm_words = bitmap_words + (rank ? 1 : 0);  // rank: one zero word for dropped wedges
-  const size_t dyn = sizeof(uint32_t) * (size_t)bm_words + (rank ? sizeof(uint16_t) * ((bm_words + 1) & ~1) : 0);
+  const size_t dyn =
+      sizeof(uint32_t) * (size_t)bm_words + (rank && MODE != 1 ? sizeof(uint16_t) * ((bm_words + 1) & ~1) : 0);
   Context& c = ctx();
   const int occ_r = occupancy_of(krows, 64 * WR_WAVES, 0, c.device);
   const int occ_i = occupancy_of(kitems, THREADS, dyn, c.device);
@@ -2168,7 +2176,19 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
                        ctx().prof_dev.as<unsigned long long>() + 1);
   const int bm = (M.ncols <= MXM_BITMAP_MAX_COLS) ? (int)ceil_div(M.ncols, 32) : (int)(MXM_FILTER_COLS / 32);
   if (mode == 1) {
-    dispatch_pat<int64_t, 1>(A, B, M, nullptr, sums, row_group, w, bm, s);
+    // (the exact bitmap while it fits the workgroup's LDS beside ~8 KB of
+    // the kernel's own arrays)
+    const int64_t words = ceil_div(M.ncols, 32);
+    const bool fits = M.ncols <= MXM_SUM_BITMAP_MAX_COLS && 4 * (words + 1) + 16384 <= ctx().lds_per_block;
+    const int bm_sum = fits ? (int)words : bm;
+    if (bm_sum > 16384 && A.val == nullptr && B.val == nullptr) {
+      // a bitmap of more than 64 KB leaves room for one workgroup per CU:
+      // 1024-entry items keep 16 wavefronts on it (R-MAT 20 triangle count)
+      launch_binned<int64_t, true, true, 1, 1024, false>(A, B, M, nullptr, sums, row_group, nullptr, w, bm_sum, s,
+                                                         nullptr, 0);
+      return;
+    }
+    dispatch_pat<int64_t, 1>(A, B, M, nullptr, sums, row_group, w, bm_sum, s);
     return;
   }
   switch (dtype) {
