@@ -1724,8 +1724,12 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
   // to 2^18 columns: R-MAT 18 k-truss k = 3 12.1 -> 7.1 ms)
   // a sum takes the rank instantiation for any exact bitmap: its bitmap
   // test is the same, without rank bases
+  // (1024-entry items of 4-byte supports: up to 2^19 columns, 96 KB of
+  // bitmap and rank bases beside their 44 KB of arrays)
   const int rank_words = MODE == 1 ? (int)(MXM_SUM_BITMAP_MAX_COLS / 32)
-                                   : (sizeof(T) == 4 ? MXM_RANK_MAX_WORDS_POS : MXM_RANK_MAX_WORDS);
+                                   : (sizeof(T) == 4 ? (THREADS >= 1024 ? 2 * MXM_RANK_MAX_WORDS_POS
+                                                                        : MXM_RANK_MAX_WORDS_POS)
+                                                     : MXM_RANK_MAX_WORDS);
   const bool rank = bitmap_words > 0 && bitmap_words <= rank_words && (MODE == 1 || w.max_mask_row < 65536) &&
                     bm_and == -1;
   auto kitems = rank ? k_mxm_items<T, PA, PB, MODE, THREADS, BATCHED, true>
@@ -2052,9 +2056,18 @@ void support_symmetric_dev(const DCsr& C, const int32_t* mirror, int32_t* S, Mxm
   if (cx.support_tri > 0 || (cx.support_tri < 0 && C.nnz >= tri_min)) {
     // each triangle i > k > j once, from the wedge (i, k, j) of C<L> = L*L
     // (L = the strict lower half): +1 to (i, j) (mask), (i, k) (A) and
-    // (k, j) (B); then the upper half from the lower
-    launch_binned<int32_t, true, true, 2, MXM_WIDE, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s, nullptr,
-                                                            0, nullptr, nullptr, split);
+    // (k, j) (B); then the upper half from the lower.  Up to 2^19 columns
+    // the mask positions come from an exact bitmap with rank bases (1024-entry
+    // items past 2^18 columns: R-MAT 19 k-truss); wider graphs filter and
+    // search
+    const int64_t words = ceil_div(C.ncols, 32);
+    if (C.ncols > MXM_BITMAP_MAX_COLS && words <= 2 * MXM_RANK_MAX_WORDS_POS && w.max_mask_row < 65536 &&
+        6 * (words + 2) + 48 * 1024 <= cx.lds_per_block)
+      launch_binned<int32_t, true, true, 2, 1024, false>(C, C, C, S, nullptr, nullptr, nullptr, w, (int)words, s,
+                                                         nullptr, 0, nullptr, nullptr, split);
+    else
+      launch_binned<int32_t, true, true, 2, MXM_WIDE, false>(C, C, C, S, nullptr, nullptr, nullptr, w, bm, s,
+                                                              nullptr, 0, nullptr, nullptr, split);
     const unsigned g = (unsigned)std::min<int64_t>(grid_for(C.nnz, 256, 8192), 4 * (int64_t)cx.num_cu);
     hipLaunchKernelGGL(k_sup_mirror, dim3(g), dim3(256), 0, s, C.rp, n, mirror, S, w.skip);
     GA_HIP(hipGetLastError());

@@ -213,10 +213,12 @@ def _wide_graph(G, n_wide=(1 << 19) + 1000, seed=11):
     return G.CSR.from_edges(n_wide, u, v)
 
 
-@pytest.mark.parametrize("n_wide", [1 << 18, (1 << 19) + 1000])
+@pytest.mark.parametrize("n_wide", [1 << 18, (1 << 19) - 1000, (1 << 19) + 1000])
 def test_wide_matrix_filter_paths(gpu, G, oracle, n_wide):
     """2^18 columns: the exact bitmap (sums count its bits, products with
-    positions take the rank lookup); wider: the filter and a search."""
+    positions take the rank lookup); up to 2^19: the supports' 1024-entry
+    items with rank bases; wider: sums on 1024-entry items over an exact
+    bitmap, products with positions through the filter and a search."""
     g = _wide_graph(G, n_wide)
     print(f"case wide n={g.n} nnz={g.nnz} maxdeg={g.degrees().max()}", flush=True)
     got = gpu.mxm_masked(g, g, g, dtype=np.int32)
