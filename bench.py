@@ -31,7 +31,7 @@ warm-up, median of several runs) that regenerates the same seeded inputs
 and never touches the GPU; the GPU results are checked against the oracle
 in this process.
 
---workload NAME runs one workload alone (also: tc, ktruss_one, ewise, copy).
+--workload NAME runs one workload alone (also: tc, tc_wide, ktruss_one, ewise, copy).
 """
 from __future__ import annotations
 
@@ -290,6 +290,46 @@ def bench_tc(args) -> dict:
         out["config"]["oracle_triangles_equal"] = True
         out["cpu_baseline"] = cpu_baseline("tc", args)
     return out
+
+
+def bench_tc_wide(args) -> dict:
+    """Triangle counts C<L> = L*L on R-MAT graphs of 2^18..2^20 vertices: the
+    survey's R-MAT 18 probe family (edge factor 16, seed 1) and the served
+    pool's scale 18, 19 and 20 graphs, resident, one call per step; each case
+    with its wedges, kernel roofline and the oracle's count.  value =
+    undirected edges / s of R-MAT 18 (edge factor 16)."""
+    from graphalgo_amd import api, graphs as G, serving as S
+    pool = {c["scale"]: c for c in S.pool_configs()}
+    cases = [("rmat18_ef16_s1", dict(scale=18, edge_factor=16, seed=1))]
+    for sc in (18, 19, 20):
+        c = pool[sc]
+        cases.append((f"rmat{sc}_ef{c['edge_factor']}_s{c['seed']}",
+                      dict(scale=sc, edge_factor=c["edge_factor"], seed=c["seed"])))
+    O = None if args.no_cpu else _oracle()
+    rows, head = [], None
+    for name, cfg in cases:
+        g = G.rmat(**cfg)
+        R = api.Graph(g)
+        dt, t, timing, acct = _measure(R.triangle_count, args.steps, args.warmup)
+        R.close()
+        rl = _roofline(timing, acct, "k_mxm_items (C<L> = L*L, sum mode)")
+        row = {"graph": name, "n": g.n, "m_undirected": g.m_undirected, "ms": round(dt * 1e3, 3),
+               "edges_per_s": g.m_undirected / dt, "triangles": t, "wedges": acct["wedges"],
+               "g_wedges_per_s": round(acct["wedges"] / dt / 1e9, 1), "kernel_us": rl["avg_launch_us"],
+               "frac": rl["frac"]}
+        if O is not None:
+            want = O.tc(g)
+            assert t == want, f"{name}: GPU triangle count {t} differs from the oracle's {want}"
+            row["oracle_triangles_equal"] = True
+        rows.append(row)
+        _log(f"tc_wide {name}: {row['ms']} ms, {row['g_wedges_per_s']} G wedges/s, frac {row['frac']}")
+        if head is None:
+            head = (g.m_undirected, dt, rl)
+    m, dt, rl = head
+    return {"value": m / dt, "unit": "edges/s", "ms_per_step": dt * 1e3, "dtype": "int32",
+            "config": {"workload": "triangle count C<L>=L*L on R-MAT graphs of scale 18..20 (headline: scale 18, "
+                                   "edge factor 16, seed 1)", "cases": rows},
+            "roofline": rl}
 
 
 def cpu_tc(args) -> dict:
@@ -735,7 +775,8 @@ def bench_ktruss_one(args) -> dict:
             "cases": rows}
 
 
-WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "tc_er": bench_tc_er, "copy": bench_copy, "ewise": bench_ewise,
+WORKLOADS = {"ktruss": bench_ktruss, "tc": bench_tc, "tc_er": bench_tc_er, "tc_wide": bench_tc_wide,
+             "copy": bench_copy, "ewise": bench_ewise,
              "batch": bench_batch, "bc": bench_bc, "serve": bench_serve, "ktruss_one": bench_ktruss_one,
              "mxm": bench_mxm}
 
