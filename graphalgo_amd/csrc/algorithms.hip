@@ -624,12 +624,10 @@ constexpr int PEEL_SEGS = 32;
 constexpr int PEEL_CNT_STRIDE = 32;  // ints from one counter to the next (128 bytes)
 
 // A list item is an edge (i, j), i > j: e = (its canonical entry (row i,
-// column j), the transposed entry, i, j), b = the row bounds of i and j
-// (rp[i], rp[i + 1], rp[j], rp[j + 1]).  Carrying the endpoints and their
-// bounds saves a round two dependent loads (both columns, then the row
-// pointers) before it can touch the rows.
+// column j), the transposed entry, i, j).  Carrying the endpoints saves a
+// round the dependent load of both columns before the row bounds.
 struct alignas(16) PeelItem {
-  int4 e, b;
+  int4 e;
 };
 
 struct PeelLists {
@@ -831,7 +829,7 @@ __device__ inline void peel_scan_body(const int32_t* __restrict__ rp, const int3
         stamp[p] = r;
         stamp[mm[u]] = r;
         if (slot >= 0)
-          list[slot] = PeelItem{make_int4((int32_t)p, mm[u], i, j), make_int4(rp[i], rp[i + 1], rp[j], rp[j + 1])};
+          list[slot] = PeelItem{make_int4((int32_t)p, mm[u], i, j)};
       }
       at += __popcll(mk[u]);
     }
@@ -897,12 +895,12 @@ struct PeelEdge {
   int c, mc, ab, ae, arow, bb, be, brow, stride, m;
 };
 
-__device__ inline PeelEdge peel_edge(const PeelItem& it, int cap) {
+__device__ inline PeelEdge peel_edge(const PeelItem& it, const int32_t* __restrict__ rp, int cap) {
   PeelEdge e;
   e.c = it.e.x;
   e.mc = it.e.y;
   const int i = it.e.z, j = it.e.w;
-  const int ib = it.b.x, ie = it.b.y, jb = it.b.z, je = it.b.w;
+  const int ib = rp[i], ie = rp[i + 1], jb = rp[j], je = rp[j + 1];
   const bool pi = (ie - ib) <= (je - jb);  // probe row i
   e.ab = pi ? ib : jb;
   e.ae = pi ? ie : je;
@@ -983,16 +981,13 @@ __device__ inline void peel_probe(const PeelEdge& e, int q, const PeelQ& v, cons
   const bool xa = sa > r && atomicSub(&sup[cid_a], 1) == thr;  // edge (arow, w): entry q lies in row arow
   const bool xb = sb > r && atomicSub(&sup[cid_b], 1) == thr;  // edge (brow, w): entry pos lies in row brow
   if (!xa && !xb) return;
-  const int wb = rp[w], we = rp[w + 1];  // in flight with the list claim that follows
   if (xa) {
     push_a.e = make_int4(cid_a, cid_a == q ? mq : q, max(e.arow, w), min(e.arow, w));
-    push_a.b = e.arow > w ? make_int4(e.ab, e.ae, wb, we) : make_int4(wb, we, e.ab, e.ae);
     stamp[q] = r + 1;
     stamp[mq] = r + 1;
   }
   if (xb) {
     push_b.e = make_int4(cid_b, cid_b == pos ? mp : pos, max(e.brow, w), min(e.brow, w));
-    push_b.b = e.brow > w ? make_int4(e.bb, e.be, wb, we) : make_int4(wb, we, e.bb, e.be);
     stamp[pos] = r + 1;
     stamp[mp] = r + 1;
   }
@@ -1057,7 +1052,7 @@ __device__ inline void peel_round_body(const PeelItem* __restrict__ cur, const P
     for (int64_t v = blockIdx.x * wpb + (threadIdx.x >> 6); v < nv; v += nw) {
       const int64_t t = v / wpe;
       const int part = (int)(v - t * wpe);
-      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), PEEL_WSTAGE);
+      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), g.rp, PEEL_WSTAGE);
       if (part == 0 && lane == 0) {
         if (g.truss) {
           g.truss[g.eid[e.c]] = tval;
@@ -1092,7 +1087,7 @@ __device__ inline void peel_round_body(const PeelItem* __restrict__ cur, const P
     const int split = max(PEEL_SPLIT, min(PEEL_SPLIT_MAX, (int)(gridDim.x / max(nc, 1))));
     for (int64_t b = blockIdx.x; b < (int64_t)nc * split; b += gridDim.x) {
       const int t = (int)(b / split), part = (int)(b % split);
-      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), PEEL_BSTAGE);
+      const PeelEdge e = peel_edge(peel_item(cur, cv, L, t, lane), g.rp, PEEL_BSTAGE);
       if (part == 0 && threadIdx.x == 0) {
         if (g.truss) {
           g.truss[g.eid[e.c]] = tval;
@@ -1293,7 +1288,7 @@ __global__ void k_peel_remap(PeelLists L, const PeelCtl* __restrict__ ctl, const
     if (t < n) {
       const PeelItem it = list[slot];
       const int c = newpos[it.e.x], m = newpos[it.e.y], i = it.e.z, j = it.e.w;  // the endpoints do not move
-      list[slot] = PeelItem{make_int4(c, m, i, j), make_int4(nrp[i], nrp[i + 1], nrp[j], nrp[j + 1])};
+      list[slot] = PeelItem{make_int4(c, m, i, j)};
       stamp[c] = r1;
       stamp[m] = r1;
     }
