@@ -37,6 +37,7 @@ constexpr int64_t MXM_BITMAP_MAX_COLS = int64_t(1) << 18;  // 32 KB bitmap
 #endif
 constexpr int MXM_G = GA_MXM_G;  // lanes that read adjacent B columns (one 64-byte run)
 constexpr int MXM_K = GA_MXM_K;  // wedges per lane per trip (gathers in flight)
+constexpr int MXM_KV = 6;        // the same for valued products (no spill at 64 VGPRs, where 8 spills ten)
 constexpr int MXM_RANK_MAX_WORDS = 4096;                     // rank lookup up to 2^17 columns
 constexpr int MXM_RANK_MAX_WORDS_POS = 8192;                 // products that need positions: up to 2^18
 // sums (triangle counts) need membership only: an exact bitmap up to 2^20
@@ -64,12 +65,14 @@ constexpr int MXM_CTR_WORDS = 2 * MXM_NSEG * MXM_CTR_STRIDE + 32;  // item segme
 // entry scan); MCAP = mask entries staged in LDS.  BATCHED: the operands are
 // many graphs stored back to back with graph-local column indices; row i
 // belongs to graph row_group[i] whose rows start at gbase[row_group[i]].
-// (pattern products with 4-byte or no accumulators: at most 64 VGPRs, eight
-// wavefronts per SIMD to hide the dependent loads; valued ones would spill
-// under that cap, 8-byte accumulators are bound by LDS instead)
+// (products with 4-byte or no accumulators: at most 64 VGPRs, eight
+// wavefronts per SIMD to hide the dependent loads; valued ones fit with six
+// wedges per trip: C<A> = A*A of R-MAT 16 with int32 values 2.43 ms at 81
+// VGPRs and five wavefronts, 1.63 ms at 63 and eight; 8-byte accumulators
+// are bound by LDS instead)
 template <typename T, bool PAT_A, bool PAT_B, int MODE, int THREADS, bool BATCHED, bool RANK>
 __global__ __launch_bounds__(THREADS)
-__attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1 || sizeof(T) == 4) ? 8 : 1)))) void k_mxm_items(
+__attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : ((MODE == 1 || sizeof(T) == 4) ? 8 : 1)))) void k_mxm_items(
     const int32_t* __restrict__ Arp, const int32_t* __restrict__ Acol, const T* __restrict__ Aval,
     const int32_t* __restrict__ Brp, const int32_t* __restrict__ Bcol, const T* __restrict__ Bval,
     const int32_t* __restrict__ Mrp, const int32_t* __restrict__ Mcol, T* __restrict__ Cval,
@@ -329,14 +332,16 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
     // contiguous wedge range [g0, g1) of span = ceil(total / NG) rounded up to
     // whole trips (small items leave the last groups idle rather than run
     // every group on a partly filled trip).  A trip of the group covers its
-    // next MXM_G * MXM_K wedges, lane r taking every MXM_G-th of them: the
+    // next MXM_G * KW wedges, lane r taking every MXM_G-th of them: the
     // MXM_G lanes read adjacent B columns (64-byte runs) and a lane's wedges
     // are MXM_G apart, so its entry only moves forward by a short linear
     // walk.  One binary search per lane and item finds the entry of its
     // first wedge; there is none per trip.
     const int grp = tid / MXM_G, rl = tid % MXM_G;
     constexpr int MXM_NG = MXM_THREADS / MXM_G;
-    constexpr int MXM_RUN = MXM_G * MXM_K;
+    // valued products: fewer wedges per trip (registers for the values)
+    constexpr int KW = (PAT_A && PAT_B) ? MXM_K : MXM_KV;
+    constexpr int MXM_RUN = MXM_G * KW;
     const long long span = (((long long)total + MXM_NG - 1) / MXM_NG + MXM_RUN - 1) / MXM_RUN * MXM_RUN;
     const int g0 = (int)min<long long>(total, span * grp);
     const int g1 = (int)min<long long>(total, span * (grp + 1));
@@ -362,9 +367,9 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
         constexpr bool FULL = decltype(full_c)::value;
         int jb = s_jb[e];
         int nb = s_eoff[e + 1];
-        int cc[MXM_K], jj[MXM_K], ee[MXM_K];
+        int cc[KW], jj[KW], ee[KW];
 #pragma unroll
-        for (int u = 0; u < MXM_K; ++u) {
+        for (int u = 0; u < KW; ++u) {
           // wedges past the range re-read its last one (in bounds) and are dropped
           const bool ok = FULL || p0 + u * MXM_G < g1;
           const int w = ok ? p0 + u * MXM_G : g1 - 1;
@@ -394,18 +399,18 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
         };
         if (RANK) {
           // all bitmap words, then all rank bases, then the hits: the LDS
-          // reads of the MXM_K wedges are in flight together
-          uint32_t wd[MXM_K];
-          int wi[MXM_K], pre[MXM_K];
+          // reads of the KW wedges are in flight together
+          uint32_t wd[KW];
+          int wi[KW], pre[KW];
 #pragma unroll
-          for (int u = 0; u < MXM_K; ++u) {
+          for (int u = 0; u < KW; ++u) {
             wi[u] = cc[u] >> 5;  // a dropped wedge reads the zero word
             wd[u] = s_bitmap[wi[u]];
           }
 #pragma unroll
-          for (int u = 0; u < MXM_K; ++u) pre[u] = MODE == 1 ? 0 : s_wpre[wi[u]];  // a sum needs no positions
+          for (int u = 0; u < KW; ++u) pre[u] = MODE == 1 ? 0 : s_wpre[wi[u]];  // a sum needs no positions
 #pragma unroll
-          for (int u = 0; u < MXM_K; ++u) {
+          for (int u = 0; u < KW; ++u) {
             const uint32_t bit = 1u << (cc[u] & 31);
             if (wd[u] & bit) {
               accumulate(pre[u] + __popc(wd[u] & (bit - 1u)),
@@ -418,7 +423,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? 6 : (PAT_A && PAT_B && (MODE == 1
           }
         } else {
 #pragma unroll
-          for (int u = 0; u < MXM_K; ++u) {
+          for (int u = 0; u < KW; ++u) {
             const int c = cc[u];
             if (c < 0) continue;
             if (use_bm && !((s_bitmap[(c & bm_and) >> 5] >> (c & 31)) & 1u)) continue;
