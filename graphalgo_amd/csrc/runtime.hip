@@ -31,6 +31,7 @@ void* Lease::take(size_t b) {
 namespace {
 struct PinnedScratch {
   void* p = nullptr;
+  void* dev = nullptr;  // the device's address of p (mapped allocations)
   ~PinnedScratch() {
     if (p) (void)hipHostFree(p);
   }
@@ -45,8 +46,11 @@ void* pinned_scratch() {
 }
 
 long long* progress_words(long long** dev) {
-  if (!t_progress.p) GA_HIP(hipHostMalloc(&t_progress.p, 256, hipHostMallocMapped | hipHostMallocCoherent));
-  GA_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(dev), t_progress.p, 0));
+  if (!t_progress.p) {
+    GA_HIP(hipHostMalloc(&t_progress.p, 256, hipHostMallocMapped | hipHostMallocCoherent));
+    GA_HIP(hipHostGetDevicePointer(&t_progress.dev, t_progress.p, 0));
+  }
+  *dev = static_cast<long long*>(t_progress.dev);
   return static_cast<long long*>(t_progress.p);
 }
 

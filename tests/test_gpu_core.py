@@ -104,6 +104,26 @@ def test_tc_er_config(gpu, G, golden):
     assert gpu.triangle_count(er) == rec["tc"]
 
 
+def test_tc_er_resident_repeats(gpu, G, golden, oracle):
+    """Repeated counts of resident graphs: from the second call on, a graph
+    runs its kept plan (statically split short rows) and each call's publish
+    kernel leaves the sum and the claim counters zero for the next; two
+    graphs' counts alternate, each with its own buffers."""
+    rec = golden["generated"]["er_1000_9990_s1"]
+    er = G.erdos_renyi(**rec["cfg"])
+    other = G.erdos_renyi(n=700, m=9000, seed=5)
+    want_other = oracle.tc(other)
+    R, Ro = gpu.Graph(er), gpu.Graph(other)
+    try:
+        for it in range(6):
+            print(f"case tc er resident call {it}", flush=True)
+            assert R.triangle_count() == rec["tc"]
+            assert Ro.triangle_count() == want_other
+    finally:
+        R.close()
+        Ro.close()
+
+
 @pytest.mark.parametrize("key", ["rmat16_ef16_s1", "rmat18_ef16_s1"])
 def test_tc_rmat_full_size(gpu, G, golden, key):
     rec = golden["generated"][key]
