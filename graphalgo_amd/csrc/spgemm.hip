@@ -2199,9 +2199,10 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
     const int64_t words = ceil_div(M.ncols, 32);
     const bool fits = M.ncols <= MXM_SUM_BITMAP_MAX_COLS && 4 * (words + 1) + 16384 <= ctx().lds_per_block;
     const int bm_sum = fits ? (int)words : bm;
-    if (bm_sum > 16384 && A.val == nullptr && B.val == nullptr) {
-      // a bitmap of more than 64 KB leaves room for one workgroup per CU:
-      // 1024-entry items keep 16 wavefronts on it (R-MAT 20 triangle count)
+    if (bm_sum > 8192 && A.val == nullptr && B.val == nullptr) {
+      // a bitmap of more than 32 KB leaves room for one or two workgroups per
+      // CU: 1024-entry items keep 16 wavefronts on each (R-MAT 19 count 4.51
+      // -> 3.99 ms against 512-entry items; R-MAT 20's 128 KB bitmap)
       launch_binned<int64_t, true, true, 1, 1024, false>(A, B, M, nullptr, sums, row_group, nullptr, w, bm_sum, s,
                                                          nullptr, 0);
       return;
