@@ -2207,11 +2207,12 @@ void mxm_masked_dev(const DCsr& A, const DCsr& B, const DCsr& M, int dtype, int 
                                                          nullptr, 0);
       return;
     }
-    if (fits && bm_sum <= 2048 && A.val == nullptr && B.val == nullptr) {
-      // an exact bitmap of at most 8 KB: 256-entry items, twice the workgroups of
-      // 512 (short rows leave most of a wide item idle: R-MAT 16 count 258 ->
-      // 205 us; with the 32 KB bitmap of R-MAT 18 the LDS holds too few of
-      // them, 1.06 -> 1.21 ms, so wider graphs keep 512)
+    if (fits && bm_sum <= 4096 && A.val == nullptr && B.val == nullptr) {
+      // an exact bitmap of at most 16 KB: 256-entry items, twice the
+      // workgroups of 512 (short rows leave most of a wide item idle: R-MAT
+      // 16 count 258 -> 205 us, the served pool's scale-17 graph 0.61 -> 0.46
+      // ms; with the 32 KB bitmap of R-MAT 18 the LDS holds too few of them,
+      // 1.06 -> 1.21 ms, so wider graphs keep 512)
       launch_binned<int64_t, true, true, 1, 256, false>(A, B, M, nullptr, sums, row_group, nullptr, w, bm_sum, s,
                                                         nullptr, 0);
       return;
