@@ -1863,7 +1863,7 @@ struct ga_graph {
   OwnedCsr A;
   OwnedCsr L;
   bool have_L = false;
-  bool sums_clean = false;  // sums[0] is zero (the last count's publish kernel cleared it)
+  bool sums_clean = false;  // the sum and the arrival count are zero (the last count left them so)
   TrussWork tw;
   MxmWork mw;
   BcWork bw;
@@ -2040,14 +2040,30 @@ ga_status ga_graph_triangle_count(ga_graph* g, int64_t* triangles) {
       lower_triangle_dev(g->A.view(), g->L, g->scan_tmp, s);
       g->have_L = true;
     }
-    g->sums.ensure(64);
-    if (!g->sums_clean) fill_u64(g->sums.p, 0ull, 1, s);
+    g->sums.ensure(sizeof(unsigned long long) * (SUM_PUB_TICKET + 1));
+    if (!g->sums_clean) fill_u64(g->sums.p, 0ull, SUM_PUB_TICKET + 1, s);
     g->sums_clean = false;
     const DCsr vL = g->L.view();
+    // the count reaches the host through pinned memory: written by the count
+    // launch itself when its kept plan is one statically split kernel, else
+    // by a kernel that also clears the sum and the claim counters
+    long long* hdev = nullptr;
+    long long* h = progress_words(&hdev) + 4;  // words 4 (value) and 5 (sequence)
+    const long long seq = next_host_seq();
+    g->mw.pub_host = hdev + 4;
+    g->mw.pub_seq = seq;
+    g->mw.pub_done = false;
+    struct PubOff {
+      MxmWork& w;
+      ~PubOff() { w.pub_host = nullptr; }
+    } pub_off{g->mw};
     mxm_masked_dev(vL, vL, vL, GA_INT64, 1, nullptr, g->sums.as<unsigned long long>(), nullptr, g->mw, s);
-    // the count reaches the host through pinned memory, written by a kernel
-    // that also clears the sum and the claim counters for the next call
-    *triangles = (int64_t)publish_sum_and_wait(g->mw, g->sums.as<unsigned long long>(), s);
+    if (g->mw.pub_done) {
+      wait_host_word(&h[1], seq, s);
+      *triangles = (int64_t)__atomic_load_n(&h[0], __ATOMIC_RELAXED);
+    } else {
+      *triangles = (int64_t)publish_sum_and_wait(g->mw, g->sums.as<unsigned long long>(), s);
+    }
     g->sums_clean = true;
   });
 }

@@ -337,7 +337,17 @@ struct MxmWork {
   // short rows and work items of the kept plan, read back once when it is
   // built (-1 = unknown): an empty kernel of a kept plan is not launched
   int64_t plan_rows = -1, plan_items = -1;
+  // A one-target sum may publish its total itself: with pub_host set, a
+  // launch whose kept plan is one statically split short-row kernel has its
+  // last workgroup write the total to pub_host[0] and pub_seq to pub_host[1]
+  // (the sum and an arrival count in the sums buffer: words 0 and
+  // SUM_PUB_TICKET, zero before and after), and sets pub_done; otherwise the
+  // caller publishes with a kernel of its own.
+  long long* pub_host = nullptr;
+  long long pub_seq = 0;
+  bool pub_done = false;
 };
+constexpr int SUM_PUB_TICKET = 16;  // the arrival count's word: a 128-byte line after the sum
 
 // C<M> = A*B (structural mask).  mode 0: per-entry values into Cval (dtype);
 // mode 1: pattern inputs, int64 sum of all C entries accumulated into

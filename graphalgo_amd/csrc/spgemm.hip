@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     const int32_t* __restrict__ Mend, unsigned long long* __restrict__ sums, const int32_t* __restrict__ row_group,
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
     unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int chunk,
-    int static_rows) {
+    int static_rows, long long* __restrict__ pub_host, long long pub_seq) {
   // the slot bitmaps of every wave, then WR_BMW words that are never set
   // (tested by lanes past a chunk's last wedge); records address them by
   // word offset from the array's start
@@ -1347,7 +1347,25 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     if (threadIdx.x == 0) {
       unsigned long long tot = 0;
       for (int w = 0; w < WR_WAVES; ++w) tot += s_red[w];
-      if (tot) atomicAdd(&sums[0], tot);
+      if (!pub_host) {
+        if (tot) atomicAdd(&sums[0], tot);
+      } else {
+        // the launch publishes its own total.  Device-scope atomics complete
+        // at the one point every workgroup sees; the arrival ticket's add
+        // takes the sum add's returned value as an operand, so it is issued
+        // only once the sum add has completed, and the last arrival reads
+        // the total after every add (no release fence: those cost more than
+        // the publish launch they replace)
+        const unsigned long long old = atomicAdd(&sums[0], tot);
+        const unsigned long long inc = 1ull + (old == ~0ull ? 1ull : 0ull);  // depends on the add's result
+        if (atomicAdd(&sums[SUM_PUB_TICKET], inc) + 1 == (unsigned long long)gridDim.x) {
+          const unsigned long long total =
+              __hip_atomic_exchange(&sums[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&sums[SUM_PUB_TICKET], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&pub_host[0], (long long)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          __hip_atomic_store(&pub_host[1], pub_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+      }
     }
   }
   if (wedge_counter && lane == 0 && wedges) atomicAdd(wedge_counter, wedges);
@@ -1783,10 +1801,18 @@ static void launch_binned(const DCsr& A, const DCsr& B, const DCsr& M, void* Cva
       // small graph's count)
       const int static_rows =
           (keep_plan && w.plan_rows >= 0 && w.plan_rows <= grid * WR_WAVES * (int64_t)chunk) ? 1 : 0;
+      // one statically split kernel is the whole launch: it publishes the
+      // total itself (no claim counter touched, nothing left to clear)
+      const bool pub = w.pub_host && static_rows && !run_items && !BATCHED && !row_group && !wc;
       hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * WR_WAVES), 0, s, A.rp, A.col, B.rp, B.col, M.rp,
                          M.col, Mend, sums, row_group, gbase, (const int32_t*)w.rows.as<int32_t>(),
                          (const int32_t*)(w.rstart.as<int32_t>() + m),
-                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, chunk, static_rows);
+                         w.counter.as<unsigned int>() + MXM_NSEG * MXM_CTR_STRIDE, wc, chunk, static_rows,
+                         pub ? w.pub_host : (long long*)nullptr, w.pub_seq);
+      if (pub) {
+        w.pub_done = true;
+        w.counters_clean = true;  // cleared before the launch, untouched by it
+      }
     };
     if (sum_sub == 2) {
       if (trunc) launch_wave(k_sum_wave<BATCHED, 4, true>);
