@@ -1088,6 +1088,9 @@ __global__ __launch_bounds__(64 * WR_WAVES) __attribute__((amdgpu_waves_per_eu(8
     const int32_t* __restrict__ gbase, const int32_t* __restrict__ rows, const int32_t* __restrict__ n_rows_ptr,
     unsigned int* __restrict__ counter, unsigned long long* __restrict__ wedge_counter, int chunk,
     int static_rows, long long* __restrict__ pub_host, long long pub_seq) {
+  // trips of 64 K wedges tile a window of 32 SW_WINW exactly (a trip reads
+  // 2 K start-bit words of the window as 16-byte groups)
+  static_assert((32 * SW_WINW) % (64 * K) == 0 && (2 * K) % 4 == 0, "K must divide the start-bit window");
   // the slot bitmaps of every wave, then WR_BMW words that are never set
   // (tested by lanes past a chunk's last wedge); records address them by
   // word offset from the array's start
