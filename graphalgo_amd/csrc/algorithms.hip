@@ -599,7 +599,7 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
 // never scan the graph: only a new k does (k_peel_scan, entry-parallel: an
 // entry is canonical when its transposed entry lies before it).  Dead
 // entries stay in the rows until a compaction at a k boundary drops them
-// (when they are at least half of the stored entries).
+// (when they are at least a third of the stored entries).
 constexpr int32_t PEEL_ALIVE = 0x7fffffff;
 
 // Counters of a peeling run that change from round to round.
