@@ -601,6 +601,11 @@ static void truss_steps(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tr
 // entries stay in the rows until a compaction at a k boundary drops them
 // (when they are at least a third of the stored entries).
 constexpr int32_t PEEL_ALIVE = 0x7fffffff;
+// A tick reports its progress to pinned host memory every PEEL_REPORT ticks
+// and when the run ends (every 8: 10.92 ms per R-MAT 16 decomposition, every
+// 4: 11.01, every 2: 11.04).  The host keeps more ticks than that queued, or
+// it would wait for a report that no queued tick makes.
+constexpr int64_t PEEL_REPORT = 8;
 
 // Counters of a peeling run that change from round to round.
 struct PeelDyn {
@@ -1218,7 +1223,7 @@ __global__ __launch_bounds__(256) void k_peel_tick(int64_t t, PeelLists L, PeelC
       ctl->nnz_at[rec_kidx] = rec_alive;
       ctl->passes_at[rec_kidx] = rec_passes;
     }
-    if ((t & 3) == 0 || d.fin) peel_publish(host, cv.total, d.alive, d.fin, t, 0);
+    if ((t & (PEEL_REPORT - 1)) == 0 || d.fin) peel_publish(host, cv.total, d.alive, d.fin, t, 0);
   }
   if (d.fin || cv.over) return;
   const int nl = (int)((t + 1) & 3);
@@ -1540,10 +1545,11 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
     out.kmax = kmax;
   } else {
     // device-driven: ticks queued a few ahead of the device, which reports
-    // every fourth tick and when the run ends; to compact, the host drains
+    // every PEEL_REPORT ticks and when the run ends; to compact, the host drains
     // the queue through k_peel_drain and compacts when the dead entries are
     // at least a third of the stored ones
     constexpr int64_t kInFlight = 12;
+    static_assert(kInFlight > PEEL_REPORT && (PEEL_REPORT & (PEEL_REPORT - 1)) == 0, "a queued tick must report");
     const int64_t max_ticks = nnz0 / 2 + n + nk + 64;  // a tick removes an edge or moves k on
     __atomic_store_n(&hw[1], (long long)nnz0, __ATOMIC_RELAXED);
     __atomic_store_n(&hw[2], 0ll, __ATOMIC_RELAXED);
