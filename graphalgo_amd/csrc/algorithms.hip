@@ -1391,7 +1391,9 @@ static void truss_peel(const DCsr& A, bool all_k, const int32_t* ks, int nk, Tru
   // is known; rounds smaller than the grid spread each edge over several
   // wavefronts (peel_round_body)
   // (4 workgroups per CU measured best on R-MAT 16 against 2, 6 and 8)
-  const unsigned gtick = std::min<unsigned>(gscan, 4 * (unsigned)c.num_cu);
+  // six workgroups of 256 threads per CU (R-MAT 16 decomposition 10.81 ms;
+  // three 11.25, four 10.91, five 10.84, eight 11.70)
+  const unsigned gtick = std::min<unsigned>(gscan, 6 * (unsigned)c.num_cu);
   auto timed = [&](auto&& launch) {  // a round launch, bracketed by events when profiling
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.prof_events_on) {
